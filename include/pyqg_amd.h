@@ -1,0 +1,135 @@
+/* C-ABI of the MI355X pyqg time-stepping core (libqgk).
+ *
+ * The library replaces the hot path of pyqg's `cdef class PseudoSpectralKernel`
+ * (pyqg/kernel.pyx:32-637): it owns the model state in HBM and runs the
+ * inversion, advection, friction, parameterization adds, AB3 step and the 2-D
+ * real FFTs as HIP kernels for gfx950, in FP64/complex128.
+ *
+ * Handles are "buckets": M independent members ("ensemble members") of one
+ * grid size (nz layers, ny x nx points).  A single pyqg model is a bucket with
+ * M = 1 created with QGK_FULL_STATE, which also keeps every intermediate field
+ * the reference exposes (ph, u, v, uh, vh, uq, vq, uqh, vqh, ...).
+ *
+ * Layouts (C order, host and device alike):
+ *   real field     double [M][nz][ny][nx]
+ *   spectral field double complex (interleaved re,im) [M][nz][nl][nk],
+ *                  nl = ny, nk = nx/2 + 1
+ * nx and ny must be powers of two in [8, 1024].
+ *
+ * Every function returns 0 on success or a negative QGK_E* code; the message
+ * of the last failure on the calling thread is qgk_last_error().  Calls are
+ * synchronous with respect to the host unless the name ends in _async.
+ */
+#ifndef PYQG_AMD_H
+#define PYQG_AMD_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qgk_bucket qgk_bucket;
+
+enum {
+  QGK_OK = 0,
+  QGK_E_ARG = -1,     /* bad argument (maps to ValueError) */
+  QGK_E_HIP = -2,     /* HIP runtime failure */
+  QGK_E_STATE = -3,   /* field not allocated for this handle (maps to AttributeError) */
+  QGK_E_NOMEM = -4
+};
+
+/* creation flags */
+enum {
+  QGK_FULL_STATE = 1, /* keep every reference-visible intermediate field */
+  QGK_HAS_Q_PARAM = 2, /* allocate dq, dqh   (kernel.pyx:174-178) */
+  QGK_HAS_UV_PARAM = 4 /* allocate du, dv, duh, dvh (kernel.pyx:164-172) */
+};
+
+/* field ids for qgk_put / qgk_get (the state arrays of kernel.pyx:37-67) */
+enum {
+  QGK_Q = 0, QGK_QH, QGK_PH, QGK_U, QGK_V, QGK_UH, QGK_VH, QGK_UQ, QGK_VQ, QGK_UQH,
+  QGK_VQH, QGK_DQHDT, QGK_DQHDT_P, QGK_DQHDT_PP, QGK_DQ, QGK_DQH, QGK_DU, QGK_DV,
+  QGK_DUH, QGK_DVH, QGK_NFIELDS
+};
+
+/* constant ids for qgk_set_const (kernel.pyx:84-104, property setters 528-575) */
+enum {
+  QGK_KK = 0,   /* double[nk]            zonal wavenumbers */
+  QGK_LL,       /* double[nl]            meridional wavenumbers */
+  QGK_K2L2,     /* double[nl][nk]        kk^2 + ll^2 (public _k2l2) */
+  QGK_FILTR,    /* double[nl][nk]        spectral filter */
+  QGK_A,        /* double[nz][nz][nl][nk] real inversion matrix */
+  QGK_UBG,      /* double[nz]            background zonal flow */
+  QGK_QY,       /* double[nz]            background PV gradient */
+  QGK_NCONSTS
+};
+
+/* Replaces PseudoSpectralKernel.__init__ (kernel.pyx:120-206) for nmembers
+ * members; all fields start at zero, filtr at one, rek at zero. */
+int qgk_create(int nmembers, int nz, int ny, int nx, int flags, qgk_bucket** out);
+int qgk_destroy(qgk_bucket* b);
+const char* qgk_last_error(void);
+
+/* Constant upload.  member < 0: one table shared by every member; member >= 0:
+ * that member's own table.  Replaces the kk/ll/_k2l2/filtr/a/Ubg/Qy setters
+ * (kernel.pyx:536-575). */
+int qgk_set_const(qgk_bucket* b, int which, int member, const double* host);
+/* Per-member tables packed member-major: host holds nmembers consecutive
+ * tables of constant `which`; member m uses the m-th. */
+int qgk_set_const_all(qgk_bucket* b, int which, const double* host);
+/* Two-layer closed-form inversion (qg_model.py:146-161) evaluated in the kernel
+ * from F1, F2 instead of a table; used by ensembles of QGModels. */
+int qgk_set_inversion_2layer(qgk_bucket* b, int member, double F1, double F2);
+/* Bottom drag rek (public attribute, kernel.pyx:101). member < 0: all. */
+int qgk_set_rek(qgk_bucket* b, int member, double rek);
+
+/* Field transfer, all members at once (host array of the full bucket size).
+ * Replaces the property getters / setters' copies (kernel.pyx:576-637). */
+int qgk_put(qgk_bucket* b, int field, const void* host);
+int qgk_get(qgk_bucket* b, int field, void* host);
+
+/* Per-phase operations, one per reference method. */
+int qgk_invert(qgk_bucket* b);                  /* kernel.pyx:312-355 */
+int qgk_advect(qgk_bucket* b);                  /* kernel.pyx:357-395 */
+int qgk_friction(qgk_bucket* b);                /* kernel.pyx:444-461 */
+int qgk_add_q_param(qgk_bucket* b, const double* dq_host);                      /* 424-442 */
+int qgk_add_uv_param(qgk_bucket* b, const double* du_host, const double* dv_host); /* 397-422 */
+/* AB update + filter + history rotation + q = irfft2(qh) (kernel.pyx:463-526).
+ * dt1..dt3 are the Adams-Bashforth weights for this step (kernel.pyx:482-500). */
+int qgk_timestep(qgk_bucket* b, double dt1, double dt2, double dt3);
+/* The q setter (rfft2 of q into qh, kernel.pyx:576-582) and the qh setter's
+ * inverse transform (kernel.pyx:583-591). */
+int qgk_q_to_qh(qgk_bucket* b);
+int qgk_qh_to_q(qgk_bucket* b);
+
+/* Generic transforms of one member-sized array (nz planes), copy in / copy
+ * out (kernel.pyx:291-309).  nplanes lets callers batch any number of planes. */
+int qgk_fft(qgk_bucket* b, int nplanes, const double* in_real, double* out_complex);
+int qgk_ifft(qgk_bucket* b, int nplanes, const double* in_complex, double* out_real);
+
+/* Fused stepping: nsteps full steps of _step_forward (model.py:406-436) without
+ * parameterizations, two kernels per step.  coeffs holds nsteps*3 AB weights
+ * (same for every member).  Afterwards qh and the history are current; the
+ * intermediate fields of QGK_FULL_STATE handles are not (qgk_materialize
+ * recomputes them by running the last step's phases).  use_graph != 0 replays
+ * the two-kernel step as a captured hipGraph once the weights stop changing. */
+int qgk_step(qgk_bucket* b, int nsteps, const double* coeffs, int use_graph);
+
+/* Kernel timing for the measurement harness: when enabled, the fused step
+ * records HIP events around each of its kernels; qgk_timing returns the
+ * summed milliseconds and launch counts of the row-phase and column-phase
+ * kernels since the last reset. */
+int qgk_timing_enable(qgk_bucket* b, int enable);
+int qgk_timing(qgk_bucket* b, double* ms_row, long* n_row, double* ms_col, long* n_col);
+int qgk_sync(qgk_bucket* b);
+
+/* Multi-bucket stepping for uneven ensembles: nsteps fused steps of every
+ * bucket, all launched on the first bucket's stream (one graph when
+ * use_graph != 0). */
+int qgk_step_buckets(qgk_bucket** bs, int nbuckets, int nsteps, const double* coeffs,
+                     int use_graph);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,176 @@
+"""ctypes binding of libqgk (include/pyqg_amd.h).
+
+The shared library is built in-tree by pyqg_amd.build.  There is no fallback:
+if it is missing or cannot be loaded, importing this module raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_qgk.so")
+
+# error codes
+E_ARG, E_HIP, E_STATE, E_NOMEM = -1, -2, -3, -4
+# flags
+FULL_STATE, HAS_Q_PARAM, HAS_UV_PARAM = 1, 2, 4
+# field ids
+FIELDS = ["q", "qh", "ph", "u", "v", "uh", "vh", "uq", "vq", "uqh", "vqh", "dqhdt",
+          "dqhdt_p", "dqhdt_pp", "dq", "dqh", "du", "dv", "duh", "dvh"]
+FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
+REAL_FIELDS = {"q", "u", "v", "uq", "vq", "dq", "du", "dv"}
+# constant ids
+CONSTS = ["kk", "ll", "k2l2", "filtr", "a", "Ubg", "Qy"]
+CONST_ID = {name: i for i, name in enumerate(CONSTS)}
+
+EXPORTS = ["qgk_create", "qgk_destroy", "qgk_last_error", "qgk_set_const", "qgk_set_const_all",
+           "qgk_set_inversion_2layer", "qgk_set_rek", "qgk_put", "qgk_get", "qgk_invert",
+           "qgk_advect", "qgk_friction", "qgk_add_q_param", "qgk_add_uv_param",
+           "qgk_timestep", "qgk_q_to_qh", "qgk_qh_to_q", "qgk_fft", "qgk_ifft", "qgk_step",
+           "qgk_timing_enable", "qgk_timing", "qgk_sync", "qgk_step_buckets"]
+
+
+class QGKError(RuntimeError):
+    pass
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"libqgk is not built ({LIB_PATH} missing): run `python pyqg_amd/build.py` "
+            "(needs hipcc); pyqg_amd has no CPU fallback")
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I, D, L = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+    DP = ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "qgk_create": (I, [I, I, I, I, I, ctypes.POINTER(P)]),
+        "qgk_destroy": (I, [P]),
+        "qgk_last_error": (ctypes.c_char_p, []),
+        "qgk_set_const": (I, [P, I, I, P]),
+        "qgk_set_const_all": (I, [P, I, P]),
+        "qgk_set_inversion_2layer": (I, [P, I, D, D]),
+        "qgk_set_rek": (I, [P, I, D]),
+        "qgk_put": (I, [P, I, P]),
+        "qgk_get": (I, [P, I, P]),
+        "qgk_invert": (I, [P]),
+        "qgk_advect": (I, [P]),
+        "qgk_friction": (I, [P]),
+        "qgk_add_q_param": (I, [P, P]),
+        "qgk_add_uv_param": (I, [P, P, P]),
+        "qgk_timestep": (I, [P, D, D, D]),
+        "qgk_q_to_qh": (I, [P]),
+        "qgk_qh_to_q": (I, [P]),
+        "qgk_fft": (I, [P, I, P, P]),
+        "qgk_ifft": (I, [P, I, P, P]),
+        "qgk_step": (I, [P, I, P, I]),
+        "qgk_timing_enable": (I, [P, I]),
+        "qgk_timing": (I, [P, DP, ctypes.POINTER(L), DP, ctypes.POINTER(L)]),
+        "qgk_sync": (I, [P]),
+        "qgk_step_buckets": (I, [ctypes.POINTER(P), I, I, P, I]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(rc):
+    if rc == 0:
+        return
+    msg = lib.qgk_last_error().decode(errors="replace")
+    if rc == E_ARG:
+        raise ValueError(msg)
+    if rc == E_STATE:
+        raise AttributeError(msg)
+    if rc == E_NOMEM:
+        raise MemoryError(msg)
+    raise QGKError(msg)
+
+
+def ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Bucket:
+    """Owning wrapper of one qgk_bucket handle."""
+
+    def __init__(self, nmembers, nz, ny, nx, flags):
+        h = ctypes.c_void_p()
+        check(lib.qgk_create(int(nmembers), int(nz), int(ny), int(nx), int(flags),
+                             ctypes.byref(h)))
+        self.h = h
+        self.M, self.nz, self.ny, self.nx = int(nmembers), int(nz), int(ny), int(nx)
+        self.nl, self.nk = self.ny, self.nx // 2 + 1
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.qgk_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, field):
+        if field in REAL_FIELDS:
+            return (self.M, self.nz, self.ny, self.nx)
+        return (self.M, self.nz, self.nl, self.nk)
+
+    def put(self, field, arr):
+        dtype = np.float64 if field in REAL_FIELDS else np.complex128
+        a = np.ascontiguousarray(arr, dtype=dtype)
+        if a.size != int(np.prod(self.shape(field))):
+            raise ValueError(f"{field}: expected {self.shape(field)}, got {np.shape(arr)}")
+        check(lib.qgk_put(self.h, FIELD_ID[field], ptr(a)))
+
+    def get(self, field, out=None):
+        dtype = np.float64 if field in REAL_FIELDS else np.complex128
+        if out is None:
+            out = np.empty(self.shape(field), dtype)
+        assert out.flags.c_contiguous and out.dtype == dtype
+        check(lib.qgk_get(self.h, FIELD_ID[field], ptr(out)))
+        return out
+
+    def set_const(self, name, arr, member=-1):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        check(lib.qgk_set_const(self.h, CONST_ID[name], int(member), ptr(a)))
+
+    def set_const_all(self, name, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        check(lib.qgk_set_const_all(self.h, CONST_ID[name], ptr(a)))
+
+    def set_inversion_2layer(self, F1, F2, member=-1):
+        check(lib.qgk_set_inversion_2layer(self.h, int(member), float(F1), float(F2)))
+
+    def set_rek(self, rek, member=-1):
+        check(lib.qgk_set_rek(self.h, int(member), float(rek)))
+
+    def step(self, coeffs, use_graph=False):
+        c = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(-1, 3)
+        check(lib.qgk_step(self.h, c.shape[0], ptr(c), int(bool(use_graph))))
+
+    def timing_enable(self, on=True):
+        check(lib.qgk_timing_enable(self.h, int(bool(on))))
+
+    def timing(self):
+        mr, mc = ctypes.c_double(), ctypes.c_double()
+        nr, nc = ctypes.c_long(), ctypes.c_long()
+        check(lib.qgk_timing(self.h, ctypes.byref(mr), ctypes.byref(nr), ctypes.byref(mc),
+                             ctypes.byref(nc)))
+        return mr.value, nr.value, mc.value, nc.value
+
+    def sync(self):
+        check(lib.qgk_sync(self.h))
+
+
+def step_buckets(buckets, coeffs, use_graph=False):
+    c = np.ascontiguousarray(coeffs, dtype=np.float64).reshape(-1, 3)
+    arr = (ctypes.c_void_p * len(buckets))(*[b.h for b in buckets])
+    check(lib.qgk_step_buckets(arr, len(buckets), c.shape[0], ptr(c), int(bool(use_graph))))

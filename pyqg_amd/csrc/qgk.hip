@@ -1,0 +1,1401 @@
+// libqgk: MI355X (gfx950) FP64 time-stepping core for pyqg's PseudoSpectralKernel.
+//
+// Data flow of one step (pyqg/model.py:406-436 composed with kernel.pyx:312-526):
+//   spectral qh --(inversion, velocities)--> ph, uh, vh
+//   --(2-D c2r: y pass on columns, x pass on rows)--> u, v, q real
+//   --(flux)--> uq = (u+Ubg) q, vq = v q
+//   --(2-D r2c: x pass on rows, y pass on columns)--> uqh, vqh
+//   --> dqhdt = -(ik uqh + il vqh + ikQy ph) [+ rek k2l2 ph bottom]
+//   --> qh = filtr (qh + dt1 dqhdt + dt2 dqhdt_p + dt3 dqhdt_pp), history shift.
+//
+// Two execution paths share the same device code:
+//  * per-phase kernels mirroring each reference method, keeping every
+//    intermediate field the reference exposes (the drop-in path);
+//  * a fused step of two kernels: a row kernel (x c2r of q,u,v -> flux -> x r2c)
+//    and a column kernel (y FFT of the fluxes -> tendency -> drag -> AB3 +
+//    filter -> inversion of the new qh -> y IFFT of q,u,v for the next step).
+//    Between them only the x-mixed ("half transformed") fields travel through
+//    HBM; ph, u, v, uq, vq and the real q never leave the chip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fft_block.hpp"
+#include "../../include/pyqg_amd.h"
+
+using qgk::cd;
+using qgk::cadd;
+using qgk::csub;
+using qgk::cmul;
+using qgk::cscale;
+using qgk::cconj;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPC(expr)                                                                  \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(QGK_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                              \
+  } while (0)
+
+// Per-member constants seen by every kernel.  Table pointers may be shared
+// between members.  a == nullptr selects the closed-form two-layer inversion.
+struct MemberParams {
+  const double* kk;     // [nk]
+  const double* ll;     // [nl]
+  const double* k2l2;   // [nl][nk]
+  const double* filtr;  // [nl][nk]
+  const double* a;      // [nz][nz][nl][nk] or nullptr
+  const double* Ubg;    // [nz]
+  const double* Qy;     // [nz]
+  double F1, F2;        // closed-form two-layer inversion
+  double rek;
+  double pad;
+};
+
+struct StepCoeffs {
+  double dt1, dt2, dt3;
+};
+
+// --------------------------------------------------------------------------
+// launch geometry per transform length
+// --------------------------------------------------------------------------
+template <int N>
+struct Geo {
+  static constexpr int E = N <= 256 ? (N < 8 ? N : 8) : 16;  // elements per thread
+  static constexpr int T = N / E;                            // threads per transform
+  static constexpr int TK = N >= 1024 ? 4 : 8;               // columns per column tile
+  static constexpr int RPraw = 128 / T;
+  static constexpr int RP = RPraw < 1 ? 1 : RPraw;           // row pairs per row block
+};
+
+__device__ __forceinline__ double ll_of(const MemberParams& P, int l) { return P.ll[l]; }
+
+// inversion at one spectral point: ph[k2] = sum_k1 a[k2,k1] qh[k1] (kernel.pyx:331-338)
+__device__ __forceinline__ cd invert_point(const MemberParams& P, int nz, int k2,
+                                           const cd* qh_pt, size_t plane, size_t pt) {
+#pragma clang fp contract(off)
+  // qh_pt: pointer to qh[0] at this point; layer stride = plane
+  if (P.a) {
+    cd acc = {0.0, 0.0};
+    for (int k1 = 0; k1 < nz; ++k1) {
+      double a = P.a[((size_t)k2 * nz + k1) * plane + pt];
+      cd q = qh_pt[k1 * plane];
+      // complex(a, 0) * q, as the reference multiplies by a complex matrix
+      acc.x = acc.x + a * q.x;
+      acc.y = acc.y + a * q.y;
+    }
+    return acc;
+  }
+  // two-layer closed form (qg_model.py:152-161); zero at kappa = 0
+  double w = P.k2l2[pt];
+  double det = w * (w + P.F1 + P.F2);
+  if (det == 0.0) return {0.0, 0.0};
+  double di = 1.0 / det;
+  double a0, a1;
+  if (k2 == 0) {
+    a0 = -(w + P.F2) * di;
+    a1 = -P.F1 * di;
+  } else {
+    a0 = -P.F2 * di;
+    a1 = -(w + P.F1) * di;
+  }
+  cd q0 = qh_pt[0], q1 = qh_pt[plane];
+  cd acc;
+  acc.x = 0.0 + a0 * q0.x;
+  acc.y = 0.0 + a0 * q0.y;
+  acc.x = acc.x + a1 * q1.x;
+  acc.y = acc.y + a1 * q1.y;
+  return acc;
+}
+
+// ==========================================================================
+// Column (y) transforms of spectral planes [nl][nk]
+// ==========================================================================
+template <int NY, int SIGN>
+__global__ __launch_bounds__(Geo<NY>::T * Geo<NY>::TK) void k_col_fft(
+    const cd* in, cd* out, int nk, const cd* __restrict__ tw) {
+  using G = Geo<NY>;
+  __shared__ cd lds[NY * G::TK];
+  const int c = threadIdx.x % G::TK;
+  const int t = threadIdx.x / G::TK;
+  const int k = blockIdx.x * G::TK + c;
+  const size_t plane = (size_t)blockIdx.y * NY * nk;
+  const bool ok = k < nk;
+  cd x[G::E];
+#pragma unroll
+  for (int s = 0; s < G::E; ++s)
+    x[s] = ok ? in[plane + (size_t)(t + s * G::T) * nk + k] : cd{0.0, 0.0};
+  qgk::fft_block<NY, G::E, SIGN, G::TK>(x, lds, t, c, tw);
+  if (ok) {
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) out[plane + (size_t)(t + s * G::T) * nk + k] = x[s];
+  }
+}
+
+// ==========================================================================
+// Row (x) real transforms.  Two rows are carried by one complex transform:
+// z = x_j + i x_{j+1}.
+// ==========================================================================
+
+// Hermitian extension of a half spectrum row h[0..N/2] at position p, with the
+// imaginary parts of bins 0 and N/2 discarded (numpy / pocketfft c2r semantics).
+template <int NX>
+__device__ __forceinline__ cd hext(const cd* __restrict__ h, int p) {
+  cd v;
+  if (p <= NX / 2) {
+    v = h[p];
+    if (p == 0 || p == NX / 2) v.y = 0.0;
+  } else {
+    v = h[NX - p];
+    v.y = -v.y;
+  }
+  return v;
+}
+
+// c2r of planes [ny][nk] -> [ny][nx], scaled by `scale`
+template <int NX>
+__global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_c2r(
+    const cd* __restrict__ in, double* __restrict__ out, int ny, double scale,
+    const cd* __restrict__ tw) {
+  using G = Geo<NX>;
+  constexpr int NK = NX / 2 + 1;
+  __shared__ cd lds[NX * G::RP];
+  const int t = threadIdx.x % G::T;
+  const int rp = threadIdx.x / G::T;
+  const int jp = blockIdx.x * G::RP + rp;
+  const bool ok = 2 * jp < ny;
+  const int j0 = ok ? 2 * jp : 0;
+  const cd* r0 = in + ((size_t)blockIdx.y * ny + j0) * NK;
+  const cd* r1 = r0 + NK;
+  cd x[G::E];
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) {
+    const int p = t + s * G::T;
+    cd a = hext<NX>(r0, p), b = hext<NX>(r1, p);
+    x[s] = {a.x - b.y, a.y + b.x};
+  }
+  qgk::fft_block<NX, G::E, +1, G::RP>(x, lds, t, rp, tw);
+  if (ok) {
+    double* o0 = out + ((size_t)blockIdx.y * ny + j0) * NX;
+    double* o1 = o0 + NX;
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) {
+      const int p = t + s * G::T;
+      o0[p] = x[s].x * scale;
+      o1[p] = x[s].y * scale;
+    }
+  }
+}
+
+// Splits the transform Z of z = x0 + i x1 (held in LDS, interleave RP) into
+// the half spectra of x0 and x1 and writes them.
+template <int NX, int RP>
+__device__ __forceinline__ void split_store(const cd* lds, int t, int rp, bool ok, cd* o0,
+                                            cd* o1) {
+  using G = Geo<NX>;
+  if (!ok) return;
+  for (int k = t; k <= NX / 2; k += G::T) {
+    cd A = lds[k * RP + rp];
+    cd B = lds[((NX - k) & (NX - 1)) * RP + rp];
+    o0[k] = {0.5 * (A.x + B.x), 0.5 * (A.y - B.y)};
+    o1[k] = {0.5 * (A.y + B.y), -0.5 * (A.x - B.x)};
+  }
+}
+
+// r2c of planes [ny][nx] -> [ny][nk]
+template <int NX>
+__global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_r2c(
+    const double* __restrict__ in, cd* __restrict__ out, int ny, const cd* __restrict__ tw) {
+  using G = Geo<NX>;
+  constexpr int NK = NX / 2 + 1;
+  __shared__ cd lds[NX * G::RP];
+  const int t = threadIdx.x % G::T;
+  const int rp = threadIdx.x / G::T;
+  const int jp = blockIdx.x * G::RP + rp;
+  const bool ok = 2 * jp < ny;
+  const int j0 = ok ? 2 * jp : 0;
+  const double* i0 = in + ((size_t)blockIdx.y * ny + j0) * NX;
+  const double* i1 = i0 + NX;
+  cd x[G::E];
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) {
+    const int p = t + s * G::T;
+    x[s] = {i0[p], i1[p]};
+  }
+  qgk::fft_block<NX, G::E, -1, G::RP>(x, lds, t, rp, tw);
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) lds[(t + s * G::T) * G::RP + rp] = x[s];
+  __syncthreads();
+  cd* o0 = out + ((size_t)blockIdx.y * ny + j0) * NK;
+  split_store<NX, G::RP>(lds, t, rp, ok, o0, o0 + NK);
+}
+
+// ==========================================================================
+// Pointwise phases (drop-in path).  Contraction is disabled so every product
+// and sum rounds exactly as the reference's loops do.
+// ==========================================================================
+
+// _invert, spectral part (kernel.pyx:323-347): ph, uh = -il ph, vh = ik ph
+__global__ void k_invert(const cd* __restrict__ qh, cd* __restrict__ ph, cd* __restrict__ uh,
+                         cd* __restrict__ vh, const MemberParams* __restrict__ params, int nz,
+                         int nl, int nk, int M) {
+#pragma clang fp contract(off)
+  const size_t plane = (size_t)nl * nk;
+  const size_t total = (size_t)M * plane;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const int m = (int)(idx / plane);
+    const size_t pt = idx - (size_t)m * plane;
+    const int l = (int)(pt / nk), k = (int)(pt - (size_t)l * nk);
+    const MemberParams& P = params[m];
+    const size_t base = (size_t)m * nz * plane + pt;
+    const double kk = P.kk[k], ll = P.ll[l];
+    for (int z = 0; z < nz; ++z) {
+      cd p = invert_point(P, nz, z, qh + (size_t)m * nz * plane + pt, plane, pt);
+      ph[base + z * plane] = p;
+      // -(1j*ll) * p and (1j*kk) * p
+      uh[base + z * plane] = {ll * p.y, -ll * p.x};
+      vh[base + z * plane] = {-kk * p.y, kk * p.x};
+    }
+  }
+}
+
+// _do_advection flux (kernel.pyx:372-378): uq = (u + Ubg) q, vq = v q
+__global__ void k_flux(const double* __restrict__ u, const double* __restrict__ v,
+                       const double* __restrict__ q, double* __restrict__ uq,
+                       double* __restrict__ vq, const MemberParams* __restrict__ params, int nz,
+                       size_t plane, int M) {
+#pragma clang fp contract(off)
+  const size_t total = (size_t)M * nz * plane;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t pl = idx / plane;
+    const int m = (int)(pl / nz), z = (int)(pl - (size_t)m * nz);
+    const double qq = q[idx];
+    uq[idx] = (u[idx] + params[m].Ubg[z]) * qq;
+    vq[idx] = v[idx] * qq;
+  }
+}
+
+// _do_advection divergence (kernel.pyx:386-394):
+// dqhdt = -(ik uqh + il vqh + ikQy ph)
+__global__ void k_divergence(const cd* __restrict__ uqh, const cd* __restrict__ vqh,
+                             const cd* __restrict__ ph, cd* __restrict__ dqhdt,
+                             const MemberParams* __restrict__ params, int nz, int nl, int nk,
+                             int M) {
+#pragma clang fp contract(off)
+  const size_t plane = (size_t)nl * nk;
+  const size_t total = (size_t)M * nz * plane;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t pl = idx / plane;
+    const size_t pt = idx - pl * plane;
+    const int m = (int)(pl / nz), z = (int)(pl - (size_t)m * nz);
+    const int l = (int)(pt / nk), k = (int)(pt - (size_t)l * nk);
+    const MemberParams& P = params[m];
+    const double kk = P.kk[k], ll = P.ll[l];
+    const double kq = kk * P.Qy[z];
+    const cd a = uqh[idx], b = vqh[idx], p = ph[idx];
+    cd s = {-kk * a.y + -ll * b.y, kk * a.x + ll * b.x};
+    s = {s.x + -kq * p.y, s.y + kq * p.x};
+    dqhdt[idx] = {-s.x, -s.y};
+  }
+}
+
+// _do_friction (kernel.pyx:444-461): bottom layer, dqhdt += (rek k2l2) ph
+__global__ void k_friction(const cd* __restrict__ ph, cd* __restrict__ dqhdt,
+                           const MemberParams* __restrict__ params, int nz, int nl, int nk,
+                           int M) {
+#pragma clang fp contract(off)
+  const size_t plane = (size_t)nl * nk;
+  const size_t total = (size_t)M * plane;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const int m = (int)(idx / plane);
+    const size_t pt = idx - (size_t)m * plane;
+    const MemberParams& P = params[m];
+    if (P.rek == 0.0) continue;
+    const size_t o = ((size_t)m * nz + nz - 1) * plane + pt;
+    const double c = P.rek * P.k2l2[pt];
+    const cd p = ph[o];
+    cd d = dqhdt[o];
+    dqhdt[o] = {d.x + c * p.x, d.y + c * p.y};
+  }
+}
+
+// q parameterization add (kernel.pyx:435-441): dqhdt += dqh
+__global__ void k_add(cd* __restrict__ dqhdt, const cd* __restrict__ dqh, size_t total) {
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    cd d = dqhdt[idx], e = dqh[idx];
+    dqhdt[idx] = {d.x + e.x, d.y + e.y};
+  }
+}
+
+// uv parameterization add (kernel.pyx:412-421): dqhdt += (-il duh + ik dvh)
+__global__ void k_add_curl(cd* __restrict__ dqhdt, const cd* __restrict__ duh,
+                           const cd* __restrict__ dvh, const MemberParams* __restrict__ params,
+                           int nz, int nl, int nk, int M) {
+#pragma clang fp contract(off)
+  const size_t plane = (size_t)nl * nk;
+  const size_t total = (size_t)M * nz * plane;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t pl = idx / plane;
+    const size_t pt = idx - pl * plane;
+    const int m = (int)(pl / nz);
+    const int l = (int)(pt / nk), k = (int)(pt - (size_t)l * nk);
+    const double kk = params[m].kk[k], ll = params[m].ll[l];
+    const cd a = duh[idx], b = dvh[idx];
+    const cd c = {ll * a.y + -kk * b.y, -ll * a.x + kk * b.x};
+    const cd d = dqhdt[idx];
+    dqhdt[idx] = {d.x + c.x, d.y + c.y};
+  }
+}
+
+// _forward_timestep spectral part (kernel.pyx:502-512).  The history rotates by
+// pointer: the new dqhdt_p is written into the old dqhdt_pp buffer.
+__global__ void k_ab_step(cd* __restrict__ qh, const cd* __restrict__ d,
+                          const cd* __restrict__ p, cd* __restrict__ pp_to_newp,
+                          const MemberParams* __restrict__ params, StepCoeffs c, int nz,
+                          int nl, int nk, int M) {
+#pragma clang fp contract(off)
+  const size_t plane = (size_t)nl * nk;
+  const size_t total = (size_t)M * nz * plane;
+  for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t pl = idx / plane;
+    const size_t pt = idx - pl * plane;
+    const int m = (int)(pl / nz);
+    const double f = params[m].filtr[pt];
+    const cd q = qh[idx], dd = d[idx], pv = p[idx], ppv = pp_to_newp[idx];
+    double rx = q.x + c.dt1 * dd.x;
+    double ry = q.y + c.dt1 * dd.y;
+    rx = rx + c.dt2 * pv.x;
+    ry = ry + c.dt2 * pv.y;
+    rx = rx + c.dt3 * ppv.x;
+    ry = ry + c.dt3 * ppv.y;
+    qh[idx] = {f * rx, f * ry};
+    pp_to_newp[idx] = dd;
+  }
+}
+
+// ==========================================================================
+// Fused step
+// ==========================================================================
+
+// Row phase: per (member, layer, row pair): x-c2r of q, u, v, flux, x-r2c.
+// In:  Qm, Um, Vm  [M][nz][ny][nk] (y already inverse transformed)
+// Out: UQm, VQm    [M][nz][ny][nk] (x already forward transformed)
+template <int NX>
+__global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_flux(
+    const cd* __restrict__ Qm, const cd* __restrict__ Um, const cd* __restrict__ Vm,
+    cd* __restrict__ UQm, cd* __restrict__ VQm, const MemberParams* __restrict__ params,
+    int nz, int ny, double scale, const cd* __restrict__ tw) {
+  using G = Geo<NX>;
+  constexpr int NK = NX / 2 + 1;
+  __shared__ cd lds[NX * G::RP];
+  const int t = threadIdx.x % G::T;
+  const int rp = threadIdx.x / G::T;
+  const int jp = blockIdx.x * G::RP + rp;
+  const bool ok = 2 * jp < ny;
+  const int j0 = ok ? 2 * jp : 0;
+  const int plane_id = blockIdx.y;  // m * nz + z
+  const int m = plane_id / nz, z = plane_id - m * nz;
+  const double ubg = params[m].Ubg[z];
+  const size_t row0 = ((size_t)plane_id * ny + j0) * NK;
+
+  // q on rows j0 (real part) and j0+1 (imaginary part)
+  cd zq[G::E];
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) {
+    const int p = t + s * G::T;
+    cd a = hext<NX>(Qm + row0, p), b = hext<NX>(Qm + row0 + NK, p);
+    zq[s] = {a.x - b.y, a.y + b.x};
+  }
+  qgk::fft_block<NX, G::E, +1, G::RP>(zq, lds, t, rp, tw);
+
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const size_t row = row0 + (size_t)r * NK;
+    cd w[G::E];
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) {
+      const int p = t + s * G::T;
+      cd a = hext<NX>(Um + row, p), b = hext<NX>(Vm + row, p);
+      w[s] = {a.x - b.y, a.y + b.x};
+    }
+    qgk::fft_block<NX, G::E, +1, G::RP>(w, lds, t, rp, tw);
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) {
+      const double q = (r == 0 ? zq[s].x : zq[s].y) * scale;
+      const double u = w[s].x * scale, v = w[s].y * scale;
+      w[s] = {(u + ubg) * q, v * q};
+    }
+    qgk::fft_block<NX, G::E, -1, G::RP>(w, lds, t, rp, tw);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) lds[(t + s * G::T) * G::RP + rp] = w[s];
+    __syncthreads();
+    split_store<NX, G::RP>(lds, t, rp, ok, UQm + row, VQm + row);
+  }
+}
+
+// Column phase, part C: from qh (all layers) build ph, uh, vh of one layer at
+// this thread's points and inverse-transform q, u, v along y into Qm, Um, Vm.
+template <int NY>
+__device__ __forceinline__ void col_prologue_layer(const cd* __restrict__ qh,
+                                                   cd* __restrict__ Qm, cd* __restrict__ Um,
+                                                   cd* __restrict__ Vm, const MemberParams& P,
+                                                   int nz, int z, int nk, int t, int c, int k,
+                                                   bool ok, size_t mbase, cd* lds,
+                                                   const cd* __restrict__ tw) {
+  using G = Geo<NY>;
+  const size_t plane = (size_t)NY * nk;
+  const size_t zoff = mbase + (size_t)z * plane;
+  const int kc = ok ? k : 0;
+  cd x[G::E];
+  cd ph[G::E];
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) {
+    const size_t pt = (size_t)(t + s * G::T) * nk + kc;
+    x[s] = ok ? qh[zoff + pt] : cd{0.0, 0.0};
+    ph[s] = ok ? invert_point(P, nz, z, qh + mbase + pt, plane, pt) : cd{0.0, 0.0};
+  }
+  qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+  if (ok) {
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) Qm[zoff + (size_t)(t + s * G::T) * nk + k] = x[s];
+  }
+  // uh = -(i ll) ph
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) {
+    const double ll = P.ll[t + s * G::T];
+    x[s] = {ll * ph[s].y, -ll * ph[s].x};
+  }
+  qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+  if (ok) {
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) Um[zoff + (size_t)(t + s * G::T) * nk + k] = x[s];
+  }
+  // vh = (i kk) ph
+  const double kk = P.kk[kc];
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) x[s] = {-kk * ph[s].y, kk * ph[s].x};
+  qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+  if (ok) {
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) Vm[zoff + (size_t)(t + s * G::T) * nk + k] = x[s];
+  }
+}
+
+template <int NY>
+__global__ __launch_bounds__(Geo<NY>::T * Geo<NY>::TK) void k_col_prologue(
+    const cd* __restrict__ qh, cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
+    const MemberParams* __restrict__ params, int nz, int nk, const cd* __restrict__ tw) {
+  using G = Geo<NY>;
+  __shared__ cd lds[NY * G::TK];
+  const int c = threadIdx.x % G::TK;
+  const int t = threadIdx.x / G::TK;
+  const int k = blockIdx.x * G::TK + c;
+  const int m = blockIdx.y;
+  const bool ok = k < nk;
+  const MemberParams& P = params[m];
+  const size_t mbase = (size_t)m * nz * NY * nk;
+  for (int z = 0; z < nz; ++z)
+    col_prologue_layer<NY>(qh, Qm, Um, Vm, P, nz, z, nk, t, c, k, ok, mbase, lds, tw);
+}
+
+// Column phase of the fused step.  Per (member, column tile):
+//  A. y-FFT of UQm, VQm per layer; S = ik uqh + il vqh parked in `dscr`.
+//  B. per point: ph = a qh; dqhdt = -(S + ikQy ph) [+ drag]; AB3 + filter;
+//     qh <- qh'; new dqhdt written into the dqhdt_pp buffer (it becomes dqhdt_p).
+//  C. the next step's prologue from qh'.
+template <int NY>
+__global__ __launch_bounds__(Geo<NY>::T * Geo<NY>::TK) void k_col_step(
+    cd* __restrict__ qh, cd* __restrict__ dscr, const cd* __restrict__ p,
+    cd* __restrict__ pp_to_newp, const cd* __restrict__ UQm, const cd* __restrict__ VQm,
+    cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
+    const MemberParams* __restrict__ params, StepCoeffs cf, int nz, int nk,
+    const cd* __restrict__ tw) {
+  using G = Geo<NY>;
+  __shared__ cd lds[NY * G::TK];
+  const int c = threadIdx.x % G::TK;
+  const int t = threadIdx.x / G::TK;
+  const int k = blockIdx.x * G::TK + c;
+  const int m = blockIdx.y;
+  const bool ok = k < nk;
+  const int kc = ok ? k : 0;
+  const MemberParams& P = params[m];
+  const size_t plane = (size_t)NY * nk;
+  const size_t mbase = (size_t)m * nz * plane;
+  const double kk = P.kk[kc];
+
+  // ---- A
+  for (int z = 0; z < nz; ++z) {
+    const size_t zoff = mbase + (size_t)z * plane;
+    cd xa[G::E], xb[G::E];
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) {
+      const size_t pt = (size_t)(t + s * G::T) * nk + kc;
+      xa[s] = ok ? UQm[zoff + pt] : cd{0.0, 0.0};
+      xb[s] = ok ? VQm[zoff + pt] : cd{0.0, 0.0};
+    }
+    qgk::fft_block<NY, G::E, -1, G::TK>(xa, lds, t, c, tw);
+    qgk::fft_block<NY, G::E, -1, G::TK>(xb, lds, t, c, tw);
+    if (ok) {
+#pragma unroll
+      for (int s = 0; s < G::E; ++s) {
+        const double ll = P.ll[t + s * G::T];
+        // ik uqh + il vqh (kernel.pyx:391-392)
+        cd S = {-kk * xa[s].y + -ll * xb[s].y, kk * xa[s].x + ll * xb[s].x};
+        dscr[zoff + (size_t)(t + s * G::T) * nk + k] = S;
+      }
+    }
+  }
+
+  // ---- B (qh' parked in Qm until every layer's ph is formed from the old qh)
+  if (ok) {
+#pragma unroll 1
+    for (int s = 0; s < G::E; ++s) {
+      const size_t pt = (size_t)(t + s * G::T) * nk + k;
+      const double f = P.filtr[pt];
+      for (int z = 0; z < nz; ++z) {
+        const size_t o = mbase + (size_t)z * plane + pt;
+        const cd ph = invert_point(P, nz, z, qh + mbase + pt, plane, pt);
+        const double kq = kk * P.Qy[z];
+        const cd S = dscr[o];
+        cd d = {S.x + -kq * ph.y, S.y + kq * ph.x};
+        d = {-d.x, -d.y};
+        if (z == nz - 1 && P.rek != 0.0) {
+          const double cr = P.rek * P.k2l2[pt];
+          d = {d.x + cr * ph.x, d.y + cr * ph.y};
+        }
+        const cd q = qh[o], pv = p[o], ppv = pp_to_newp[o];
+        double rx = q.x + cf.dt1 * d.x;
+        double ry = q.y + cf.dt1 * d.y;
+        rx = rx + cf.dt2 * pv.x;
+        ry = ry + cf.dt2 * pv.y;
+        rx = rx + cf.dt3 * ppv.x;
+        ry = ry + cf.dt3 * ppv.y;
+        Qm[o] = {f * rx, f * ry};
+        pp_to_newp[o] = d;
+      }
+      for (int z = 0; z < nz; ++z) {
+        const size_t o = mbase + (size_t)z * plane + pt;
+        qh[o] = Qm[o];
+      }
+    }
+  }
+
+  // ---- C
+  for (int z = 0; z < nz; ++z)
+    col_prologue_layer<NY>(qh, Qm, Um, Vm, P, nz, z, nk, t, c, k, ok, mbase, lds, tw);
+}
+
+// ==========================================================================
+// host side
+// ==========================================================================
+
+std::vector<cd> make_twiddles(int n) {
+  std::vector<cd> tw(n);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int m = 0; m < n; ++m) {
+    long double th = two_pi * (long double)m / (long double)n;
+    tw[m] = {(double)cosl(th), (double)(-sinl(th))};
+  }
+  return tw;
+}
+
+bool pow2_in_range(int n) { return n >= 8 && n <= 1024 && (n & (n - 1)) == 0; }
+
+}  // namespace
+
+struct qgk_bucket {
+  int M = 0, nz = 0, ny = 0, nx = 0, nl = 0, nk = 0, flags = 0;
+  size_t nplane_c = 0, nplane_r = 0;  // points per spectral / real plane
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  // state
+  cd* field_c[QGK_NFIELDS] = {};
+  double* field_r[QGK_NFIELDS] = {};
+  cd* hist[3] = {};  // rotating: hist[iD] dqhdt, hist[iP] dqhdt_p, hist[iPP] dqhdt_pp
+  int iD = 0, iP = 1, iPP = 2;
+  cd *Qm = nullptr, *Um = nullptr, *Vm = nullptr, *UQm = nullptr, *VQm = nullptr;
+  cd* ctmp = nullptr;      // spectral scratch (M*nz planes)
+  double* rtmp = nullptr;  // real scratch (M*nz planes)
+  bool primed = false;     // Qm/Um/Vm match the current qh
+  bool q_stale = false;    // the real q lags qh (after fused steps)
+  // constants
+  std::vector<MemberParams> hp;
+  MemberParams* dp = nullptr;
+  bool params_dirty = true;
+  double* shared_tab[QGK_NCONSTS] = {};
+  std::vector<std::vector<double*>> own_tab;  // [which][member]
+  cd* tw_x = nullptr;
+  cd* tw_y = nullptr;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  struct Pending {
+    hipEvent_t a, b;
+    int kind;
+  };
+  std::vector<Pending> pending;
+  double ms_row = 0, ms_col = 0;
+  long n_row = 0, n_col = 0;
+  // graph of one fused step (per parity of the history rotation)
+  hipGraphExec_t gexec[3] = {};
+  StepCoeffs gcoef = {0, 0, 0};
+  bool gvalid = false;
+};
+
+namespace {
+
+int dev_alloc(qgk_bucket* b, void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess)
+    return fail(QGK_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  b->allocs.push_back(*p);
+  e = hipMemsetAsync(*p, 0, bytes, b->stream);
+  if (e != hipSuccess) return fail(QGK_E_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
+  return 0;
+}
+
+bool is_real_field(int f) {
+  return f == QGK_Q || f == QGK_U || f == QGK_V || f == QGK_UQ || f == QGK_VQ || f == QGK_DQ ||
+         f == QGK_DU || f == QGK_DV;
+}
+
+size_t const_len(const qgk_bucket* b, int which) {
+  switch (which) {
+    case QGK_KK: return b->nk;
+    case QGK_LL: return b->nl;
+    case QGK_K2L2:
+    case QGK_FILTR: return b->nplane_c;
+    case QGK_A: return (size_t)b->nz * b->nz * b->nplane_c;
+    case QGK_UBG:
+    case QGK_QY: return b->nz;
+  }
+  return 0;
+}
+
+const double*& param_slot(MemberParams& p, int which) {
+  switch (which) {
+    case QGK_KK: return p.kk;
+    case QGK_LL: return p.ll;
+    case QGK_K2L2: return p.k2l2;
+    case QGK_FILTR: return p.filtr;
+    case QGK_A: return p.a;
+    case QGK_UBG: return p.Ubg;
+    default: return p.Qy;
+  }
+}
+
+int upload_params(qgk_bucket* b) {
+  if (!b->params_dirty) return 0;
+  HIPC(hipMemcpyAsync(b->dp, b->hp.data(), sizeof(MemberParams) * b->M, hipMemcpyHostToDevice,
+                      b->stream));
+  b->params_dirty = false;
+  return 0;
+}
+
+cd* D(qgk_bucket* b) { return b->hist[b->iD]; }
+cd* Pp(qgk_bucket* b) { return b->hist[b->iP]; }
+cd* PP(qgk_bucket* b) { return b->hist[b->iPP]; }
+
+// ---- event timing
+int t_begin(qgk_bucket* b, hipEvent_t* ev) {
+  if (!b->timing) return 0;
+  HIPC(hipEventCreate(ev));
+  HIPC(hipEventRecord(*ev, b->stream));
+  return 0;
+}
+int t_end(qgk_bucket* b, hipEvent_t ev0, int kind) {
+  if (!b->timing) return 0;
+  hipEvent_t ev1;
+  HIPC(hipEventCreate(&ev1));
+  HIPC(hipEventRecord(ev1, b->stream));
+  b->pending.push_back({ev0, ev1, kind});
+  return 0;
+}
+int t_collect(qgk_bucket* b) {
+  for (auto& p : b->pending) {
+    HIPC(hipEventSynchronize(p.b));
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, p.a, p.b));
+    if (p.kind == 0) {
+      b->ms_row += ms;
+      b->n_row++;
+    } else {
+      b->ms_col += ms;
+      b->n_col++;
+    }
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
+  b->pending.clear();
+  return 0;
+}
+
+// ---- dispatch helpers
+int grid_1d(size_t total) {
+  size_t g = (total + 255) / 256;
+  if (g > 65535 * 8) g = 65535 * 8;
+  return (int)(g ? g : 1);
+}
+
+template <int SIGN>
+int launch_col_fft(qgk_bucket* b, const cd* in, cd* out, int nplanes) {
+  const int ny = b->ny, nk = b->nk;
+#define QGK_COL(NY)                                                                     \
+  case NY: {                                                                            \
+    using G = Geo<NY>;                                                                  \
+    dim3 grid((nk + G::TK - 1) / G::TK, nplanes);                                       \
+    hipLaunchKernelGGL((k_col_fft<NY, SIGN>), grid, dim3(G::T * G::TK), 0, b->stream, in, \
+                       out, nk, b->tw_y);                                               \
+    break;                                                                              \
+  }
+  switch (ny) {
+    QGK_COL(8) QGK_COL(16) QGK_COL(32) QGK_COL(64) QGK_COL(128) QGK_COL(256) QGK_COL(512)
+    QGK_COL(1024)
+    default: return fail(QGK_E_ARG, "unsupported ny=%d", ny);
+  }
+#undef QGK_COL
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+int launch_row_c2r(qgk_bucket* b, const cd* in, double* out, int nplanes) {
+  const int nx = b->nx, ny = b->ny;
+  const double scale = 1.0 / ((double)nx * (double)ny);
+#define QGK_ROW(NX)                                                                       \
+  case NX: {                                                                              \
+    using G = Geo<NX>;                                                                    \
+    dim3 grid((ny / 2 + G::RP - 1) / G::RP, nplanes);                                     \
+    hipLaunchKernelGGL((k_row_c2r<NX>), grid, dim3(G::T * G::RP), 0, b->stream, in, out, ny, \
+                       scale, b->tw_x);                                                   \
+    break;                                                                                \
+  }
+  switch (nx) {
+    QGK_ROW(8) QGK_ROW(16) QGK_ROW(32) QGK_ROW(64) QGK_ROW(128) QGK_ROW(256) QGK_ROW(512)
+    QGK_ROW(1024)
+    default: return fail(QGK_E_ARG, "unsupported nx=%d", nx);
+  }
+#undef QGK_ROW
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+int launch_row_r2c(qgk_bucket* b, const double* in, cd* out, int nplanes) {
+  const int nx = b->nx, ny = b->ny;
+#define QGK_ROW(NX)                                                                        \
+  case NX: {                                                                               \
+    using G = Geo<NX>;                                                                     \
+    dim3 grid((ny / 2 + G::RP - 1) / G::RP, nplanes);                                      \
+    hipLaunchKernelGGL((k_row_r2c<NX>), grid, dim3(G::T * G::RP), 0, b->stream, in, out, ny, \
+                       b->tw_x);                                                           \
+    break;                                                                                 \
+  }
+  switch (nx) {
+    QGK_ROW(8) QGK_ROW(16) QGK_ROW(32) QGK_ROW(64) QGK_ROW(128) QGK_ROW(256) QGK_ROW(512)
+    QGK_ROW(1024)
+    default: return fail(QGK_E_ARG, "unsupported nx=%d", nx);
+  }
+#undef QGK_ROW
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+// 2-D transforms of nplanes planes: rfftn / irfftn over the last two axes
+int r2c2d(qgk_bucket* b, const double* in, cd* out, int nplanes) {
+  int rc = launch_row_r2c(b, in, out, nplanes);
+  if (rc) return rc;
+  return launch_col_fft<-1>(b, out, out, nplanes);
+}
+
+int c2r2d(qgk_bucket* b, const cd* in, double* out, int nplanes) {
+  // y pass into the spectral scratch, x pass into out
+  if (nplanes > b->M * b->nz) return fail(QGK_E_ARG, "too many planes for scratch");
+  int rc = launch_col_fft<+1>(b, in, b->ctmp, nplanes);
+  if (rc) return rc;
+  return launch_row_c2r(b, b->ctmp, out, nplanes);
+}
+
+int launch_row_flux(qgk_bucket* b) {
+  const int nx = b->nx, ny = b->ny;
+  const double scale = 1.0 / ((double)nx * (double)ny);
+  const int nplanes = b->M * b->nz;
+#define QGK_ROW(NX)                                                                         \
+  case NX: {                                                                                \
+    using G = Geo<NX>;                                                                      \
+    dim3 grid((ny / 2 + G::RP - 1) / G::RP, nplanes);                                       \
+    hipLaunchKernelGGL((k_row_flux<NX>), grid, dim3(G::T * G::RP), 0, b->stream, b->Qm, b->Um, \
+                       b->Vm, b->UQm, b->VQm, b->dp, b->nz, ny, scale, b->tw_x);             \
+    break;                                                                                  \
+  }
+  switch (nx) {
+    QGK_ROW(8) QGK_ROW(16) QGK_ROW(32) QGK_ROW(64) QGK_ROW(128) QGK_ROW(256) QGK_ROW(512)
+    QGK_ROW(1024)
+    default: return fail(QGK_E_ARG, "unsupported nx=%d", nx);
+  }
+#undef QGK_ROW
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+int launch_col_prologue(qgk_bucket* b) {
+  const int ny = b->ny, nk = b->nk;
+#define QGK_COL(NY)                                                                      \
+  case NY: {                                                                             \
+    using G = Geo<NY>;                                                                   \
+    dim3 grid((nk + G::TK - 1) / G::TK, b->M);                                           \
+    hipLaunchKernelGGL((k_col_prologue<NY>), grid, dim3(G::T * G::TK), 0, b->stream,       \
+                       b->field_c[QGK_QH], b->Qm, b->Um, b->Vm, b->dp, b->nz, nk, b->tw_y); \
+    break;                                                                               \
+  }
+  switch (ny) {
+    QGK_COL(8) QGK_COL(16) QGK_COL(32) QGK_COL(64) QGK_COL(128) QGK_COL(256) QGK_COL(512)
+    QGK_COL(1024)
+    default: return fail(QGK_E_ARG, "unsupported ny=%d", ny);
+  }
+#undef QGK_COL
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+int launch_col_step(qgk_bucket* b, StepCoeffs cf) {
+  const int ny = b->ny, nk = b->nk;
+#define QGK_COL(NY)                                                                         \
+  case NY: {                                                                                \
+    using G = Geo<NY>;                                                                      \
+    dim3 grid((nk + G::TK - 1) / G::TK, b->M);                                              \
+    hipLaunchKernelGGL((k_col_step<NY>), grid, dim3(G::T * G::TK), 0, b->stream,              \
+                       b->field_c[QGK_QH], D(b), Pp(b), PP(b), b->UQm, b->VQm, b->Qm, b->Um,  \
+                       b->Vm, b->dp, cf, b->nz, nk, b->tw_y);                               \
+    break;                                                                                  \
+  }
+  switch (ny) {
+    QGK_COL(8) QGK_COL(16) QGK_COL(32) QGK_COL(64) QGK_COL(128) QGK_COL(256) QGK_COL(512)
+    QGK_COL(1024)
+    default: return fail(QGK_E_ARG, "unsupported ny=%d", ny);
+  }
+#undef QGK_COL
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+void rotate_history(qgk_bucket* b) {
+  // after a step: new dqhdt_p lives in the old dqhdt_pp buffer
+  int oldP = b->iP, oldPP = b->iPP;
+  b->iP = oldPP;
+  b->iPP = oldP;
+}
+
+// one fused step (row kernel + column kernel); qh must be primed
+int fused_step(qgk_bucket* b, StepCoeffs cf) {
+  hipEvent_t e0 = nullptr;
+  int rc = t_begin(b, &e0);
+  if (rc) return rc;
+  rc = launch_row_flux(b);
+  if (rc) return rc;
+  rc = t_end(b, e0, 0);
+  if (rc) return rc;
+  rc = t_begin(b, &e0);
+  if (rc) return rc;
+  rc = launch_col_step(b, cf);
+  if (rc) return rc;
+  rc = t_end(b, e0, 1);
+  if (rc) return rc;
+  rotate_history(b);
+  return 0;
+}
+
+int check_field(qgk_bucket* b, int f) {
+  if (f < 0 || f >= QGK_NFIELDS) return fail(QGK_E_ARG, "bad field id %d", f);
+  if (f == QGK_DQHDT || f == QGK_DQHDT_P || f == QGK_DQHDT_PP || f == QGK_QH) return 0;
+  if (is_real_field(f) ? b->field_r[f] == nullptr : b->field_c[f] == nullptr)
+    return fail(QGK_E_STATE, "field %d is not allocated for this handle", f);
+  return 0;
+}
+
+cd* spectral_ptr(qgk_bucket* b, int f) {
+  switch (f) {
+    case QGK_DQHDT: return D(b);
+    case QGK_DQHDT_P: return Pp(b);
+    case QGK_DQHDT_PP: return PP(b);
+    default: return b->field_c[f];
+  }
+}
+
+int refresh_q(qgk_bucket* b) {
+  if (!b->q_stale) return 0;
+  int rc = c2r2d(b, b->field_c[QGK_QH], b->field_r[QGK_Q], b->M * b->nz);
+  if (rc) return rc;
+  b->q_stale = false;
+  return 0;
+}
+
+int need_full(qgk_bucket* b, const char* what) {
+  if (!(b->flags & QGK_FULL_STATE))
+    return fail(QGK_E_STATE, "%s needs a handle created with QGK_FULL_STATE", what);
+  return 0;
+}
+
+}  // namespace
+
+// ==========================================================================
+// C-ABI
+// ==========================================================================
+extern "C" {
+
+const char* qgk_last_error(void) { return g_err.c_str(); }
+
+int qgk_create(int nmembers, int nz, int ny, int nx, int flags, qgk_bucket** out) {
+  if (!out) return fail(QGK_E_ARG, "out is NULL");
+  *out = nullptr;
+  if (nmembers < 1 || nz < 1) return fail(QGK_E_ARG, "nmembers and nz must be >= 1");
+  if (!pow2_in_range(nx) || !pow2_in_range(ny))
+    return fail(QGK_E_ARG, "nx=%d, ny=%d: only powers of two in [8, 1024] are supported", nx, ny);
+  int ndev = 0;
+  hipError_t he = hipGetDeviceCount(&ndev);
+  if (he != hipSuccess || ndev == 0)
+    return fail(QGK_E_HIP, "no HIP device available (%s)", hipGetErrorString(he));
+  qgk_bucket* b = new qgk_bucket();
+  b->M = nmembers;
+  b->nz = nz;
+  b->ny = ny;
+  b->nx = nx;
+  b->nl = ny;
+  b->nk = nx / 2 + 1;
+  b->flags = flags;
+  b->nplane_c = (size_t)b->nl * b->nk;
+  b->nplane_r = (size_t)ny * nx;
+  int rc = 0;
+  auto cleanup = [&](int code) {
+    qgk_destroy(b);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(QGK_E_HIP, "hipStreamCreate failed"));
+  const size_t nc = (size_t)nmembers * nz * b->nplane_c * sizeof(cd);
+  const size_t nr = (size_t)nmembers * nz * b->nplane_r * sizeof(double);
+  auto ac = [&](cd** p) { return dev_alloc(b, (void**)p, nc); };
+  auto ar = [&](double** p) { return dev_alloc(b, (void**)p, nr); };
+  if ((rc = ac(&b->field_c[QGK_QH])) || (rc = ac(&b->hist[0])) || (rc = ac(&b->hist[1])) ||
+      (rc = ac(&b->hist[2])) || (rc = ac(&b->Qm)) || (rc = ac(&b->Um)) || (rc = ac(&b->Vm)) ||
+      (rc = ac(&b->UQm)) || (rc = ac(&b->VQm)) || (rc = ac(&b->ctmp)) || (rc = ar(&b->rtmp)) ||
+      (rc = ar(&b->field_r[QGK_Q])))
+    return cleanup(rc);
+  if (flags & QGK_FULL_STATE) {
+    const int cf[] = {QGK_PH, QGK_UH, QGK_VH, QGK_UQH, QGK_VQH};
+    const int rf[] = {QGK_U, QGK_V, QGK_UQ, QGK_VQ};
+    for (int f : cf)
+      if ((rc = ac(&b->field_c[f]))) return cleanup(rc);
+    for (int f : rf)
+      if ((rc = ar(&b->field_r[f]))) return cleanup(rc);
+  }
+  if (flags & QGK_HAS_Q_PARAM) {
+    if ((rc = ar(&b->field_r[QGK_DQ])) || (rc = ac(&b->field_c[QGK_DQH]))) return cleanup(rc);
+  }
+  if (flags & QGK_HAS_UV_PARAM) {
+    if ((rc = ar(&b->field_r[QGK_DU])) || (rc = ar(&b->field_r[QGK_DV])) ||
+        (rc = ac(&b->field_c[QGK_DUH])) || (rc = ac(&b->field_c[QGK_DVH])))
+      return cleanup(rc);
+  }
+  // twiddles
+  {
+    auto twx = make_twiddles(nx), twy = make_twiddles(ny);
+    if ((rc = dev_alloc(b, (void**)&b->tw_x, nx * sizeof(cd))) ||
+        (rc = dev_alloc(b, (void**)&b->tw_y, ny * sizeof(cd))))
+      return cleanup(rc);
+    if (hipMemcpyAsync(b->tw_x, twx.data(), nx * sizeof(cd), hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(b->tw_y, twy.data(), ny * sizeof(cd), hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+      return cleanup(fail(QGK_E_HIP, "twiddle upload failed"));
+  }
+  // default constants: zeros, filtr = 1 (shared tables)
+  b->own_tab.assign(QGK_NCONSTS, std::vector<double*>(nmembers, nullptr));
+  for (int w = 0; w < QGK_NCONSTS; ++w) {
+    const size_t n = const_len(b, w);
+    if ((rc = dev_alloc(b, (void**)&b->shared_tab[w], n * sizeof(double)))) return cleanup(rc);
+    if (w == QGK_FILTR) {
+      // same stream as the zero fill above, then wait: `ones` is pageable and local
+      std::vector<double> ones(n, 1.0);
+      if (hipMemcpyAsync(b->shared_tab[w], ones.data(), n * sizeof(double),
+                         hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+          hipStreamSynchronize(b->stream) != hipSuccess)
+        return cleanup(fail(QGK_E_HIP, "filter init failed"));
+    }
+  }
+  b->hp.assign(nmembers, MemberParams{});
+  for (auto& p : b->hp) {
+    for (int w = 0; w < QGK_NCONSTS; ++w) param_slot(p, w) = b->shared_tab[w];
+    p.rek = 0.0;
+  }
+  if ((rc = dev_alloc(b, (void**)&b->dp, sizeof(MemberParams) * nmembers))) return cleanup(rc);
+  b->params_dirty = true;
+  if (hipStreamSynchronize(b->stream) != hipSuccess)
+    return cleanup(fail(QGK_E_HIP, "stream sync failed"));
+  *out = b;
+  return 0;
+}
+
+int qgk_destroy(qgk_bucket* b) {
+  if (!b) return 0;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  for (auto& g : b->gexec)
+    if (g) hipGraphExecDestroy(g);
+  for (auto& p : b->pending) {
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
+  for (void* p : b->allocs) hipFree(p);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+int qgk_set_const(qgk_bucket* b, int which, int member, const double* host) {
+  if (!b || !host) return fail(QGK_E_ARG, "NULL argument");
+  if (which < 0 || which >= QGK_NCONSTS) return fail(QGK_E_ARG, "bad constant id %d", which);
+  if (member >= b->M) return fail(QGK_E_ARG, "member %d out of range", member);
+  const size_t bytes = const_len(b, which) * sizeof(double);
+  double* dst;
+  if (member < 0) {
+    dst = b->shared_tab[which];
+    for (int m = 0; m < b->M; ++m) param_slot(b->hp[m], which) = dst;
+  } else {
+    double*& own = b->own_tab[which][member];
+    if (!own) {
+      int rc = dev_alloc(b, (void**)&own, bytes);
+      if (rc) return rc;
+    }
+    dst = own;
+    param_slot(b->hp[member], which) = dst;
+  }
+  if (which == QGK_A) b->gvalid = false;
+  b->params_dirty = true;
+  HIPC(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, b->stream));
+  HIPC(hipStreamSynchronize(b->stream));
+  b->primed = false;
+  return 0;
+}
+
+int qgk_set_const_all(qgk_bucket* b, int which, const double* host) {
+  if (!b || !host) return fail(QGK_E_ARG, "NULL argument");
+  if (which < 0 || which >= QGK_NCONSTS) return fail(QGK_E_ARG, "bad constant id %d", which);
+  const size_t n = const_len(b, which);
+  double* tab = nullptr;
+  int rc = dev_alloc(b, (void**)&tab, n * b->M * sizeof(double));
+  if (rc) return rc;
+  HIPC(hipMemcpyAsync(tab, host, n * b->M * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  HIPC(hipStreamSynchronize(b->stream));
+  for (int m = 0; m < b->M; ++m) param_slot(b->hp[m], which) = tab + (size_t)m * n;
+  b->params_dirty = true;
+  b->primed = false;
+  b->gvalid = false;
+  return 0;
+}
+
+int qgk_set_inversion_2layer(qgk_bucket* b, int member, double F1, double F2) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (b->nz != 2) return fail(QGK_E_ARG, "closed-form inversion needs nz == 2");
+  if (member >= b->M) return fail(QGK_E_ARG, "member %d out of range", member);
+  for (int m = (member < 0 ? 0 : member); m < (member < 0 ? b->M : member + 1); ++m) {
+    b->hp[m].a = nullptr;
+    b->hp[m].F1 = F1;
+    b->hp[m].F2 = F2;
+  }
+  b->params_dirty = true;
+  b->primed = false;
+  b->gvalid = false;
+  return 0;
+}
+
+int qgk_set_rek(qgk_bucket* b, int member, double rek) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (member >= b->M) return fail(QGK_E_ARG, "member %d out of range", member);
+  for (int m = (member < 0 ? 0 : member); m < (member < 0 ? b->M : member + 1); ++m)
+    b->hp[m].rek = rek;
+  b->params_dirty = true;
+  b->gvalid = false;
+  return 0;
+}
+
+int qgk_put(qgk_bucket* b, int field, const void* host) {
+  if (!b || !host) return fail(QGK_E_ARG, "NULL argument");
+  int rc = check_field(b, field);
+  if (rc) return rc;
+  const size_t n = (size_t)b->M * b->nz;
+  if (is_real_field(field)) {
+    HIPC(hipMemcpyAsync(b->field_r[field], host, n * b->nplane_r * sizeof(double),
+                        hipMemcpyHostToDevice, b->stream));
+  } else {
+    HIPC(hipMemcpyAsync(spectral_ptr(b, field), host, n * b->nplane_c * sizeof(cd),
+                        hipMemcpyHostToDevice, b->stream));
+  }
+  HIPC(hipStreamSynchronize(b->stream));
+  if (field == QGK_QH) b->primed = false;
+  if (field == QGK_Q) b->q_stale = false;
+  return 0;
+}
+
+int qgk_get(qgk_bucket* b, int field, void* host) {
+  if (!b || !host) return fail(QGK_E_ARG, "NULL argument");
+  int rc = check_field(b, field);
+  if (rc) return rc;
+  if (field == QGK_Q && (rc = refresh_q(b))) return rc;
+  const size_t n = (size_t)b->M * b->nz;
+  if (is_real_field(field)) {
+    HIPC(hipMemcpyAsync(host, b->field_r[field], n * b->nplane_r * sizeof(double),
+                        hipMemcpyDeviceToHost, b->stream));
+  } else {
+    HIPC(hipMemcpyAsync(host, spectral_ptr(b, field), n * b->nplane_c * sizeof(cd),
+                        hipMemcpyDeviceToHost, b->stream));
+  }
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_invert(qgk_bucket* b) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  int rc = need_full(b, "_invert");
+  if (rc || (rc = upload_params(b))) return rc;
+  const size_t pts = (size_t)b->M * b->nplane_c;
+  hipLaunchKernelGGL(k_invert, dim3(grid_1d(pts)), dim3(256), 0, b->stream, b->field_c[QGK_QH],
+                     b->field_c[QGK_PH], b->field_c[QGK_UH], b->field_c[QGK_VH], b->dp, b->nz,
+                     b->nl, b->nk, b->M);
+  HIPC(hipGetLastError());
+  const int np = b->M * b->nz;
+  if ((rc = c2r2d(b, b->field_c[QGK_UH], b->field_r[QGK_U], np))) return rc;
+  if ((rc = c2r2d(b, b->field_c[QGK_VH], b->field_r[QGK_V], np))) return rc;
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_advect(qgk_bucket* b) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  int rc = need_full(b, "_do_advection");
+  if (rc || (rc = upload_params(b)) || (rc = refresh_q(b))) return rc;
+  const int np = b->M * b->nz;
+  const size_t rpts = (size_t)np * b->nplane_r;
+  hipLaunchKernelGGL(k_flux, dim3(grid_1d(rpts)), dim3(256), 0, b->stream, b->field_r[QGK_U],
+                     b->field_r[QGK_V], b->field_r[QGK_Q], b->field_r[QGK_UQ],
+                     b->field_r[QGK_VQ], b->dp, b->nz, b->nplane_r, b->M);
+  HIPC(hipGetLastError());
+  if ((rc = r2c2d(b, b->field_r[QGK_UQ], b->field_c[QGK_UQH], np))) return rc;
+  if ((rc = r2c2d(b, b->field_r[QGK_VQ], b->field_c[QGK_VQH], np))) return rc;
+  const size_t cpts = (size_t)np * b->nplane_c;
+  hipLaunchKernelGGL(k_divergence, dim3(grid_1d(cpts)), dim3(256), 0, b->stream,
+                     b->field_c[QGK_UQH], b->field_c[QGK_VQH], b->field_c[QGK_PH], D(b), b->dp,
+                     b->nz, b->nl, b->nk, b->M);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_friction(qgk_bucket* b) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  int rc = need_full(b, "_do_friction");
+  if (rc || (rc = upload_params(b))) return rc;
+  const size_t pts = (size_t)b->M * b->nplane_c;
+  hipLaunchKernelGGL(k_friction, dim3(grid_1d(pts)), dim3(256), 0, b->stream,
+                     b->field_c[QGK_PH], D(b), b->dp, b->nz, b->nl, b->nk, b->M);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_add_q_param(qgk_bucket* b, const double* dq_host) {
+  if (!b || !dq_host) return fail(QGK_E_ARG, "NULL argument");
+  if (!b->field_r[QGK_DQ]) return fail(QGK_E_STATE, "dq is not allocated (no q parameterization)");
+  const int np = b->M * b->nz;
+  HIPC(hipMemcpyAsync(b->field_r[QGK_DQ], dq_host, (size_t)np * b->nplane_r * sizeof(double),
+                      hipMemcpyHostToDevice, b->stream));
+  int rc = r2c2d(b, b->field_r[QGK_DQ], b->field_c[QGK_DQH], np);
+  if (rc) return rc;
+  const size_t cpts = (size_t)np * b->nplane_c;
+  hipLaunchKernelGGL(k_add, dim3(grid_1d(cpts)), dim3(256), 0, b->stream, D(b),
+                     b->field_c[QGK_DQH], cpts);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_add_uv_param(qgk_bucket* b, const double* du_host, const double* dv_host) {
+  if (!b || !du_host || !dv_host) return fail(QGK_E_ARG, "NULL argument");
+  if (!b->field_r[QGK_DU])
+    return fail(QGK_E_STATE, "du/dv are not allocated (no uv parameterization)");
+  int rc = upload_params(b);
+  if (rc) return rc;
+  const int np = b->M * b->nz;
+  const size_t bytes = (size_t)np * b->nplane_r * sizeof(double);
+  HIPC(hipMemcpyAsync(b->field_r[QGK_DU], du_host, bytes, hipMemcpyHostToDevice, b->stream));
+  HIPC(hipMemcpyAsync(b->field_r[QGK_DV], dv_host, bytes, hipMemcpyHostToDevice, b->stream));
+  if ((rc = r2c2d(b, b->field_r[QGK_DU], b->field_c[QGK_DUH], np))) return rc;
+  if ((rc = r2c2d(b, b->field_r[QGK_DV], b->field_c[QGK_DVH], np))) return rc;
+  const size_t cpts = (size_t)np * b->nplane_c;
+  hipLaunchKernelGGL(k_add_curl, dim3(grid_1d(cpts)), dim3(256), 0, b->stream, D(b),
+                     b->field_c[QGK_DUH], b->field_c[QGK_DVH], b->dp, b->nz, b->nl, b->nk, b->M);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_timestep(qgk_bucket* b, double dt1, double dt2, double dt3) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  int rc = upload_params(b);
+  if (rc) return rc;
+  const int np = b->M * b->nz;
+  const size_t cpts = (size_t)np * b->nplane_c;
+  StepCoeffs c = {dt1, dt2, dt3};
+  hipLaunchKernelGGL(k_ab_step, dim3(grid_1d(cpts)), dim3(256), 0, b->stream,
+                     b->field_c[QGK_QH], D(b), Pp(b), PP(b), b->dp, c, b->nz, b->nl, b->nk,
+                     b->M);
+  HIPC(hipGetLastError());
+  rotate_history(b);
+  b->primed = false;
+  b->q_stale = false;
+  if ((rc = c2r2d(b, b->field_c[QGK_QH], b->field_r[QGK_Q], np))) return rc;
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_q_to_qh(qgk_bucket* b) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  int rc = r2c2d(b, b->field_r[QGK_Q], b->field_c[QGK_QH], b->M * b->nz);
+  if (rc) return rc;
+  b->primed = false;
+  b->q_stale = false;
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_qh_to_q(qgk_bucket* b) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  int rc = c2r2d(b, b->field_c[QGK_QH], b->field_r[QGK_Q], b->M * b->nz);
+  if (rc) return rc;
+  b->q_stale = false;
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_fft(qgk_bucket* b, int nplanes, const double* in_real, double* out_complex) {
+  if (!b || !in_real || !out_complex) return fail(QGK_E_ARG, "NULL argument");
+  if (nplanes < 1 || nplanes > b->M * b->nz)
+    return fail(QGK_E_ARG, "nplanes must be in [1, %d]", b->M * b->nz);
+  HIPC(hipMemcpyAsync(b->rtmp, in_real, (size_t)nplanes * b->nplane_r * sizeof(double),
+                      hipMemcpyHostToDevice, b->stream));
+  // keep the generic transform off every state buffer: UQm is free between steps
+  // only in the fused path, so use the dedicated scratch pair (rtmp -> VQm is
+  // not safe either); the row pass writes its half spectra into ctmp.
+  int rc = launch_row_r2c(b, b->rtmp, b->ctmp, nplanes);
+  if (rc) return rc;
+  if ((rc = launch_col_fft<-1>(b, b->ctmp, b->ctmp, nplanes))) return rc;
+  HIPC(hipMemcpyAsync(out_complex, b->ctmp, (size_t)nplanes * b->nplane_c * sizeof(cd),
+                      hipMemcpyDeviceToHost, b->stream));
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_ifft(qgk_bucket* b, int nplanes, const double* in_complex, double* out_real) {
+  if (!b || !in_complex || !out_real) return fail(QGK_E_ARG, "NULL argument");
+  if (nplanes < 1 || nplanes > b->M * b->nz)
+    return fail(QGK_E_ARG, "nplanes must be in [1, %d]", b->M * b->nz);
+  HIPC(hipMemcpyAsync(b->ctmp, in_complex, (size_t)nplanes * b->nplane_c * sizeof(cd),
+                      hipMemcpyHostToDevice, b->stream));
+  int rc = launch_col_fft<+1>(b, b->ctmp, b->ctmp, nplanes);
+  if (rc) return rc;
+  if ((rc = launch_row_c2r(b, b->ctmp, b->rtmp, nplanes))) return rc;
+  HIPC(hipMemcpyAsync(out_real, b->rtmp, (size_t)nplanes * b->nplane_r * sizeof(double),
+                      hipMemcpyDeviceToHost, b->stream));
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_timing_enable(qgk_bucket* b, int enable) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  int rc = t_collect(b);
+  if (rc) return rc;
+  b->timing = enable != 0;
+  b->ms_row = b->ms_col = 0;
+  b->n_row = b->n_col = 0;
+  return 0;
+}
+
+int qgk_timing(qgk_bucket* b, double* ms_row, long* n_row, double* ms_col, long* n_col) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  int rc = t_collect(b);
+  if (rc) return rc;
+  if (ms_row) *ms_row = b->ms_row;
+  if (n_row) *n_row = b->n_row;
+  if (ms_col) *ms_col = b->ms_col;
+  if (n_col) *n_col = b->n_col;
+  return 0;
+}
+
+int qgk_sync(qgk_bucket* b) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_step_buckets(qgk_bucket** bs, int nb, int nsteps, const double* coeffs, int use_graph) {
+  if (!bs || nb < 1 || nsteps < 0 || (nsteps > 0 && !coeffs))
+    return fail(QGK_E_ARG, "bad arguments");
+  (void)use_graph;
+  for (int i = 0; i < nb; ++i) {
+    qgk_bucket* b = bs[i];
+    if (!b) return fail(QGK_E_ARG, "NULL bucket");
+    int rc = upload_params(b);
+    if (rc) return rc;
+    if (!b->primed) {
+      if ((rc = launch_col_prologue(b))) return rc;
+      b->primed = true;
+    }
+  }
+  for (int s = 0; s < nsteps; ++s) {
+    StepCoeffs cf = {coeffs[3 * s], coeffs[3 * s + 1], coeffs[3 * s + 2]};
+    for (int i = 0; i < nb; ++i) {
+      int rc = fused_step(bs[i], cf);
+      if (rc) return rc;
+    }
+  }
+  for (int i = 0; i < nb; ++i) {
+    // the fused step leaves the real q stale; it is refreshed when next needed
+    if (nsteps > 0) bs[i]->q_stale = true;
+    HIPC(hipStreamSynchronize(bs[i]->stream));
+  }
+  return 0;
+}
+
+int qgk_step(qgk_bucket* b, int nsteps, const double* coeffs, int use_graph) {
+  return qgk_step_buckets(&b, 1, nsteps, coeffs, use_graph);
+}
+
+}  // extern "C"

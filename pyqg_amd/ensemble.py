@@ -1,0 +1,176 @@
+"""Ensembles: many independent pyqg models stepped together on one GPU.
+
+pyqg has no ensemble concept (users build separate Model objects); here the
+members of one grid size share one libqgk handle ("bucket"), so one fused step
+of every member is two kernel launches per bucket.  Members of different nx
+go to different buckets; all buckets are stepped in one call.
+
+Each member follows exactly the per-model semantics of the reference
+(pyqg/model.py:406-436 with kernel.pyx:312-526): the same constants as the
+corresponding QGModel / LayeredModel would build, the same AB start-up and the
+same accumulated time t.
+"""
+import numpy as np
+
+from . import _lib
+from . import constants as C
+from ._lib import Bucket
+
+
+def _as_array(v, n):
+    a = np.asarray(v, dtype=np.float64)
+    return np.full(n, float(a)) if a.ndim == 0 else a.astype(np.float64)
+
+
+class Ensemble:
+    """A set of independent models with a common time step dt."""
+
+    def __init__(self, dt):
+        self._dt = float(dt)
+        self.t = 0.0
+        self.tc = 0
+        self.ablevel = 0
+        self.buckets = []      # Bucket handles, one per (nz, nx)
+        self.members = []      # member index -> (bucket index, position in bucket)
+        self.member_nx = []
+
+    @property
+    def dt(self):
+        return self._dt
+
+    @dt.setter
+    def dt(self, dt):
+        self._dt = float(dt)
+        self.ablevel = 0
+
+    @property
+    def nmembers(self):
+        return len(self.members)
+
+    # ------------------------------------------------------------ builders
+    @classmethod
+    def qg(cls, nx, beta=1.5e-11, rd=15000.0, delta=0.25, rek=5.787e-7, L=1e6, H1=500.,
+           U1=0.025, U2=0.0, dt=3600., filterfac=23.6, nmembers=None):
+        """Two-layer QGModels (qg_model.py:60-161).  Every argument may be a
+        scalar or one value per member; members are bucketed by nx."""
+        if nmembers is None:
+            sizes = [np.size(v) for v in (nx, beta, rd, delta, rek, L, U1, U2)]
+            nmembers = max(sizes)
+        n = int(nmembers)
+        nxs = np.full(n, int(nx)) if np.ndim(nx) == 0 else np.asarray(nx, dtype=int)
+        p = {k: _as_array(v, n) for k, v in dict(beta=beta, rd=rd, delta=delta, rek=rek, L=L,
+                                                  H1=H1, U1=U1, U2=U2,
+                                                  filterfac=filterfac).items()}
+        ens = cls(dt)
+        ens.member_nx = list(nxs)
+        ens.members = [None] * n
+        for size in sorted(set(nxs.tolist())):
+            idx = np.nonzero(nxs == size)[0]
+            b = Bucket(len(idx), 2, size, size, 0)
+            nk = size // 2 + 1
+            kk = np.empty((len(idx), nk))
+            ll = np.empty((len(idx), size))
+            k2l2 = np.empty((len(idx), size, nk))
+            filt = np.empty((len(idx), size, nk))
+            Ubg = np.empty((len(idx), 2))
+            Qy = np.empty((len(idx), 2))
+            for j, m in enumerate(idx):
+                g = C.grid(size, size, p["L"][m], p["L"][m])
+                kk[j], ll[j], k2l2[j] = g["kk"], g["ll"], g["k2l2"]
+                filt[j] = C.spectral_filter(g["k"], g["l"], g["dx"], g["dy"], p["filterfac"][m])
+                F1, F2, Qy[j], Ubg[j] = C.qg_background(p["beta"][m], p["rd"][m], p["delta"][m],
+                                                        p["U1"][m], p["U2"][m])
+                b.set_inversion_2layer(F1, F2, member=j)
+                b.set_rek(p["rek"][m], member=j)
+                ens.members[m] = (len(ens.buckets), j)
+            ens._set_table(b, "kk", kk)
+            ens._set_table(b, "ll", ll)
+            ens._set_table(b, "k2l2", k2l2)
+            ens._set_table(b, "filtr", filt)
+            ens._set_table(b, "Ubg", Ubg)
+            ens._set_table(b, "Qy", Qy)
+            ens.buckets.append(b)
+        ens.params = p
+        return ens
+
+    @classmethod
+    def layered(cls, nmembers, nx, nz, H, rho, U, L=1e6, beta=1.5e-11, rek=5.787e-7,
+                f=0.0001236812857687059, g=9.81, dt=3600., filterfac=23.6):
+        """nmembers LayeredModels with one shared stratification
+        (layered_model.py:72-229); the inversion table is shared by all."""
+        n, nz = int(nmembers), int(nz)
+        Hi, rhoi, Ubg = (np.asarray(v, dtype=np.float64) for v in (H, rho, U))
+        gr = C.grid(nx, nx, L, L)
+        S, _ = C.layered_stretching(nz, Hi, rhoi, g, f * f)
+        Qy = beta - np.dot(S, Ubg)
+        a = C.layered_inversion(S, gr["wv2"])
+        ens = cls(dt)
+        b = Bucket(n, nz, nx, nx, 0)
+        b.set_const("kk", gr["kk"])
+        b.set_const("ll", gr["ll"])
+        b.set_const("k2l2", gr["k2l2"])
+        b.set_const("filtr", C.spectral_filter(gr["k"], gr["l"], gr["dx"], gr["dy"], filterfac))
+        b.set_const("a", a)
+        b.set_const("Ubg", Ubg)
+        b.set_const("Qy", Qy)
+        b.set_rek(rek)
+        ens.buckets = [b]
+        ens.members = [(0, j) for j in range(n)]
+        ens.member_nx = [nx] * n
+        ens.S, ens.Qy, ens.a = S, Qy, a
+        return ens
+
+    @staticmethod
+    def _set_table(b, name, per_member):
+        """Upload one shared table when every member's is identical, else one
+        table per member."""
+        if all(np.array_equal(per_member[0], x) for x in per_member[1:]):
+            b.set_const(name, per_member[0])
+        else:
+            b.set_const_all(name, per_member)
+
+    # ------------------------------------------------------------ state
+    def _bucket_members(self, bi):
+        return [m for m, (b, _) in enumerate(self.members) if b == bi]
+
+    def set_q(self, q):
+        """Initial PV of every member: a sequence of (nz, ny, nx) arrays (or one
+        array with a leading member axis when all members share a size).  Sets
+        qh = rfft2(q) as the reference's q setter does (kernel.pyx:576-582)."""
+        for bi, b in enumerate(self.buckets):
+            mem = self._bucket_members(bi)
+            arr = np.stack([np.asarray(q[m], dtype=np.float64) for m in mem])
+            b.put("q", arr)
+            _lib.check(_lib.lib.qgk_q_to_qh(b.h))
+
+    def get(self, field, member=None):
+        """A field ("q", "qh", "dqhdt_p", ...) of one member, or a list for all."""
+        out = [b.get(field) for b in self.buckets]
+        if member is not None:
+            bi, j = self.members[member]
+            return out[bi][j]
+        return [out[bi][j] for (bi, j) in self.members]
+
+    # ------------------------------------------------------------ stepping
+    def step(self, nsteps=1, use_graph=False):
+        """nsteps fused steps of every member."""
+        nsteps = int(nsteps)
+        if nsteps <= 0:
+            return
+        coeffs = np.empty((nsteps, 3))
+        for s in range(nsteps):
+            w, self.ablevel = C.ab_weights(self.ablevel, self._dt)
+            coeffs[s] = w
+        _lib.step_buckets(self.buckets, coeffs, use_graph)
+        for _ in range(nsteps):
+            self.t += self._dt
+        self.tc += nsteps
+
+    def model_days(self, nsteps):
+        """Simulated model-days of nsteps steps summed over members."""
+        return self.nmembers * nsteps * self._dt / 86400.
+
+    def close(self):
+        for b in self.buckets:
+            b.close()
+        self.buckets = []

@@ -1,0 +1,441 @@
+"""Model driver and the four physics front-ends on top of the HIP kernel.
+
+The classes keep the constructor signatures, initialisation order and stepping
+order of pyqg's Model / QGModel / LayeredModel / BTModel / SQGModel
+(pyqg/model.py:101-258, 406-436; qg_model.py; layered_model.py; bt_model.py;
+sqg_model.py).  They only build the constants the kernel consumes (wavenumbers,
+filter, background, inversion matrix) and orchestrate the kernel's phases;
+all per-step arithmetic happens in libqgk.  Time-averaged diagnostics, xarray
+output and linear stability analysis are not part of this package.
+"""
+import logging
+import warnings
+
+import numpy as np
+from numpy import pi
+
+from . import constants as C
+from .kernel import PseudoSpectralKernel
+
+
+class Model(PseudoSpectralKernel):
+    """Generic pseudo-spectral model (pyqg/model.py:22-258)."""
+
+    def __init__(self, nz=1, nx=64, ny=None, L=1e6, W=None, dt=7200., twrite=1000.,
+                 tmax=1576800000., tavestart=315360000., taveint=86400., useAB2=False,
+                 rek=5.787e-7, filterfac=23.6, f=None, g=9.81, q_parameterization=None,
+                 uv_parameterization=None, parameterization=None, diagnostics_list='all',
+                 ntd=1, log_level=1, logfile=None):
+        if ny is None:
+            ny = nx
+        if W is None:
+            W = L
+        # infer the type of an explicit parameterization object (model.py:201-212)
+        if parameterization is not None:
+            ptype = getattr(parameterization, "parameterization_type", None)
+            if ptype == 'uv_parameterization':
+                assert uv_parameterization is None
+                uv_parameterization = parameterization
+            elif ptype == 'q_parameterization':
+                assert q_parameterization is None
+                q_parameterization = parameterization
+            else:
+                raise ValueError(f"unknown parameterization type {ptype}")
+
+        PseudoSpectralKernel.__init__(self, nz, ny, nx, ntd,
+                                      has_q_param=int(q_parameterization is not None),
+                                      has_uv_param=int(uv_parameterization is not None))
+        self.L, self.W = L, W
+        self.dt = dt
+        self.twrite = twrite
+        self.tmax = tmax
+        self.tavestart = tavestart
+        self.taveint = taveint
+        self.logfile = logfile
+        self.log_level = log_level
+        self.useAB2 = useAB2
+        self.ntd = ntd
+        self.rek = rek
+        self.filterfac = filterfac
+        self.g = g
+        if f:
+            self.f = f
+            self.f2 = f ** 2
+        self.q_parameterization = q_parameterization
+        self.uv_parameterization = uv_parameterization
+        self.diagnostics_list = diagnostics_list
+        self._diag_warned = False
+
+        self._initialize_logger()
+        self._initialize_grid()
+        self._initialize_background()
+        self._initialize_forcing()
+        self._initialize_filter()
+        self._initialize_time()
+        self._initialize_inversion_matrix()
+
+    # ------------------------------------------------------------ running
+    def run_with_snapshots(self, tsnapstart=0., tsnapint=432000.):
+        """Yield model time every tsnapint once t >= tsnapstart (model.py:260-278)."""
+        tsnapints = np.ceil(tsnapint / self.dt)
+        while self.t < self.tmax:
+            self._step_forward()
+            if self.t >= tsnapstart and (self.tc % tsnapints) == 0:
+                yield self.t
+
+    def run(self):
+        """Run until t >= tmax (model.py:280-283).
+
+        Without parameterizations the steps between two status reports run in
+        the fused two-kernel form; the step that a status report observes runs
+        phase by phase, so KE / CFL see the same fields as the reference's."""
+        if self.q_parameterization is not None or self.uv_parameterization is not None:
+            while self.t < self.tmax:
+                self._step_forward()
+            return
+        # how many steps until t >= tmax with the reference's accumulated time
+        t, n = self.t, 0
+        while t < self.tmax:
+            t += self.dt
+            n += 1
+        while n > 0:
+            chunk = n
+            if self.log_level:
+                to_status = int(self.twrite - (self.tc % self.twrite))
+                chunk = min(n, max(to_status, 1))
+            t_before = self.t
+            self._fused_steps(chunk)
+            self._warn_diagnostics(t_before)
+            self._print_status()
+            n -= chunk
+
+    def _step_forward(self):
+        """One step in the reference's order (model.py:406-436)."""
+        self._invert()
+        self._do_advection()
+        self._do_friction()
+        self._do_external_forcing()
+        if self.uv_parameterization is not None:
+            self._do_uv_subgrid_parameterization()
+        if self.q_parameterization is not None:
+            self._do_q_subgrid_parameterization()
+        self._calc_diagnostics()
+        self._forward_timestep()
+        self._print_status()
+
+    def _do_external_forcing(self):
+        pass
+
+    def _calc_diagnostics(self):
+        if (self.t >= self.dt) and (self.t >= self.tavestart):
+            self._warn_diagnostics(self.t)
+
+    def _warn_diagnostics(self, t):
+        if t >= self.tavestart and not self._diag_warned:
+            warnings.warn("pyqg_amd does not accumulate time-averaged diagnostics; "
+                          "the state is stepped without them", RuntimeWarning)
+            self._diag_warned = True
+
+    def _print_status(self):
+        """Log KE and CFL every twrite steps; assert CFL < 1 (model.py:564-574)."""
+        if self.log_level and (self.tc % self.twrite) == 0:
+            self.ke = self._calc_ke()
+            self.cfl = self._calc_cfl()
+            self.logger.info('Step: %i, Time: %3.2e, KE: %3.2e, CFL: %4.3f',
+                             self.tc, self.t, self.ke, self.cfl)
+            assert self.cfl < 1., self.logger.error('CFL condition violated')
+
+    # ------------------------------------------------------------ set-up
+    def _initialize_time(self):
+        self.taveints = np.ceil(self.taveint / self.dt)
+
+    def _initialize_grid(self):
+        """Grids and wavenumbers (model.py:446-482)."""
+        g = C.grid(self.nx, self.ny, self.L, self.W)
+        self.x, self.y = g["x"], g["y"]
+        self.dk, self.dl = g["dk"], g["dl"]
+        self.ll = g["ll"]
+        self.kk = g["kk"]
+        self.k, self.l = g["k"], g["l"]
+        self.ik = 1j * self.k
+        self.il = 1j * self.l
+        self.dx, self.dy, self.M = g["dx"], g["dy"], g["M"]
+        self.wv2, self.wv, self.wv2i = g["wv2"], g["wv"], g["wv2i"]
+
+    def _initialize_filter(self):
+        """Exponential cutoff filter of Arbic & Flierl (2003) (model.py:496-503)."""
+        self.filtr = C.spectral_filter(self.k, self.l, self.dx, self.dy, self.filterfac)
+
+    def _filter(self, q):
+        return self.filtr * q
+
+    def _initialize_background(self):
+        raise NotImplementedError('needs to be implemented by Model subclass')
+
+    def _initialize_inversion_matrix(self):
+        raise NotImplementedError('needs to be implemented by Model subclass')
+
+    def _initialize_forcing(self):
+        pass
+
+    def _initialize_logger(self):
+        self.logger = logging.getLogger(__name__)
+        if self.logfile is not None:
+            handler = logging.FileHandler(filename=self.logfile, mode='w')
+        else:
+            handler = logging.StreamHandler()
+        handler.setFormatter(logging.Formatter('%(levelname)s: %(message)s'))
+        if not self.logger.handlers:
+            self.logger.addHandler(handler)
+        self.logger.setLevel(self.log_level * 10)
+        self.logger.propagate = False
+
+    # ------------------------------------------------------------ helpers
+    def spec_var(self, ph):
+        """Variance of a field from its rfft2 coefficients (model.py:832-838)."""
+        var_dens = 2. * np.abs(ph) ** 2 / self.M ** 2
+        var_dens[..., 0] = var_dens[..., 0] / 2.
+        var_dens[..., -1] = var_dens[..., -1] / 2.
+        return var_dens.sum()
+
+    def _advect(self, q, u=None, v=None):
+        """Spectral advective tendency of real q, u, v (model.py:538-558)."""
+        u = self.u if u is None else u
+        v = self.v if v is None else v
+        uq, vq = u * q, v * q
+        is_2d = uq.ndim == 2
+        if is_2d:
+            uq = np.tile(uq[np.newaxis], (self.nz, 1, 1))
+            vq = np.tile(vq[np.newaxis], (self.nz, 1, 1))
+        tend = self.ik * self.fft(uq) + self.il * self.fft(vq)
+        return tend[0] if is_2d else tend
+
+    def set_q(self, q):
+        warnings.warn("Method deprecated. Set model.q directly instead. ", DeprecationWarning)
+        self.q = q
+
+    def set_qh(self, qh):
+        warnings.warn("Method deprecated. Set model.qh directly instead. ", DeprecationWarning)
+        self.qh = qh
+
+    def vertical_modes(self):
+        """Deformation radii and pressure modes from the stretching matrix
+        (model.py:286-305)."""
+        evals, evecs = np.linalg.eig(-self.S)
+        asort = evals.argsort()
+        self.kdi2 = evals[asort]
+        self.radii = np.zeros_like(self.kdi2)
+        self.radii[0] = np.sqrt(self.g * self.H) / np.abs(self.f)
+        self.radii[1:] = 1. / np.sqrt(self.kdi2[1:])
+        self.pmodes = evecs[:, asort]
+        Ai = (self.H / (self.Hi[:, np.newaxis] * (self.pmodes ** 2)).sum(axis=0)) ** 0.5
+        self.pmodes = Ai[np.newaxis, :] * self.pmodes
+
+    @property
+    def parameterization(self):
+        if self.q_parameterization and self.uv_parameterization:
+            warnings.warn("Model has multiple parameterizations, but only returning PV")
+        return self.q_parameterization or self.uv_parameterization
+
+    def _calc_cfl(self):
+        return np.abs(np.hstack([self.u + self.Ubg[:, np.newaxis, np.newaxis], self.v])
+                      ).max() * self.dt / self.dx
+
+    def _calc_ke(self):
+        ke = 0.
+        for j in range(self.nz):
+            ke += .5 * self.Hi[j] * self.spec_var(self.wv * self.ph[j])
+        return ke.sum() / self.H
+
+
+class QGModel(Model):
+    """Two-layer QG model (pyqg/qg_model.py:17-209)."""
+
+    def __init__(self, beta=1.5e-11, rd=15000.0, delta=0.25, H1=500, U1=0.025, U2=0.0, **kwargs):
+        self.beta = beta
+        self.rd = rd
+        self.delta = delta
+        self.Hi = np.array([H1, H1 / delta])
+        self.U1 = U1
+        self.U2 = U2
+        super().__init__(nz=2, **kwargs)
+        # initial PV anomalies (qg_model.py:103-106)
+        self.set_q1q2(1e-7 * np.random.rand(self.ny, self.nx)
+                      + 1e-6 * (np.ones((self.ny, 1)) * np.random.rand(1, self.nx)),
+                      np.zeros_like(self.x))
+
+    def _initialize_background(self):
+        """qg_model.py:110-138"""
+        self.H = self.Hi.sum()
+        self.set_U1U2(self.U1, self.U2)
+        self.U = self.U1 - self.U2
+        self.Vbg = np.zeros_like(self.Ubg)
+        self.F1, self.F2, Qy, _ = C.qg_background(self.beta, self.rd, self.delta, self.U1,
+                                                  self.U2)
+        self.Qy1, self.Qy2 = Qy
+        self.Qy = Qy
+        self.ikQy1 = self.Qy1 * 1j * self.k
+        self.ikQy2 = self.Qy2 * 1j * self.k
+        self.ikQy = np.vstack([self.ikQy1[np.newaxis], self.ikQy2[np.newaxis]])
+        self.ilQx = 0.
+        self.del1 = self.delta / (self.delta + 1.)
+        self.del2 = (self.delta + 1.) ** -1
+
+    @property
+    def S(self):
+        return np.array([[-self.F1, self.F1], [self.F2, -self.F2]]).astype(np.float64)
+
+    def _initialize_inversion_matrix(self):
+        """Closed-form 2x2 inverse of (S - kappa^2 I), zero at kappa = 0
+        (qg_model.py:146-161)."""
+        self.a = C.qg_inversion(self.wv2, self.F1, self.F2)
+
+    def set_q1q2(self, q1, q2, check=False):
+        self.q = np.vstack([q1[np.newaxis, :, :], q2[np.newaxis, :, :]])
+        if check:
+            np.testing.assert_allclose(self.q[0], q1)
+
+    def set_U1U2(self, U1, U2):
+        self.U1 = U1
+        self.U2 = U2
+        self.Ubg = np.array([U1, U2])
+
+
+class LayeredModel(Model):
+    """N-layer QG model (pyqg/layered_model.py:17-229)."""
+
+    def __init__(self, beta=1.5e-11, nz=3, rd=15000.0, f=0.0001236812857687059, H=None, U=None,
+                 V=None, rho=None, delta=None, **kwargs):
+        self.beta = beta
+        self.rd = rd
+        self.delta = delta
+        if U is None:
+            U = (np.arange(nz) * 0.025)[::-1]
+        if V is None:
+            V = np.zeros(nz)
+        if H is None:
+            H = [500] + [1750 for _ in range(nz - 1)]
+        if rho is None:
+            rho = np.arange(nz) * 0.3 + 1025
+        self.Ubg = np.array(U)
+        self.Vbg = np.array(V)
+        self.Hi = np.array(H)
+        self.rhoi = np.array(rho)
+        super().__init__(nz=nz, f=f, **kwargs)
+        self.vertical_modes()
+        self.set_q(1e-7 * np.vstack([np.random.randn(self.nx, self.ny)[np.newaxis]
+                                     for _ in range(nz)]))
+
+    def _initialize_stretching_matrix(self):
+        """Stretching matrix S (layered_model.py:136-168)."""
+        if self.nz == 2 and self.rd and self.delta:
+            self.del1 = self.delta / (self.delta + 1.)
+            self.del2 = (self.delta + 1.) ** -1
+            self.Us = self.Ubg[0] - self.Ubg[1]
+            self.F1 = self.rd ** -2 / (1. + self.delta)
+            self.F2 = self.delta * self.F1
+        self.S, _ = C.layered_stretching(self.nz, self.Hi, self.rhoi, self.g,
+                                         getattr(self, "f2", None), self.rd, self.delta)
+
+    def _initialize_background(self):
+        """layered_model.py:170-209"""
+        self.H = self.Hi.sum()
+        if self.nz != 2:
+            self.gpi = self.g * (self.rhoi[1:] - self.rhoi[:-1]) / self.rhoi[:-1]
+            self.f2gpi = (self.f2 / self.gpi)[:, np.newaxis, np.newaxis]
+            assert self.gpi.size == self.nz - 1, "Invalid size of gpi"
+            assert np.all(self.gpi > 0.), "Buoyancy jump has negative sign!"
+            assert self.Hi.size == self.nz, 'size of Hi does not match number of vertical levels nz'
+            assert self.rhoi.size == self.nz, 'size of rhoi does not match number of vertical levels nz'
+            assert self.Ubg.size == self.nz, 'size of Ubg does not match number of vertical levels nz'
+            assert self.Vbg.size == self.nz, 'size of Vbg does not match number of vertical levels nz'
+        else:
+            self.f2gpi = np.array(self.rd ** -2 * (self.Hi[0] * self.Hi[1]) / self.H
+                                  )[np.newaxis, np.newaxis]
+        self._initialize_stretching_matrix()
+        self.Qy = self.beta - np.dot(self.S, self.Ubg)
+        self.Qx = np.dot(self.S, self.Vbg)
+        self.ikQy = self.Qy[:, np.newaxis, np.newaxis] * 1j * self.k
+        self.ilQx = self.Qx[:, np.newaxis, np.newaxis] * 1j * self.l
+
+    def _initialize_inversion_matrix(self):
+        """a = (S - kappa^2 I)^-1 per wavenumber, zero at kappa = 0
+        (layered_model.py:211-229)."""
+        self.a = C.layered_inversion(self.S, self.wv2)
+
+
+class BTModel(Model):
+    """Single-layer (equivalent) barotropic model (pyqg/bt_model.py:6-104)."""
+
+    def __init__(self, beta=0., rd=None, H=1., U=0., **kwargs):
+        self.beta = beta
+        self.rd = rd
+        self.H = H
+        self.Hi = np.array(H)[np.newaxis, ...]
+        self.U = U
+        self.nz = 1
+        self.kd2 = rd ** -2 if rd else 0.
+        super().__init__(**kwargs)
+        self.set_q(1e-3 * np.random.rand(1, self.ny, self.nx))
+
+    def _initialize_background(self):
+        self.Qy = np.asarray(self.beta)[np.newaxis, ...]
+        self.set_U(self.U)
+        self.ikQy = self.Qy * 1j * self.k
+        self.ilQx = 0.
+
+    def _initialize_inversion_matrix(self):
+        if self.rd:
+            self.a = -(1. / (self.wv2 + self.kd2))[np.newaxis, np.newaxis, :, :]
+        else:
+            self.a = -self.wv2i[np.newaxis, np.newaxis, :, :]
+
+    def set_U(self, U):
+        self.Ubg = np.asarray(U)[np.newaxis, ...]
+
+    def _calc_diagnostics(self):
+        if (self.t >= self.dt) and (self.tc % self.taveints == 0):
+            self._warn_diagnostics(self.tavestart)
+
+    def _calc_cfl(self):
+        return np.abs(np.hstack([self.u + self.Ubg, self.v])).max() * self.dt / self.dx
+
+    def _calc_ke(self):
+        return (.5 * self.spec_var(self.wv * self.ph)).sum()
+
+
+class SQGModel(Model):
+    """Surface QG model (pyqg/sqg_model.py:6-79)."""
+
+    def __init__(self, beta=0., Nb=1., f_0=1., H=1., U=0., **kwargs):
+        self.beta = beta
+        self.Nb = Nb
+        self.f_0 = f_0
+        self.H = H
+        self.Hi = np.array(H)[np.newaxis, ...]
+        self.U = U
+        self.nz = 1
+        super().__init__(**kwargs)
+        self.set_q(1e-3 * np.random.rand(1, self.ny, self.nx))
+
+    def _initialize_background(self):
+        self.Qy = np.asarray(self.beta)[np.newaxis, ...]
+        self.set_U(self.U)
+        self.ikQy = self.Qy * 1j * self.k
+        self.ilQx = 0.
+
+    def _initialize_inversion_matrix(self):
+        self.a = np.asarray(self.f_0 / self.Nb * np.sqrt(self.wv2i))[np.newaxis, np.newaxis, :, :]
+
+    def set_U(self, U):
+        self.Ubg = np.asarray(U)[np.newaxis, ...]
+
+    def _calc_diagnostics(self):
+        if (self.t >= self.dt) and (self.tc % self.taveints == 0):
+            self._warn_diagnostics(self.tavestart)
+
+    def _calc_cfl(self):
+        return np.abs(np.hstack([self.u + self.Ubg, self.v])).max() * self.dt / self.dx
+
+    def _calc_ke(self):
+        return (.5 * self.spec_var(self.wv * self.ph)).sum()

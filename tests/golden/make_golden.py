@@ -1,0 +1,249 @@
+"""Writes the golden vectors under tests/golden/*.npz from the reference pyqg.
+
+Runs only in the development container, where /root/reference exists.  The
+reference kernel (pyqg/kernel.pyx) is compiled by build_reference.sh in its
+numpy.fft fallback form (pyfftw is absent offline); the reference's own Python
+model classes then run on top of it.  Nothing here is imported by the package,
+the GPU tests, smoke() or bench.py: they read only the .npz files written here.
+
+What these vectors pin, and why an import is needed for them: the composed
+time step Model._step_forward (pyqg/model.py:406-436) = _invert ->
+_do_advection -> _do_friction -> [uv/q parameterization] -> _forward_timestep
+(pyqg/kernel.pyx:312-526), with the reference's exact AB start-up, filter
+placement and numpy c2r handling, over 1..1000 steps.  The closed-form tests of
+the reference pin each phase separately, and its long-run statistics pin the
+composition only to 1e-2..1e-5; the 1e-10 trajectory bar needs the trajectory.
+
+Usage:  bash tests/golden/build_reference.sh && python3 tests/golden/make_golden.py
+"""
+import glob
+import importlib.util
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_ROOT = "/root/reference"
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from pyqg_amd.constants import exponential_stratification as stratification_nz10  # noqa: E402
+
+
+def import_reference():
+    sys.dont_write_bytecode = True
+    so = glob.glob(os.path.join(HERE, "_refbuild", "kernel*.so"))
+    if not so:
+        raise SystemExit("run tests/golden/build_reference.sh first")
+    spec = importlib.util.spec_from_file_location("pyqg.kernel", so[0])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["pyqg.kernel"] = mod
+    spec.loader.exec_module(mod)
+    sys.path.insert(0, REF_ROOT)
+    import warnings
+    warnings.simplefilter("ignore")
+    import pyqg
+    return pyqg
+
+
+QG_PHYS = dict(L=1e6, H1=500.0, delta=0.25, U1=0.025, U2=0.0, beta=1.5e-11,
+               rd=15000.0, rek=5.787e-7, filterfac=23.6)
+
+STATE_FIELDS = ["q", "qh", "ph", "u", "v", "uq", "vq", "uqh", "vqh",
+                "dqhdt", "dqhdt_p", "dqhdt_pp"]
+
+
+def snap(m, fields, prefix, out):
+    for f in fields:
+        out[f"{prefix}{f}"] = np.array(getattr(m, f)).copy()
+    out[f"{prefix}t"] = np.float64(m.t)
+    out[f"{prefix}tc"] = np.int64(m.tc)
+    out[f"{prefix}ablevel"] = np.int64(m.ablevel)
+
+
+def constants(m, out, a_full=True):
+    for f in ["kk", "ll", "filtr", "Qy", "Ubg", "_k2l2"]:
+        out[f"const_{f}"] = np.array(getattr(m, f)).copy()
+    a = np.array(m.a)
+    if a_full:
+        out["const_a"] = a.real.copy()
+    else:
+        out["const_a_5_5"] = a[:, :, 5, 5].real.copy()
+        out["const_a_3_7"] = a[:, :, 3, 7].real.copy()
+        out["const_a_m2_1"] = a[:, :, -2, 1].real.copy()
+
+
+def qg_startup(pyqg):
+    """QGModel nx=32, seeded 1e-7 N(0,1) IC: every state field after steps 1 and 3."""
+    out = {}
+    m = pyqg.QGModel(nx=32, dt=7200.0, log_level=0, **QG_PHYS)
+    q0 = 1e-7 * np.random.default_rng(0).standard_normal((2, 32, 32))
+    m.q = q0
+    out["q0"] = q0
+    out["qh0"] = np.array(m.qh).copy()
+    constants(m, out)
+    for step in range(1, 4):
+        m._step_forward()
+        if step in (1, 3):
+            snap(m, STATE_FIELDS, f"s{step}_", out)
+    np.savez(os.path.join(HERE, "qg_nx32_startup.npz"), **out)
+
+
+def qg_config1(pyqg):
+    """QGModel nx=64, dt=7200 (the CPU-runnable workload): q at steps 100 and 1000."""
+    out = {}
+    m = pyqg.QGModel(nx=64, dt=7200.0, log_level=0, **QG_PHYS)
+    q0 = 1e-7 * np.random.default_rng(0).standard_normal((2, 64, 64))
+    m.q = q0
+    out["q0"] = q0
+    for step in range(1, 1001):
+        m._step_forward()
+        if step in (100, 1000):
+            out[f"s{step}_q"] = np.array(m.q).copy()
+            out[f"s{step}_t"] = np.float64(m.t)
+    np.savez(os.path.join(HERE, "qg_nx64_cfg1.npz"), **out)
+
+
+LAYERED3 = dict(nz=3, U=[0.05, 0.025, 0.0], V=[0.0, 0.0, 0.0], H=[500, 1750, 1750],
+                rho=[1025.0, 1025.3, 1025.6], L=1e6, rek=5.787e-7, g=9.81, dt=3600.0,
+                beta=1.5e-11, rd=15000.0, f=0.0001236812857687059)
+
+
+def layered_nz3(pyqg):
+    """LayeredModel nz=3 from the reference's spun-up fixture state: q after 1 and 20 steps."""
+    out = {}
+    m = pyqg.LayeredModel(nx=64, log_level=0, **LAYERED3)
+    q0 = np.load(os.path.join(REF_ROOT, "pyqg/tests/fixtures/LayeredModel_nx64_q.npy"))
+    m.q = q0
+    out["q0"] = q0
+    constants(m, out, a_full=False)
+    out["const_S"] = np.array(m.S)
+    for step in range(1, 21):
+        m._step_forward()
+        if step == 1:
+            snap(m, ["q", "ph", "dqhdt"], "s1_", out)
+        if step == 20:
+            snap(m, ["q", "ph"], "s20_", out)
+    np.savez(os.path.join(HERE, "layered_nz3_nx64.npz"), **out)
+
+
+def layered_nz10(pyqg):
+    """LayeredModel nz=10 (exponential stratification, 20 km first radius) at nx=32:
+    inversion-matrix samples and q after 10 steps."""
+    out = {}
+    H, rho, U = stratification_nz10()
+    kw = dict(nz=10, H=H, rho=rho, U=U, V=np.zeros(10), L=2e6, beta=1.5e-11,
+              rek=5.787e-7, dt=3600.0, f=0.0001236812857687059, g=9.81)
+    m = pyqg.LayeredModel(nx=32, log_level=0, **kw)
+    out["in_H"], out["in_rho"], out["in_U"] = H, rho, U
+    q0 = 1e-7 * np.random.default_rng(3).standard_normal((10, 32, 32))
+    m.q = q0
+    out["q0"] = q0
+    constants(m, out, a_full=False)
+    out["const_radii"] = np.array(m.radii)
+    for step in range(1, 11):
+        m._step_forward()
+    snap(m, ["q"], "s10_", out)
+    np.savez(os.path.join(HERE, "layered_nz10_nx32.npz"), **out)
+
+
+def gaussian_ic(m):
+    """Two-Gaussian streamfunction of the reference's BT/SQG regression runs
+    (pyqg/tests/test_reference_solns.py:104-106)."""
+    return (np.exp(-(2. * (m.x - 1.75 * np.pi / 2)) ** 2. - (2. * (m.y - np.pi)) ** 2)
+            + np.exp(-(2. * (m.x - 2.25 * np.pi / 2)) ** 2. - (2. * (m.y - np.pi)) ** 2))
+
+
+def bt_sqg(pyqg):
+    """BTModel and SQGModel nx=64 two-Gaussian runs: q after 100 steps."""
+    out = {}
+    m = pyqg.BTModel(L=2. * np.pi, nx=64, tmax=5, beta=0., H=1., rek=0., rd=None,
+                     dt=0.0025, twrite=1000, log_level=0)
+    p = gaussian_ic(m)
+    ph = m.fft(p[np.newaxis, ...])
+    KEaux = m.spec_var(m.filtr * m.wv * ph) / 2.
+    qi = m.ifft(-m.wv2 * (ph / np.sqrt(KEaux)))
+    m.q = qi
+    out["bt_q0"] = qi
+    out["bt_a"] = np.array(m.a).real.copy()
+    out["bt_filtr"] = np.array(m.filtr).copy()
+    for _ in range(100):
+        m._step_forward()
+    out["bt_s100_q"] = np.array(m.q).copy()
+    out["bt_s100_t"] = np.float64(m.t)
+
+    m = pyqg.SQGModel(L=2. * np.pi, nx=64, tmax=5., beta=0., H=1., dt=0.0025,
+                      twrite=1000, log_level=0)
+    p = gaussian_ic(m)
+    ph = m.fft(p[np.newaxis, :, :])
+    KEaux = m.spec_var(m.filtr * m.wv * ph) / 2.
+    qi = m.ifft(m.wv * (ph / np.sqrt(KEaux)))
+    m.q = qi
+    out["sqg_q0"] = qi
+    out["sqg_a"] = np.array(m.a).real.copy()
+    for _ in range(100):
+        m._step_forward()
+    out["sqg_s100_q"] = np.array(m.q).copy()
+    np.savez(os.path.join(HERE, "bt_sqg_nx64.npz"), **out)
+
+
+def q_param(m):
+    """Deterministic PV parameterization used by the golden parameterized run."""
+    return -2e-7 * m.q + 1e-13 * np.cos(2 * np.pi * m.x / m.L)[np.newaxis]
+
+
+def uv_param(m):
+    """Deterministic velocity parameterization (nonzero curl) for the golden run."""
+    return 1e-6 * m.v, -1e-6 * m.u + 1e-9 * np.sin(2 * np.pi * m.y / m.W)[np.newaxis]
+
+
+def qg_param(pyqg):
+    """QGModel nx=32 with both q and uv parameterizations: tendency pieces after 3 steps."""
+    out = {}
+    m = pyqg.QGModel(nx=32, dt=7200.0, log_level=0, q_parameterization=q_param,
+                     uv_parameterization=uv_param, **QG_PHYS)
+    q0 = 1e-7 * np.random.default_rng(1).standard_normal((2, 32, 32))
+    m.q = q0
+    out["q0"] = q0
+    for _ in range(3):
+        m._step_forward()
+    snap(m, ["q", "qh", "dqhdt", "dqh", "duh", "dvh"], "s3_", out)
+    np.savez(os.path.join(HERE, "qg_param_nx32.npz"), **out)
+
+
+def long_runs(pyqg):
+    """Full-length statistics of the reference's regression runs
+    (pyqg/tests/test_reference_solns.py:9-175), as the numpy-fallback build computes
+    them, to pin the oracle tighter than the published rtol of 1e-2..1e-5."""
+    out = {}
+    year = 360 * 86400.
+    m = pyqg.QGModel(nx=32, L=1e6, beta=1.5e-11, rek=5.787e-7, rd=30000.0, delta=0.25,
+                     U1=0.05, U2=0.0, filterfac=18.4, dt=12800., tmax=3 * year,
+                     tavestart=10 * year, taveint=12800., useAB2=True, log_level=0)
+    m.set_q1q2((1e-6 * np.cos(2 * 5 * np.pi * m.x / m.L)
+                + 1e-7 * np.cos(2 * 5 * np.pi * m.y / m.W)), np.zeros_like(m.x))
+    m.run()
+    out["two_layer_t"] = np.float64(m.t)
+    out["two_layer_q1norm"] = np.float64((np.array(m.q)[0] ** 2).sum())
+    for name, cls, kw, sign in (
+            ("bt", pyqg.BTModel, dict(rek=0., rd=None), -1),
+            ("sqg", pyqg.SQGModel, dict(), 0)):
+        m = cls(L=2. * np.pi, nx=64, tmax=5, beta=0., H=1., dt=0.0025, twrite=1000,
+                log_level=0, **kw)
+        ph = m.fft(gaussian_ic(m)[np.newaxis])
+        KEaux = m.spec_var(m.filtr * m.wv * ph) / 2.
+        ph = ph / np.sqrt(KEaux)
+        m.q = m.ifft(-m.wv2 * ph if sign < 0 else m.wv * ph)
+        m.run()
+        out[f"{name}_t"] = np.float64(m.t)
+        out[f"{name}_qnorm"] = np.float64((np.array(m.q) ** 2).sum())
+        out[f"{name}_pnorm"] = np.float64((m.ifft(np.array(m.ph)) ** 2).sum())
+        out[f"{name}_ke"] = np.float64(m._calc_ke())
+    np.savez(os.path.join(HERE, "long_runs.npz"), **out)
+
+
+if __name__ == "__main__":
+    ref = import_reference()
+    for fn in (qg_startup, qg_config1, layered_nz3, layered_nz10, bt_sqg, qg_param,
+               long_runs):
+        fn(ref)
+        print("wrote", fn.__name__)

@@ -1,0 +1,159 @@
+"""GPU trajectories against the CPU oracle on identical seeded inputs.
+
+The bar: relative L2 of q <= 1e-10 after 100 steps (before chaotic
+divergence), per-phase fields <= 1e-12, at sizes the oracle finishes in
+seconds; the nx=512, dt=1800 two-layer run is the single-GPU parity workload.
+"""
+import numpy as np
+import pytest
+
+from conftest import QG_PHYS, rel_l2
+from oracle.qg_oracle import OracleLayered, OracleQG
+
+pytestmark = pytest.mark.gpu
+
+TOL_TRAJ = 1e-10
+
+
+@pytest.fixture(scope="module")
+def pq(gpu):
+    import pyqg_amd
+    return pyqg_amd
+
+
+def seeded_q(nz, n, seed):
+    return 1e-7 * np.random.default_rng(seed).standard_normal((nz, n, n))
+
+
+def test_phases_match_oracle_on_turbulent_state(pq):
+    """Each reference method on the spun-up LayeredModel fixture state."""
+    from conftest import golden
+    g = golden("layered_nz3_nx64.npz")
+    kw = dict(nz=3, U=[0.05, 0.025, 0.0], V=[0.0, 0.0, 0.0], H=[500, 1750, 1750],
+              rho=[1025.0, 1025.3, 1025.6], L=1e6, rek=5.787e-7, g=9.81, dt=3600.0,
+              beta=1.5e-11, rd=15000.0, f=0.0001236812857687059)
+    m = pq.LayeredModel(nx=64, log_level=0, **kw)
+    o = OracleLayered(nx=64, **kw)
+    o.a = np.ascontiguousarray(m.a.real)   # same constants on both sides
+    m.q = g["q0"]
+    o.q = g["q0"]
+    for step in range(3):
+        for phase, fields in (("_invert", ["ph", "uh", "vh", "u", "v"]),
+                              ("_do_advection", ["uq", "vq", "uqh", "vqh", "dqhdt"]),
+                              ("_do_friction", ["dqhdt"]),
+                              ("_forward_timestep", ["qh", "q", "dqhdt_p", "dqhdt_pp"])):
+            getattr(m, phase)()
+            getattr(o, phase)()
+            for f in fields:
+                ref = getattr(o, f)
+                if np.abs(ref).max() == 0:
+                    continue
+                assert rel_l2(getattr(m, f), ref) < 1e-12, (step, phase, f)
+
+
+@pytest.mark.parametrize("path", ["methods", "fused"])
+def test_qg_nx64_100_steps(pq, path):
+    m = pq.QGModel(nx=64, dt=7200.0, log_level=0, **QG_PHYS)
+    o = OracleQG(nx=64, dt=7200.0, **QG_PHYS)
+    q0 = seeded_q(2, 64, 0)
+    m.q = q0
+    o.q = q0
+    for _ in range(100):
+        o._step_forward()
+    if path == "methods":
+        for _ in range(100):
+            m._step_forward()
+    else:
+        m._fused_steps(100)
+    assert m.t == o.t and m.tc == 100
+    assert rel_l2(m.q, o.q) < TOL_TRAJ
+    assert rel_l2(m.dqhdt_p, o.dqhdt_p) < TOL_TRAJ
+    assert rel_l2(m.ph, o.ph) < TOL_TRAJ
+
+
+def test_qg_nx512_dt1800_config(pq):
+    """Two-layer nx=512, dt=1800, seeded 1e-7 Gaussian q: 1e-10 at step 100."""
+    print("nx=512 fused 100 steps", flush=True)
+    m = pq.QGModel(nx=512, dt=1800.0, log_level=0, **QG_PHYS)
+    o = OracleQG(nx=512, dt=1800.0, **QG_PHYS)
+    q0 = seeded_q(2, 512, 0)
+    m.q = q0
+    o.q = q0
+    m._fused_steps(100)
+    for _ in range(100):
+        o._step_forward()
+    err = rel_l2(m.q, o.q)
+    print(f"nx=512 step-100 rel L2 = {err:.3e}", flush=True)
+    assert err < TOL_TRAJ
+
+
+def test_uneven_qg_ensemble_matches_oracle(pq):
+    """Members of nx in {64,128,256} with their own beta, rd, delta, rek."""
+    rng = np.random.default_rng(1)
+    n = 6
+    nx = np.array([64, 128, 256, 64, 128, 256])
+    beta = rng.uniform(0.5e-11, 2.5e-11, n)
+    rd = rng.uniform(1e4, 2.5e4, n)
+    delta = rng.uniform(0.1, 0.5, n)
+    rek = rng.uniform(3e-7, 1e-6, n)
+    ens = pq.Ensemble.qg(nx, beta=beta, rd=rd, delta=delta, rek=rek, dt=3600.)
+    q0 = [seeded_q(2, int(nx[i]), 100 + i) for i in range(n)]
+    ens.set_q(q0)
+    nsteps = 30
+    ens.step(nsteps)
+    for i in range(n):
+        o = OracleQG(nx=int(nx[i]), dt=3600., beta=beta[i], rd=rd[i], delta=delta[i],
+                     rek=rek[i], L=1e6, H1=500., U1=0.025, U2=0.0, filterfac=23.6)
+        o.q = q0[i]
+        for _ in range(nsteps):
+            o._step_forward()
+        assert rel_l2(ens.get("q", i), o.q) < TOL_TRAJ, i
+        assert rel_l2(ens.get("qh", i), o.qh) < TOL_TRAJ, i
+    assert ens.tc == nsteps
+
+
+def test_ensemble_chunked_steps_equal_single_call(pq):
+    ens1 = pq.Ensemble.qg(np.full(4, 64), dt=3600.)
+    ens2 = pq.Ensemble.qg(np.full(4, 64), dt=3600.)
+    q0 = [seeded_q(2, 64, 7 + i) for i in range(4)]
+    ens1.set_q(q0)
+    ens2.set_q(q0)
+    ens1.step(10)
+    for _ in range(5):
+        ens2.step(2)
+    for i in range(4):
+        assert np.array_equal(ens1.get("qh", i), ens2.get("qh", i))
+
+
+def test_layered_nz10_ensemble_shared_inversion(pq):
+    H, rho, U = pq.constants.exponential_stratification()
+    nx, n, nsteps = 64, 3, 20
+    ens = pq.Ensemble.layered(n, nx, 10, H, rho, U, L=2e6, dt=3600.)
+    q0 = [seeded_q(10, nx, 30 + i) for i in range(n)]
+    ens.set_q(q0)
+    ens.step(nsteps)
+    o = OracleLayered(nz=10, nx=nx, H=H, rho=rho, U=U, V=np.zeros(10), L=2e6, beta=1.5e-11,
+                      rek=5.787e-7, dt=3600.0, f=0.0001236812857687059, g=9.81)
+    np.testing.assert_allclose(ens.a, o.a.real, rtol=1e-9, atol=1e-9 * np.abs(o.a).max())
+    for i in range(n):
+        o2 = OracleLayered(nz=10, nx=nx, H=H, rho=rho, U=U, V=np.zeros(10), L=2e6,
+                           beta=1.5e-11, rek=5.787e-7, dt=3600.0, f=0.0001236812857687059,
+                           g=9.81)
+        o2.a = ens.a
+        o2.q = q0[i]
+        for _ in range(nsteps):
+            o2._step_forward()
+        assert rel_l2(ens.get("q", i), o2.q) < TOL_TRAJ, i
+
+
+def test_full_size_invariants_headline_ensemble(pq):
+    """256 members of nx=256 (the measured workload): members with identical
+    inputs stay bit-identical, and the state stays finite and bounded."""
+    ens = pq.Ensemble.qg(np.full(256, 256), dt=3600.)
+    q0 = [seeded_q(2, 256, i % 8) for i in range(256)]
+    ens.set_q(q0)
+    ens.step(20)
+    qh = ens.get("qh")
+    for i in range(8, 256, 37):
+        assert np.array_equal(qh[i], qh[i % 8])
+    assert all(np.isfinite(x).all() for x in qh)

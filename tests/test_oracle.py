@@ -1,0 +1,323 @@
+"""Pins the CPU oracle (oracle/qg_oracle.py) before it is trusted as the checker.
+
+* golden vectors written from the reference build (tests/golden/make_golden.py);
+* the regression values the reference's own tests hold
+  (pyqg/tests/test_reference_solns.py);
+* closed forms restated from pyqg/tests/test_model.py, test_advection.py,
+  test_fft2.py, test_layered.py;
+* a brute-force DFT for the transforms.
+"""
+import numpy as np
+import pytest
+
+from conftest import QG_PHYS, golden, rel_l2
+from oracle.qg_oracle import (OracleBT, OracleLayered, OracleQG, OracleSQG, ab_coefficients)
+
+STATE = ["q", "qh", "ph", "u", "v", "uq", "vq", "uqh", "vqh", "dqhdt", "dqhdt_p", "dqhdt_pp"]
+
+
+def close(a, b, rtol=1e-13):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = max(np.abs(b).max(), 1e-300)
+    assert np.abs(a - b).max() <= rtol * scale, (np.abs(a - b).max(), scale)
+
+
+# ----------------------------------------------------------------- golden
+def test_qg_startup_all_fields_match_reference():
+    g = golden("qg_nx32_startup.npz")
+    m = OracleQG(nx=32, dt=7200.0, **QG_PHYS)
+    for f in ["kk", "ll", "filtr", "Qy", "Ubg", "_k2l2"]:
+        close(getattr(m, f), g["const_" + f], 0)
+    close(m.a.real, g["const_a"], 0)
+    m.q = g["q0"]
+    close(m.qh, g["qh0"], 0)
+    for step in range(1, 4):
+        m._step_forward()
+        if step in (1, 3):
+            for f in STATE:
+                close(getattr(m, f), g[f"s{step}_{f}"], 1e-14)
+            assert m.t == g[f"s{step}_t"] and m.tc == g[f"s{step}_tc"]
+            assert m.ablevel == g[f"s{step}_ablevel"]
+
+
+def test_qg_config1_trajectory():
+    g = golden("qg_nx64_cfg1.npz")
+    m = OracleQG(nx=64, dt=7200.0, **QG_PHYS)
+    m.q = g["q0"]
+    for step in range(1, 1001):
+        m._step_forward()
+        if step in (100, 1000):
+            assert rel_l2(m.q, g[f"s{step}_q"]) < 1e-13
+            assert m.t == g[f"s{step}_t"]
+
+
+LAYERED3 = dict(nz=3, U=[0.05, 0.025, 0.0], V=[0.0, 0.0, 0.0], H=[500, 1750, 1750],
+                rho=[1025.0, 1025.3, 1025.6], L=1e6, rek=5.787e-7, g=9.81, dt=3600.0,
+                beta=1.5e-11, rd=15000.0, f=0.0001236812857687059)
+
+
+def test_layered_nz3_from_spun_up_fixture():
+    g = golden("layered_nz3_nx64.npz")
+    m = OracleLayered(nx=64, **LAYERED3)
+    close(m.S, g["const_S"], 1e-15)
+    close(m.Qy, g["const_Qy"], 1e-15)
+    for key in ("5_5", "3_7", "m2_1"):
+        i, j = [int(x.replace("m", "-")) for x in key.split("_")]
+        close(m.a[:, :, i, j].real, g["const_a_" + key], 1e-13)
+    m.q = g["q0"]
+    m._step_forward()
+    for f in ("q", "ph", "dqhdt"):
+        close(getattr(m, f), g["s1_" + f], 1e-12)
+    for _ in range(19):
+        m._step_forward()
+    assert rel_l2(m.q, g["s20_q"]) < 1e-12
+    assert rel_l2(m.ph, g["s20_ph"]) < 1e-12
+
+
+def test_layered_nz10():
+    g = golden("layered_nz10_nx32.npz")
+    m = OracleLayered(nz=10, nx=32, H=g["in_H"], rho=g["in_rho"], U=g["in_U"],
+                      V=np.zeros(10), L=2e6, beta=1.5e-11, rek=5.787e-7, dt=3600.0,
+                      f=0.0001236812857687059, g=9.81)
+    close(m.a[:, :, 5, 5].real, g["const_a_5_5"], 1e-12)
+    close(m.a[:, :, -2, 1].real, g["const_a_m2_1"], 1e-12)
+    m.q = g["q0"]
+    for _ in range(10):
+        m._step_forward()
+    assert rel_l2(m.q, g["s10_q"]) < 1e-11
+
+
+def test_bt_sqg_100_steps():
+    g = golden("bt_sqg_nx64.npz")
+    m = OracleBT(L=2. * np.pi, nx=64, beta=0., H=1., rek=0., rd=None, dt=0.0025)
+    close(m.a.real, g["bt_a"], 0)
+    close(m.filtr, g["bt_filtr"], 0)
+    m.q = g["bt_q0"]
+    for _ in range(100):
+        m._step_forward()
+    assert rel_l2(m.q, g["bt_s100_q"]) < 1e-13
+    assert m.t == g["bt_s100_t"]
+    m = OracleSQG(L=2. * np.pi, nx=64, beta=0., H=1., dt=0.0025)
+    close(m.a.real, g["sqg_a"], 0)
+    m.q = g["sqg_q0"]
+    for _ in range(100):
+        m._step_forward()
+    assert rel_l2(m.q, g["sqg_s100_q"]) < 1e-13
+
+
+def q_param(m):
+    return -2e-7 * m.q + 1e-13 * np.cos(2 * np.pi * m.x / m.L)[np.newaxis]
+
+
+def uv_param(m):
+    return 1e-6 * m.v, -1e-6 * m.u + 1e-9 * np.sin(2 * np.pi * m.y / m.W)[np.newaxis]
+
+
+def test_parameterized_steps():
+    g = golden("qg_param_nx32.npz")
+    m = OracleQG(nx=32, dt=7200.0, q_parameterization=q_param, uv_parameterization=uv_param,
+                 **QG_PHYS)
+    m.q = g["q0"]
+    for _ in range(3):
+        m._step_forward()
+    for f in ("q", "qh", "dqhdt", "dqh", "duh", "dvh"):
+        close(getattr(m, f), g["s3_" + f], 1e-13)
+
+
+# ----------------------------------------------------- long regression runs
+def _gaussian(m):
+    return (np.exp(-(2. * (m.x - 1.75 * np.pi / 2)) ** 2. - (2. * (m.y - np.pi)) ** 2)
+            + np.exp(-(2. * (m.x - 2.25 * np.pi / 2)) ** 2. - (2. * (m.y - np.pi)) ** 2))
+
+
+@pytest.mark.parametrize("kind", ["bt", "sqg"])
+def test_long_runs_match_reference_values(kind):
+    """test_reference_solns.py:97-175: 2000 steps; values held by the reference
+    (rtol 1e-5 / 1e-4) and the reference build's own values (tighter)."""
+    lr = golden("long_runs.npz")
+    if kind == "bt":
+        m = OracleBT(L=2. * np.pi, nx=64, tmax=5, beta=0., H=1., rek=0., rd=None, dt=0.0025)
+        held = dict(qnorm=89101.741238768518, pnorm=1493.217664248918, ke=0.9950360837282386)
+        rtol = 1e-5
+    else:
+        m = OracleSQG(L=2. * np.pi, nx=64, tmax=5., beta=0., H=1., dt=0.0025)
+        held = dict(qnorm=7847.5169609881032, pnorm=1346.8874163575524, ke=0.9579493831013508)
+        rtol = 1e-4
+    ph = m.fft(_gaussian(m)[np.newaxis])
+    KEaux = m.spec_var(m.filtr * m.wv * ph) / 2.
+    ph = ph / np.sqrt(KEaux)
+    m.q = m.ifft(-m.wv2 * ph if kind == "bt" else m.wv * ph)
+    m.run()
+    assert m.t == 5.000000000000082
+    got = dict(qnorm=(m.q ** 2).sum(), pnorm=(m.ifft(m.ph) ** 2).sum(), ke=m._calc_ke())
+    for k, v in held.items():
+        np.testing.assert_allclose(got[k], v, rtol=rtol)
+        np.testing.assert_allclose(got[k], lr[f"{kind}_{k}"], rtol=1e-11)
+
+
+def test_two_layer_regression():
+    """test_reference_solns.py:9-51 (q1 norm after 3 model years, rtol 1e-2)."""
+    lr = golden("long_runs.npz")
+    year = 360 * 86400.
+    m = OracleQG(nx=32, L=1e6, beta=1.5e-11, rek=5.787e-7, rd=30000.0, delta=0.25, U1=0.05,
+                 U2=0.0, filterfac=18.4, dt=12800., tmax=3 * year)
+    m.set_q1q2(1e-6 * np.cos(2 * 5 * np.pi * m.x / m.L) + 1e-7 * np.cos(2 * 5 * np.pi * m.y / m.W),
+               np.zeros_like(m.x))
+    m.run()
+    assert m.t == 93312000.0
+    q1norm = (m.q[0] ** 2).sum()
+    np.testing.assert_allclose(q1norm, 9.561430503712755e-08, rtol=1e-2)
+    np.testing.assert_allclose(q1norm, lr["two_layer_q1norm"], rtol=1e-9)
+
+
+# ------------------------------------------------------------ closed forms
+def _closed_form_model():
+    return OracleQG(beta=0., U1=0., U2=0., filterfac=0.)
+
+
+def test_fft_bin_amplitudes():
+    """test_model.py:16-65"""
+    m = _closed_form_model()
+    amp = (m.nx / 2) ** 2 * m.ny ** 2
+    for kw in range(1, m.nx // 8):
+        for lw in range(1, m.ny // 8):
+            k, l = 2 * np.pi * kw / m.L, 2 * np.pi * lw / m.W
+            m.set_q1q2(np.cos(k * m.x), np.sin(l * m.y))
+            a = np.real(m.qh * m.qh.conj())
+            np.testing.assert_allclose(a[0, 0, kw], amp, 1e-15)
+            np.testing.assert_allclose(a[1, lw, 0], amp, 1e-15)
+            np.testing.assert_allclose(a[1, -lw, 0], amp, 1e-15)
+            a[0, 0, kw] = a[1, lw, 0] = a[1, -lw, 0] = 0
+            assert np.abs(a).max() <= 1e-15 * amp ** 0 + 1e-15
+
+
+def test_inversion_barotropic():
+    """test_model.py:67-94"""
+    m = _closed_form_model()
+    for kw in range(1, m.nx // 8):
+        for lw in range(1, m.ny // 8):
+            k, l = 2 * np.pi * kw / m.L, 2 * np.pi * lw / m.W
+            q = np.cos(k * m.x) + np.sin(l * m.y)
+            m.set_q1q2(q, q)
+            m._invert()
+            for z in range(2):
+                np.testing.assert_allclose(m.u[z], l ** -1 * np.cos(l * m.y), 1e-13)
+                np.testing.assert_allclose(m.v[z], k ** -1 * np.sin(k * m.x), 1e-13)
+
+
+def test_advection_closed_form():
+    """test_model.py:108-158: energy only at 2k and 2l."""
+    m = _closed_form_model()
+    amp = (m.nx / 2) ** 2 * m.ny ** 2
+    for kw in range(1, m.nx // 8):
+        for lw in range(1, m.ny // 8):
+            k, l = 2 * np.pi * kw / m.L, 2 * np.pi * lw / m.W
+            m.set_q1q2(np.cos(k * m.x), np.sin(l * m.y))
+            m._u = np.stack([np.sin(k * m.x), np.zeros_like(m.x)])
+            m._v = np.stack([np.zeros_like(m.y), np.cos(l * m.y)])
+            m.Ubg = np.array([0., 0.])
+            m._do_advection()
+            t = np.real(m.dqhdt * m.dqhdt.conj())
+            np.testing.assert_allclose(t[0, 0, 2 * kw], k ** 2 * amp, 1e-14)
+            np.testing.assert_allclose(t[1, 2 * lw, 0], l ** 2 * amp, 1e-14)
+            np.testing.assert_allclose(t[1, -2 * lw, 0], l ** 2 * amp, 1e-14)
+            t[0, 0, 2 * kw] = t[1, 2 * lw, 0] = t[1, -2 * lw, 0] = 0
+            assert np.abs(t).max() <= 1e-14
+
+
+def test_friction_closed_form():
+    """test_model.py:160-183"""
+    m = _closed_form_model()
+    m.set_q1q2(np.cos(2 * np.pi * 3 / m.L * m.x), np.sin(2 * np.pi * 2 / m.W * m.y))
+    m._invert()
+    m._dqhdt[:] = 0.
+    m._do_friction()
+    np.testing.assert_allclose(m.dqhdt[-1], m.rek * m.wv2 * m.ph[-1], 1e-15)
+    assert np.all(m.dqhdt[:-1] == 0)
+
+
+def test_timestepping_startup():
+    """test_model.py:314-340: AB1, AB2, AB3 with a constant tendency."""
+    m = _closed_form_model()
+    m.set_q1q2(np.zeros_like(m.x), np.zeros_like(m.x))
+    rng = np.random.default_rng(5)
+    d = rng.random(m.dqhdt.shape) + 1j * rng.random(m.dqhdt.shape)
+    m._dqhdt[:] = d
+    for n in (1, 2, 3):
+        m._forward_timestep()
+        np.testing.assert_allclose(m.qh, n * m.dt * d)
+
+
+def test_plane_wave_jacobian_vanishes():
+    """test_advection.py:4-52 (plane-wave Jacobian, nx=256)."""
+    m = OracleQG(L=2. * np.pi, nx=256, U1=0)
+    for kx, ky in [(1., 2.), (5., 0.), (10., 10.), (0., 21.), (24., 12.), (2., 49.)]:
+        m.set_q1q2(np.cos(kx * m.x + ky * m.y), np.zeros_like(m.x))
+        m._invert()
+        uq, vq = m.u * m.q, m.v * m.q
+        jac = m.ifft(1j * m.k * m.fft(uq) + 1j * m.l * m.fft(vq))
+        assert np.abs(jac).sum() * m.dx * m.dy / m.L ** 2 < 1e-13
+
+
+def test_ab_coefficients():
+    assert ab_coefficients(0, 2.0) == (2.0, 0.0, 0.0)
+    assert ab_coefficients(1, 2.0) == (3.0, -1.0, 0.0)
+    assert ab_coefficients(5, 12.0) == (23.0, -16.0, 5.0)
+
+
+# ------------------------------------------------------- transforms pinned
+def test_transforms_against_brute_force_dft():
+    rng = np.random.default_rng(0)
+    nz, n = 2, 8
+    x = rng.standard_normal((nz, n, n))
+    j = np.arange(n)
+    W = np.exp(-2j * np.pi * np.outer(j, j) / n)
+    full = np.einsum("ab,zbc,cd->zad", W, x, W)
+    m = OracleQG(nx=n, **QG_PHYS)
+    close(m.fft(x), full[:, :, : n // 2 + 1], 1e-14)
+    # c2r discards the imaginary parts of the kx = 0 and kx = n/2 columns
+    X = rng.standard_normal((nz, n, n // 2 + 1)) + 1j * rng.standard_normal((nz, n, n // 2 + 1))
+    Y = np.fft.ifft(X, axis=-2)
+    Y[..., 0] = Y[..., 0].real
+    Y[..., -1] = Y[..., -1].real
+    ext = np.concatenate([Y, np.conj(Y[..., 1:-1][..., ::-1])], axis=-1)
+    brute = np.einsum("zak,kb->zab", ext, np.conj(W)).real / n
+    close(m.ifft(X), brute, 1e-14)
+
+
+def test_parseval():
+    """test_fft2.py:8-34"""
+    m = OracleQG(L=2. * np.pi, beta=0., U1=0., U2=0., filterfac=0.)
+    rng = np.random.default_rng(1)
+    p = rng.standard_normal((m.nz, m.ny, m.nx))
+    p -= p.mean(axis=(-2, -1))[:, np.newaxis, np.newaxis]
+    ph = m.fft(p)
+    var_spec = 2. * np.abs(ph) ** 2 / m.M ** 2
+    var_spec[..., 0] /= 2
+    var_spec[..., -1] /= 2
+    assert np.abs((p.var(axis=(-2, -1)) - var_spec.sum(axis=(-2, -1)))
+                  / p.var(axis=(-2, -1))).sum() < 1e-14
+
+
+def test_layered_inversion_is_matrix_inverse():
+    """a(k,l) (S - kappa^2 I) = I away from kappa = 0; the 3x3 cofactor check of
+    test_layered.py:60-84."""
+    m = OracleLayered(nx=16, **{k: v for k, v in LAYERED3.items() if k != "nz"}, nz=3)
+    for (j, i) in [(5, 5), (1, 0), (3, 7), (-2, 1)]:
+        M = m.S - np.eye(3) * m.wv2[j, i]
+        np.testing.assert_allclose(m.a[:, :, j, i].real @ M, np.eye(3), atol=1e-12)
+    M = m.S - np.eye(3) * m.wv2[5, 5]
+    cof = np.array([[np.linalg.det(np.delete(np.delete(M, r, 0), c, 1)) * (-1) ** (r + c)
+                     for c in range(3)] for r in range(3)])
+    np.testing.assert_allclose(m.a[:, :, 5, 5].real, cof.T / np.linalg.det(M), rtol=1e-12)
+    assert np.all(m.a[:, :, 0, 0] == 0)
+
+
+def test_planted_mistake_is_caught():
+    """A wrong sign in the friction term must break the golden comparison."""
+    g = golden("qg_nx32_startup.npz")
+    m = OracleQG(nx=32, dt=7200.0, **QG_PHYS)
+    m.q = g["q0"]
+    m.rek = -m.rek
+    m._step_forward()
+    assert np.abs(m.dqhdt - g["s1_dqhdt"]).max() > 1e-10 * np.abs(g["s1_dqhdt"]).max()
