@@ -45,7 +45,7 @@ def _cpu_worker(args):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(nsteps=60):
+def cpu_baseline(nsteps=500):
     """The oracle (NumPy FP64, numpy.fft = the reference's no-pyfftw path), one
     member per process, one thread per process, on the host cores."""
     import concurrent.futures as cf
@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--members", type=int, default=MEMBERS)
+    ap.add_argument("--copy-probe", type=int, default=1 << 30,
+                    help="bytes moved by the copy kernel that calibrates HBM counters (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the ensemble runs on a single MI355X here "
@@ -96,7 +98,9 @@ def main():
     M = args.members
     ens = pyqg_amd.Ensemble.qg(np.full(M, NX), dt=DT, **{k: v for k, v in PHYS.items()})
     ens.set_q([1e-7 * np.random.default_rng(s).standard_normal((NZ, NX, NX)) for s in range(M)])
-    ens.step(max(args.warmup, 3))        # AB start-up + warm-up, untimed
+    ens.step(3)                           # AB start-up, untimed
+    tune = ens.autotune(reps=5, cache_path=False)
+    ens.step(max(args.warmup - 3, 1))     # warm-up with the tuned kernels, untimed
     b = ens.buckets[0]
     b.sync()
 
@@ -106,6 +110,12 @@ def main():
     wall = time.perf_counter() - t0
     ms_row, n_row, ms_col, n_col = b.timing()
     b.timing_enable(False)
+
+    copy = None
+    if args.copy_probe:
+        ms = pyqg_amd._lib.copy_probe(args.copy_probe)
+        copy = {"bytes": args.copy_probe, "ms": ms,
+                "GB/s": 2 * args.copy_probe / (ms * 1e-3) / 1e9}
 
     qh = ens.get("qh", 0)
     if not np.isfinite(qh).all():
@@ -138,7 +148,10 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "fused step = k_row_flux + k_col_step",
                      "algorithmic_bytes_per_launch": bytes_step,
-                     "row_kernel_ms": row_ms, "col_kernel_ms": col_ms},
+                     "row_kernel_ms": row_ms, "col_kernel_ms": col_ms,
+                     "copy_kernel": copy,
+                     "variant": {"row_radix": b.variant()[0], "col_radix": b.variant()[1],
+                                 "autotune": list(tune.values())[0]}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

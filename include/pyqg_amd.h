@@ -123,6 +123,49 @@ int qgk_timing_enable(qgk_bucket* b, int enable);
 int qgk_timing(qgk_bucket* b, double* ms_row, long* n_row, double* ms_col, long* n_col);
 int qgk_sync(qgk_bucket* b);
 
+/* Device-to-device field transfer (dptr is device memory of the full bucket
+ * size), for pipelines that keep data on the GPU. */
+int qgk_put_device(qgk_bucket* b, int field, const void* dptr);
+int qgk_get_device(qgk_bucket* b, int field, void* dptr);
+
+/* Learned-closure support; no counterpart in the reference (pyqg removed its
+ * subgrid_forcing, pyqg/model.py:397-402).
+ * qgk_coarsen: spectral coarse-graining of a field of `hi` into a field of `lo`
+ *   (same members and layers, square grids): the modes both grids resolve,
+ *   scaled by (n_lo/n_hi)^2 and multiplied by filter_lo[nl_lo][nk_lo] (NULL:
+ *   sharp truncation); the coarse Nyquist row and column are zero.
+ * qgk_subgrid_forcing: for the current state of `hi` (full state): the coarse
+ *   state C(qh) into lo's qh (and q, u, v after lo's inversion), and the
+ *   subgrid PV forcing S = C(T_hi(q)) - T_lo(C(q)) into lo's dqh (spectral) and
+ *   dq (real), T = the model tendency of _invert + _do_advection.
+ * qgk_pack_inputs: lo's q, u, v (real) as an NHWC BF16 network input with
+ *   channels f*nz + z (f = q, u, v) times inv_scale[3*nz], zero-padded to cinp.
+ * qgk_add_forcing_device: dq = net_out[..., z] * scale[z] (network output FP32
+ *   NHWC with coutp channels), then dqhdt += rfft2(dq), as
+ *   _do_q_subgrid_parameterization does with a host array (kernel.pyx:424-442). */
+int qgk_coarsen(qgk_bucket* hi, qgk_bucket* lo, int field_hi, int field_lo,
+                const double* filter_lo);
+int qgk_subgrid_forcing(qgk_bucket* hi, qgk_bucket* lo, const double* filter_lo);
+int qgk_pack_inputs(qgk_bucket* b, const double* inv_scale, int cinp, void* dst);
+int qgk_add_forcing_device(qgk_bucket* b, const float* net_out, int coutp, const double* scale);
+
+/* Register radix (4, 8 or 16 complex values per thread) of the fused row and
+ * column kernels; it fixes the Stockham pass structure and the tile shapes. */
+int qgk_set_variant(qgk_bucket* b, int erow, int ecol);
+int qgk_get_variant(qgk_bucket* b, int* erow, int* ecol);
+/* Autotuner: times every radix for the row and the column kernel on this
+ * bucket (reps launches each; the model state is saved and restored), selects
+ * the fastest, and reports the per-candidate milliseconds (radix 4, 8, 16; -1
+ * where a radix exceeds the length).  Not in the reference (no counterpart). */
+int qgk_autotune(qgk_bucket* b, int reps, int* erow, int* ecol, double* ms_row3,
+                 double* ms_col3);
+/* "<name>:<gcnArch>:<CUs>" of the current device, the autotuner cache key. */
+int qgk_device_name(char* buf, int len);
+
+/* Measurement aid: average time of a plain device-to-device copy kernel
+ * (nbytes read + nbytes written) over iters launches. */
+int qgk_copy_probe(size_t nbytes, int iters, double* ms_per_copy);
+
 /* Multi-bucket stepping for uneven ensembles: nsteps fused steps of every
  * bucket, all launched on the first bucket's stream (one graph when
  * use_graph != 0). */

@@ -28,7 +28,10 @@ EXPORTS = ["qgk_create", "qgk_destroy", "qgk_last_error", "qgk_set_const", "qgk_
            "qgk_set_inversion_2layer", "qgk_set_rek", "qgk_put", "qgk_get", "qgk_invert",
            "qgk_advect", "qgk_friction", "qgk_add_q_param", "qgk_add_uv_param",
            "qgk_timestep", "qgk_q_to_qh", "qgk_qh_to_q", "qgk_fft", "qgk_ifft", "qgk_step",
-           "qgk_timing_enable", "qgk_timing", "qgk_sync", "qgk_step_buckets"]
+           "qgk_timing_enable", "qgk_timing", "qgk_sync", "qgk_step_buckets",
+           "qgk_copy_probe", "qgk_set_variant", "qgk_get_variant", "qgk_autotune",
+           "qgk_device_name", "qgk_put_device", "qgk_get_device", "qgk_coarsen",
+           "qgk_subgrid_forcing", "qgk_pack_inputs", "qgk_add_forcing_device"]
 
 
 class QGKError(RuntimeError):
@@ -68,6 +71,17 @@ def _load():
         "qgk_timing": (I, [P, DP, ctypes.POINTER(L), DP, ctypes.POINTER(L)]),
         "qgk_sync": (I, [P]),
         "qgk_step_buckets": (I, [ctypes.POINTER(P), I, I, P, I]),
+        "qgk_copy_probe": (I, [ctypes.c_size_t, I, DP]),
+        "qgk_set_variant": (I, [P, I, I]),
+        "qgk_get_variant": (I, [P, ctypes.POINTER(I), ctypes.POINTER(I)]),
+        "qgk_autotune": (I, [P, I, ctypes.POINTER(I), ctypes.POINTER(I), DP, DP]),
+        "qgk_device_name": (I, [ctypes.c_char_p, I]),
+        "qgk_put_device": (I, [P, I, P]),
+        "qgk_get_device": (I, [P, I, P]),
+        "qgk_coarsen": (I, [P, P, I, I, P]),
+        "qgk_subgrid_forcing": (I, [P, P, P]),
+        "qgk_pack_inputs": (I, [P, P, I, P]),
+        "qgk_add_forcing_device": (I, [P, P, I, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -168,6 +182,33 @@ class Bucket:
 
     def sync(self):
         check(lib.qgk_sync(self.h))
+
+    def set_variant(self, erow, ecol):
+        check(lib.qgk_set_variant(self.h, int(erow), int(ecol)))
+
+    def variant(self):
+        r, c = ctypes.c_int(), ctypes.c_int()
+        check(lib.qgk_get_variant(self.h, ctypes.byref(r), ctypes.byref(c)))
+        return r.value, c.value
+
+    def autotune(self, reps=5):
+        r, c = ctypes.c_int(), ctypes.c_int()
+        mr, mc = (ctypes.c_double * 3)(), (ctypes.c_double * 3)()
+        check(lib.qgk_autotune(self.h, int(reps), ctypes.byref(r), ctypes.byref(c), mr, mc))
+        return (r.value, c.value), list(mr), list(mc)
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    check(lib.qgk_device_name(buf, 256))
+    return buf.value.decode()
+
+
+def copy_probe(nbytes, iters=20):
+    """Milliseconds per launch of a plain copy kernel moving nbytes in and out."""
+    ms = ctypes.c_double()
+    check(lib.qgk_copy_probe(int(nbytes), int(iters), ctypes.byref(ms)))
+    return ms.value
 
 
 def step_buckets(buckets, coeffs, use_graph=False):

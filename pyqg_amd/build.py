@@ -14,6 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_qgk.so")
+# (source, shared library) pairs, built in parallel
+TARGETS = [("qgk.hip", "_qgk.so"), ("cnn.hip", "_qgcnn.so")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -35,17 +37,27 @@ def hipcc():
 
 
 def build(verbose=False, extra=()):
-    src = os.path.join(CSRC, "qgk.hip")
-    tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), *extra, src, "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {res.returncode}")
-    os.replace(tmp, OUT)
-    return OUT
+    procs = []
+    for src, lib in TARGETS:
+        out = os.path.join(HERE, lib)
+        tmp = out + ".tmp"
+        cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), *extra,
+               os.path.join(CSRC, src), "-o", tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                       text=True), tmp, out))
+    failed = False
+    for p, tmp, out in procs:
+        log, _ = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(log)
+            failed = True
+        else:
+            os.replace(tmp, out)
+    if failed:
+        raise RuntimeError("hipcc failed")
+    return [out for _, _, out in procs]
 
 
 if __name__ == "__main__":

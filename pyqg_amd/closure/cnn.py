@@ -1,0 +1,227 @@
+"""Fully convolutional subgrid-forcing network on libqgcnn's kernels.
+
+Every forward, backward and optimizer operation is a libqgcnn kernel (BF16
+operands, FP32 accumulation on the matrix cores); torch tensors are only the
+device-memory containers and provide the stream.  Activations are NHWC BF16
+with channels padded to multiples of 16; weights [cout][25][cin].
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+import torch
+
+_HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_HERE, "_qgcnn.so")
+
+DEFAULT_CHANNELS = (6, 128, 64, 32, 32, 32, 32, 32, 2)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libqgcnn is not built ({LIB_PATH} missing): run "
+                          "`python pyqg_amd/build.py`")
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I, F, Z = ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_size_t
+    sig = {
+        "qgcnn_last_error": (ctypes.c_char_p, []),
+        "qgcnn_conv_fwd": (I, [P, I, I, I, I, P, P, I, I, P, P, P, P]),
+        "qgcnn_conv_wgrad": (I, [P, P, I, I, I, I, I, P, P]),
+        "qgcnn_flip_transpose": (I, [P, P, I, I, P]),
+        "qgcnn_bias_grad": (I, [P, Z, I, P, P]),
+        "qgcnn_mse": (I, [P, P, Z, I, I, F, P, P, P]),
+        "qgcnn_f32_to_bf16": (I, [P, P, Z, P]),
+        "qgcnn_bf16_to_f32": (I, [P, P, Z, P]),
+        "qgcnn_adam": (I, [P, P, P, P, P, Z, F, F, F, F, I, F, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+EXPORTS = ["qgcnn_last_error", "qgcnn_conv_fwd", "qgcnn_conv_wgrad", "qgcnn_flip_transpose",
+           "qgcnn_bias_grad", "qgcnn_mse", "qgcnn_f32_to_bf16", "qgcnn_bf16_to_f32",
+           "qgcnn_adam"]
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib.qgcnn_last_error().decode(errors="replace")
+        raise (ValueError if rc == -1 else RuntimeError)(msg)
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def pad16(c):
+    return ((c + 15) // 16) * 16
+
+
+# ---------------------------------------------------------------- primitives
+def conv_fwd(x, w, bias, coutp, relu=True, mask=None, out_bf16=True, out_f32=False):
+    """x: (B, H, 64, cinp) bf16; w: (coutp, 25, cinp) bf16; returns (y_bf16, y_f32)."""
+    B, H, W, cinp = x.shape
+    y = torch.empty((B, H, W, coutp), dtype=torch.bfloat16, device=x.device) if out_bf16 else None
+    y32 = torch.empty((B, H, W, coutp), dtype=torch.float32, device=x.device) if out_f32 else None
+    check(lib.qgcnn_conv_fwd(_p(x), B, H, W, cinp, _p(w), _p(bias), coutp, int(relu), _p(mask),
+                             _p(y), _p(y32), _stream()))
+    return y, y32
+
+
+def conv_wgrad(dy, x, dw):
+    B, H, W, coutp = dy.shape
+    cinp = x.shape[-1]
+    check(lib.qgcnn_conv_wgrad(_p(dy), _p(x), B, H, W, cinp, coutp, _p(dw), _stream()))
+
+
+def flip_transpose(w, wt):
+    coutp, _, cinp = w.shape
+    check(lib.qgcnn_flip_transpose(_p(w), _p(wt), cinp, coutp, _stream()))
+
+
+def bias_grad(dy, db):
+    coutp = dy.shape[-1]
+    check(lib.qgcnn_bias_grad(_p(dy), dy.numel() // coutp, coutp, _p(db), _stream()))
+
+
+def mse(pred, target, grad=None, gscale=1.0):
+    """pred (B,H,W,coutp) f32, target (B,H,W,nch) f32 -> loss tensor (1,) f32."""
+    loss = torch.zeros(1, dtype=torch.float32, device=pred.device)
+    npix = pred.numel() // pred.shape[-1]
+    check(lib.qgcnn_mse(_p(pred), _p(target), npix, pred.shape[-1], target.shape[-1],
+                        float(gscale), _p(grad), _p(loss), _stream()))
+    return loss
+
+
+# ---------------------------------------------------------------- network
+class CNN:
+    """5x5 periodic convolutions with ReLU between them; channels[0] inputs,
+    channels[-1] outputs (128/64/32 hidden channels by default)."""
+
+    def __init__(self, channels=DEFAULT_CHANNELS, seed=0, device="cuda"):
+        self.channels = tuple(int(c) for c in channels)
+        self.chp = [pad16(c) for c in self.channels]
+        self.device = device
+        g = torch.Generator().manual_seed(seed)
+        self.w32, self.b32, self.wb, self.wt = [], [], [], []
+        for l in range(len(self.channels) - 1):
+            cin, cout = self.channels[l], self.channels[l + 1]
+            cinp, coutp = self.chp[l], self.chp[l + 1]
+            bound = 1.0 / math.sqrt(cin * 25)          # PyTorch's conv default init
+            w = torch.zeros((coutp, 25, cinp))
+            w[:cout, :, :cin] = (torch.rand((cout, 25, cin), generator=g) * 2 - 1) * bound
+            b = torch.zeros(coutp)
+            b[:cout] = (torch.rand(cout, generator=g) * 2 - 1) * bound
+            self.w32.append(w.to(device))
+            self.b32.append(b.to(device))
+            self.wb.append(self.w32[-1].to(torch.bfloat16))
+            self.wt.append(torch.empty((cinp, 25, coutp), dtype=torch.bfloat16, device=device))
+        self._adam = None
+        self.step_count = 0
+
+    @property
+    def nlayers(self):
+        return len(self.channels) - 1
+
+    def n_params(self):
+        return sum((self.channels[l + 1] * self.channels[l] * 25 + self.channels[l + 1])
+                   for l in range(self.nlayers))
+
+    def refresh_bf16(self):
+        for l in range(self.nlayers):
+            self.wb[l] = self.w32[l].to(torch.bfloat16)
+
+    def pad_input(self, x):
+        """(B, H, W, channels[0]) float -> padded bf16 NHWC."""
+        B, H, W, c = x.shape
+        xp = torch.zeros((B, H, W, self.chp[0]), dtype=torch.bfloat16, device=x.device)
+        xp[..., :c] = x.to(torch.bfloat16)
+        return xp
+
+    def forward(self, xp, keep=False):
+        """xp: padded bf16 input.  Returns the FP32 output (B,H,W,coutp) and,
+        with keep, every BF16 activation for the backward pass."""
+        acts = [xp]
+        h = xp
+        for l in range(self.nlayers):
+            last = l == self.nlayers - 1
+            y, y32 = conv_fwd(h, self.wb[l], self.b32[l], self.chp[l + 1], relu=not last,
+                              out_bf16=not last or keep, out_f32=last)
+            if not last:
+                h = y
+                if keep:
+                    acts.append(y)
+            else:
+                out = y32
+                if keep:
+                    acts.append(y)
+        return (out, acts) if keep else out
+
+    def backward(self, acts, dout):
+        """dout: bf16 gradient of the loss w.r.t. the last (pre-activation) output.
+        Returns FP32 (dW, db) per layer."""
+        dws, dbs = [], []
+        d = dout
+        for l in reversed(range(self.nlayers)):
+            dw = torch.zeros_like(self.w32[l])
+            db = torch.zeros_like(self.b32[l])
+            conv_wgrad(d, acts[l], dw)
+            bias_grad(d, db)
+            dws.append(dw)
+            dbs.append(db)
+            if l > 0:
+                flip_transpose(self.wb[l], self.wt[l])
+                d, _ = conv_fwd(d, self.wt[l], None, self.chp[l], relu=False, mask=acts[l])
+        return dws[::-1], dbs[::-1]
+
+    def adam_step(self, dws, dbs, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+        if self._adam is None:
+            self._adam = [(torch.zeros_like(w), torch.zeros_like(w), torch.zeros_like(b),
+                           torch.zeros_like(b)) for w, b in zip(self.w32, self.b32)]
+        self.step_count += 1
+        for l in range(self.nlayers):
+            mw, vw, mb, vb = self._adam[l]
+            check(lib.qgcnn_adam(_p(self.w32[l]), _p(dws[l]), _p(mw), _p(vw), _p(self.wb[l]),
+                                 self.w32[l].numel(), lr, b1, b2, eps, self.step_count, wd,
+                                 _stream()))
+            check(lib.qgcnn_adam(_p(self.b32[l]), _p(dbs[l]), _p(mb), _p(vb), None,
+                                 self.b32[l].numel(), lr, b1, b2, eps, self.step_count, 0.0,
+                                 _stream()))
+
+    def train_step(self, xp, target, lr=1e-3):
+        """One Adam step on the MSE between output channels and target
+        (B,H,W,channels[-1]) FP32; returns the loss tensor."""
+        out, acts = self.forward(xp, keep=True)
+        grad = torch.empty(out.shape, dtype=torch.bfloat16, device=out.device)
+        loss = mse(out, target, grad)
+        dws, dbs = self.backward(acts, grad)
+        self.adam_step(dws, dbs, lr=lr)
+        return loss
+
+    # ------------------------------------------------------------ persistence
+    def state(self):
+        return {"channels": np.array(self.channels),
+                **{f"w{l}": self.w32[l].cpu().numpy() for l in range(self.nlayers)},
+                **{f"b{l}": self.b32[l].cpu().numpy() for l in range(self.nlayers)}}
+
+    def save(self, path):
+        np.savez(path, **self.state())
+
+    @classmethod
+    def load(cls, path, device="cuda"):
+        d = np.load(path, allow_pickle=False)
+        net = cls(tuple(int(c) for c in d["channels"]), device=device)
+        for l in range(net.nlayers):
+            net.w32[l] = torch.from_numpy(d[f"w{l}"]).to(device)
+            net.b32[l] = torch.from_numpy(d[f"b{l}"]).to(device)
+        net.refresh_bf16()
+        return net

@@ -1,0 +1,162 @@
+"""End-to-end learned-closure experiment (two-layer eddy regime).
+
+    python -m pyqg_amd.closure.experiment [--members 256] [--samples 51200] ...
+
+1. A high-resolution ensemble (nx=256, dt=3600 s) is spun up and sampled; at
+   each sample libqgk computes the coarse inputs (q, u, v at nx=64) and the
+   subgrid PV forcing, and the time-mean KE spectrum of the high-resolution
+   flow is accumulated.
+2. The CNN (5x5 convolutions, 128/64/32 channels) is trained on the GPU with
+   libqgcnn's BF16 forward/backward kernels; offline skill is measured on
+   held-out pairs.
+3. Coarse ensembles (nx=64) are run with and without the network, started
+   from the coarse-grained high-resolution states; their time-mean KE spectra
+   are compared with the high-resolution one over the coarse grid's resolved
+   wavenumbers, and the cost per simulated day is compared.
+Prints one JSON line.
+"""
+import argparse
+import json
+import time
+
+import numpy as np
+import torch
+
+from .. import _lib
+from . import cnn as cnn_mod
+from . import data as data_mod
+from . import param as param_mod
+from . import spectra
+from . import train as train_mod
+
+
+def run(members=256, samples=51200, spinup_days=1080, every=24, epochs=8, batch=64, lr=1e-3,
+        online_members=128, online_days=720, avg_days=360, dt_lo=14400., seed=0, log=print):
+    out = {"config": dict(members=members, samples=samples, spinup_days=spinup_days,
+                          every=every, epochs=epochs, batch=batch, lr=lr,
+                          online_members=online_members, online_days=online_days,
+                          avg_days=avg_days, dt_hi=3600., dt_lo=dt_lo)}
+    # ---------------------------------------------------------- 1. data
+    gen = data_mod.PairGenerator(nmembers=members, n_hi=256, n_lo=64, dt=3600., seed=seed)
+    t0 = time.perf_counter()
+    gen.spin_up(spinup_days * 24)
+    hb = gen.hi.buckets[0]
+    torch.cuda.synchronize()
+    n_hold = max(samples // 10, members)
+    xs, ys = [], []
+    from .. import constants as C
+    g_hi = C.grid(256, 256, gen.phys["L"], gen.phys["L"])
+    g_lo = C.grid(64, 64, gen.phys["L"], gen.phys["L"])
+    Hi = np.array([gen.phys["H1"], gen.phys["H1"] / gen.phys["delta"]])
+    kmax_lo = g_lo["kk"].max()
+    e_hi = None
+    n = 0
+    t_hi_steps = 0.0
+    while n < samples + n_hold:
+        ts = time.perf_counter()
+        gen.hi.step(every)
+        hb.sync()
+        t_hi_steps += time.perf_counter() - ts
+        x, y = gen.sample()
+        xs.append(x)
+        ys.append(y)
+        ph = hb.get("ph")
+        spec = np.mean([spectra.ke_spectrum(p, g_hi["kk"], g_hi["ll"], g_hi["wv2"], Hi,
+                                            kmax_lo)[1] for p in ph], axis=0)
+        e_hi = spec if e_hi is None else e_hi + spec
+        n += x.shape[0]
+    nsnap = len(xs)
+    e_hi /= nsnap
+    x = torch.cat(xs)
+    y = torch.cat(ys)
+    del xs, ys
+    x_tr, y_tr = x[:samples], y[:samples]
+    x_te, y_te = x[samples:], y[samples:]
+    hi_state = gen.hi.get("q")
+    out["data_seconds"] = time.perf_counter() - t0
+    # cost of the high-resolution model per simulated day (all members)
+    hi_steps = nsnap * every
+    out["hi_res_ms_per_step"] = 1e3 * t_hi_steps / hi_steps
+    log(f"data: {x_tr.shape[0]} train + {x_te.shape[0]} held-out pairs in "
+        f"{out['data_seconds']:.1f} s")
+
+    # ---------------------------------------------------------- 2. training
+    scales = param_mod.Scales.fit(x_tr, y_tr)
+    net = cnn_mod.CNN(seed=seed)
+    out["n_params"] = net.n_params()
+    info = train_mod.train(net, x_tr, y_tr, scales, epochs=epochs, batch=batch, lr=lr,
+                           log=log)
+    out["train"] = {"steps": info["steps"], "seconds": info["seconds"],
+                    "final_loss": info["history"][-1][1] if info["history"] else None}
+    out["offline"] = train_mod.evaluate(net, x_te, y_te, scales)
+    flop_per_sample = 2 * 64 * 64 * 25 * sum(
+        net.channels[l] * net.channels[l + 1] for l in range(net.nlayers))
+    out["train"]["tflops"] = 3 * flop_per_sample * info["steps"] * batch / info["seconds"] / 1e12
+    log(f"offline R^2 = {out['offline']['r2']:.3f}")
+    del x, y, x_tr, y_tr
+
+    # ---------------------------------------------------------- 3. online
+    p = param_mod.CNNParameterization(net, scales)
+    runs = {}
+    for name, prm in (("plain", None), ("cnn", p)):
+        ce = param_mod.ClosedEnsemble(online_members, 64, dt_lo, gen.phys, param=prm, seed=seed)
+        # start from coarse-grained high-resolution states
+        _lib.check(_lib.lib.qgk_coarsen(hb.h, gen.lo.buckets[0].h, _lib.FIELD_ID["qh"],
+                                        _lib.FIELD_ID["qh"], _lib.ptr(gen.G)))
+        qlo = gen.lo.get("q")
+        ce.ens.set_q([qlo[i % members] for i in range(online_members)])
+        per_day = int(round(86400. / dt_lo))
+        nsteps = online_days * per_day
+        navg = avg_days * per_day
+        e = None
+        cnt = 0
+        t1 = time.perf_counter()
+        for s in range(nsteps):
+            ce.step(1)
+            if s >= nsteps - navg and s % per_day == 0:
+                ph = ce.ph()
+                sp = np.mean([spectra.ke_spectrum(q, g_lo["kk"], g_lo["ll"], g_lo["wv2"], Hi,
+                                                  kmax_lo)[1] for q in ph], axis=0)
+                e = sp if e is None else e + sp
+                cnt += 1
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
+        e /= cnt
+        finite = bool(np.isfinite(ce.ens.get("q", 0)).all())
+        runs[name] = {"ke_spectrum_error": spectra.spectrum_error(e, e_hi),
+                      "ms_per_step": 1e3 * wall / nsteps, "finite": finite,
+                      "total_ke": float(e.sum())}
+        log(f"{name}: KE-spectrum error {runs[name]['ke_spectrum_error']:.3f}, "
+            f"{runs[name]['ms_per_step']:.2f} ms/step")
+    out["online"] = runs
+    out["hi_res_total_ke"] = float(e_hi.sum())
+    # cost per member and simulated day
+    hi_cost = out["hi_res_ms_per_step"] / members * 86400. / 3600.
+    lo_cost = runs["cnn"]["ms_per_step"] / online_members * 86400. / dt_lo
+    out["cost_ratio_hi_over_cnn"] = hi_cost / lo_cost
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--samples", type=int, default=51200)
+    ap.add_argument("--spinup-days", type=int, default=1080)
+    ap.add_argument("--every", type=int, default=24)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--online-members", type=int, default=128)
+    ap.add_argument("--online-days", type=int, default=720)
+    ap.add_argument("--avg-days", type=int, default=360)
+    ap.add_argument("--dt-lo", type=float, default=14400.)
+    a = ap.parse_args()
+    res = run(members=a.members, samples=a.samples, spinup_days=a.spinup_days, every=a.every,
+              epochs=a.epochs, batch=a.batch, lr=a.lr, online_members=a.online_members,
+              online_days=a.online_days, avg_days=a.avg_days, dt_lo=a.dt_lo,
+              log=lambda s: print(s, flush=True))
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,116 @@
+"""The trained CNN as a pyqg q parameterization.
+
+Two ways in:
+  * `CNNParameterization(net, scales)` is a callable with
+    parameterization_type = "q_parameterization" (pyqg/parameterizations.py:
+    4-130): m -> dq (nz, ny, nx), usable by pyqg_amd's models and by pyqg's own
+    (the host-callback path of kernel.pyx:424-442);
+  * `ClosedEnsemble` steps an ensemble of coarse models with the network
+    inside the step on the GPU: per step the inversion, the packing of q, u, v
+    into the network input, the network, and dqhdt += rfft2(S) run as device
+    kernels with no host round trip, in the reference's step order
+    (pyqg/model.py:406-436: invert, advection, friction, q parameterization,
+    timestep).
+"""
+import numpy as np
+import torch
+
+from .. import _lib
+from .. import constants as C
+from .cnn import CNN
+
+
+class Scales:
+    """Per-channel normalisation: network input = field / in_scale, forcing =
+    network output * out_scale."""
+
+    def __init__(self, in_scale, out_scale):
+        self.in_scale = np.asarray(in_scale, dtype=np.float64)
+        self.out_scale = np.asarray(out_scale, dtype=np.float64)
+
+    @classmethod
+    def fit(cls, x, y):
+        """RMS of every input and target channel of training tensors (NHWC)."""
+        xs = torch.sqrt((x.double() ** 2).mean(dim=(0, 1, 2))).cpu().numpy()
+        ys = torch.sqrt((y.double() ** 2).mean(dim=(0, 1, 2))).cpu().numpy()
+        return cls(xs, ys)
+
+
+class CNNParameterization:
+    """q parameterization from a trained network (host callback path)."""
+
+    parameterization_type = "q_parameterization"
+
+    def __init__(self, net: CNN, scales: Scales):
+        self.net = net
+        self.scales = scales
+
+    def predict(self, x):
+        """x: (B, H, W, 6) FP32 physical inputs on the GPU -> (B, H, W, 2) FP32."""
+        xin = x / torch.as_tensor(self.scales.in_scale, dtype=torch.float32, device=x.device)
+        out = self.net.forward(self.net.pad_input(xin))
+        nout = self.net.channels[-1]
+        return out[..., :nout] * torch.as_tensor(self.scales.out_scale, dtype=torch.float32,
+                                                 device=x.device)
+
+    def __call__(self, m):
+        q, u, v = (np.asarray(a) for a in (m.q, m.u, m.v))
+        x = np.concatenate([q, u, v], axis=0).transpose(1, 2, 0)[np.newaxis]
+        y = self.predict(torch.as_tensor(x, dtype=torch.float32, device="cuda"))
+        return y[0].permute(2, 0, 1).double().cpu().numpy()
+
+
+class ClosedEnsemble:
+    """Coarse two-layer QG ensemble stepped with the CNN closure on the GPU.
+
+    With `param=None` it is the plain coarse model stepped phase by phase (the
+    same order and kernels, without the forcing), the baseline the closure is
+    compared with."""
+
+    def __init__(self, nmembers, n, dt, phys, param=None, seed=0):
+        from ..ensemble import Ensemble
+        self.ens = Ensemble.qg(np.full(nmembers, n), dt=dt, full_state=True,
+                               q_param=param is not None, **phys)
+        self.b = self.ens.buckets[0]
+        self.param = param
+        self.n = n
+        rng = np.random.default_rng(seed)
+        self.ens.set_q([1e-7 * rng.standard_normal((2, n, n)) for _ in range(nmembers)])
+        if param is not None:
+            self.cinp = param.net.chp[0]
+            self.xin = torch.empty((nmembers, n, n, self.cinp), dtype=torch.bfloat16,
+                                   device="cuda")
+            self.inv_in = np.ascontiguousarray(1.0 / param.scales.in_scale)
+            self.out_scale = np.ascontiguousarray(param.scales.out_scale)
+        self.ablevel = 0
+
+    @property
+    def t(self):
+        return self.ens.t
+
+    def step(self, nsteps=1):
+        h = self.b.h
+        L = _lib.lib
+        for _ in range(int(nsteps)):
+            _lib.check(L.qgk_invert(h))
+            _lib.check(L.qgk_advect(h))
+            _lib.check(L.qgk_friction(h))
+            if self.param is not None:
+                _lib.check(L.qgk_pack_inputs(h, _lib.ptr(self.inv_in), self.cinp,
+                                             ctypes_p(self.xin)))
+                out = self.param.net.forward(self.xin)
+                torch.cuda.current_stream().synchronize()
+                _lib.check(L.qgk_add_forcing_device(h, ctypes_p(out), out.shape[-1],
+                                                    _lib.ptr(self.out_scale)))
+            w, self.ablevel = C.ab_weights(self.ablevel, self.ens.dt)
+            _lib.check(L.qgk_timestep(h, *w))
+            self.ens.t += self.ens.dt
+            self.ens.tc += 1
+
+    def ph(self):
+        return self.b.get("ph")
+
+
+def ctypes_p(t):
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr())

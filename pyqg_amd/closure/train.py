@@ -1,0 +1,56 @@
+"""Training and offline evaluation of the closure network (libqgcnn kernels)."""
+import math
+import time
+
+import numpy as np
+import torch
+
+from .cnn import CNN
+from .param import Scales
+
+
+def _normalized(x, y, scales, idx, net):
+    dev = x.device
+    xs = torch.as_tensor(scales.in_scale, dtype=torch.float32, device=dev)
+    ys = torch.as_tensor(scales.out_scale, dtype=torch.float32, device=dev)
+    return net.pad_input(x[idx] / xs), (y[idx] / ys).contiguous()
+
+
+def train(net: CNN, x, y, scales: Scales, epochs=4, batch=64, lr=1e-3, lr_min=1e-5, seed=0,
+          log=None):
+    """Adam on the MSE of normalised forcing, cosine learning-rate decay.
+    x (N, H, W, 6) and y (N, H, W, 2) FP32 tensors on the GPU."""
+    n = x.shape[0]
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    steps = epochs * max(n // batch, 1)
+    it = 0
+    history = []
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        perm = torch.randperm(n, generator=g)
+        for s in range(max(n // batch, 1)):
+            idx = perm[s * batch:(s + 1) * batch].to(x.device)
+            xb, yb = _normalized(x, y, scales, idx, net)
+            cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * it / steps))
+            loss = net.train_step(xb, yb, lr=cur)
+            it += 1
+            if it % 50 == 0 or it == steps:
+                history.append((it, float(loss)))
+                if log:
+                    log(f"step {it}/{steps} loss {float(loss):.4f}")
+    torch.cuda.synchronize()
+    return {"steps": it, "seconds": time.perf_counter() - t0, "history": history}
+
+
+@torch.no_grad()
+def evaluate(net: CNN, x, y, scales: Scales, batch=128):
+    """Offline skill on held-out pairs: normalised MSE of the network and of
+    the no-closure prediction (zero forcing), and R^2 = 1 - their ratio."""
+    se, ss = 0.0, 0.0
+    for s in range(0, x.shape[0], batch):
+        idx = torch.arange(s, min(s + batch, x.shape[0]), device=x.device)
+        xb, yb = _normalized(x, y, scales, idx, net)
+        out = net.forward(xb)[..., :yb.shape[-1]]
+        se += float(((out - yb) ** 2).sum())
+        ss += float((yb ** 2).sum())
+    return {"mse_net": se / ss, "mse_zero": 1.0, "r2": 1.0 - se / ss}

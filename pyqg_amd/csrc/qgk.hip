@@ -22,6 +22,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -88,7 +89,10 @@ struct Geo {
   static constexpr int RP = RPraw < 1 ? 1 : RPraw;           // row pairs per row block
 };
 
-__device__ __forceinline__ double ll_of(const MemberParams& P, int l) { return P.ll[l]; }
+// Row stride of spectral arrays: nx/2+1 complex values padded to a multiple of
+// 8 (128 bytes), so every row starts on a cache line and column tiles of 8
+// read whole lines.
+__host__ __device__ constexpr int ldk_of(int nx) { return ((nx / 2 + 1) + 7) / 8 * 8; }
 
 // inversion at one spectral point: ph[k2] = sum_k1 a[k2,k1] qh[k1] (kernel.pyx:331-338)
 __device__ __forceinline__ cd invert_point(const MemberParams& P, int nz, int k2,
@@ -133,22 +137,22 @@ __device__ __forceinline__ cd invert_point(const MemberParams& P, int nz, int k2
 // ==========================================================================
 template <int NY, int SIGN>
 __global__ __launch_bounds__(Geo<NY>::T * Geo<NY>::TK) void k_col_fft(
-    const cd* in, cd* out, int nk, const cd* __restrict__ tw) {
+    const cd* in, cd* out, int nk, int ldk, const cd* __restrict__ tw) {
   using G = Geo<NY>;
   __shared__ cd lds[NY * G::TK];
   const int c = threadIdx.x % G::TK;
   const int t = threadIdx.x / G::TK;
   const int k = blockIdx.x * G::TK + c;
-  const size_t plane = (size_t)blockIdx.y * NY * nk;
+  const size_t plane = (size_t)blockIdx.y * NY * ldk;
   const bool ok = k < nk;
   cd x[G::E];
 #pragma unroll
   for (int s = 0; s < G::E; ++s)
-    x[s] = ok ? in[plane + (size_t)(t + s * G::T) * nk + k] : cd{0.0, 0.0};
+    x[s] = ok ? in[plane + (size_t)(t + s * G::T) * ldk + k] : cd{0.0, 0.0};
   qgk::fft_block<NY, G::E, SIGN, G::TK>(x, lds, t, c, tw);
   if (ok) {
 #pragma unroll
-    for (int s = 0; s < G::E; ++s) out[plane + (size_t)(t + s * G::T) * nk + k] = x[s];
+    for (int s = 0; s < G::E; ++s) out[plane + (size_t)(t + s * G::T) * ldk + k] = x[s];
   }
 }
 
@@ -178,7 +182,7 @@ __global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_c2r(
     const cd* __restrict__ in, double* __restrict__ out, int ny, double scale,
     const cd* __restrict__ tw) {
   using G = Geo<NX>;
-  constexpr int NK = NX / 2 + 1;
+  constexpr int NK = ldk_of(NX);  // row stride
   __shared__ cd lds[NX * G::RP];
   const int t = threadIdx.x % G::T;
   const int rp = threadIdx.x / G::T;
@@ -227,7 +231,7 @@ template <int NX>
 __global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_r2c(
     const double* __restrict__ in, cd* __restrict__ out, int ny, const cd* __restrict__ tw) {
   using G = Geo<NX>;
-  constexpr int NK = NX / 2 + 1;
+  constexpr int NK = ldk_of(NX);  // row stride
   __shared__ cd lds[NX * G::RP];
   const int t = threadIdx.x % G::T;
   const int rp = threadIdx.x / G::T;
@@ -405,16 +409,42 @@ __global__ void k_ab_step(cd* __restrict__ qh, const cd* __restrict__ d,
 // Fused step
 // ==========================================================================
 
+// Launch geometry of the fused kernels for transform length N with E elements
+// per thread: T = N/E threads per transform, TK columns per column tile, RP row
+// pairs per row block.  E sets the register radix of the Stockham passes and is
+// the autotuned parameter.
+template <int N, int E_>
+struct FGeo {
+  static constexpr int E = E_ < N ? E_ : N;
+  static constexpr int T = N / E;
+  static constexpr int TKraw = 512 / T;  // 8 columns = one 128-byte line per row
+  static constexpr int TK = TKraw < 1 ? 1 : (TKraw > 8 ? 8 : TKraw);
+  static constexpr int RPraw = 128 / T;
+  static constexpr int RP = RPraw < 1 ? 1 : RPraw;
+};
+
+template <int NX, int T, int RP>
+__device__ __forceinline__ void split_store_t(const cd* lds, int t, int rp, bool ok, cd* o0,
+                                              cd* o1) {
+  if (!ok) return;
+  for (int k = t; k <= NX / 2; k += T) {
+    cd A = lds[k * RP + rp];
+    cd B = lds[((NX - k) & (NX - 1)) * RP + rp];
+    o0[k] = {0.5 * (A.x + B.x), 0.5 * (A.y - B.y)};
+    o1[k] = {0.5 * (A.y + B.y), -0.5 * (A.x - B.x)};
+  }
+}
+
 // Row phase: per (member, layer, row pair): x-c2r of q, u, v, flux, x-r2c.
 // In:  Qm, Um, Vm  [M][nz][ny][nk] (y already inverse transformed)
 // Out: UQm, VQm    [M][nz][ny][nk] (x already forward transformed)
-template <int NX>
-__global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_flux(
+template <int NX, int E>
+__global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP)) void k_row_flux(
     const cd* __restrict__ Qm, const cd* __restrict__ Um, const cd* __restrict__ Vm,
     cd* __restrict__ UQm, cd* __restrict__ VQm, const MemberParams* __restrict__ params,
     int nz, int ny, double scale, const cd* __restrict__ tw) {
-  using G = Geo<NX>;
-  constexpr int NK = NX / 2 + 1;
+  using G = FGeo<NX, E>;
+  constexpr int NK = ldk_of(NX);  // row stride
   __shared__ cd lds[NX * G::RP];
   const int t = threadIdx.x % G::T;
   const int rp = threadIdx.x / G::T;
@@ -458,35 +488,37 @@ __global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_flux(
 #pragma unroll
     for (int s = 0; s < G::E; ++s) lds[(t + s * G::T) * G::RP + rp] = w[s];
     __syncthreads();
-    split_store<NX, G::RP>(lds, t, rp, ok, UQm + row, VQm + row);
+    split_store_t<NX, G::T, G::RP>(lds, t, rp, ok, UQm + row, VQm + row);
   }
 }
 
-// Column phase, part C: from qh (all layers) build ph, uh, vh of one layer at
-// this thread's points and inverse-transform q, u, v along y into Qm, Um, Vm.
-template <int NY>
+// ---- any nz: the inversion re-reads qh from global memory (cache hits)
+
+// From qh (all layers) build ph, uh, vh of layer z at this thread's points and
+// inverse-transform q, u, v along y into Qm, Um, Vm.
+template <int NY, int E>
 __device__ __forceinline__ void col_prologue_layer(const cd* __restrict__ qh,
                                                    cd* __restrict__ Qm, cd* __restrict__ Um,
                                                    cd* __restrict__ Vm, const MemberParams& P,
-                                                   int nz, int z, int nk, int t, int c, int k,
+                                                   int nz, int z, int nk, int ldk, int t, int c, int k,
                                                    bool ok, size_t mbase, cd* lds,
                                                    const cd* __restrict__ tw) {
-  using G = Geo<NY>;
-  const size_t plane = (size_t)NY * nk;
+  using G = FGeo<NY, E>;
+  const size_t plane = (size_t)NY * ldk;
   const size_t zoff = mbase + (size_t)z * plane;
   const int kc = ok ? k : 0;
   cd x[G::E];
   cd ph[G::E];
 #pragma unroll
   for (int s = 0; s < G::E; ++s) {
-    const size_t pt = (size_t)(t + s * G::T) * nk + kc;
+    const size_t pt = (size_t)(t + s * G::T) * ldk + kc;
     x[s] = ok ? qh[zoff + pt] : cd{0.0, 0.0};
     ph[s] = ok ? invert_point(P, nz, z, qh + mbase + pt, plane, pt) : cd{0.0, 0.0};
   }
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
   if (ok) {
 #pragma unroll
-    for (int s = 0; s < G::E; ++s) Qm[zoff + (size_t)(t + s * G::T) * nk + k] = x[s];
+    for (int s = 0; s < G::E; ++s) Qm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
   }
   // uh = -(i ll) ph
 #pragma unroll
@@ -497,7 +529,7 @@ __device__ __forceinline__ void col_prologue_layer(const cd* __restrict__ qh,
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
   if (ok) {
 #pragma unroll
-    for (int s = 0; s < G::E; ++s) Um[zoff + (size_t)(t + s * G::T) * nk + k] = x[s];
+    for (int s = 0; s < G::E; ++s) Um[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
   }
   // vh = (i kk) ph
   const double kk = P.kk[kc];
@@ -506,15 +538,15 @@ __device__ __forceinline__ void col_prologue_layer(const cd* __restrict__ qh,
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
   if (ok) {
 #pragma unroll
-    for (int s = 0; s < G::E; ++s) Vm[zoff + (size_t)(t + s * G::T) * nk + k] = x[s];
+    for (int s = 0; s < G::E; ++s) Vm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
   }
 }
 
-template <int NY>
-__global__ __launch_bounds__(Geo<NY>::T * Geo<NY>::TK) void k_col_prologue(
+template <int NY, int E>
+__global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_prologue(
     const cd* __restrict__ qh, cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
-    const MemberParams* __restrict__ params, int nz, int nk, const cd* __restrict__ tw) {
-  using G = Geo<NY>;
+    const MemberParams* __restrict__ params, int nz, int nk, int ldk, const cd* __restrict__ tw) {
+  using G = FGeo<NY, E>;
   __shared__ cd lds[NY * G::TK];
   const int c = threadIdx.x % G::TK;
   const int t = threadIdx.x / G::TK;
@@ -522,24 +554,25 @@ __global__ __launch_bounds__(Geo<NY>::T * Geo<NY>::TK) void k_col_prologue(
   const int m = blockIdx.y;
   const bool ok = k < nk;
   const MemberParams& P = params[m];
-  const size_t mbase = (size_t)m * nz * NY * nk;
+  const size_t mbase = (size_t)m * nz * NY * ldk;
   for (int z = 0; z < nz; ++z)
-    col_prologue_layer<NY>(qh, Qm, Um, Vm, P, nz, z, nk, t, c, k, ok, mbase, lds, tw);
+    col_prologue_layer<NY, E>(qh, Qm, Um, Vm, P, nz, z, nk, ldk, t, c, k, ok, mbase, lds, tw);
 }
 
-// Column phase of the fused step.  Per (member, column tile):
-//  A. y-FFT of UQm, VQm per layer; S = ik uqh + il vqh parked in `dscr`.
-//  B. per point: ph = a qh; dqhdt = -(S + ikQy ph) [+ drag]; AB3 + filter;
-//     qh <- qh'; new dqhdt written into the dqhdt_pp buffer (it becomes dqhdt_p).
-//  C. the next step's prologue from qh'.
-template <int NY>
-__global__ __launch_bounds__(Geo<NY>::T * Geo<NY>::TK) void k_col_step(
-    cd* __restrict__ qh, cd* __restrict__ dscr, const cd* __restrict__ p,
+// Column phase of the fused step.  Per (member, column tile), layer by layer:
+//  y-FFT of UQm, VQm -> S = ik uqh + il vqh; ph = a qh_old; dqhdt = -(S + ikQy ph)
+//  [+ drag on the bottom layer]; AB3 + filter -> qh_new; the new dqhdt goes into
+//  the dqhdt_pp buffer (it becomes dqhdt_p).  qh is double-buffered, so every
+//  layer's inversion reads the old state while new layers are written.  Then
+//  the next step's prologue (inversion of qh_new, y IFFT of q, u, v) runs.
+template <int NY, int E>
+__global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step(
+    const cd* __restrict__ qh_old, cd* __restrict__ qh_new, const cd* __restrict__ p,
     cd* __restrict__ pp_to_newp, const cd* __restrict__ UQm, const cd* __restrict__ VQm,
     cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
-    const MemberParams* __restrict__ params, StepCoeffs cf, int nz, int nk,
+    const MemberParams* __restrict__ params, StepCoeffs cf, int nz, int nk, int ldk,
     const cd* __restrict__ tw) {
-  using G = Geo<NY>;
+  using G = FGeo<NY, E>;
   __shared__ cd lds[NY * G::TK];
   const int c = threadIdx.x % G::TK;
   const int t = threadIdx.x / G::TK;
@@ -548,70 +581,335 @@ __global__ __launch_bounds__(Geo<NY>::T * Geo<NY>::TK) void k_col_step(
   const bool ok = k < nk;
   const int kc = ok ? k : 0;
   const MemberParams& P = params[m];
-  const size_t plane = (size_t)NY * nk;
+  const size_t plane = (size_t)NY * ldk;
   const size_t mbase = (size_t)m * nz * plane;
   const double kk = P.kk[kc];
 
-  // ---- A
   for (int z = 0; z < nz; ++z) {
     const size_t zoff = mbase + (size_t)z * plane;
     cd xa[G::E], xb[G::E];
 #pragma unroll
     for (int s = 0; s < G::E; ++s) {
-      const size_t pt = (size_t)(t + s * G::T) * nk + kc;
+      const size_t pt = (size_t)(t + s * G::T) * ldk + kc;
       xa[s] = ok ? UQm[zoff + pt] : cd{0.0, 0.0};
       xb[s] = ok ? VQm[zoff + pt] : cd{0.0, 0.0};
     }
     qgk::fft_block<NY, G::E, -1, G::TK>(xa, lds, t, c, tw);
     qgk::fft_block<NY, G::E, -1, G::TK>(xb, lds, t, c, tw);
     if (ok) {
+      const double kq = kk * P.Qy[z];
+      const bool drag = (z == nz - 1) && P.rek != 0.0;
 #pragma unroll
       for (int s = 0; s < G::E; ++s) {
-        const double ll = P.ll[t + s * G::T];
+        const int l = t + s * G::T;
+        const size_t pt = (size_t)l * ldk + k;
+        const size_t o = zoff + pt;
+        const double ll = P.ll[l];
         // ik uqh + il vqh (kernel.pyx:391-392)
-        cd S = {-kk * xa[s].y + -ll * xb[s].y, kk * xa[s].x + ll * xb[s].x};
-        dscr[zoff + (size_t)(t + s * G::T) * nk + k] = S;
-      }
-    }
-  }
-
-  // ---- B (qh' parked in Qm until every layer's ph is formed from the old qh)
-  if (ok) {
-#pragma unroll 1
-    for (int s = 0; s < G::E; ++s) {
-      const size_t pt = (size_t)(t + s * G::T) * nk + k;
-      const double f = P.filtr[pt];
-      for (int z = 0; z < nz; ++z) {
-        const size_t o = mbase + (size_t)z * plane + pt;
-        const cd ph = invert_point(P, nz, z, qh + mbase + pt, plane, pt);
-        const double kq = kk * P.Qy[z];
-        const cd S = dscr[o];
+        const cd S = {-kk * xa[s].y + -ll * xb[s].y, kk * xa[s].x + ll * xb[s].x};
+        const cd ph = invert_point(P, nz, z, qh_old + mbase + pt, plane, pt);
         cd d = {S.x + -kq * ph.y, S.y + kq * ph.x};
         d = {-d.x, -d.y};
-        if (z == nz - 1 && P.rek != 0.0) {
+        if (drag) {
           const double cr = P.rek * P.k2l2[pt];
           d = {d.x + cr * ph.x, d.y + cr * ph.y};
         }
-        const cd q = qh[o], pv = p[o], ppv = pp_to_newp[o];
+        const double f = P.filtr[pt];
+        const cd q = qh_old[o], pv = p[o], ppv = pp_to_newp[o];
         double rx = q.x + cf.dt1 * d.x;
         double ry = q.y + cf.dt1 * d.y;
         rx = rx + cf.dt2 * pv.x;
         ry = ry + cf.dt2 * pv.y;
         rx = rx + cf.dt3 * ppv.x;
         ry = ry + cf.dt3 * ppv.y;
-        Qm[o] = {f * rx, f * ry};
+        qh_new[o] = {f * rx, f * ry};
         pp_to_newp[o] = d;
-      }
-      for (int z = 0; z < nz; ++z) {
-        const size_t o = mbase + (size_t)z * plane + pt;
-        qh[o] = Qm[o];
       }
     }
   }
-
-  // ---- C
   for (int z = 0; z < nz; ++z)
-    col_prologue_layer<NY>(qh, Qm, Um, Vm, P, nz, z, nk, t, c, k, ok, mbase, lds, tw);
+    col_prologue_layer<NY, E>(qh_new, Qm, Um, Vm, P, nz, z, nk, ldk, t, c, k, ok, mbase, lds, tw);
+}
+
+// ---- nz <= 2: every field is read exactly once; the layers' qh and ph at the
+// thread's points stay in registers.
+
+// ph[z] = sum_k1 a[z,k1] q[k1] with the reference's summation order
+template <int NZ>
+__device__ __forceinline__ cd invert_regs(const MemberParams& P, int z, const cd* q,
+                                          size_t plane, size_t pt) {
+#pragma clang fp contract(off)
+  if (P.a) {
+    cd acc = {0.0, 0.0};
+#pragma unroll
+    for (int k1 = 0; k1 < NZ; ++k1) {
+      const double a = P.a[((size_t)z * NZ + k1) * plane + pt];
+      acc.x = acc.x + a * q[k1].x;
+      acc.y = acc.y + a * q[k1].y;
+    }
+    return acc;
+  }
+  if constexpr (NZ == 2) {
+    const double w = P.k2l2[pt];
+    const double det = w * (w + P.F1 + P.F2);
+    if (det == 0.0) return {0.0, 0.0};
+    const double di = 1.0 / det;
+    const double a0 = z == 0 ? -(w + P.F2) * di : -P.F2 * di;
+    const double a1 = z == 0 ? -P.F1 * di : -(w + P.F1) * di;
+    cd acc;
+    acc.x = 0.0 + a0 * q[0].x;
+    acc.y = 0.0 + a0 * q[0].y;
+    acc.x = acc.x + a1 * q[1].x;
+    acc.y = acc.y + a1 * q[1].y;
+    return acc;
+  } else {
+    return {0.0, 0.0};
+  }
+}
+
+// inversion + y IFFT of q, u, v for all layers from register-held qh
+template <int NY, int NZ, int E>
+__device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
+                                                  cd* __restrict__ Qm, cd* __restrict__ Um,
+                                                  cd* __restrict__ Vm, const MemberParams& P,
+                                                  int nk, int ldk, int t, int c, int k, bool ok,
+                                                  size_t mbase, cd* lds,
+                                                  const cd* __restrict__ tw) {
+  using G = FGeo<NY, E>;
+  const size_t plane = (size_t)NY * ldk;
+  const int kc = ok ? k : 0;
+  const double kk = P.kk[kc];
+#pragma unroll
+  for (int z = 0; z < NZ; ++z) {
+    const size_t zoff = mbase + (size_t)z * plane;
+    cd x[G::E], ph[G::E];
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) {
+      cd qs[NZ];
+#pragma unroll
+      for (int k1 = 0; k1 < NZ; ++k1) qs[k1] = Q[k1][s];
+      ph[s] = ok ? invert_regs<NZ>(P, z, qs, plane, (size_t)(t + s * G::T) * ldk + kc)
+                 : cd{0.0, 0.0};
+      x[s] = Q[z][s];
+    }
+    qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+    if (ok) {
+#pragma unroll
+      for (int s = 0; s < G::E; ++s) Qm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+    }
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) {
+      const double ll = P.ll[t + s * G::T];
+      x[s] = {ll * ph[s].y, -ll * ph[s].x};
+    }
+    qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+    if (ok) {
+#pragma unroll
+      for (int s = 0; s < G::E; ++s) Um[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+    }
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) x[s] = {-kk * ph[s].y, kk * ph[s].x};
+    qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+    if (ok) {
+#pragma unroll
+      for (int s = 0; s < G::E; ++s) Vm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+    }
+  }
+}
+
+template <int NY, int NZ, int E>
+__global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_prologue_reg(
+    const cd* __restrict__ qh, cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
+    const MemberParams* __restrict__ params, int nk, int ldk, const cd* __restrict__ tw) {
+  using G = FGeo<NY, E>;
+  __shared__ cd lds[NY * G::TK];
+  const int c = threadIdx.x % G::TK;
+  const int t = threadIdx.x / G::TK;
+  const int k = blockIdx.x * G::TK + c;
+  const int m = blockIdx.y;
+  const bool ok = k < nk;
+  const int kc = ok ? k : 0;
+  const MemberParams& P = params[m];
+  const size_t plane = (size_t)NY * ldk;
+  const size_t mbase = (size_t)m * NZ * plane;
+  cd Q[NZ][G::E];
+#pragma unroll
+  for (int z = 0; z < NZ; ++z)
+#pragma unroll
+    for (int s = 0; s < G::E; ++s)
+      Q[z][s] = ok ? qh[mbase + z * plane + (size_t)(t + s * G::T) * ldk + kc] : cd{0.0, 0.0};
+  col_prologue_regs<NY, NZ, G::E>(Q, Qm, Um, Vm, P, nk, ldk, t, c, k, ok, mbase, lds, tw);
+}
+
+template <int NY, int NZ, int E>
+__global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step_reg(
+    const cd* __restrict__ qh_old, cd* __restrict__ qh_new, const cd* __restrict__ p,
+    cd* __restrict__ pp_to_newp, const cd* __restrict__ UQm, const cd* __restrict__ VQm,
+    cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
+    const MemberParams* __restrict__ params, StepCoeffs cf, int nk, int ldk,
+    const cd* __restrict__ tw) {
+  using G = FGeo<NY, E>;
+  __shared__ cd lds[NY * G::TK];
+  const int c = threadIdx.x % G::TK;
+  const int t = threadIdx.x / G::TK;
+  const int k = blockIdx.x * G::TK + c;
+  const int m = blockIdx.y;
+  const bool ok = k < nk;
+  const int kc = ok ? k : 0;
+  const MemberParams& P = params[m];
+  const size_t plane = (size_t)NY * ldk;
+  const size_t mbase = (size_t)m * NZ * plane;
+  const double kk = P.kk[kc];
+
+  cd Q[NZ][G::E];
+  cd Q0old[NZ == 2 ? G::E : 1];
+#pragma unroll
+  for (int z = 0; z < NZ; ++z)
+#pragma unroll
+    for (int s = 0; s < G::E; ++s)
+      Q[z][s] = ok ? qh_old[mbase + z * plane + (size_t)(t + s * G::T) * ldk + kc]
+                   : cd{0.0, 0.0};
+
+#pragma unroll
+  for (int z = 0; z < NZ; ++z) {
+    const size_t zoff = mbase + (size_t)z * plane;
+    cd xa[G::E], xb[G::E];
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) {
+      const size_t pt = (size_t)(t + s * G::T) * ldk + kc;
+      xa[s] = ok ? UQm[zoff + pt] : cd{0.0, 0.0};
+      xb[s] = ok ? VQm[zoff + pt] : cd{0.0, 0.0};
+    }
+    qgk::fft_block<NY, G::E, -1, G::TK>(xa, lds, t, c, tw);
+    qgk::fft_block<NY, G::E, -1, G::TK>(xb, lds, t, c, tw);
+    if (ok) {
+      const double kq = kk * P.Qy[z];
+      const bool drag = (z == NZ - 1) && P.rek != 0.0;
+#pragma unroll
+      for (int s = 0; s < G::E; ++s) {
+        const int l = t + s * G::T;
+        const size_t pt = (size_t)l * ldk + k;
+        const size_t o = zoff + pt;
+        const double ll = P.ll[l];
+        const cd S = {-kk * xa[s].y + -ll * xb[s].y, kk * xa[s].x + ll * xb[s].x};
+        // ph from the old qh of every layer (layer 0 was already replaced in Q
+        // when z == 1; its old value is kept in Q0old)
+        cd qs[NZ];
+#pragma unroll
+        for (int k1 = 0; k1 < NZ; ++k1) qs[k1] = Q[k1][s];
+        if constexpr (NZ == 2) {
+          if (z == 1) qs[0] = Q0old[s];
+        }
+        const cd ph = invert_regs<NZ>(P, z, qs, plane, pt);
+        cd d = {S.x + -kq * ph.y, S.y + kq * ph.x};
+        d = {-d.x, -d.y};
+        if (drag) {
+          const double cr = P.rek * P.k2l2[pt];
+          d = {d.x + cr * ph.x, d.y + cr * ph.y};
+        }
+        const double f = P.filtr[pt];
+        const cd q = Q[z][s], pv = p[o], ppv = pp_to_newp[o];
+        double rx = q.x + cf.dt1 * d.x;
+        double ry = q.y + cf.dt1 * d.y;
+        rx = rx + cf.dt2 * pv.x;
+        ry = ry + cf.dt2 * pv.y;
+        rx = rx + cf.dt3 * ppv.x;
+        ry = ry + cf.dt3 * ppv.y;
+        const cd qn = {f * rx, f * ry};
+        qh_new[o] = qn;
+        pp_to_newp[o] = d;
+        if constexpr (NZ == 2) {
+          if (z == 0) Q0old[s] = q;
+        }
+        Q[z][s] = qn;
+      }
+    }
+  }
+  col_prologue_regs<NY, NZ, G::E>(Q, Qm, Um, Vm, P, nk, ldk, t, c, k, ok, mbase, lds, tw);
+}
+
+// ==========================================================================
+// Learned-closure support: coarse-graining between grids, NHWC packing
+// ==========================================================================
+
+// lo[m][z][l][k] = s * G[l][k] * hi[m][z][l'][k] for the modes both grids
+// resolve (|l| < nlo/2, k < nlo/2); the coarse grid's Nyquist row and column
+// are set to zero.  s = (nlo/nhi)^2 keeps physical amplitudes (the transforms
+// are unnormalized sums over the grid).
+__global__ void k_coarsen(const cd* __restrict__ hi, cd* __restrict__ lo,
+                          const double* __restrict__ G, int nplanes, int nhi, int nlo,
+                          double s) {
+  const int nkh = ldk_of(nhi), nkl = ldk_of(nlo);
+  const size_t total = (size_t)nplanes * nlo * nkl;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(i % nkl);
+    const int l = (int)((i / nkl) % nlo);
+    const size_t pl = i / ((size_t)nkl * nlo);
+    cd v = {0.0, 0.0};
+    if (k < nlo / 2 && l != nlo / 2) {
+      const int lw = l < nlo / 2 ? l : l - nlo;
+      const int lh = lw >= 0 ? lw : lw + nhi;
+      const cd h = hi[(pl * nhi + lh) * nkh + k];
+      const double g = G ? G[(size_t)l * nkl + k] * s : s;
+      v = {h.x * g, h.y * g};
+    }
+    lo[i] = v;
+  }
+}
+
+// a - b (spectral), into out
+__global__ void k_csub(const cd* __restrict__ a, const cd* __restrict__ b, cd* __restrict__ out,
+                       size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = {a[i].x - b[i].x, a[i].y - b[i].y};
+}
+
+// NHWC BF16 network input from real fields: channel c = f * nz + z of field f
+// (f over q, u, v) scaled by inv_scale[c]; channels cin..cinp-1 are zero.
+__global__ void k_pack_nhwc(const double* __restrict__ q, const double* __restrict__ u,
+                            const double* __restrict__ v, const double* __restrict__ inv_scale,
+                            unsigned short* __restrict__ out, int M, int nz, int npix, int cinp) {
+  const size_t total = (size_t)M * npix * cinp;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % cinp);
+    const size_t mp = i / cinp;
+    const int p = (int)(mp % npix);
+    const int m = (int)(mp / npix);
+    float val = 0.f;
+    if (c < 3 * nz) {
+      const int f = c / nz, z = c % nz;
+      const double* src = f == 0 ? q : (f == 1 ? u : v);
+      val = (float)(src[((size_t)m * nz + z) * npix + p] * inv_scale[c]);
+    }
+    unsigned int bits = __float_as_uint(val);
+    bits += 0x7fffu + ((bits >> 16) & 1u);
+    out[i] = (unsigned short)(bits >> 16);
+  }
+}
+
+// dq[m][z][p] = net[m][p][z] * scale[z] (network output, FP32 NHWC, coutp channels)
+__global__ void k_unpack_forcing(const float* __restrict__ net, const double* __restrict__ scale,
+                                 double* __restrict__ dq, int M, int nz, int npix, int coutp) {
+  const size_t total = (size_t)M * nz * npix;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int p = (int)(i % npix);
+    const int z = (int)((i / npix) % nz);
+    const int m = (int)(i / ((size_t)npix * nz));
+    dq[i] = (double)net[((size_t)m * npix + p) * coutp + z] * scale[z];
+  }
+}
+
+// Plain streaming copy (16 B per lane), the calibration kernel for HBM traffic
+// counters and the measured copy bandwidth.
+__global__ void k_stream_copy(const double2* __restrict__ in, double2* __restrict__ out,
+                              size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = in[i];
 }
 
 // ==========================================================================
@@ -634,6 +932,7 @@ bool pow2_in_range(int n) { return n >= 8 && n <= 1024 && (n & (n - 1)) == 0; }
 
 struct qgk_bucket {
   int M = 0, nz = 0, ny = 0, nx = 0, nl = 0, nk = 0, flags = 0;
+  int ldk = 0;  // row stride of spectral arrays (nk padded to a multiple of 8)
   size_t nplane_c = 0, nplane_r = 0;  // points per spectral / real plane
   hipStream_t stream = nullptr;
   std::vector<void*> allocs;
@@ -641,6 +940,7 @@ struct qgk_bucket {
   cd* field_c[QGK_NFIELDS] = {};
   double* field_r[QGK_NFIELDS] = {};
   cd* hist[3] = {};  // rotating: hist[iD] dqhdt, hist[iP] dqhdt_p, hist[iPP] dqhdt_pp
+  cd* qh_alt = nullptr;  // the fused step writes qh here, then the two swap
   int iD = 0, iP = 1, iPP = 2;
   cd *Qm = nullptr, *Um = nullptr, *Vm = nullptr, *UQm = nullptr, *VQm = nullptr;
   cd* ctmp = nullptr;      // spectral scratch (M*nz planes)
@@ -669,6 +969,9 @@ struct qgk_bucket {
   hipGraphExec_t gexec[3] = {};
   StepCoeffs gcoef = {0, 0, 0};
   bool gvalid = false;
+  // register radix of the fused row and column kernels (autotuned)
+  int erow = 8, ecol = 8;
+  double* small = nullptr;  // 256 doubles of per-call constants (closure scales)
 };
 
 namespace {
@@ -691,7 +994,7 @@ bool is_real_field(int f) {
 
 size_t const_len(const qgk_bucket* b, int which) {
   switch (which) {
-    case QGK_KK: return b->nk;
+    case QGK_KK: return b->ldk;
     case QGK_LL: return b->nl;
     case QGK_K2L2:
     case QGK_FILTR: return b->nplane_c;
@@ -775,7 +1078,7 @@ int launch_col_fft(qgk_bucket* b, const cd* in, cd* out, int nplanes) {
     using G = Geo<NY>;                                                                  \
     dim3 grid((nk + G::TK - 1) / G::TK, nplanes);                                       \
     hipLaunchKernelGGL((k_col_fft<NY, SIGN>), grid, dim3(G::T * G::TK), 0, b->stream, in, \
-                       out, nk, b->tw_y);                                               \
+                       out, nk, b->ldk, b->tw_y);                                       \
     break;                                                                              \
   }
   switch (ny) {
@@ -844,65 +1147,94 @@ int c2r2d(qgk_bucket* b, const cd* in, double* out, int nplanes) {
   return launch_row_c2r(b, b->ctmp, out, nplanes);
 }
 
+// Calls L::go<N, E>(args...) for a power-of-two length n and register radix e
+// (4, 8 or 16, clamped to n).
+template <class L, class... A>
+int with_ne(int n, int e, A... a) {
+#define QGK_NE(N)                                                     \
+  case N:                                                             \
+    switch (e) {                                                      \
+      case 4: L::template go<N, (4 < N ? 4 : N)>(a...); return 0;     \
+      case 8: L::template go<N, (8 < N ? 8 : N)>(a...); return 0;     \
+      case 16: L::template go<N, (16 < N ? 16 : N)>(a...); return 0;  \
+      default: return fail(QGK_E_ARG, "unsupported radix %d", e);     \
+    }
+  switch (n) {
+    QGK_NE(8) QGK_NE(16) QGK_NE(32) QGK_NE(64) QGK_NE(128) QGK_NE(256) QGK_NE(512)
+    QGK_NE(1024)
+    default: return fail(QGK_E_ARG, "unsupported transform length %d", n);
+  }
+#undef QGK_NE
+}
+
+struct RowFluxL {
+  template <int NX, int E>
+  static void go(qgk_bucket* b) {
+    using G = FGeo<NX, E>;
+    const double scale = 1.0 / ((double)b->nx * (double)b->ny);
+    dim3 grid((b->ny / 2 + G::RP - 1) / G::RP, b->M * b->nz);
+    hipLaunchKernelGGL((k_row_flux<NX, E>), grid, dim3(G::T * G::RP), 0, b->stream, b->Qm,
+                       b->Um, b->Vm, b->UQm, b->VQm, b->dp, b->nz, b->ny, scale, b->tw_x);
+  }
+};
+
+struct ColPrologueL {
+  template <int NY, int E>
+  static void go(qgk_bucket* b) {
+    using G = FGeo<NY, E>;
+    dim3 grid((b->nk + G::TK - 1) / G::TK, b->M), block(G::T * G::TK);
+    const cd* qh = b->field_c[QGK_QH];
+    if (b->nz == 1)
+      hipLaunchKernelGGL((k_col_prologue_reg<NY, 1, E>), grid, block, 0, b->stream, qh, b->Qm,
+                         b->Um, b->Vm, b->dp, b->nk, b->ldk, b->tw_y);
+    else if (b->nz == 2)
+      hipLaunchKernelGGL((k_col_prologue_reg<NY, 2, E>), grid, block, 0, b->stream, qh, b->Qm,
+                         b->Um, b->Vm, b->dp, b->nk, b->ldk, b->tw_y);
+    else
+      hipLaunchKernelGGL((k_col_prologue<NY, E>), grid, block, 0, b->stream, qh, b->Qm, b->Um,
+                         b->Vm, b->dp, b->nz, b->nk, b->ldk, b->tw_y);
+  }
+};
+
+struct ColStepL {
+  template <int NY, int E>
+  static void go(qgk_bucket* b, StepCoeffs cf) {
+    using G = FGeo<NY, E>;
+    dim3 grid((b->nk + G::TK - 1) / G::TK, b->M), block(G::T * G::TK);
+    const cd* qo = b->field_c[QGK_QH];
+    cd* qn = b->qh_alt;
+    if (b->nz == 1)
+      hipLaunchKernelGGL((k_col_step_reg<NY, 1, E>), grid, block, 0, b->stream, qo, qn, Pp(b),
+                         PP(b), b->UQm, b->VQm, b->Qm, b->Um, b->Vm, b->dp, cf, b->nk, b->ldk,
+                         b->tw_y);
+    else if (b->nz == 2)
+      hipLaunchKernelGGL((k_col_step_reg<NY, 2, E>), grid, block, 0, b->stream, qo, qn, Pp(b),
+                         PP(b), b->UQm, b->VQm, b->Qm, b->Um, b->Vm, b->dp, cf, b->nk, b->ldk,
+                         b->tw_y);
+    else
+      hipLaunchKernelGGL((k_col_step<NY, E>), grid, block, 0, b->stream, qo, qn, Pp(b), PP(b),
+                         b->UQm, b->VQm, b->Qm, b->Um, b->Vm, b->dp, cf, b->nz, b->nk, b->ldk,
+                         b->tw_y);
+  }
+};
+
 int launch_row_flux(qgk_bucket* b) {
-  const int nx = b->nx, ny = b->ny;
-  const double scale = 1.0 / ((double)nx * (double)ny);
-  const int nplanes = b->M * b->nz;
-#define QGK_ROW(NX)                                                                         \
-  case NX: {                                                                                \
-    using G = Geo<NX>;                                                                      \
-    dim3 grid((ny / 2 + G::RP - 1) / G::RP, nplanes);                                       \
-    hipLaunchKernelGGL((k_row_flux<NX>), grid, dim3(G::T * G::RP), 0, b->stream, b->Qm, b->Um, \
-                       b->Vm, b->UQm, b->VQm, b->dp, b->nz, ny, scale, b->tw_x);             \
-    break;                                                                                  \
-  }
-  switch (nx) {
-    QGK_ROW(8) QGK_ROW(16) QGK_ROW(32) QGK_ROW(64) QGK_ROW(128) QGK_ROW(256) QGK_ROW(512)
-    QGK_ROW(1024)
-    default: return fail(QGK_E_ARG, "unsupported nx=%d", nx);
-  }
-#undef QGK_ROW
+  int rc = with_ne<RowFluxL>(b->nx, b->erow, b);
+  if (rc) return rc;
   HIPC(hipGetLastError());
   return 0;
 }
 
 int launch_col_prologue(qgk_bucket* b) {
-  const int ny = b->ny, nk = b->nk;
-#define QGK_COL(NY)                                                                      \
-  case NY: {                                                                             \
-    using G = Geo<NY>;                                                                   \
-    dim3 grid((nk + G::TK - 1) / G::TK, b->M);                                           \
-    hipLaunchKernelGGL((k_col_prologue<NY>), grid, dim3(G::T * G::TK), 0, b->stream,       \
-                       b->field_c[QGK_QH], b->Qm, b->Um, b->Vm, b->dp, b->nz, nk, b->tw_y); \
-    break;                                                                               \
-  }
-  switch (ny) {
-    QGK_COL(8) QGK_COL(16) QGK_COL(32) QGK_COL(64) QGK_COL(128) QGK_COL(256) QGK_COL(512)
-    QGK_COL(1024)
-    default: return fail(QGK_E_ARG, "unsupported ny=%d", ny);
-  }
-#undef QGK_COL
+  int rc = with_ne<ColPrologueL>(b->ny, b->ecol, b);
+  if (rc) return rc;
   HIPC(hipGetLastError());
   return 0;
 }
 
 int launch_col_step(qgk_bucket* b, StepCoeffs cf) {
-  const int ny = b->ny, nk = b->nk;
-#define QGK_COL(NY)                                                                         \
-  case NY: {                                                                                \
-    using G = Geo<NY>;                                                                      \
-    dim3 grid((nk + G::TK - 1) / G::TK, b->M);                                              \
-    hipLaunchKernelGGL((k_col_step<NY>), grid, dim3(G::T * G::TK), 0, b->stream,              \
-                       b->field_c[QGK_QH], D(b), Pp(b), PP(b), b->UQm, b->VQm, b->Qm, b->Um,  \
-                       b->Vm, b->dp, cf, b->nz, nk, b->tw_y);                               \
-    break;                                                                                  \
-  }
-  switch (ny) {
-    QGK_COL(8) QGK_COL(16) QGK_COL(32) QGK_COL(64) QGK_COL(128) QGK_COL(256) QGK_COL(512)
-    QGK_COL(1024)
-    default: return fail(QGK_E_ARG, "unsupported ny=%d", ny);
-  }
-#undef QGK_COL
+  int rc = with_ne<ColStepL>(b->ny, b->ecol, b, cf);
+  if (rc) return rc;
   HIPC(hipGetLastError());
   return 0;
 }
@@ -930,6 +1262,7 @@ int fused_step(qgk_bucket* b, StepCoeffs cf) {
   rc = t_end(b, e0, 1);
   if (rc) return rc;
   rotate_history(b);
+  std::swap(b->field_c[QGK_QH], b->qh_alt);
   return 0;
 }
 
@@ -956,6 +1289,37 @@ int refresh_q(qgk_bucket* b) {
   if (rc) return rc;
   b->q_stale = false;
   return 0;
+}
+
+// Host tables are laid out with nk columns; device tables with ldk.  Expand
+// `rows` rows of nk values into rows of ldk values (zero padding).
+std::vector<double> pad_rows(const double* host, size_t rows, int nk, int ldk) {
+  std::vector<double> out(rows * ldk, 0.0);
+  for (size_t r = 0; r < rows; ++r)
+    std::memcpy(out.data() + r * ldk, host + r * nk, nk * sizeof(double));
+  return out;
+}
+
+// rows of an (m members x) table in host layout for constant `which`
+size_t const_rows(const qgk_bucket* b, int which) {
+  switch (which) {
+    case QGK_KK: return 1;
+    case QGK_K2L2:
+    case QGK_FILTR: return b->nl;
+    case QGK_A: return (size_t)b->nz * b->nz * b->nl;
+    default: return 0;  // LL, UBG, QY: not padded
+  }
+}
+
+// spectral copies between a dense host layout [rows][nk] and the padded device one
+hipError_t copy_spectral(qgk_bucket* b, void* dst, const void* src, size_t rows,
+                         hipMemcpyKind kind) {
+  const size_t dense = (size_t)b->nk * sizeof(cd), padded = (size_t)b->ldk * sizeof(cd);
+  const size_t dpitch = kind == hipMemcpyHostToDevice || kind == hipMemcpyDeviceToDevice ?
+                        padded : dense;
+  const size_t spitch = kind == hipMemcpyDeviceToHost || kind == hipMemcpyDeviceToDevice ?
+                        padded : dense;
+  return hipMemcpy2DAsync(dst, dpitch, src, spitch, dense, rows, kind, b->stream);
 }
 
 int need_full(qgk_bucket* b, const char* what) {
@@ -990,8 +1354,9 @@ int qgk_create(int nmembers, int nz, int ny, int nx, int flags, qgk_bucket** out
   b->nx = nx;
   b->nl = ny;
   b->nk = nx / 2 + 1;
+  b->ldk = ldk_of(nx);
   b->flags = flags;
-  b->nplane_c = (size_t)b->nl * b->nk;
+  b->nplane_c = (size_t)b->nl * b->ldk;
   b->nplane_r = (size_t)ny * nx;
   int rc = 0;
   auto cleanup = [&](int code) {
@@ -1004,7 +1369,7 @@ int qgk_create(int nmembers, int nz, int ny, int nx, int flags, qgk_bucket** out
   const size_t nr = (size_t)nmembers * nz * b->nplane_r * sizeof(double);
   auto ac = [&](cd** p) { return dev_alloc(b, (void**)p, nc); };
   auto ar = [&](double** p) { return dev_alloc(b, (void**)p, nr); };
-  if ((rc = ac(&b->field_c[QGK_QH])) || (rc = ac(&b->hist[0])) || (rc = ac(&b->hist[1])) ||
+  if ((rc = ac(&b->field_c[QGK_QH])) || (rc = ac(&b->qh_alt)) || (rc = ac(&b->hist[0])) || (rc = ac(&b->hist[1])) ||
       (rc = ac(&b->hist[2])) || (rc = ac(&b->Qm)) || (rc = ac(&b->Um)) || (rc = ac(&b->Vm)) ||
       (rc = ac(&b->UQm)) || (rc = ac(&b->VQm)) || (rc = ac(&b->ctmp)) || (rc = ar(&b->rtmp)) ||
       (rc = ar(&b->field_r[QGK_Q])))
@@ -1058,6 +1423,7 @@ int qgk_create(int nmembers, int nz, int ny, int nx, int flags, qgk_bucket** out
     p.rek = 0.0;
   }
   if ((rc = dev_alloc(b, (void**)&b->dp, sizeof(MemberParams) * nmembers))) return cleanup(rc);
+  if ((rc = dev_alloc(b, (void**)&b->small, 256 * sizeof(double)))) return cleanup(rc);
   b->params_dirty = true;
   if (hipStreamSynchronize(b->stream) != hipSuccess)
     return cleanup(fail(QGK_E_HIP, "stream sync failed"));
@@ -1100,8 +1466,15 @@ int qgk_set_const(qgk_bucket* b, int which, int member, const double* host) {
   }
   if (which == QGK_A) b->gvalid = false;
   b->params_dirty = true;
-  HIPC(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, b->stream));
-  HIPC(hipStreamSynchronize(b->stream));
+  const size_t rows = const_rows(b, which);
+  if (rows) {
+    std::vector<double> tab = pad_rows(host, rows, b->nk, b->ldk);
+    HIPC(hipMemcpyAsync(dst, tab.data(), bytes, hipMemcpyHostToDevice, b->stream));
+    HIPC(hipStreamSynchronize(b->stream));
+  } else {
+    HIPC(hipMemcpyAsync(dst, host, bytes, hipMemcpyHostToDevice, b->stream));
+    HIPC(hipStreamSynchronize(b->stream));
+  }
   b->primed = false;
   return 0;
 }
@@ -1113,7 +1486,14 @@ int qgk_set_const_all(qgk_bucket* b, int which, const double* host) {
   double* tab = nullptr;
   int rc = dev_alloc(b, (void**)&tab, n * b->M * sizeof(double));
   if (rc) return rc;
-  HIPC(hipMemcpyAsync(tab, host, n * b->M * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  const size_t rows = const_rows(b, which);
+  std::vector<double> buf;
+  const double* src = host;
+  if (rows) {
+    buf = pad_rows(host, rows * b->M, b->nk, b->ldk);
+    src = buf.data();
+  }
+  HIPC(hipMemcpyAsync(tab, src, n * b->M * sizeof(double), hipMemcpyHostToDevice, b->stream));
   HIPC(hipStreamSynchronize(b->stream));
   for (int m = 0; m < b->M; ++m) param_slot(b->hp[m], which) = tab + (size_t)m * n;
   b->params_dirty = true;
@@ -1156,8 +1536,7 @@ int qgk_put(qgk_bucket* b, int field, const void* host) {
     HIPC(hipMemcpyAsync(b->field_r[field], host, n * b->nplane_r * sizeof(double),
                         hipMemcpyHostToDevice, b->stream));
   } else {
-    HIPC(hipMemcpyAsync(spectral_ptr(b, field), host, n * b->nplane_c * sizeof(cd),
-                        hipMemcpyHostToDevice, b->stream));
+    HIPC(copy_spectral(b, spectral_ptr(b, field), host, n * b->nl, hipMemcpyHostToDevice));
   }
   HIPC(hipStreamSynchronize(b->stream));
   if (field == QGK_QH) b->primed = false;
@@ -1175,8 +1554,7 @@ int qgk_get(qgk_bucket* b, int field, void* host) {
     HIPC(hipMemcpyAsync(host, b->field_r[field], n * b->nplane_r * sizeof(double),
                         hipMemcpyDeviceToHost, b->stream));
   } else {
-    HIPC(hipMemcpyAsync(host, spectral_ptr(b, field), n * b->nplane_c * sizeof(cd),
-                        hipMemcpyDeviceToHost, b->stream));
+    HIPC(copy_spectral(b, host, spectral_ptr(b, field), n * b->nl, hipMemcpyDeviceToHost));
   }
   HIPC(hipStreamSynchronize(b->stream));
   return 0;
@@ -1189,7 +1567,7 @@ int qgk_invert(qgk_bucket* b) {
   const size_t pts = (size_t)b->M * b->nplane_c;
   hipLaunchKernelGGL(k_invert, dim3(grid_1d(pts)), dim3(256), 0, b->stream, b->field_c[QGK_QH],
                      b->field_c[QGK_PH], b->field_c[QGK_UH], b->field_c[QGK_VH], b->dp, b->nz,
-                     b->nl, b->nk, b->M);
+                     b->nl, b->ldk, b->M);
   HIPC(hipGetLastError());
   const int np = b->M * b->nz;
   if ((rc = c2r2d(b, b->field_c[QGK_UH], b->field_r[QGK_U], np))) return rc;
@@ -1213,7 +1591,7 @@ int qgk_advect(qgk_bucket* b) {
   const size_t cpts = (size_t)np * b->nplane_c;
   hipLaunchKernelGGL(k_divergence, dim3(grid_1d(cpts)), dim3(256), 0, b->stream,
                      b->field_c[QGK_UQH], b->field_c[QGK_VQH], b->field_c[QGK_PH], D(b), b->dp,
-                     b->nz, b->nl, b->nk, b->M);
+                     b->nz, b->nl, b->ldk, b->M);
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(b->stream));
   return 0;
@@ -1225,7 +1603,7 @@ int qgk_friction(qgk_bucket* b) {
   if (rc || (rc = upload_params(b))) return rc;
   const size_t pts = (size_t)b->M * b->nplane_c;
   hipLaunchKernelGGL(k_friction, dim3(grid_1d(pts)), dim3(256), 0, b->stream,
-                     b->field_c[QGK_PH], D(b), b->dp, b->nz, b->nl, b->nk, b->M);
+                     b->field_c[QGK_PH], D(b), b->dp, b->nz, b->nl, b->ldk, b->M);
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(b->stream));
   return 0;
@@ -1261,7 +1639,7 @@ int qgk_add_uv_param(qgk_bucket* b, const double* du_host, const double* dv_host
   if ((rc = r2c2d(b, b->field_r[QGK_DV], b->field_c[QGK_DVH], np))) return rc;
   const size_t cpts = (size_t)np * b->nplane_c;
   hipLaunchKernelGGL(k_add_curl, dim3(grid_1d(cpts)), dim3(256), 0, b->stream, D(b),
-                     b->field_c[QGK_DUH], b->field_c[QGK_DVH], b->dp, b->nz, b->nl, b->nk, b->M);
+                     b->field_c[QGK_DUH], b->field_c[QGK_DVH], b->dp, b->nz, b->nl, b->ldk, b->M);
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(b->stream));
   return 0;
@@ -1275,7 +1653,7 @@ int qgk_timestep(qgk_bucket* b, double dt1, double dt2, double dt3) {
   const size_t cpts = (size_t)np * b->nplane_c;
   StepCoeffs c = {dt1, dt2, dt3};
   hipLaunchKernelGGL(k_ab_step, dim3(grid_1d(cpts)), dim3(256), 0, b->stream,
-                     b->field_c[QGK_QH], D(b), Pp(b), PP(b), b->dp, c, b->nz, b->nl, b->nk,
+                     b->field_c[QGK_QH], D(b), Pp(b), PP(b), b->dp, c, b->nz, b->nl, b->ldk,
                      b->M);
   HIPC(hipGetLastError());
   rotate_history(b);
@@ -1317,8 +1695,7 @@ int qgk_fft(qgk_bucket* b, int nplanes, const double* in_real, double* out_compl
   int rc = launch_row_r2c(b, b->rtmp, b->ctmp, nplanes);
   if (rc) return rc;
   if ((rc = launch_col_fft<-1>(b, b->ctmp, b->ctmp, nplanes))) return rc;
-  HIPC(hipMemcpyAsync(out_complex, b->ctmp, (size_t)nplanes * b->nplane_c * sizeof(cd),
-                      hipMemcpyDeviceToHost, b->stream));
+  HIPC(copy_spectral(b, out_complex, b->ctmp, (size_t)nplanes * b->nl, hipMemcpyDeviceToHost));
   HIPC(hipStreamSynchronize(b->stream));
   return 0;
 }
@@ -1327,8 +1704,7 @@ int qgk_ifft(qgk_bucket* b, int nplanes, const double* in_complex, double* out_r
   if (!b || !in_complex || !out_real) return fail(QGK_E_ARG, "NULL argument");
   if (nplanes < 1 || nplanes > b->M * b->nz)
     return fail(QGK_E_ARG, "nplanes must be in [1, %d]", b->M * b->nz);
-  HIPC(hipMemcpyAsync(b->ctmp, in_complex, (size_t)nplanes * b->nplane_c * sizeof(cd),
-                      hipMemcpyHostToDevice, b->stream));
+  HIPC(copy_spectral(b, b->ctmp, in_complex, (size_t)nplanes * b->nl, hipMemcpyHostToDevice));
   int rc = launch_col_fft<+1>(b, b->ctmp, b->ctmp, nplanes);
   if (rc) return rc;
   if ((rc = launch_row_c2r(b, b->ctmp, b->rtmp, nplanes))) return rc;
@@ -1391,6 +1767,282 @@ int qgk_step_buckets(qgk_bucket** bs, int nb, int nsteps, const double* coeffs, 
     if (nsteps > 0) bs[i]->q_stale = true;
     HIPC(hipStreamSynchronize(bs[i]->stream));
   }
+  return 0;
+}
+
+int qgk_copy_probe(size_t nbytes, int iters, double* ms_per_copy) {
+  if (nbytes < 16 || iters < 1 || !ms_per_copy) return fail(QGK_E_ARG, "bad arguments");
+  const size_t n = nbytes / 16;
+  double2 *a = nullptr, *c = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  if (hipMalloc((void**)&a, n * 16) != hipSuccess || hipMalloc((void**)&c, n * 16) != hipSuccess ||
+      hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(QGK_E_HIP, "copy probe set-up failed");
+  } else {
+    hipMemsetAsync(a, 0, n * 16, st);
+    const int blocks = 256 * 8 * 4;
+    hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, st, a, c, n);
+    hipEventRecord(e0, st);
+    for (int i = 0; i < iters; ++i)
+      hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, st, a, c, n);
+    hipEventRecord(e1, st);
+    float ms = 0;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = fail(QGK_E_HIP, "copy probe timing failed");
+    *ms_per_copy = ms / iters;
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st) hipStreamDestroy(st);
+  if (a) hipFree(a);
+  if (c) hipFree(c);
+  return rc;
+}
+
+int qgk_set_variant(qgk_bucket* b, int erow, int ecol) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  auto okr = [](int e) { return e == 4 || e == 8 || e == 16; };
+  if (!okr(erow) || !okr(ecol)) return fail(QGK_E_ARG, "radix must be 4, 8 or 16");
+  b->erow = erow;
+  b->ecol = ecol;
+  b->gvalid = false;
+  return 0;
+}
+
+int qgk_get_variant(qgk_bucket* b, int* erow, int* ecol) {
+  if (!b || !erow || !ecol) return fail(QGK_E_ARG, "NULL argument");
+  *erow = b->erow;
+  *ecol = b->ecol;
+  return 0;
+}
+
+int qgk_autotune(qgk_bucket* b, int reps, int* erow, int* ecol, double* ms_row3,
+                 double* ms_col3) {
+  if (!b || reps < 1 || !erow || !ecol) return fail(QGK_E_ARG, "bad arguments");
+  int rc = upload_params(b);
+  if (rc) return rc;
+  const size_t bytes = (size_t)b->M * b->nz * b->nplane_c * sizeof(cd);
+  cd* save = nullptr;
+  HIPC(hipMalloc((void**)&save, 3 * bytes));
+  HIPC(hipMemcpyAsync(save, b->field_c[QGK_QH], bytes, hipMemcpyDeviceToDevice, b->stream));
+  HIPC(hipMemcpyAsync(save + bytes / sizeof(cd), Pp(b), bytes, hipMemcpyDeviceToDevice,
+                      b->stream));
+  HIPC(hipMemcpyAsync(save + 2 * (bytes / sizeof(cd)), PP(b), bytes, hipMemcpyDeviceToDevice,
+                      b->stream));
+  const int cand[3] = {4, 8, 16};
+  double best_r = 1e30, best_c = 1e30;
+  int br = b->erow, bc = b->ecol;
+  const StepCoeffs zero = {0.0, 0.0, 0.0};  // qh <- filtr qh: timing only
+  hipEvent_t e0, e1;
+  HIPC(hipEventCreate(&e0));
+  HIPC(hipEventCreate(&e1));
+  const int keep_r = b->erow, keep_c = b->ecol;
+  for (int i = 0; i < 3 && rc == 0; ++i) {
+    const int e = cand[i];
+    const bool valid_r = e <= b->nx && !(i > 0 && cand[i - 1] >= b->nx);
+    const bool valid_c = e <= b->ny && !(i > 0 && cand[i - 1] >= b->ny);
+    float ms = 0;
+    // row kernel
+    b->erow = e;
+    if (valid_r) {
+      if ((rc = launch_row_flux(b))) break;
+      HIPC(hipEventRecord(e0, b->stream));
+      for (int r = 0; r < reps && rc == 0; ++r) rc = launch_row_flux(b);
+      HIPC(hipEventRecord(e1, b->stream));
+      HIPC(hipEventSynchronize(e1));
+      HIPC(hipEventElapsedTime(&ms, e0, e1));
+      if (ms_row3) ms_row3[i] = ms / reps;
+      if (ms / reps < best_r) {
+        best_r = ms / reps;
+        br = e;
+      }
+    } else if (ms_row3) {
+      ms_row3[i] = -1.0;
+    }
+    // column kernel (writes qh_alt and dqhdt_pp; restored below)
+    b->ecol = e;
+    if (valid_c) {
+      if ((rc = launch_col_step(b, zero))) break;
+      HIPC(hipEventRecord(e0, b->stream));
+      for (int r = 0; r < reps && rc == 0; ++r) rc = launch_col_step(b, zero);
+      HIPC(hipEventRecord(e1, b->stream));
+      HIPC(hipEventSynchronize(e1));
+      HIPC(hipEventElapsedTime(&ms, e0, e1));
+      if (ms_col3) ms_col3[i] = ms / reps;
+      if (ms / reps < best_c) {
+        best_c = ms / reps;
+        bc = e;
+      }
+    } else if (ms_col3) {
+      ms_col3[i] = -1.0;
+    }
+  }
+  b->erow = keep_r;
+  b->ecol = keep_c;
+  HIPC(hipMemcpyAsync(b->field_c[QGK_QH], save, bytes, hipMemcpyDeviceToDevice, b->stream));
+  HIPC(hipMemcpyAsync(Pp(b), save + bytes / sizeof(cd), bytes, hipMemcpyDeviceToDevice,
+                      b->stream));
+  HIPC(hipMemcpyAsync(PP(b), save + 2 * (bytes / sizeof(cd)), bytes, hipMemcpyDeviceToDevice,
+                      b->stream));
+  HIPC(hipStreamSynchronize(b->stream));
+  hipFree(save);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  b->erow = br;
+  b->ecol = bc;
+  b->primed = false;  // Qm/Um/Vm were overwritten by the trials
+  b->gvalid = false;
+  *erow = br;
+  *ecol = bc;
+  return 0;
+}
+
+int qgk_device_name(char* buf, int len) {
+  if (!buf || len < 1) return fail(QGK_E_ARG, "bad buffer");
+  hipDeviceProp_t prop;
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  HIPC(hipGetDeviceProperties(&prop, dev));
+  snprintf(buf, len, "%s:%s:%d", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+  return 0;
+}
+
+int qgk_put_device(qgk_bucket* b, int field, const void* dptr) {
+  if (!b || !dptr) return fail(QGK_E_ARG, "NULL argument");
+  int rc = check_field(b, field);
+  if (rc) return rc;
+  const size_t n = (size_t)b->M * b->nz;
+  if (is_real_field(field)) {
+    HIPC(hipMemcpyAsync(b->field_r[field], dptr, n * b->nplane_r * sizeof(double),
+                        hipMemcpyDeviceToDevice, b->stream));
+  } else {
+    HIPC(hipMemcpy2DAsync(spectral_ptr(b, field), b->ldk * sizeof(cd), dptr, b->nk * sizeof(cd),
+                          b->nk * sizeof(cd), n * b->nl, hipMemcpyDeviceToDevice, b->stream));
+  }
+  HIPC(hipStreamSynchronize(b->stream));
+  if (field == QGK_QH) b->primed = false;
+  if (field == QGK_Q) b->q_stale = false;
+  return 0;
+}
+
+int qgk_get_device(qgk_bucket* b, int field, void* dptr) {
+  if (!b || !dptr) return fail(QGK_E_ARG, "NULL argument");
+  int rc = check_field(b, field);
+  if (rc) return rc;
+  if (field == QGK_Q && (rc = refresh_q(b))) return rc;
+  const size_t n = (size_t)b->M * b->nz;
+  if (is_real_field(field)) {
+    HIPC(hipMemcpyAsync(dptr, b->field_r[field], n * b->nplane_r * sizeof(double),
+                        hipMemcpyDeviceToDevice, b->stream));
+  } else {
+    HIPC(hipMemcpy2DAsync(dptr, b->nk * sizeof(cd), spectral_ptr(b, field), b->ldk * sizeof(cd),
+                          b->nk * sizeof(cd), n * b->nl, hipMemcpyDeviceToDevice, b->stream));
+  }
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_coarsen(qgk_bucket* hi, qgk_bucket* lo, int field_hi, int field_lo,
+                const double* filter_lo) {
+  if (!hi || !lo) return fail(QGK_E_ARG, "NULL handle");
+  if (hi->M != lo->M || hi->nz != lo->nz || hi->nx != hi->ny || lo->nx != lo->ny ||
+      lo->nx > hi->nx)
+    return fail(QGK_E_ARG, "coarsening needs equal members/layers and square grids, lo <= hi");
+  if (is_real_field(field_hi) || is_real_field(field_lo))
+    return fail(QGK_E_ARG, "coarsening acts on spectral fields");
+  int rc = check_field(hi, field_hi);
+  if (rc || (rc = check_field(lo, field_lo))) return rc;
+  double* G = nullptr;
+  std::vector<double> gpad;
+  if (filter_lo) {
+    gpad = pad_rows(filter_lo, lo->nl, lo->nk, lo->ldk);
+    HIPC(hipMalloc((void**)&G, lo->nplane_c * sizeof(double)));
+    HIPC(hipMemcpyAsync(G, gpad.data(), lo->nplane_c * sizeof(double), hipMemcpyHostToDevice,
+                        lo->stream));
+  }
+  HIPC(hipStreamSynchronize(hi->stream));
+  const double s = ((double)lo->nx * lo->ny) / ((double)hi->nx * hi->ny);
+  const size_t n = (size_t)lo->M * lo->nz * lo->nplane_c;
+  hipLaunchKernelGGL(k_coarsen, dim3(grid_1d(n)), dim3(256), 0, lo->stream,
+                     spectral_ptr(hi, field_hi), spectral_ptr(lo, field_lo), G, lo->M * lo->nz,
+                     hi->nx, lo->nx, s);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(lo->stream));
+  if (G) hipFree(G);
+  if (field_lo == QGK_QH) {
+    lo->primed = false;
+    lo->q_stale = true;
+  }
+  return 0;
+}
+
+int qgk_subgrid_forcing(qgk_bucket* hi, qgk_bucket* lo, const double* filter_lo) {
+  if (!hi || !lo) return fail(QGK_E_ARG, "NULL handle");
+  if (!(hi->flags & QGK_FULL_STATE) || !(lo->flags & QGK_FULL_STATE) || !lo->field_r[QGK_DQ])
+    return fail(QGK_E_STATE,
+                "subgrid forcing needs full-state handles and dq on the coarse one");
+  int rc;
+  // high-resolution tendency (advection + beta; friction is linear and cancels)
+  if ((rc = qgk_invert(hi)) || (rc = qgk_advect(hi))) return rc;
+  // coarse state and its own tendency
+  if ((rc = qgk_coarsen(hi, lo, QGK_QH, QGK_QH, filter_lo)) || (rc = refresh_q(lo)) ||
+      (rc = qgk_invert(lo)) || (rc = qgk_advect(lo)))
+    return rc;
+  // filtered high-resolution tendency into dqh, then S_h = C(T_hi) - T_lo(C q)
+  if ((rc = qgk_coarsen(hi, lo, QGK_DQHDT, QGK_DQH, filter_lo))) return rc;
+  const size_t n = (size_t)lo->M * lo->nz * lo->nplane_c;
+  hipLaunchKernelGGL(k_csub, dim3(grid_1d(n)), dim3(256), 0, lo->stream, lo->field_c[QGK_DQH],
+                     D(lo), lo->field_c[QGK_DQH], n);
+  HIPC(hipGetLastError());
+  rc = c2r2d(lo, lo->field_c[QGK_DQH], lo->field_r[QGK_DQ], lo->M * lo->nz);
+  if (rc) return rc;
+  HIPC(hipStreamSynchronize(lo->stream));
+  return 0;
+}
+
+int qgk_pack_inputs(qgk_bucket* b, const double* inv_scale, int cinp, void* dst) {
+  if (!b || !inv_scale || !dst) return fail(QGK_E_ARG, "NULL argument");
+  if (!(b->flags & QGK_FULL_STATE)) return fail(QGK_E_STATE, "needs a full-state handle");
+  if (cinp < 3 * b->nz) return fail(QGK_E_ARG, "cinp too small");
+  int rc = refresh_q(b);
+  if (rc) return rc;
+  if (3 * b->nz > 256) return fail(QGK_E_ARG, "too many layers for the packing constants");
+  double* sc = b->small;
+  HIPC(hipMemcpyAsync(sc, inv_scale, 3 * b->nz * sizeof(double), hipMemcpyHostToDevice,
+                      b->stream));
+  const int npix = (int)b->nplane_r;
+  const size_t n = (size_t)b->M * npix * cinp;
+  hipLaunchKernelGGL(k_pack_nhwc, dim3(grid_1d(n)), dim3(256), 0, b->stream, b->field_r[QGK_Q],
+                     b->field_r[QGK_U], b->field_r[QGK_V], sc, (unsigned short*)dst, b->M, b->nz,
+                     npix, cinp);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_add_forcing_device(qgk_bucket* b, const float* net_out, int coutp, const double* scale) {
+  if (!b || !net_out || !scale) return fail(QGK_E_ARG, "NULL argument");
+  if (!b->field_r[QGK_DQ]) return fail(QGK_E_STATE, "dq is not allocated (no q parameterization)");
+  if (coutp < b->nz) return fail(QGK_E_ARG, "coutp too small");
+  if (b->nz > 256) return fail(QGK_E_ARG, "too many layers for the forcing scales");
+  double* sc = b->small;
+  HIPC(hipMemcpyAsync(sc, scale, b->nz * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  const int np = b->M * b->nz;
+  const size_t n = (size_t)np * b->nplane_r;
+  hipLaunchKernelGGL(k_unpack_forcing, dim3(grid_1d(n)), dim3(256), 0, b->stream, net_out, sc,
+                     b->field_r[QGK_DQ], b->M, b->nz, (int)b->nplane_r, coutp);
+  HIPC(hipGetLastError());
+  int rc = r2c2d(b, b->field_r[QGK_DQ], b->field_c[QGK_DQH], np);
+  if (rc) return rc;
+  const size_t cpts = (size_t)np * b->nplane_c;
+  hipLaunchKernelGGL(k_add, dim3(grid_1d(cpts)), dim3(256), 0, b->stream, D(b),
+                     b->field_c[QGK_DQH], cpts);
+  HIPC(hipGetLastError());
+  HIPC(hipStreamSynchronize(b->stream));
   return 0;
 }
 

@@ -10,6 +10,9 @@ Each member follows exactly the per-model semantics of the reference
 corresponding QGModel / LayeredModel would build, the same AB start-up and the
 same accumulated time t.
 """
+import json
+import os
+
 import numpy as np
 
 from . import _lib
@@ -50,9 +53,12 @@ class Ensemble:
     # ------------------------------------------------------------ builders
     @classmethod
     def qg(cls, nx, beta=1.5e-11, rd=15000.0, delta=0.25, rek=5.787e-7, L=1e6, H1=500.,
-           U1=0.025, U2=0.0, dt=3600., filterfac=23.6, nmembers=None):
+           U1=0.025, U2=0.0, dt=3600., filterfac=23.6, nmembers=None, full_state=False,
+           q_param=False):
         """Two-layer QGModels (qg_model.py:60-161).  Every argument may be a
-        scalar or one value per member; members are bucketed by nx."""
+        scalar or one value per member; members are bucketed by nx.  full_state
+        keeps every intermediate field (needed by the per-phase operations and
+        the closure pipeline); q_param allocates dq / dqh."""
         if nmembers is None:
             sizes = [np.size(v) for v in (nx, beta, rd, delta, rek, L, U1, U2)]
             nmembers = max(sizes)
@@ -66,7 +72,8 @@ class Ensemble:
         ens.members = [None] * n
         for size in sorted(set(nxs.tolist())):
             idx = np.nonzero(nxs == size)[0]
-            b = Bucket(len(idx), 2, size, size, 0)
+            flags = (_lib.FULL_STATE if full_state else 0) | (_lib.HAS_Q_PARAM if q_param else 0)
+            b = Bucket(len(idx), 2, size, size, flags)
             nk = size // 2 + 1
             kk = np.empty((len(idx), nk))
             ll = np.empty((len(idx), size))
@@ -165,6 +172,41 @@ class Ensemble:
         for _ in range(nsteps):
             self.t += self._dt
         self.tc += nsteps
+
+    def autotune(self, reps=5, cache_path=None):
+        """Pick the fastest register radix of the fused row and column kernels
+        for every bucket (pyqg_amd._lib.Bucket.autotune); results are cached
+        per device and bucket shape in a JSON file (default
+        ~/.cache/pyqg_amd/autotune.json; pass cache_path=False to disable)."""
+        if cache_path is None:
+            cache_path = os.path.join(os.path.expanduser("~"), ".cache", "pyqg_amd",
+                                      "autotune.json")
+        cache = {}
+        if cache_path and os.path.exists(cache_path):
+            try:
+                with open(cache_path) as f:
+                    cache = json.load(f)
+            except (OSError, ValueError):
+                cache = {}
+        dev = _lib.device_name()
+        report = {}
+        for b in self.buckets:
+            key = f"{dev}|nz={b.nz}|ny={b.ny}|nx={b.nx}|M={b.M}"
+            if key in cache:
+                b.set_variant(*cache[key]["variant"])
+                report[key] = cache[key]
+                continue
+            variant, ms_row, ms_col = b.autotune(reps)
+            cache[key] = report[key] = {"variant": list(variant), "ms_row": ms_row,
+                                        "ms_col": ms_col}
+        if cache_path:
+            try:
+                os.makedirs(os.path.dirname(cache_path), exist_ok=True)
+                with open(cache_path, "w") as f:
+                    json.dump(cache, f, indent=1)
+            except OSError:
+                pass
+        return report
 
     def model_days(self, nsteps):
         """Simulated model-days of nsteps steps summed over members."""

@@ -1,0 +1,127 @@
+"""The learned-closure pipeline on the GPU: coarse-graining invariants, the
+subgrid forcing against the oracle's restatement, the device-resident closure
+step against the host-callback path, and what a trained network gains over the
+coarse model without it (offline skill on held-out pairs)."""
+import numpy as np
+import pytest
+
+from conftest import QG_PHYS, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mods(gpu):
+    import torch  # noqa: F401
+    import pyqg_amd
+    from pyqg_amd.closure import cnn, data, param, train
+    return pyqg_amd, cnn, data, param, train
+
+
+def _oracle_forcing(q_hi, n_lo, phys, G):
+    """NumPy restatement: S = C(T_hi(q)) - T_lo(C(q)) with the oracle's tendency."""
+    from oracle.qg_oracle import OracleQG
+    n_hi = q_hi.shape[-1]
+    kw = dict(L=phys["L"], H1=phys["H1"], delta=phys["delta"], U1=phys["U1"], U2=phys["U2"],
+              beta=phys["beta"], rd=phys["rd"], rek=phys["rek"], filterfac=phys["filterfac"])
+    hi = OracleQG(nx=n_hi, **kw)
+    hi.q = q_hi
+    hi._invert()
+    hi._do_advection()
+
+    def coarsen(fh):
+        out = np.zeros((fh.shape[0], n_lo, n_lo // 2 + 1), complex)
+        for l in range(n_lo):
+            if l == n_lo // 2:
+                continue
+            lw = l if l < n_lo // 2 else l - n_lo
+            out[:, l, : n_lo // 2] = fh[:, lw % n_hi, : n_lo // 2]
+        return out * (n_lo / n_hi) ** 2 * G
+
+    lo = OracleQG(nx=n_lo, **kw)
+    lo.qh = coarsen(hi.qh)
+    lo._invert()
+    lo._do_advection()
+    S = coarsen(hi.dqhdt) - lo.dqhdt
+    return lo.q, lo.u, lo.v, np.fft.irfftn(S, axes=(-2, -1))
+
+
+def test_coarsen_is_a_projection_and_commutes_with_inversion(mods):
+    pq = mods[0]
+    from pyqg_amd import _lib
+    hi = pq.Ensemble.qg(np.full(2, 128), full_state=True)
+    lo = pq.Ensemble.qg(np.full(2, 64), full_state=True)
+    rng = np.random.default_rng(0)
+    hi.set_q([rng.standard_normal((2, 128, 128)) for _ in range(2)])
+    hb, lb = hi.buckets[0], lo.buckets[0]
+    _lib.check(_lib.lib.qgk_coarsen(hb.h, lb.h, _lib.FIELD_ID["qh"], _lib.FIELD_ID["qh"], None))
+    once = lb.get("qh")
+    # sharp truncation twice (hi -> lo -> lo) changes nothing: idempotent
+    lo2 = pq.Ensemble.qg(np.full(2, 64), full_state=True)
+    _lib.check(_lib.lib.qgk_coarsen(lb.h, lo2.buckets[0].h, _lib.FIELD_ID["qh"],
+                                    _lib.FIELD_ID["qh"], None))
+    assert np.array_equal(lo2.buckets[0].get("qh"), once)
+    # commutes with the (diagonal-in-k) inversion: C(ph_hi) == a_lo C(qh_hi)
+    _lib.check(_lib.lib.qgk_invert(hb.h))
+    _lib.check(_lib.lib.qgk_invert(lb.h))
+    _lib.check(_lib.lib.qgk_coarsen(hb.h, lo2.buckets[0].h, _lib.FIELD_ID["ph"],
+                                    _lib.FIELD_ID["qh"], None))
+    assert rel_l2(lo2.buckets[0].get("qh"), lb.get("ph")) < 1e-13
+    # Parseval on the retained modes: the coarse field's variance is the
+    # retained part of the fine field's
+    q_lo = lb.get("q")
+    assert np.isfinite(q_lo).all() and q_lo.var() < hi.buckets[0].get("q").var()
+
+
+def test_subgrid_forcing_matches_oracle(mods):
+    pq, _, data, _, _ = mods
+    gen = data.PairGenerator(nmembers=2, n_hi=128, n_lo=64, seed=3)
+    gen.spin_up(30)
+    q_hi = gen.hi.get("q")
+    x, y = gen.sample()
+    x, y = x.cpu().numpy(), y.cpu().numpy()
+    for i in range(2):
+        q, u, v, S = _oracle_forcing(q_hi[i], 64, gen.phys, gen.G)
+        assert rel_l2(x[i, ..., 0:2].transpose(2, 0, 1), q) < 1e-6   # FP32 copies
+        assert rel_l2(x[i, ..., 2:4].transpose(2, 0, 1), u) < 1e-6
+        assert rel_l2(x[i, ..., 4:6].transpose(2, 0, 1), v) < 1e-6
+        assert rel_l2(y[i].transpose(2, 0, 1), S) < 1e-6
+
+
+def test_device_closure_step_equals_host_callback(mods):
+    """The device-resident closure step and pyqg_amd.QGModel with the same
+    network as a host-callback q_parameterization give the same trajectory."""
+    pq, cnn, _, param, _ = mods
+    net = cnn.CNN(channels=(6, 32, 16, 2), seed=1)
+    sc = param.Scales(in_scale=[1e-6, 1e-6, 1e-2, 1e-2, 1e-2, 1e-2], out_scale=[1e-12, 1e-12])
+    p = param.CNNParameterization(net, sc)
+    phys = dict(QG_PHYS)
+    ce = param.ClosedEnsemble(1, 64, 3600., phys, param=p, seed=4)
+    q0 = ce.ens.get("q", 0).copy()
+    m = pq.QGModel(nx=64, dt=3600., log_level=0, q_parameterization=p, **phys)
+    m.q = q0
+    for _ in range(5):
+        ce.step(1)
+        m._step_forward()
+    # the network sees BF16 inputs in both paths; FP32 packing order differs
+    assert rel_l2(ce.ens.get("q", 0), m.q) < 1e-4
+    assert ce.ens.t == m.t
+
+
+def test_trained_network_beats_no_closure(mods):
+    """Offline: on held-out pairs the trained network's forcing error is well
+    below the no-closure prediction (zero forcing), i.e. R^2 > 0.3."""
+    pq, cnn, data, param, train = mods
+    gen = data.PairGenerator(nmembers=32, n_hi=128, n_lo=64, seed=0)
+    gen.spin_up(24 * 300)
+    x, y = gen.generate(1024, every=12)
+    xt, yt = gen.generate(256, every=12)
+    sc = param.Scales.fit(x, y)
+    net = cnn.CNN(channels=(6, 64, 32, 32, 2), seed=0)
+    before = train.evaluate(net, xt, yt, sc)
+    info = train.train(net, x, y, sc, epochs=6, batch=32, lr=2e-3)
+    after = train.evaluate(net, xt, yt, sc)
+    print(f"offline R^2 untrained {before['r2']:.3f} -> trained {after['r2']:.3f} "
+          f"({info['steps']} steps, {info['seconds']:.1f} s)", flush=True)
+    assert after["r2"] > 0.3
+    assert after["mse_net"] < 0.7 * after["mse_zero"]

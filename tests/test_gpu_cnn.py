@@ -1,0 +1,172 @@
+"""libqgcnn's BF16 MFMA kernels against a torch-CPU FP32 reference of the same
+operations on the same BF16-rounded tensors (periodic 5x5 cross-correlation,
+NHWC activations, weights [cout][25][cin])."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cnn(gpu):
+    import torch  # noqa: F401
+    from pyqg_amd.closure import cnn
+    return cnn
+
+
+def ref_conv(x_nhwc, w, bias=None):
+    """torch CPU FP32: x (B,H,W,Ci), w (Co,25,Ci) -> (B,H,W,Co)"""
+    import torch
+    import torch.nn.functional as F
+    co, _, ci = w.shape
+    wt = w.reshape(co, 5, 5, ci).permute(0, 3, 1, 2)
+    x = x_nhwc.permute(0, 3, 1, 2)
+    y = F.conv2d(F.pad(x, (2, 2, 2, 2), mode="circular"), wt, bias)
+    return y.permute(0, 2, 3, 1)
+
+
+def rnd_bf16(shape, scale, gen):
+    import torch
+    return (torch.randn(shape, generator=gen) * scale).to(torch.bfloat16).float()
+
+
+def rel(a, b):
+    return float((a - b).abs().max() / b.abs().max())
+
+
+@pytest.mark.parametrize("cinp,coutp,relu", [(16, 32, True), (32, 128, True), (128, 64, False),
+                                             (64, 16, False)])
+def test_conv_forward(cnn, cinp, coutp, relu):
+    import torch
+    g = torch.Generator().manual_seed(cinp + coutp)
+    B, H, W = 2, 8, 64
+    x = rnd_bf16((B, H, W, cinp), 1.0, g)
+    w = rnd_bf16((coutp, 25, cinp), 0.1, g)
+    b = torch.randn(coutp, generator=g) * 0.1
+    y_bf, y32 = cnn.conv_fwd(x.to(torch.bfloat16).cuda(), w.to(torch.bfloat16).cuda(), b.cuda(),
+                             coutp, relu=relu, out_f32=True)
+    ref = ref_conv(x, w, b)
+    if relu:
+        ref = torch.relu(ref)
+    assert rel(y32.cpu(), ref) < 1e-5
+    assert rel(y_bf.float().cpu(), ref) < 1e-2  # BF16 output rounding
+
+
+def test_backward_data_with_relu_mask(cnn):
+    import torch
+    g = torch.Generator().manual_seed(1)
+    B, H, W, cinp, coutp = 2, 8, 64, 32, 64
+    x = torch.relu(rnd_bf16((B, H, W, cinp), 1.0, g))   # previous layer's ReLU output
+    w = rnd_bf16((coutp, 25, cinp), 0.1, g)
+    dy = rnd_bf16((B, H, W, coutp), 1.0, g)
+    wt = torch.empty((cinp, 25, coutp), dtype=torch.bfloat16, device="cuda")
+    cnn.flip_transpose(w.to(torch.bfloat16).cuda(), wt)
+    _, dx32 = cnn.conv_fwd(dy.to(torch.bfloat16).cuda(), wt, None, cinp, relu=False,
+                           mask=x.to(torch.bfloat16).cuda(), out_bf16=False, out_f32=True)
+    xr = x.clone().requires_grad_(True)
+    (ref_conv(xr, w) * dy).sum().backward()
+    ref = xr.grad * (x > 0)
+    assert rel(dx32.cpu(), ref) < 1e-5
+
+
+@pytest.mark.parametrize("cinp,coutp", [(16, 128), (128, 64), (32, 32), (32, 16)])
+def test_weight_and_bias_gradient(cnn, cinp, coutp):
+    import torch
+    g = torch.Generator().manual_seed(7 + cinp)
+    B, H, W = 2, 16, 64
+    x = rnd_bf16((B, H, W, cinp), 1.0, g)
+    w = rnd_bf16((coutp, 25, cinp), 0.1, g)
+    dy = rnd_bf16((B, H, W, coutp), 1.0, g)
+    dw = torch.zeros((coutp, 25, cinp), device="cuda")
+    db = torch.zeros(coutp, device="cuda")
+    cnn.conv_wgrad(dy.to(torch.bfloat16).cuda(), x.to(torch.bfloat16).cuda(), dw)
+    cnn.bias_grad(dy.to(torch.bfloat16).cuda(), db)
+    wr = w.clone().requires_grad_(True)
+    br = torch.zeros(coutp, requires_grad=True)
+    (ref_conv(x, wr, br) * dy).sum().backward()
+    assert rel(dw.cpu(), wr.grad) < 1e-5
+    assert rel(db.cpu(), br.grad) < 1e-5
+
+
+def test_mse_loss_and_gradient(cnn):
+    import torch
+    g = torch.Generator().manual_seed(3)
+    pred = torch.randn((2, 8, 64, 16), generator=g)
+    target = torch.randn((2, 8, 64, 2), generator=g)
+    grad = torch.empty(pred.shape, dtype=torch.bfloat16, device="cuda")
+    loss = cnn.mse(pred.cuda(), target.cuda(), grad)
+    ref = ((pred[..., :2] - target) ** 2).mean()
+    assert abs(float(loss) - float(ref)) < 1e-5 * float(ref)
+    gref = torch.zeros_like(pred)
+    gref[..., :2] = 2 * (pred[..., :2] - target) / target.numel()
+    assert rel(grad.float().cpu(), gref) < 1e-2
+
+
+def test_network_gradients_match_torch(cnn):
+    """Whole small network: loss and every layer's gradient against torch
+    autograd, with activations and activation-gradients rounded to BF16 at
+    the same points as the kernels store them."""
+    import torch
+    net = cnn.CNN(channels=(6, 32, 16, 2), seed=4)
+    g = torch.Generator().manual_seed(11)
+    x = torch.randn((2, 8, 64, 6), generator=g)
+    target = torch.randn((2, 8, 64, 2), generator=g) * 0.1
+    xp = net.pad_input(x.cuda())
+    out, acts = net.forward(xp, keep=True)
+    grad = torch.empty(out.shape, dtype=torch.bfloat16, device="cuda")
+    loss = cnn.mse(out, target.cuda(), grad)
+    dws, dbs = net.backward(acts, grad)
+
+    class RoundGrad(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, a):
+            return a.to(torch.bfloat16).float()
+
+        @staticmethod
+        def backward(ctx, ga):
+            return ga.to(torch.bfloat16).float()
+
+    class RoundBack(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, a):
+            return a.clone()
+
+        @staticmethod
+        def backward(ctx, ga):
+            return ga.to(torch.bfloat16).float()
+
+    ws = [w.cpu()[: net.channels[l + 1], :, : net.channels[l]].clone().requires_grad_(True)
+          for l, w in enumerate(net.wb)]
+    ws_f = [w.float().detach().requires_grad_(True) for w in ws]
+    bs = [b.cpu()[: net.channels[l + 1]].clone().requires_grad_(True)
+          for l, b in enumerate(net.b32)]
+    h = x.to(torch.bfloat16).float()
+    for l in range(net.nlayers):
+        z = ref_conv(h, ws_f[l], bs[l])
+        if l < net.nlayers - 1:
+            h = RoundGrad.apply(torch.relu(z))
+        else:
+            h = RoundBack.apply(z)
+    ref_loss = ((h - target) ** 2).mean()
+    ref_loss.backward()
+    assert abs(float(loss) - float(ref_loss)) < 2e-3 * float(ref_loss)
+    for l in range(net.nlayers):
+        co, ci = net.channels[l + 1], net.channels[l]
+        got = dws[l].cpu()[:co, :, :ci]
+        assert rel(got, ws_f[l].grad) < 3e-2, l
+        assert rel(dbs[l].cpu()[:co], bs[l].grad) < 3e-2, l
+
+
+def test_training_reduces_loss(cnn):
+    import torch
+    net = cnn.CNN(channels=(6, 32, 32, 2), seed=5)
+    g = torch.Generator().manual_seed(2)
+    x = torch.randn((4, 16, 64, 6), generator=g)
+    # a target the network can represent: a fixed smoothing of two input channels
+    target = (x[..., :2] + torch.roll(x[..., :2], 1, dims=2)) * 0.5
+    xp = net.pad_input(x.cuda())
+    t = target.cuda()
+    first = float(net.train_step(xp, t, lr=3e-3))
+    for _ in range(150):
+        last = float(net.train_step(xp, t, lr=3e-3))
+    assert last < 0.2 * first

@@ -157,3 +157,73 @@ def test_full_size_invariants_headline_ensemble(pq):
     for i in range(8, 256, 37):
         assert np.array_equal(qh[i], qh[i % 8])
     assert all(np.isfinite(x).all() for x in qh)
+
+
+@pytest.mark.parametrize("erow,ecol", [(4, 4), (4, 8), (8, 16), (16, 4), (16, 16)])
+def test_fused_radix_variants(pq, erow, ecol):
+    """Every register radix of the fused kernels gives the oracle's trajectory
+    (two-layer register path, one-layer register path, generic nz path)."""
+    nsteps = 12
+    ens = pq.Ensemble.qg(np.array([32, 64, 64]), dt=3600.)
+    for b in ens.buckets:
+        b.set_variant(erow, ecol)
+    q0 = [seeded_q(2, n, 40 + i) for i, n in enumerate([32, 64, 64])]
+    ens.set_q(q0)
+    ens.step(nsteps)
+    for i, n in enumerate([32, 64, 64]):
+        o = OracleQG(nx=n, dt=3600., **QG_PHYS)
+        o.q = q0[i]
+        for _ in range(nsteps):
+            o._step_forward()
+        assert rel_l2(ens.get("q", i), o.q) < TOL_TRAJ
+    from oracle.qg_oracle import OracleBT
+    bt = pq.BTModel(L=2 * np.pi, nx=64, beta=0.5, rd=0.3, dt=0.005, log_level=0)
+    bt._bucket.set_variant(erow, ecol)
+    ob = OracleBT(L=2 * np.pi, nx=64, beta=0.5, rd=0.3, dt=0.005)
+    qb = 1e-2 * np.random.default_rng(3).standard_normal((1, 64, 64))
+    bt.q = qb
+    ob.q = qb
+    bt._fused_steps(nsteps)
+    for _ in range(nsteps):
+        ob._step_forward()
+    assert rel_l2(bt.q, ob.q) < TOL_TRAJ
+    kw = dict(nz=3, U=[0.05, 0.025, 0.0], V=[0.0, 0.0, 0.0], H=[500, 1750, 1750],
+              rho=[1025.0, 1025.3, 1025.6], L=1e6, rek=5.787e-7, g=9.81, dt=3600.0,
+              beta=1.5e-11, rd=15000.0, f=0.0001236812857687059)
+    lm = pq.LayeredModel(nx=32, log_level=0, **kw)
+    lm._bucket.set_variant(erow, ecol)
+    ol = OracleLayered(nx=32, **kw)
+    ol.a = np.ascontiguousarray(lm.a.real)
+    ql = seeded_q(3, 32, 9)
+    lm.q = ql
+    ol.q = ql
+    lm._fused_steps(nsteps)
+    for _ in range(nsteps):
+        ol._step_forward()
+    assert rel_l2(lm.q, ol.q) < TOL_TRAJ
+
+
+def test_autotune_keeps_state_and_parity(pq, tmp_path):
+    ens = pq.Ensemble.qg(np.array([64, 64, 128]), dt=3600.)
+    q0 = [seeded_q(2, n, 60 + i) for i, n in enumerate([64, 64, 128])]
+    ens.set_q(q0)
+    ens.step(3)
+    before = [ens.get(f) for f in ("qh", "dqhdt_p", "dqhdt_pp")]
+    report = ens.autotune(reps=2, cache_path=str(tmp_path / "tune.json"))
+    assert len(report) == 2
+    after = [ens.get(f) for f in ("qh", "dqhdt_p", "dqhdt_pp")]
+    for x, y in zip(before, after):
+        for a, b in zip(x, y):
+            assert np.array_equal(a, b)
+    ens.step(7)
+    for i, n in enumerate([64, 64, 128]):
+        o = OracleQG(nx=n, dt=3600., **QG_PHYS)
+        o.q = q0[i]
+        for _ in range(10):
+            o._step_forward()
+        assert rel_l2(ens.get("q", i), o.q) < TOL_TRAJ
+    # a second ensemble of the same shapes takes the variant from the cache
+    ens2 = pq.Ensemble.qg(np.array([64, 64, 128]), dt=3600.)
+    rep2 = ens2.autotune(cache_path=str(tmp_path / "tune.json"))
+    assert [b.variant() for b in ens2.buckets] == [b.variant() for b in ens.buckets]
+    assert rep2.keys() == report.keys()

@@ -10,11 +10,14 @@ from conftest import ROOT
 
 HEADER = os.path.join(ROOT, "include", "pyqg_amd.h")
 LIB = os.path.join(ROOT, "pyqg_amd", "_qgk.so")
+CNN_HEADER = os.path.join(ROOT, "include", "pyqg_amd_cnn.h")
+CNN_LIB = os.path.join(ROOT, "pyqg_amd", "_qgcnn.so")
 
 
-def declared():
-    text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(qgk_\w+)\s*\(", text, re.M)))
+def declared(header=HEADER, prefix="qgk_"):
+    text = open(header).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(" + prefix + r"\w+)\s*\(",
+                                 text, re.M)))
 
 
 def test_header_declares_entry_points():
@@ -55,3 +58,15 @@ def test_bad_sizes_rejected_before_device_use():
         _lib.Bucket(1, 2, 48, 48, 0)
     with pytest.raises(ValueError):
         _lib.Bucket(0, 2, 32, 32, 0)
+
+
+def test_cnn_library_exports_every_declared_symbol():
+    if not os.path.exists(CNN_LIB):
+        pytest.fail("libqgcnn is not built: run python pyqg_amd/build.py")
+    lib = ctypes.CDLL(CNN_LIB)
+    names = declared(CNN_HEADER, "qgcnn_")
+    assert len(names) >= 9
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    from pyqg_amd.closure import cnn
+    assert sorted(cnn.EXPORTS) == names
