@@ -1,13 +1,19 @@
-"""Throughput of the FP64 two-layer QG ensemble on one MI355X.
+"""Throughput of FP64 QG ensembles on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--workload W]
 
-Workload: 256 independent two-layer QGModels, nx = ny = 256, dt = 3600 s, FP64,
-seeded 1e-7 N(0,1) initial PV, stepped with the fused two-kernel step.  One
-"step" advances every member by dt.  Prints one JSON line with the metric
-(model-days per second summed over members), the HBM roofline of the fused
-step measured with HIP events over the timed region, and the CPU baseline
-(the NumPy oracle on the host cores, same physics, in the same run).
+Default workload (the headline): 256 independent two-layer QGModels,
+nx = ny = 256, dt = 3600 s, FP64, seeded 1e-7 N(0,1) initial PV, stepped with
+the fused two-kernel step.  One "step" advances every member by dt.  Prints one
+JSON line with the metric (model-days per second summed over members), the HBM
+roofline of the fused step measured with HIP events over the timed region, and
+the CPU baseline (the NumPy oracle on the host cores, same physics, same run).
+
+Other workloads (--workload):
+  uneven     256 two-layer members, nx drawn from {64, 128, 256}, beta, rd,
+             delta, rek drawn per member (default_rng(1)), dt = 3600 s
+  layered10  64 LayeredModel members, nz = 10, nx = 256, L = 2e6 m,
+             exponential stratification, first deformation radius 20 km
 """
 import argparse
 import json
@@ -25,6 +31,33 @@ PHYS = dict(L=1e6, H1=500.0, delta=0.25, U1=0.025, U2=0.0, beta=1.5e-11, rd=1500
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(radix_row, radix_col, path=None):
+    """HBM bytes per fused step from the committed rocprofv3 PMC summary of
+    the headline workload (FETCH_SIZE and WRITE_SIZE, copy-calibrated), or
+    None when the summary is missing or lacks the tuned kernels."""
+    path = path or os.path.join(ROOT, "profiles", "fused_step_qg2_nx256_256members.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    if "## HBM traffic" not in text:
+        return None
+    text = text.split("## HBM traffic", 1)[1]
+    total = 0.0
+    for name in (f"k_row_flux<256, {radix_row}>", f"k_col_step_reg<256, 2, {radix_col}>"):
+        for line in text.splitlines():
+            cells = [c.strip() for c in line.split("|")]
+            if len(cells) > 3 and cells[1] == name:
+                try:
+                    total += (float(cells[2]) + float(cells[3])) * 2 ** 20
+                    break
+                except ValueError:
+                    return None
+        else:
+            return None
+    return total
+
+
 def algorithmic_bytes(nz, nx, ny):
     """Per member-step: read qh, dqhdt_p, dqhdt_pp; write qh and the new dqhdt
     (complex128 spectral fields of nz x ny x (nx/2+1))."""
@@ -33,11 +66,21 @@ def algorithmic_bytes(nz, nx, ny):
 
 # ------------------------------------------------------------------ CPU leg
 def _cpu_worker(args):
-    nsteps, seed = args
+    nsteps, seed, workload = args
     import numpy as np
-    from oracle.qg_oracle import OracleQG
-    m = OracleQG(nx=NX, dt=DT, **PHYS)
-    m.q = 1e-7 * np.random.default_rng(seed).standard_normal((NZ, NX, NX))
+    from oracle.qg_oracle import OracleLayered, OracleQG
+    rng = np.random.default_rng(seed)
+    if workload == "layered10":
+        from pyqg_amd.constants import exponential_stratification
+        H, rho, U = exponential_stratification()
+        m = OracleLayered(nz=10, nx=256, H=H, rho=rho, U=U, V=np.zeros(10), L=2e6,
+                          beta=1.5e-11, rek=5.787e-7, dt=DT, f=0.0001236812857687059, g=9.81)
+        nz, n = 10, 256
+    else:
+        n = NX if workload == "qg256" else int(rng.choice([64, 128, 256]))
+        m = OracleQG(nx=n, dt=DT, **PHYS)
+        nz = NZ
+    m.q = 1e-7 * rng.standard_normal((nz, n, n))
     m._step_forward()
     t0 = time.perf_counter()
     for _ in range(nsteps):
@@ -45,17 +88,19 @@ def _cpu_worker(args):
     return time.perf_counter() - t0
 
 
-def cpu_baseline(nsteps=500):
+def cpu_baseline(workload="qg256", nsteps=None):
     """The oracle (NumPy FP64, numpy.fft = the reference's no-pyfftw path), one
     member per process, one thread per process, on the host cores."""
     import concurrent.futures as cf
     import multiprocessing as mp
     os.environ["OMP_NUM_THREADS"] = "1"
     os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    if nsteps is None:
+        nsteps = {"qg256": 500, "uneven": 500, "layered10": 60}[workload]
     cores = min(len(os.sched_getaffinity(0)), 16)
     with cf.ProcessPoolExecutor(max_workers=cores, mp_context=mp.get_context("spawn")) as ex:
         t0 = time.perf_counter()
-        times = list(ex.map(_cpu_worker, [(nsteps, s) for s in range(cores)]))
+        times = list(ex.map(_cpu_worker, [(nsteps, s, workload) for s in range(cores)]))
         wall = time.perf_counter() - t0
     per_member = nsteps * DT / 86400. / (sum(times) / len(times))
     model = ""
@@ -67,11 +112,14 @@ def cpu_baseline(nsteps=500):
                     break
     except OSError:
         pass
+    what = {"qg256": f"the nx={NX} two-layer QG oracle",
+            "uneven": "two-layer QG oracle members of nx drawn from {64,128,256}",
+            "layered10": "the nz=10 nx=256 LayeredModel oracle"}[workload]
     return {"value": per_member * cores, "unit": "model-days/s", "cores": cores,
             "kind": "port",
-            "sample": (f"{cores} members x {nsteps} steps of the nx={NX} two-layer QG oracle, "
-                       f"one single-threaded process per core ({model}); aggregate = "
-                       f"per-member rate x cores; {wall:.1f} s wall")}
+            "sample": (f"{cores} members x {nsteps} steps of {what}, one single-threaded "
+                       f"process per core ({model}); aggregate = mean per-member rate x "
+                       f"cores; {wall:.1f} s wall")}
 
 
 # ------------------------------------------------------------------ GPU leg
@@ -82,6 +130,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--members", type=int, default=MEMBERS)
+    ap.add_argument("--workload", choices=["qg256", "uneven", "layered10"], default="qg256")
     ap.add_argument("--copy-probe", type=int, default=1 << 30,
                     help="bytes moved by the copy kernel that calibrates HBM counters (0: skip)")
     args = ap.parse_args()
@@ -90,26 +139,57 @@ def main():
               "(members shard across GPUs with no per-step communication).", file=sys.stderr)
         sys.exit(2)
 
-    cpu = None if args.no_cpu else cpu_baseline()
+    cpu = None if args.no_cpu else cpu_baseline(args.workload)
 
     import numpy as np
     import pyqg_amd
 
-    M = args.members
-    ens = pyqg_amd.Ensemble.qg(np.full(M, NX), dt=DT, **{k: v for k, v in PHYS.items()})
-    ens.set_q([1e-7 * np.random.default_rng(s).standard_normal((NZ, NX, NX)) for s in range(M)])
+    if args.workload == "qg256":
+        M = args.members
+        ens = pyqg_amd.Ensemble.qg(np.full(M, NX), dt=DT, **{k: v for k, v in PHYS.items()})
+        nxs, nz = np.full(M, NX), NZ
+        workload = (f"QGModel two-layer ensemble: {M} members, nx=ny={NX}, dt={DT:g} s, "
+                    f"L=1e6 m, rd=15 km, delta=0.25, U1=0.025 m/s, beta=1.5e-11, "
+                    f"rek=5.787e-7, FP64, fused 2-kernel step")
+    elif args.workload == "uneven":
+        M = 256
+        rng = np.random.default_rng(1)
+        nxs = rng.choice([64, 128, 256], size=M)
+        beta = rng.uniform(0.5e-11, 2.5e-11, M)
+        rd = rng.uniform(1e4, 2.5e4, M)
+        delta = rng.uniform(0.1, 0.5, M)
+        rek = rng.uniform(3e-7, 1e-6, M)
+        ens = pyqg_amd.Ensemble.qg(nxs, beta=beta, rd=rd, delta=delta, rek=rek, dt=DT)
+        nz = NZ
+        workload = (f"uneven two-layer QG ensemble: 256 members, nx from {{64,128,256}} "
+                    f"({int((nxs == 64).sum())}/{int((nxs == 128).sum())}/"
+                    f"{int((nxs == 256).sum())}), per-member beta, rd, delta, rek, dt=3600 s, FP64")
+    else:
+        M = 64
+        H, rho, U = pyqg_amd.constants.exponential_stratification()
+        ens = pyqg_amd.Ensemble.layered(M, 256, 10, H, rho, U, L=2e6, dt=DT)
+        nxs, nz = np.full(M, 256), 10
+        workload = ("LayeredModel ensemble: 64 members, nz=10, nx=ny=256, L=2e6 m, exponential "
+                    "stratification (H ~ exp(-z/800 m), 20 km first radius), dt=3600 s, FP64")
+    ens.set_q([1e-7 * np.random.default_rng(s).standard_normal((nz, int(n), int(n)))
+               for s, n in enumerate(nxs)])
     ens.step(3)                           # AB start-up, untimed
     tune = ens.autotune(reps=5, cache_path=False)
     ens.step(max(args.warmup - 3, 1))     # warm-up with the tuned kernels, untimed
-    b = ens.buckets[0]
-    b.sync()
-
-    b.timing_enable(True)
+    for b in ens.buckets:
+        b.sync()
+        b.timing_enable(True)
     t0 = time.perf_counter()
     ens.step(args.steps)                  # returns after a device synchronize
     wall = time.perf_counter() - t0
-    ms_row, n_row, ms_col, n_col = b.timing()
-    b.timing_enable(False)
+    step_ms = 0.0
+    per_bucket = []
+    for b in ens.buckets:
+        ms_row, n_row, ms_col, n_col = b.timing()
+        b.timing_enable(False)
+        per_bucket.append({"nx": b.nx, "members": b.M, "row_kernel_ms": ms_row / n_row,
+                           "col_kernel_ms": ms_col / n_col, "variant": list(b.variant())})
+        step_ms += ms_row / n_row + ms_col / n_col
 
     copy = None
     if args.copy_probe:
@@ -117,16 +197,17 @@ def main():
         copy = {"bytes": args.copy_probe, "ms": ms,
                 "GB/s": 2 * args.copy_probe / (ms * 1e-3) / 1e9}
 
-    qh = ens.get("qh", 0)
-    if not np.isfinite(qh).all():
-        raise SystemExit("non-finite state after the timed steps")
+    for m in (0, M - 1):
+        if not np.isfinite(ens.get("qh", m)).all():
+            raise SystemExit("non-finite state after the timed steps")
 
     model_days = M * args.steps * DT / 86400.
     value = model_days / wall
-    row_ms, col_ms = ms_row / n_row, ms_col / n_col
-    step_ms = row_ms + col_ms
-    bytes_step = algorithmic_bytes(NZ, NX, NX) * M
+    bytes_step = float(sum(algorithmic_bytes(nz, int(n), int(n)) for n in nxs))
     achieved = bytes_step / (step_ms * 1e-3) / 1e9
+    traffic = None
+    if args.workload == "qg256" and M == MEMBERS:
+        traffic = measured_traffic(*ens.buckets[0].variant())
     out = {
         "metric": METRIC,
         "value": value,
@@ -140,18 +221,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded 1e-7 N(0,1) initial PV per member)",
-        "config": {"workload": f"QGModel two-layer ensemble: {M} members, nx=ny={NX}, "
-                               f"dt={DT:g} s, L=1e6 m, rd=15 km, delta=0.25, U1=0.025 m/s, "
-                               f"beta=1.5e-11, rek=5.787e-7, FP64, fused 2-kernel step",
-                   "members": M, "nx": NX, "nz": NZ, "dt": DT},
+        "config": {"workload": workload, "members": M, "nz": nz,
+                   "nx": int(nxs[0]) if args.workload != "uneven" else "64/128/256",
+                   "dt": DT},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "fused step = k_row_flux + k_col_step",
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "fused step = k_row_flux + k_col_step (per bucket, summed)",
                      "algorithmic_bytes_per_launch": bytes_step,
-                     "row_kernel_ms": row_ms, "col_kernel_ms": col_ms,
-                     "copy_kernel": copy,
-                     "variant": {"row_radix": b.variant()[0], "col_radix": b.variant()[1],
-                                 "autotune": list(tune.values())[0]}},
+                     "kernel_ms_per_step": step_ms, "buckets": per_bucket,
+                     "copy_kernel": copy, "autotune": tune},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -10,7 +10,9 @@ import math
 import os
 
 import numpy as np
-import torch
+
+from .. import _lib as _qgk  # noqa: F401  (ROCm runtime before torch's)
+import torch  # noqa: E402
 
 _HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_HERE, "_qgcnn.so")

@@ -20,9 +20,9 @@ import json
 import time
 
 import numpy as np
-import torch
 
-from .. import _lib
+from .. import _lib  # loads the ROCm runtime before torch does
+import torch  # noqa: E402
 from . import cnn as cnn_mod
 from . import data as data_mod
 from . import param as param_mod

@@ -144,14 +144,18 @@ __device__ __forceinline__ void stockham_pass(cd (&x)[E], cd* lds, int t, int la
 #pragma unroll
   for (int i = 0; i < NB; ++i) {
     if constexpr (L > 1) {
+      // input s is twiddled by w^s, w = exp(SIGN 2 pi i j / (L R)): one table
+      // load, the higher powers by multiplication (error grows ~ s ulp)
       const int b = t + i * T;
       const int j = b % L;
       constexpr int STEP = N / (L * R);
+      cd w = tw[j * STEP];
+      if constexpr (SIGN > 0) w.y = -w.y;
+      cd ws = w;
 #pragma unroll
       for (int s = 1; s < R; ++s) {
-        cd w = tw[j * s * STEP];
-        if constexpr (SIGN > 0) w.y = -w.y;
-        y[i][s] = cmul(y[i][s], w);
+        y[i][s] = cmul(y[i][s], ws);
+        if (s + 1 < R) ws = cmul(ws, w);
       }
     }
     DFT<R, SIGN>::run(y[i]);

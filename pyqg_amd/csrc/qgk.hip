@@ -436,8 +436,11 @@ __device__ __forceinline__ void split_store_t(const cd* lds, int t, int rp, bool
 }
 
 // Row phase: per (member, layer, row pair): x-c2r of q, u, v, flux, x-r2c.
-// In:  Qm, Um, Vm  [M][nz][ny][nk] (y already inverse transformed)
-// Out: UQm, VQm    [M][nz][ny][nk] (x already forward transformed)
+// In:  Qm, Um, Vm  [M][nz][ny][ldk] (y already inverse transformed)
+// Out: UQm, VQm    [M][nz][ny][ldk] (x already forward transformed)
+// Each complex transform carries the two rows of the pair (row j in the real
+// part, row j+1 in the imaginary part), so the six input rows are loaded at
+// once and the five transforms run back to back.
 template <int NX, int E>
 __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP)) void k_row_flux(
     const cd* __restrict__ Qm, const cd* __restrict__ Um, const cd* __restrict__ Vm,
@@ -456,40 +459,43 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP)) void k_row_flux
   const double ubg = params[m].Ubg[z];
   const size_t row0 = ((size_t)plane_id * ny + j0) * NK;
 
-  // q on rows j0 (real part) and j0+1 (imaginary part)
-  cd zq[G::E];
+  cd zq[G::E], zu[G::E], zv[G::E];
 #pragma unroll
   for (int s = 0; s < G::E; ++s) {
     const int p = t + s * G::T;
     cd a = hext<NX>(Qm + row0, p), b = hext<NX>(Qm + row0 + NK, p);
     zq[s] = {a.x - b.y, a.y + b.x};
+    a = hext<NX>(Um + row0, p);
+    b = hext<NX>(Um + row0 + NK, p);
+    zu[s] = {a.x - b.y, a.y + b.x};
+    a = hext<NX>(Vm + row0, p);
+    b = hext<NX>(Vm + row0 + NK, p);
+    zv[s] = {a.x - b.y, a.y + b.x};
   }
   qgk::fft_block<NX, G::E, +1, G::RP>(zq, lds, t, rp, tw);
-
+  qgk::fft_block<NX, G::E, +1, G::RP>(zu, lds, t, rp, tw);
+  qgk::fft_block<NX, G::E, +1, G::RP>(zv, lds, t, rp, tw);
+  // fluxes of both rows: uq = (u + Ubg) q, vq = v q (kernel.pyx:372-378)
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const size_t row = row0 + (size_t)r * NK;
-    cd w[G::E];
-#pragma unroll
-    for (int s = 0; s < G::E; ++s) {
-      const int p = t + s * G::T;
-      cd a = hext<NX>(Um + row, p), b = hext<NX>(Vm + row, p);
-      w[s] = {a.x - b.y, a.y + b.x};
-    }
-    qgk::fft_block<NX, G::E, +1, G::RP>(w, lds, t, rp, tw);
-#pragma unroll
-    for (int s = 0; s < G::E; ++s) {
-      const double q = (r == 0 ? zq[s].x : zq[s].y) * scale;
-      const double u = w[s].x * scale, v = w[s].y * scale;
-      w[s] = {(u + ubg) * q, v * q};
-    }
-    qgk::fft_block<NX, G::E, -1, G::RP>(w, lds, t, rp, tw);
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < G::E; ++s) lds[(t + s * G::T) * G::RP + rp] = w[s];
-    __syncthreads();
-    split_store_t<NX, G::T, G::RP>(lds, t, rp, ok, UQm + row, VQm + row);
+  for (int s = 0; s < G::E; ++s) {
+    const double q0 = zq[s].x * scale, q1 = zq[s].y * scale;
+    const double u0 = zu[s].x * scale, u1 = zu[s].y * scale;
+    const double v0 = zv[s].x * scale, v1 = zv[s].y * scale;
+    zu[s] = {(u0 + ubg) * q0, (u1 + ubg) * q1};
+    zv[s] = {v0 * q0, v1 * q1};
   }
+  qgk::fft_block<NX, G::E, -1, G::RP>(zu, lds, t, rp, tw);
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) lds[(t + s * G::T) * G::RP + rp] = zu[s];
+  __syncthreads();
+  split_store_t<NX, G::T, G::RP>(lds, t, rp, ok, UQm + row0, UQm + row0 + NK);
+  qgk::fft_block<NX, G::E, -1, G::RP>(zv, lds, t, rp, tw);
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) lds[(t + s * G::T) * G::RP + rp] = zv[s];
+  __syncthreads();
+  split_store_t<NX, G::T, G::RP>(lds, t, rp, ok, VQm + row0, VQm + row0 + NK);
 }
 
 // ---- any nz: the inversion re-reads qh from global memory (cache hits)
@@ -972,6 +978,18 @@ struct qgk_bucket {
   // register radix of the fused row and column kernels (autotuned)
   int erow = 8, ecol = 8;
   double* small = nullptr;  // 256 doubles of per-call constants (closure scales)
+  hipStream_t lst = nullptr;  // stream override for fused-step launches (graph capture)
+  long gen = 0;               // bumped whenever launch arguments change (graph cache key)
+  // hipGraph of two fused steps of a set of buckets, replayed while the AB
+  // weights are constant (owned by the first bucket of the set)
+  struct GraphCache {
+    std::vector<qgk_bucket*> bs;
+    std::vector<long> gens;
+    std::vector<cd*> qh0;
+    std::vector<int> iP0;
+    StepCoeffs cf = {0, 0, 0};
+    hipGraphExec_t exec = nullptr;
+  } graph;
 };
 
 namespace {
@@ -1173,7 +1191,8 @@ struct RowFluxL {
     using G = FGeo<NX, E>;
     const double scale = 1.0 / ((double)b->nx * (double)b->ny);
     dim3 grid((b->ny / 2 + G::RP - 1) / G::RP, b->M * b->nz);
-    hipLaunchKernelGGL((k_row_flux<NX, E>), grid, dim3(G::T * G::RP), 0, b->stream, b->Qm,
+    hipLaunchKernelGGL((k_row_flux<NX, E>), grid, dim3(G::T * G::RP), 0,
+                       b->lst ? b->lst : b->stream, b->Qm,
                        b->Um, b->Vm, b->UQm, b->VQm, b->dp, b->nz, b->ny, scale, b->tw_x);
   }
 };
@@ -1204,15 +1223,15 @@ struct ColStepL {
     const cd* qo = b->field_c[QGK_QH];
     cd* qn = b->qh_alt;
     if (b->nz == 1)
-      hipLaunchKernelGGL((k_col_step_reg<NY, 1, E>), grid, block, 0, b->stream, qo, qn, Pp(b),
+      hipLaunchKernelGGL((k_col_step_reg<NY, 1, E>), grid, block, 0, b->lst ? b->lst : b->stream, qo, qn, Pp(b),
                          PP(b), b->UQm, b->VQm, b->Qm, b->Um, b->Vm, b->dp, cf, b->nk, b->ldk,
                          b->tw_y);
     else if (b->nz == 2)
-      hipLaunchKernelGGL((k_col_step_reg<NY, 2, E>), grid, block, 0, b->stream, qo, qn, Pp(b),
+      hipLaunchKernelGGL((k_col_step_reg<NY, 2, E>), grid, block, 0, b->lst ? b->lst : b->stream, qo, qn, Pp(b),
                          PP(b), b->UQm, b->VQm, b->Qm, b->Um, b->Vm, b->dp, cf, b->nk, b->ldk,
                          b->tw_y);
     else
-      hipLaunchKernelGGL((k_col_step<NY, E>), grid, block, 0, b->stream, qo, qn, Pp(b), PP(b),
+      hipLaunchKernelGGL((k_col_step<NY, E>), grid, block, 0, b->lst ? b->lst : b->stream, qo, qn, Pp(b), PP(b),
                          b->UQm, b->VQm, b->Qm, b->Um, b->Vm, b->dp, cf, b->nz, b->nk, b->ldk,
                          b->tw_y);
   }
@@ -1436,6 +1455,7 @@ int qgk_destroy(qgk_bucket* b) {
   if (b->stream) hipStreamSynchronize(b->stream);
   for (auto& g : b->gexec)
     if (g) hipGraphExecDestroy(g);
+  if (b->graph.exec) hipGraphExecDestroy(b->graph.exec);
   for (auto& p : b->pending) {
     hipEventDestroy(p.a);
     hipEventDestroy(p.b);
@@ -1465,6 +1485,7 @@ int qgk_set_const(qgk_bucket* b, int which, int member, const double* host) {
     param_slot(b->hp[member], which) = dst;
   }
   if (which == QGK_A) b->gvalid = false;
+  b->gen++;
   b->params_dirty = true;
   const size_t rows = const_rows(b, which);
   if (rows) {
@@ -1499,6 +1520,7 @@ int qgk_set_const_all(qgk_bucket* b, int which, const double* host) {
   b->params_dirty = true;
   b->primed = false;
   b->gvalid = false;
+  b->gen++;
   return 0;
 }
 
@@ -1514,6 +1536,7 @@ int qgk_set_inversion_2layer(qgk_bucket* b, int member, double F1, double F2) {
   b->params_dirty = true;
   b->primed = false;
   b->gvalid = false;
+  b->gen++;
   return 0;
 }
 
@@ -1524,6 +1547,7 @@ int qgk_set_rek(qgk_bucket* b, int member, double rek) {
     b->hp[m].rek = rek;
   b->params_dirty = true;
   b->gvalid = false;
+  b->gen++;
   return 0;
 }
 
@@ -1741,10 +1765,72 @@ int qgk_sync(qgk_bucket* b) {
   return 0;
 }
 
+namespace {
+
+bool graph_matches(const qgk_bucket::GraphCache& g, qgk_bucket** bs, int nb, StepCoeffs cf) {
+  if (!g.exec || (int)g.bs.size() != nb) return false;
+  if (g.cf.dt1 != cf.dt1 || g.cf.dt2 != cf.dt2 || g.cf.dt3 != cf.dt3) return false;
+  for (int i = 0; i < nb; ++i)
+    if (g.bs[i] != bs[i] || g.gens[i] != bs[i]->gen || g.qh0[i] != bs[i]->field_c[QGK_QH] ||
+        g.iP0[i] != bs[i]->iP)
+      return false;
+  return true;
+}
+
+// Capture two fused steps of every bucket on bs[0]'s stream.  The history and
+// qh buffers rotate with period two, so after the capture the host-side
+// rotation state is back where it started and the graph can be replayed.
+int capture_two_steps(qgk_bucket** bs, int nb, StepCoeffs cf) {
+  qgk_bucket::GraphCache& g = bs[0]->graph;
+  if (g.exec) {
+    hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+  }
+  hipStream_t cs = bs[0]->stream;
+  for (int i = 0; i < nb; ++i) HIPC(hipStreamSynchronize(bs[i]->stream));
+  HIPC(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int s = 0; s < 2 && rc == 0; ++s)
+    for (int i = 0; i < nb && rc == 0; ++i) {
+      bs[i]->lst = cs;
+      if ((rc = launch_row_flux(bs[i])) == 0 && (rc = launch_col_step(bs[i], cf)) == 0) {
+        rotate_history(bs[i]);
+        std::swap(bs[i]->field_c[QGK_QH], bs[i]->qh_alt);
+      }
+      bs[i]->lst = nullptr;
+    }
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(cs, &graph);
+  if (rc) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(QGK_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    g.exec = nullptr;
+    return fail(QGK_E_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+  }
+  g.bs.assign(bs, bs + nb);
+  g.gens.resize(nb);
+  g.qh0.resize(nb);
+  g.iP0.resize(nb);
+  for (int i = 0; i < nb; ++i) {
+    g.gens[i] = bs[i]->gen;
+    g.qh0[i] = bs[i]->field_c[QGK_QH];
+    g.iP0[i] = bs[i]->iP;
+  }
+  g.cf = cf;
+  return 0;
+}
+
+}  // namespace
+
 int qgk_step_buckets(qgk_bucket** bs, int nb, int nsteps, const double* coeffs, int use_graph) {
   if (!bs || nb < 1 || nsteps < 0 || (nsteps > 0 && !coeffs))
     return fail(QGK_E_ARG, "bad arguments");
-  (void)use_graph;
+  bool timing = false;
   for (int i = 0; i < nb; ++i) {
     qgk_bucket* b = bs[i];
     if (!b) return fail(QGK_E_ARG, "NULL bucket");
@@ -1754,12 +1840,40 @@ int qgk_step_buckets(qgk_bucket** bs, int nb, int nsteps, const double* coeffs, 
       if ((rc = launch_col_prologue(b))) return rc;
       b->primed = true;
     }
+    timing = timing || b->timing;
   }
-  for (int s = 0; s < nsteps; ++s) {
+  // the trailing run of steps with identical AB weights may replay a graph
+  int first_const = nsteps;
+  while (first_const > 0 && coeffs[3 * (first_const - 1)] == coeffs[3 * (nsteps - 1)] &&
+         coeffs[3 * (first_const - 1) + 1] == coeffs[3 * (nsteps - 1) + 1] &&
+         coeffs[3 * (first_const - 1) + 2] == coeffs[3 * (nsteps - 1) + 2])
+    --first_const;
+  const bool graph = use_graph && !timing && nsteps - first_const >= 2;
+  const int eager_end = graph ? first_const : nsteps;
+  for (int s = 0; s < eager_end; ++s) {
     StepCoeffs cf = {coeffs[3 * s], coeffs[3 * s + 1], coeffs[3 * s + 2]};
     for (int i = 0; i < nb; ++i) {
       int rc = fused_step(bs[i], cf);
       if (rc) return rc;
+    }
+  }
+  if (graph) {
+    const int s0 = first_const;
+    StepCoeffs cf = {coeffs[3 * s0], coeffs[3 * s0 + 1], coeffs[3 * s0 + 2]};
+    if (!graph_matches(bs[0]->graph, bs, nb, cf)) {
+      int rc = capture_two_steps(bs, nb, cf);
+      if (rc) return rc;
+    }
+    // the other buckets' pending work must precede the replays on bs[0]'s stream
+    for (int i = 1; i < nb; ++i) HIPC(hipStreamSynchronize(bs[i]->stream));
+    const int pairs = (nsteps - s0) / 2;
+    for (int r = 0; r < pairs; ++r) HIPC(hipGraphLaunch(bs[0]->graph.exec, bs[0]->stream));
+    if ((nsteps - s0) % 2) {
+      HIPC(hipStreamSynchronize(bs[0]->stream));
+      for (int i = 0; i < nb; ++i) {
+        int rc = fused_step(bs[i], cf);
+        if (rc) return rc;
+      }
     }
   }
   for (int i = 0; i < nb; ++i) {
@@ -1809,6 +1923,7 @@ int qgk_set_variant(qgk_bucket* b, int erow, int ecol) {
   b->erow = erow;
   b->ecol = ecol;
   b->gvalid = false;
+  b->gen++;
   return 0;
 }
 
@@ -1896,6 +2011,7 @@ int qgk_autotune(qgk_bucket* b, int reps, int* erow, int* ecol, double* ms_row3,
   b->ecol = bc;
   b->primed = false;  // Qm/Um/Vm were overwritten by the trials
   b->gvalid = false;
+  b->gen++;
   *erow = br;
   *ecol = bc;
   return 0;
