@@ -109,10 +109,11 @@ int qgk_ifft(qgk_bucket* b, int nplanes, const double* in_complex, double* out_r
 
 /* Fused stepping: nsteps full steps of _step_forward (model.py:406-436) without
  * parameterizations, two kernels per step.  coeffs holds nsteps*3 AB weights
- * (same for every member).  Afterwards qh and the history are current; the
- * intermediate fields of QGK_FULL_STATE handles are not (qgk_materialize
- * recomputes them by running the last step's phases).  use_graph != 0 replays
- * the two-kernel step as a captured hipGraph once the weights stop changing. */
+ * (same for every member).  Afterwards qh, q (transformed on first read) and
+ * the history are current; the other intermediate fields of QGK_FULL_STATE
+ * handles are not (the Python model runs its last step phase by phase so that
+ * they are).  use_graph != 0 replays the two-kernel step as a captured
+ * hipGraph once the weights stop changing. */
 int qgk_step(qgk_bucket* b, int nsteps, const double* coeffs, int use_graph);
 
 /* Kernel timing for the measurement harness: when enabled, the fused step

@@ -38,11 +38,33 @@ class QGKError(RuntimeError):
     pass
 
 
+def _one_hip_runtime():
+    """Keep a single HIP runtime in the process.
+
+    The PyTorch-ROCm wheel ships its own libamdhip64.so; libqgk needs
+    libamdhip64.so.7 from the ROCm install.  If both copies are mapped, each
+    brings its own HSA runtime and whichever initialises second finds no
+    device.  When torch is already imported its copy is loaded and satisfies
+    libqgk's dependency (soname libamdhip64.so.7).  Otherwise the ROCm copy is
+    loaded here under the name torch links against (libamdhip64.so), so a
+    later `import torch` binds to it instead of loading its own."""
+    import sys
+    if "torch" in sys.modules:
+        return
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            ctypes.CDLL(name, mode=ctypes.RTLD_GLOBAL)
+            return
+        except OSError:
+            continue
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"libqgk is not built ({LIB_PATH} missing): run `python pyqg_amd/build.py` "
             "(needs hipcc); pyqg_amd has no CPU fallback")
+    _one_hip_runtime()
     lib = ctypes.CDLL(LIB_PATH)
     P, I, D, L = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
     DP = ctypes.POINTER(ctypes.c_double)

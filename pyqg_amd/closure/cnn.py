@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-from .. import _lib as _qgk  # noqa: F401  (ROCm runtime before torch's)
+from .. import _lib as _qgk  # noqa: F401  (one HIP runtime for torch and the kernels)
 import torch  # noqa: E402
 
 _HERE = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))

@@ -21,7 +21,7 @@ import time
 
 import numpy as np
 
-from .. import _lib  # loads the ROCm runtime before torch does
+from .. import _lib
 import torch  # noqa: E402
 from . import cnn as cnn_mod
 from . import data as data_mod

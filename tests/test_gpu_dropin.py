@@ -331,3 +331,27 @@ def test_bt_long_run_reference_values(pq):
     np.testing.assert_allclose((m.ifft(m.ph) ** 2).sum(), 1493.217664248918, 1e-5)
     np.testing.assert_allclose(m._calc_ke(), 0.9950360837282386, 1e-5)
     np.testing.assert_allclose((m.q ** 2).sum(), lr["bt_qnorm"], rtol=1e-9)
+
+
+# --------------------------------------------------------------- runtime
+@pytest.mark.parametrize("order", ["kernels_first", "torch_first"])
+def test_one_hip_runtime_with_torch(gpu, order):
+    """libqgk and torch share one HIP runtime whichever is imported first
+    (two runtimes in one process leave the second without a device)."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    body = ("import numpy as np, pyqg_amd\n"
+            "e = pyqg_amd.Ensemble.qg(np.full(2, 64), dt=3600.)\n"
+            "e.set_q([np.zeros((2, 64, 64))] * 2); e.step(3)\n")
+    torch_op = ("import torch\n"
+                "x = torch.ones(4, device='cuda'); torch.cuda.synchronize()\n"
+                "assert float(x.sum()) == 4.0\n")
+    code = body + torch_op if order == "kernels_first" else torch_op + body + torch_op
+    code += ("maps = {l.split()[-1] for l in open('/proc/self/maps') if 'amdhip64' in l}\n"
+             "assert len(maps) == 1, maps\nprint('ok')\n")
+    print(f"runtime order {order}", flush=True)
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True,
+                       timeout=180)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stderr[-2000:]
