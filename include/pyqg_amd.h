@@ -173,6 +173,41 @@ int qgk_copy_probe(size_t nbytes, int iters, double* ms_per_copy);
 int qgk_step_buckets(qgk_bucket** bs, int nbuckets, int nsteps, const double* coeffs,
                      int use_graph);
 
+/* Time-averaged diagnostics accumulated on the device, per member, replacing
+ * the registry of pyqg/model.py:644-736 (Ensspec ... ENSparamspec),
+ * qg_diagnostics.py:6-70 (entspec ... APEgenspec), qg_model.py:213-263
+ * (APEflux, KEflux, APEgen; nz = 2) and layered_model.py:242-301
+ * (KEspec_modal ... APEflux_div).  Spectral diagnostics have shape
+ * [slots][nl][nk] per member (slots = nz for Ensspec, KEspec, KEspec_modal;
+ * nz-1 for PEspec_modal; else 1); EKE has nz values, EKEdiss and APEgen one. */
+enum {
+  QGK_D_ENSSPEC = 0, QGK_D_KESPEC, QGK_D_EKEDISS, QGK_D_KEFRICTIONSPEC, QGK_D_EKE,
+  QGK_D_DISSSPEC, QGK_D_ENSDISSSPEC, QGK_D_PARAMSPEC, QGK_D_ENSPARAMSPEC,
+  QGK_D_ENTSPEC, QGK_D_PARAMSPEC_APEFLUX, QGK_D_PARAMSPEC_KEFLUX, QGK_D_ENSFLUX,
+  QGK_D_ENSGENSPEC, QGK_D_ENSFRICTIONSPEC, QGK_D_APEGENSPEC,
+  QGK_D_APEFLUX, QGK_D_KEFLUX, QGK_D_APEGEN,
+  QGK_D_KESPEC_MODAL, QGK_D_PESPEC_MODAL, QGK_D_APESPEC, QGK_D_KEFLUX_DIV, QGK_D_APEFLUX_DIV,
+  QGK_NDIAG
+};
+/* Per-member constants of the diagnostics, 2*nz*nz + 5*nz + 5 doubles:
+ * Hi[nz], H, S[nz][nz] (stretching matrix), del1, del2, rd^-2, U1-U2,
+ * Vbg[nz], Qx[nz], f2gpi[nz] (nz-1 used), inverse of the vertical-mode
+ * matrix pmodes [nz][nz], kdi2[nz] (pyqg/model.py:284-313). */
+int qgk_diag_set_params(qgk_bucket* b, int member, const double* params);
+/* Adds every diagnostic whose bit is set in mask, from the current fields
+ * (the point of _calc_diagnostics in model.py:406-436: after inversion,
+ * advection, friction and parameterizations, before the time step).  dt and
+ * the AB weights dt1..dt3 of the coming step enter Dissspec / ENSDissspec.
+ * instant != 0 writes a one-shot evaluation to a separate set instead.
+ * Needs QGK_FULL_STATE. */
+int qgk_diag_accumulate(qgk_bucket* b, unsigned int mask, int instant, double dt, double dt1,
+                        double dt2, double dt3);
+/* Copies one diagnostic out: member >= 0 one member, -1 every member
+ * ([M][slots][...]), -2 the sum over members.  Sums, not means: the caller
+ * keeps the count. */
+int qgk_diag_read(qgk_bucket* b, int diag, int member, int instant, double* host);
+int qgk_diag_reset(qgk_bucket* b);
+
 #ifdef __cplusplus
 }
 #endif

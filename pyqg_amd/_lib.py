@@ -31,7 +31,16 @@ EXPORTS = ["qgk_create", "qgk_destroy", "qgk_last_error", "qgk_set_const", "qgk_
            "qgk_timing_enable", "qgk_timing", "qgk_sync", "qgk_step_buckets",
            "qgk_copy_probe", "qgk_set_variant", "qgk_get_variant", "qgk_autotune",
            "qgk_device_name", "qgk_put_device", "qgk_get_device", "qgk_coarsen",
-           "qgk_subgrid_forcing", "qgk_pack_inputs", "qgk_add_forcing_device"]
+           "qgk_subgrid_forcing", "qgk_pack_inputs", "qgk_add_forcing_device",
+           "qgk_diag_set_params", "qgk_diag_accumulate", "qgk_diag_read", "qgk_diag_reset"]
+
+# device diagnostics, in the order of the qgk_diag enum
+DIAGS = ["Ensspec", "KEspec", "EKEdiss", "KEfrictionspec", "EKE", "Dissspec", "ENSDissspec",
+         "paramspec", "ENSparamspec", "entspec", "paramspec_APEflux", "paramspec_KEflux",
+         "ENSflux", "ENSgenspec", "ENSfrictionspec", "APEgenspec", "APEflux", "KEflux", "APEgen",
+         "KEspec_modal", "PEspec_modal", "APEspec", "KEflux_div", "APEflux_div"]
+DIAG_ID = {name: i for i, name in enumerate(DIAGS)}
+SCALAR_DIAGS = {"EKEdiss", "EKE", "APEgen"}
 
 
 class QGKError(RuntimeError):
@@ -104,6 +113,10 @@ def _load():
         "qgk_subgrid_forcing": (I, [P, P, P]),
         "qgk_pack_inputs": (I, [P, P, I, P]),
         "qgk_add_forcing_device": (I, [P, P, I, P]),
+        "qgk_diag_set_params": (I, [P, I, P]),
+        "qgk_diag_accumulate": (I, [P, ctypes.c_uint, I, D, D, D, D]),
+        "qgk_diag_read": (I, [P, I, I, I, P]),
+        "qgk_diag_reset": (I, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -140,6 +153,7 @@ class Bucket:
         check(lib.qgk_create(int(nmembers), int(nz), int(ny), int(nx), int(flags),
                              ctypes.byref(h)))
         self.h = h
+        self.flags = int(flags)
         self.M, self.nz, self.ny, self.nx = int(nmembers), int(nz), int(ny), int(nx)
         self.nl, self.nk = self.ny, self.nx // 2 + 1
 
@@ -212,6 +226,50 @@ class Bucket:
         r, c = ctypes.c_int(), ctypes.c_int()
         check(lib.qgk_get_variant(self.h, ctypes.byref(r), ctypes.byref(c)))
         return r.value, c.value
+
+    # ---------------------------------------------------------- diagnostics
+    def diag_slots(self, name):
+        """Leading shape of one member's value of a device diagnostic."""
+        nz = self.nz
+        if name in ("Ensspec", "KEspec", "KEspec_modal", "EKE"):
+            return nz
+        if name == "PEspec_modal":
+            return nz - 1
+        return 1
+
+    def diag_set_params(self, member, params):
+        a = np.ascontiguousarray(params, dtype=np.float64)
+        n = 2 * self.nz * self.nz + 5 * self.nz + 5
+        if a.size != n:
+            raise ValueError(f"diagnostic parameters: expected {n} values, got {a.size}")
+        check(lib.qgk_diag_set_params(self.h, int(member), ptr(a)))
+
+    def diag_accumulate(self, names, dt, weights, instant=False):
+        mask = 0
+        for n in names:
+            mask |= 1 << DIAG_ID[n]
+        check(lib.qgk_diag_accumulate(self.h, mask, int(bool(instant)), float(dt),
+                                      *(float(w) for w in weights)))
+
+    def diag_read(self, name, member=-2, instant=False):
+        """Accumulated sum of one diagnostic: member >= 0 one member, -1 all
+        members (leading axis M), -2 the sum over members.  Shapes follow the
+        reference: (nz, nl, nk), (nl, nk), (nz,) or a scalar."""
+        slots = self.diag_slots(name)
+        scalar = name in SCALAR_DIAGS
+        one = (slots,) if scalar else (slots, self.nl, self.nk)
+        shape = (self.M,) + one if member == -1 else one
+        out = np.empty(shape, np.float64)
+        check(lib.qgk_diag_read(self.h, DIAG_ID[name], int(member), int(bool(instant)),
+                                ptr(out)))
+        squeeze = (name not in ("Ensspec", "KEspec", "KEspec_modal", "EKE", "PEspec_modal"))
+        if squeeze:
+            out = out.reshape(out.shape[:-3] + out.shape[-2:]) if not scalar else \
+                out.reshape(out.shape[:-1])
+        return out
+
+    def diag_reset(self):
+        check(lib.qgk_diag_reset(self.h))
 
     def autotune(self, reps=5):
         r, c = ctypes.c_int(), ctypes.c_int()

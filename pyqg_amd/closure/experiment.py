@@ -28,6 +28,7 @@ from . import data as data_mod
 from . import param as param_mod
 from . import spectra
 from . import train as train_mod
+from .. import diagnostic_tools as tools
 
 
 def run(members=256, samples=51200, spinup_days=1080, every=24, epochs=8, batch=64, lr=1e-3,
@@ -36,47 +37,41 @@ def run(members=256, samples=51200, spinup_days=1080, every=24, epochs=8, batch=
                           every=every, epochs=epochs, batch=batch, lr=lr,
                           online_members=online_members, online_days=online_days,
                           avg_days=avg_days, dt_hi=3600., dt_lo=dt_lo)}
+    L = data_mod.QG_EDDY["L"]
+    g_hi, g_lo = tools.grid(256, L), tools.grid(64, L)
     # ---------------------------------------------------------- 1. data
     gen = data_mod.PairGenerator(nmembers=members, n_hi=256, n_lo=64, dt=3600., seed=seed)
     t0 = time.perf_counter()
     gen.spin_up(spinup_days * 24)
     hb = gen.hi.buckets[0]
-    torch.cuda.synchronize()
+    hb.sync()
+    log(f"spin-up: {spinup_days} days of {members} members in {time.perf_counter() - t0:.1f} s")
     n_hold = max(samples // 10, members)
     xs, ys = [], []
-    from .. import constants as C
-    g_hi = C.grid(256, 256, gen.phys["L"], gen.phys["L"])
-    g_lo = C.grid(64, 64, gen.phys["L"], gen.phys["L"])
-    Hi = np.array([gen.phys["H1"], gen.phys["H1"] / gen.phys["delta"]])
-    kmax_lo = g_lo["kk"].max()
-    e_hi = None
     n = 0
+    nsnap = 0
     t_hi_steps = 0.0
     while n < samples + n_hold:
         ts = time.perf_counter()
         gen.hi.step(every)
         hb.sync()
         t_hi_steps += time.perf_counter() - ts
+        gen.hi.accumulate_diagnostics(["KEspec"])
         x, y = gen.sample()
         xs.append(x)
         ys.append(y)
-        ph = hb.get("ph")
-        spec = np.mean([spectra.ke_spectrum(p, g_hi["kk"], g_hi["ll"], g_hi["wv2"], Hi,
-                                            kmax_lo)[1] for p in ph], axis=0)
-        e_hi = spec if e_hi is None else e_hi + spec
         n += x.shape[0]
-    nsnap = len(xs)
-    e_hi /= nsnap
+        nsnap += 1
+        if nsnap % 50 == 0:
+            log(f"sampled {n} pairs")
+    e_hi = gen.hi.get_diagnostic("KEspec", mean=True)[0]
     x = torch.cat(xs)
     y = torch.cat(ys)
     del xs, ys
     x_tr, y_tr = x[:samples], y[:samples]
     x_te, y_te = x[samples:], y[samples:]
-    hi_state = gen.hi.get("q")
     out["data_seconds"] = time.perf_counter() - t0
-    # cost of the high-resolution model per simulated day (all members)
-    hi_steps = nsnap * every
-    out["hi_res_ms_per_step"] = 1e3 * t_hi_steps / hi_steps
+    out["hi_res_ms_per_step"] = 1e3 * t_hi_steps / (nsnap * every)
     log(f"data: {x_tr.shape[0]} train + {x_te.shape[0]} held-out pairs in "
         f"{out['data_seconds']:.1f} s")
 
@@ -90,46 +85,51 @@ def run(members=256, samples=51200, spinup_days=1080, every=24, epochs=8, batch=
                     "final_loss": info["history"][-1][1] if info["history"] else None}
     out["offline"] = train_mod.evaluate(net, x_te, y_te, scales)
     flop_per_sample = 2 * 64 * 64 * 25 * sum(
-        net.channels[l] * net.channels[l + 1] for l in range(net.nlayers))
+        net.chp[l] * net.chp[l + 1] for l in range(net.nlayers))
     out["train"]["tflops"] = 3 * flop_per_sample * info["steps"] * batch / info["seconds"] / 1e12
     log(f"offline R^2 = {out['offline']['r2']:.3f}")
     del x, y, x_tr, y_tr
 
     # ---------------------------------------------------------- 3. online
     p = param_mod.CNNParameterization(net, scales)
+    # coarse-grained high-resolution states as the common start
+    _lib.check(_lib.lib.qgk_coarsen(hb.h, gen.lo.buckets[0].h, _lib.FIELD_ID["qh"],
+                                    _lib.FIELD_ID["qh"], _lib.ptr(gen.G)))
+    qlo = gen.lo.get("q")
     runs = {}
+    spectra_lo = {}
     for name, prm in (("plain", None), ("cnn", p)):
         ce = param_mod.ClosedEnsemble(online_members, 64, dt_lo, gen.phys, param=prm, seed=seed)
-        # start from coarse-grained high-resolution states
-        _lib.check(_lib.lib.qgk_coarsen(hb.h, gen.lo.buckets[0].h, _lib.FIELD_ID["qh"],
-                                        _lib.FIELD_ID["qh"], _lib.ptr(gen.G)))
-        qlo = gen.lo.get("q")
         ce.ens.set_q([qlo[i % members] for i in range(online_members)])
         per_day = int(round(86400. / dt_lo))
         nsteps = online_days * per_day
         navg = avg_days * per_day
-        e = None
-        cnt = 0
+        cb = ce.ens.buckets[0]
         t1 = time.perf_counter()
-        for s in range(nsteps):
-            ce.step(1)
-            if s >= nsteps - navg and s % per_day == 0:
-                ph = ce.ph()
-                sp = np.mean([spectra.ke_spectrum(q, g_lo["kk"], g_lo["ll"], g_lo["wv2"], Hi,
-                                                  kmax_lo)[1] for q in ph], axis=0)
-                e = sp if e is None else e + sp
-                cnt += 1
-        torch.cuda.synchronize()
+        done = 0
+        # accumulate KEspec at the start of one step a day over the last avg_days
+        for s0 in range(nsteps - navg, nsteps, per_day):
+            if s0 > done:
+                ce.step(s0 - done)
+                done = s0
+            ce.ens.accumulate_diagnostics(["KEspec"])
+        if nsteps > done:
+            ce.step(nsteps - done)
+        cb.sync()
         wall = time.perf_counter() - t1
-        e /= cnt
+        e = ce.ens.get_diagnostic("KEspec", mean=True)[0]
+        spectra_lo[name] = e
         finite = bool(np.isfinite(ce.ens.get("q", 0)).all())
-        runs[name] = {"ke_spectrum_error": spectra.spectrum_error(e, e_hi),
+        runs[name] = {"ke_rmse": spectra.ke_errors(e, g_lo, e_hi, g_hi),
+                      "ke_rel_err": spectra.relative_errors(e, g_lo, e_hi, g_hi),
                       "ms_per_step": 1e3 * wall / nsteps, "finite": finite,
-                      "total_ke": float(e.sum())}
-        log(f"{name}: KE-spectrum error {runs[name]['ke_spectrum_error']:.3f}, "
+                      "total_ke": float(tools.spec_sum(e).sum())}
+        log(f"{name}: KE-spectrum rel. error {runs[name]['ke_rel_err']}, "
             f"{runs[name]['ms_per_step']:.2f} ms/step")
+    runs["cnn"]["similarity_vs_plain"] = spectra.similarity(runs["cnn"]["ke_rmse"],
+                                                           runs["plain"]["ke_rmse"])
     out["online"] = runs
-    out["hi_res_total_ke"] = float(e_hi.sum())
+    out["hi_res_total_ke"] = float(tools.spec_sum(e_hi).sum())
     # cost per member and simulated day
     hi_cost = out["hi_res_ms_per_step"] / members * 86400. / 3600.
     lo_cost = runs["cnn"]["ms_per_step"] / online_members * 86400. / dt_lo

@@ -63,9 +63,8 @@ class CNNParameterization:
 class ClosedEnsemble:
     """Coarse two-layer QG ensemble stepped with the CNN closure on the GPU.
 
-    With `param=None` it is the plain coarse model stepped phase by phase (the
-    same order and kernels, without the forcing), the baseline the closure is
-    compared with."""
+    With `param=None` it is the plain coarse model (fused steps), the baseline
+    the closure is compared with."""
 
     def __init__(self, nmembers, n, dt, phys, param=None, seed=0):
         from ..ensemble import Ensemble
@@ -82,13 +81,15 @@ class ClosedEnsemble:
                                    device="cuda")
             self.inv_in = np.ascontiguousarray(1.0 / param.scales.in_scale)
             self.out_scale = np.ascontiguousarray(param.scales.out_scale)
-        self.ablevel = 0
 
     @property
     def t(self):
         return self.ens.t
 
     def step(self, nsteps=1):
+        if self.param is None:
+            self.ens.step(nsteps)
+            return
         h = self.b.h
         L = _lib.lib
         for _ in range(int(nsteps)):
@@ -102,7 +103,7 @@ class ClosedEnsemble:
                 torch.cuda.current_stream().synchronize()
                 _lib.check(L.qgk_add_forcing_device(h, ctypes_p(out), out.shape[-1],
                                                     _lib.ptr(self.out_scale)))
-            w, self.ablevel = C.ab_weights(self.ablevel, self.ens.dt)
+            w, self.ens.ablevel = C.ab_weights(self.ens.ablevel, self.ens.dt)
             _lib.check(L.qgk_timestep(h, *w))
             self.ens.t += self.ens.dt
             self.ens.tc += 1

@@ -1,47 +1,46 @@
-"""Isotropic kinetic-energy spectra (the closure's evaluation metric).
+"""KE-spectrum error of a coarse model against the high-resolution truth.
 
-Follows the definitions of pyqg's KEspec diagnostic (pyqg/model.py:657-663:
-wv2 |ph|^2 / M^2 per layer) and of diagnostic_tools.calc_ispec
-(pyqg/diagnostic_tools.py:51-126: sum over annuli of width dk, counting the
-half-plane modes twice except kx = 0 and kx = nx/2), weighted by layer
-thickness.
+In pyqg's own terms (diagnostic_tools.diagnostic_differences /
+diagnostic_similarities, pyqg/diagnostic_tools.py:128-262): the time-mean
+KEspec diagnostic of each run (accumulated on the device, averaged over the
+ensemble's members) is made isotropic per layer with calc_ispec, cut to the
+wavenumbers both grids resolve, and compared by RMSE; the similarity of a
+closed run is 1 - RMSE(closed, truth) / RMSE(plain, truth).
 """
 import numpy as np
 
-
-def ke_density(ph, wv2, Hi):
-    """Layer-thickness weighted KE density on the rfft2 grid: (nl, nk)."""
-    M = ph.shape[-2] * (2 * (ph.shape[-1] - 1))
-    H = float(np.sum(Hi))
-    e = wv2[np.newaxis] * np.abs(ph) ** 2 / M ** 2
-    return np.tensordot(np.asarray(Hi, dtype=float) / H, e, axes=(0, 0))
+from ..diagnostic_tools import calc_ispec
 
 
-def isotropic(field, kk, ll, kmax=None):
-    """Sum an (nl, nk) rfft2-grid quantity over annuli of width dk = kk[1]."""
-    dk = kk[1] - kk[0]
-    k, l = np.meshgrid(kk, ll)
-    kappa = np.sqrt(k ** 2 + l ** 2)
-    w = np.full(field.shape, 2.0)
-    w[:, 0] = 1.0
-    w[:, -1] = 1.0
-    kmax = kmax if kmax is not None else kk.max()
-    edges = np.arange(0.5 * dk, kmax + dk, dk)
-    centers = 0.5 * (edges[1:] + edges[:-1])
-    spec = np.zeros(len(centers))
-    idx = np.digitize(kappa.ravel(), edges) - 1
-    ok = (idx >= 0) & (idx < len(centers))
-    np.add.at(spec, idx[ok], (w * field).ravel()[ok])
-    return centers, spec
+def layer_spectra(kespec, g):
+    """Isotropic spectrum of every layer of a (nz, nl, nk) KEspec: (kr, [nz][nkr])."""
+    kr = None
+    out = []
+    for z in range(kespec.shape[0]):
+        kr, s = calc_ispec(g, kespec[z])
+        out.append(s)
+    return kr, np.array(out)
 
 
-def ke_spectrum(ph, kk, ll, wv2, Hi, kmax=None):
-    """Isotropic KE spectrum of one state: (kappa, E(kappa))."""
-    return isotropic(ke_density(ph, wv2, Hi), kk, ll, kmax)
+def ke_errors(kespec, g, kespec_ref, g_ref):
+    """{"KEspec1": rmse, "KEspec2": rmse, ...} on the common wavenumbers."""
+    kr, s = layer_spectra(kespec, g)
+    kr_ref, s_ref = layer_spectra(kespec_ref, g_ref)
+    n = min(len(kr), len(kr_ref))
+    return {f"KEspec{z + 1}": float(np.sqrt(np.mean((s[z, :n] - s_ref[z, :n]) ** 2)))
+            for z in range(s.shape[0])}
 
 
-def spectrum_error(e_test, e_ref):
-    """Relative L2 difference of two spectra on their common bins."""
-    n = min(len(e_test), len(e_ref))
-    a, b = np.asarray(e_test[:n]), np.asarray(e_ref[:n])
-    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+def similarity(err_model, err_baseline):
+    """1 - d(model, truth) / d(baseline, truth) per entry."""
+    return {k: 1.0 - err_model[k] / err_baseline[k] for k in err_model}
+
+
+def relative_errors(kespec, g, kespec_ref, g_ref):
+    """Relative L2 difference of the per-layer isotropic spectra (common bins)."""
+    kr, s = layer_spectra(kespec, g)
+    kr_ref, s_ref = layer_spectra(kespec_ref, g_ref)
+    n = min(len(kr), len(kr_ref))
+    return {f"KEspec{z + 1}": float(np.linalg.norm(s[z, :n] - s_ref[z, :n])
+                                    / np.linalg.norm(s_ref[z, :n]))
+            for z in range(s.shape[0])}

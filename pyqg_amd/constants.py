@@ -139,6 +139,19 @@ def exponential_stratification(nz=10, Htot=4000.0, hscale=800.0, uscale=1000.0,
     return H, rho, U
 
 
+def vertical_modes(S, Hi):
+    """Deformation wavenumbers kdi2 (ascending eigenvalues of -S) and the
+    pressure modes normalised so that sum_k Hi_k pmodes_k^2 = H
+    (model.py:286-305).  Returns (kdi2, pmodes)."""
+    evals, evecs = np.linalg.eig(-S)
+    asort = evals.argsort()
+    kdi2 = evals[asort]
+    pmodes = evecs[:, asort]
+    H = np.sum(Hi)
+    Ai = (H / (Hi[:, np.newaxis] * (pmodes ** 2)).sum(axis=0)) ** 0.5
+    return kdi2, Ai[np.newaxis, :] * pmodes
+
+
 def ab_weights(ablevel, dt):
     """Adams-Bashforth weights and the next ablevel (kernel.pyx:482-500)."""
     if ablevel == 0:

@@ -909,13 +909,410 @@ __global__ void k_unpack_forcing(const float* __restrict__ net, const double* __
   }
 }
 
+// ==========================================================================
+// Diagnostics (pyqg/model.py:644-736, qg_diagnostics.py:6-70,
+// qg_model.py:213-263, layered_model.py:242-301)
+// ==========================================================================
+constexpr int kDiagNzMax = 16;
+
+// Offsets into the per-member diagnostic parameter block (doubles):
+// Hi[nz], H, S[nz][nz], del1, del2, rd^-2, U1-U2, Vbg[nz], Qx[nz],
+// f2gpi[nz] (nz-1 used), inverse of the vertical-mode matrix [nz][nz], kdi2[nz].
+struct DiagLayout {
+  int nz;
+  __host__ __device__ int Hi() const { return 0; }
+  __host__ __device__ int H() const { return nz; }
+  __host__ __device__ int S() const { return nz + 1; }
+  __host__ __device__ int del1() const { return nz + 1 + nz * nz; }
+  __host__ __device__ int del2() const { return del1() + 1; }
+  __host__ __device__ int rd2inv() const { return del1() + 2; }
+  __host__ __device__ int Ushear() const { return del1() + 3; }
+  __host__ __device__ int Vbg() const { return del1() + 4; }
+  __host__ __device__ int Qx() const { return Vbg() + nz; }
+  __host__ __device__ int f2gpi() const { return Qx() + nz; }
+  __host__ __device__ int pinv() const { return f2gpi() + nz; }
+  __host__ __device__ int kdi2() const { return pinv() + nz * nz; }
+  __host__ __device__ int len() const { return kdi2() + nz; }
+};
+
+struct DiagArgs {
+  const cd *qh, *ph, *D, *P, *PP;  // state [M][nz][nl][ldk]
+  const cd *dqh, *duh, *dvh;       // parameterization tendencies (or null)
+  const cd *Jq, *Jpxi, *J3;        // Jacobians; J3: Jptpc [M][1] (nz = 2) or JSp [M][nz]
+  const double* prm;               // [M][plen]
+  double* acc;                     // [slot][M][nl][ldk]
+  double* pts;                     // per-point APEgen terms [M][nl][ldk] (or null)
+  int M, nz, nl, nk, ldk, plen;
+  double invM2, dt, dt1, dt2, dt3;
+  int slot[QGK_NDIAG];             // first accumulator slot, -1 = not requested
+};
+
+// (a . v)_i at one spectral point: the member's inversion table or the
+// closed-form two-layer inverse (qg_model.py:146-161)
+__device__ cd inv_apply(const MemberParams& P, int nz, int i, const cd* v, size_t plane,
+                        size_t pt) {
+  if (P.a) {
+    cd acc = {0.0, 0.0};
+    for (int j = 0; j < nz; ++j) {
+      const double a = P.a[((size_t)i * nz + j) * plane + pt];
+      acc.x += a * v[j].x;
+      acc.y += a * v[j].y;
+    }
+    return acc;
+  }
+  const double w = P.k2l2[pt];
+  const double det = w * (w + P.F1 + P.F2);
+  if (det == 0.0) return {0.0, 0.0};
+  const double a0 = i == 0 ? -(w + P.F2) / det : -P.F2 / det;
+  const double a1 = i == 0 ? -P.F1 / det : -(w + P.F1) / det;
+  return {a0 * v[0].x + a1 * v[1].x, a0 * v[0].y + a1 * v[1].y};
+}
+
+__device__ __forceinline__ double re_conj_mul(cd a, cd b) {  // Re(conj(a) * b)
+  return a.x * b.x + a.y * b.y;
+}
+__device__ __forceinline__ double abs2(cd a) { return a.x * a.x + a.y * a.y; }
+
+// Every requested spectral diagnostic at every (member, l, k), added into its
+// accumulator; the per-point APEgen terms go to pts for a per-member sum.
+__global__ void k_diag_spectral(DiagArgs A, const MemberParams* __restrict__ mp) {
+  const size_t plane = (size_t)A.nl * A.ldk;
+  const size_t total = (size_t)A.M * plane;
+  const int nz = A.nz;
+  const DiagLayout L{nz};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int m = (int)(i / plane);
+    const size_t pt = i % plane;
+    const int l = (int)(pt / A.ldk), k = (int)(pt % A.ldk);
+    if (k >= A.nk) {
+      if (A.pts) A.pts[i] = 0.0;
+      continue;
+    }
+    const MemberParams& P = mp[m];
+    const double* pr = A.prm + (size_t)m * A.plen;
+    const size_t base = (size_t)m * nz * plane + pt;
+    cd qh[kDiagNzMax], ph[kDiagNzMax];
+    for (int z = 0; z < nz; ++z) {
+      qh[z] = A.qh[base + z * plane];
+      ph[z] = A.ph[base + z * plane];
+    }
+    const double kk = P.kk[k], ll = P.ll[l], wv2 = P.k2l2[pt];
+    const double H = pr[L.H()];
+    const double* Hi = pr + L.Hi();
+    const double s = A.invM2;
+    auto add = [&](int d, int j, double v) {
+      A.acc[((size_t)(A.slot[d] + j) * A.M + m) * plane + pt] += v;
+    };
+    const int bot = nz - 1;
+    if (A.slot[QGK_D_ENSSPEC] >= 0)
+      for (int z = 0; z < nz; ++z) add(QGK_D_ENSSPEC, z, abs2(qh[z]) * s);
+    if (A.slot[QGK_D_KESPEC] >= 0)
+      for (int z = 0; z < nz; ++z) add(QGK_D_KESPEC, z, wv2 * abs2(ph[z]) * s);
+    if (A.slot[QGK_D_KEFRICTIONSPEC] >= 0)
+      add(QGK_D_KEFRICTIONSPEC, 0, -P.rek * Hi[bot] / H * wv2 * abs2(ph[bot]) * s);
+    if (A.slot[QGK_D_DISSSPEC] >= 0 || A.slot[QGK_D_ENSDISSSPEC] >= 0) {
+      const double f1 = P.filtr[pt] - 1.0;
+      double de = 0.0, dz = 0.0;
+      for (int z = 0; z < nz; ++z) {
+        const size_t o = base + z * plane;
+        const cd d = A.D[o], p = A.P[o], pp = A.PP[o];
+        cd r = {qh[z].x + A.dt1 * d.x + A.dt2 * p.x + A.dt3 * pp.x,
+                qh[z].y + A.dt1 * d.y + A.dt2 * p.y + A.dt3 * pp.y};
+        r = {f1 * r.x, f1 * r.y};
+        de += Hi[z] * re_conj_mul(ph[z], r);
+        dz += Hi[z] * re_conj_mul(qh[z], r);
+      }
+      if (A.slot[QGK_D_DISSSPEC] >= 0) add(QGK_D_DISSSPEC, 0, -de / H / A.dt * s);
+      if (A.slot[QGK_D_ENSDISSSPEC] >= 0) add(QGK_D_ENSDISSSPEC, 0, dz / H / A.dt * s);
+    }
+    const bool want_param = A.slot[QGK_D_PARAMSPEC] >= 0 || A.slot[QGK_D_ENSPARAMSPEC] >= 0 ||
+                            A.slot[QGK_D_PARAMSPEC_APEFLUX] >= 0 ||
+                            A.slot[QGK_D_PARAMSPEC_KEFLUX] >= 0;
+    if (want_param) {
+      cd dp[kDiagNzMax];
+      for (int z = 0; z < nz; ++z) {
+        const size_t o = base + z * plane;
+        cd v = {0.0, 0.0};
+        if (A.duh) {  // -il duh + ik dvh
+          const cd du = A.duh[o], dv = A.dvh[o];
+          v.x += ll * du.y - kk * dv.y;
+          v.y += -ll * du.x + kk * dv.x;
+        }
+        if (A.dqh) {
+          const cd dq = A.dqh[o];
+          v.x += dq.x;
+          v.y += dq.y;
+        }
+        dp[z] = v;
+      }
+      double ps = 0.0, es = 0.0, ape = 0.0, ke = 0.0;
+      cd adp[kDiagNzMax];
+      const bool want_a = A.slot[QGK_D_PARAMSPEC_APEFLUX] >= 0 ||
+                          A.slot[QGK_D_PARAMSPEC_KEFLUX] >= 0;
+      if (want_a)
+        for (int z = 0; z < nz; ++z) adp[z] = inv_apply(P, nz, z, dp, plane, pt);
+      for (int z = 0; z < nz; ++z) {
+        const double hr = Hi[z] / H;
+        ps += hr * re_conj_mul(ph[z], dp[z]);
+        es += hr * re_conj_mul(qh[z], dp[z]);
+        if (want_a) {
+          cd t = {0.0, 0.0};
+          for (int j = 0; j < nz; ++j) {
+            const double Sij = pr[L.S() + z * nz + j];
+            t.x += Sij * adp[j].x;
+            t.y += Sij * adp[j].y;
+          }
+          ape += hr * re_conj_mul(ph[z], t);
+          ke += hr * re_conj_mul(ph[z], adp[z]);
+        }
+      }
+      if (A.slot[QGK_D_PARAMSPEC] >= 0) add(QGK_D_PARAMSPEC, 0, -ps * s);
+      if (A.slot[QGK_D_ENSPARAMSPEC] >= 0) add(QGK_D_ENSPARAMSPEC, 0, es * s);
+      if (A.slot[QGK_D_PARAMSPEC_APEFLUX] >= 0) add(QGK_D_PARAMSPEC_APEFLUX, 0, -ape * s);
+      if (A.slot[QGK_D_PARAMSPEC_KEFLUX] >= 0) add(QGK_D_PARAMSPEC_KEFLUX, 0, wv2 * ke * s);
+    }
+    if (A.slot[QGK_D_ENTSPEC] >= 0) {
+      cd b = {0.0, 0.0};
+      for (int z = 0; z < nz; ++z) {
+        b.x += Hi[z] * qh[z].x;
+        b.y += Hi[z] * qh[z].y;
+      }
+      b = {b.x / H, b.y / H};
+      add(QGK_D_ENTSPEC, 0, abs2(b) * s);
+    }
+    if (A.slot[QGK_D_ENSFLUX] >= 0) {
+      double e = 0.0;
+      for (int z = 0; z < nz; ++z) e += -Hi[z] * re_conj_mul(qh[z], A.Jq[base + z * plane]);
+      add(QGK_D_ENSFLUX, 0, e / H * s);
+    }
+    if (A.slot[QGK_D_ENSGENSPEC] >= 0 || A.slot[QGK_D_APEGENSPEC] >= 0) {
+      double eg = 0.0, ag = 0.0;
+      for (int z = 0; z < nz; ++z) {
+        cd sph = {0.0, 0.0};
+        for (int j = 0; j < nz; ++j) {
+          const double Sij = pr[L.S() + z * nz + j];
+          sph.x += Sij * ph[j].x;
+          sph.y += Sij * ph[j].y;
+        }
+        // (ilQx - ikQy) conj(Sph) ph: a purely imaginary factor i*c
+        const double c = ll * pr[L.Qx() + z] - kk * P.Qy[z];
+        const cd w = cmul(cconj(sph), ph[z]);
+        eg += Hi[z] * (-c * w.y);
+        // (Ubg k + Vbg l) Re(i conj(ph) Sph)
+        const cd w2 = cmul(cconj(ph[z]), sph);
+        ag += Hi[z] * (P.Ubg[z] * kk + pr[L.Vbg() + z] * ll) * (-w2.y);
+      }
+      if (A.slot[QGK_D_ENSGENSPEC] >= 0) add(QGK_D_ENSGENSPEC, 0, eg / H * s);
+      if (A.slot[QGK_D_APEGENSPEC] >= 0) add(QGK_D_APEGENSPEC, 0, ag / H * s);
+    }
+    if (A.slot[QGK_D_ENSFRICTIONSPEC] >= 0)
+      add(QGK_D_ENSFRICTIONSPEC, 0,
+          P.rek * Hi[bot] / H * wv2 * re_conj_mul(qh[bot], ph[bot]) * s);
+    if (A.slot[QGK_D_APEFLUX] >= 0) {  // two layers
+      const cd d = csub(ph[0], ph[1]);
+      const cd j = A.J3[(size_t)m * plane + pt];
+      add(QGK_D_APEFLUX, 0,
+          pr[L.rd2inv()] * pr[L.del1()] * pr[L.del2()] * re_conj_mul(j, d) * s);
+    }
+    if (A.slot[QGK_D_KEFLUX] >= 0) {
+      const cd j0 = A.Jpxi[base], j1 = A.Jpxi[base + plane];
+      add(QGK_D_KEFLUX, 0,
+          (pr[L.del1()] * re_conj_mul(j0, ph[0]) + pr[L.del2()] * re_conj_mul(j1, ph[1])) * s);
+    }
+    if (A.pts) {  // APEgen (qg_model.py:250-262): columns 1..nk-3 count twice
+      const cd x = {pr[L.del1()] * ph[0].x + pr[L.del2()] * ph[1].x,
+                    pr[L.del1()] * ph[0].y + pr[L.del2()] * ph[1].y};
+      const cd w = cmul(x, cconj(csub(ph[0], ph[1])));
+      const double mult = (k >= 1 && k <= A.nk - 3) ? 2.0 : 1.0;
+      A.pts[i] = mult * (-kk * w.y);
+    }
+    if (A.slot[QGK_D_KESPEC_MODAL] >= 0 || A.slot[QGK_D_PESPEC_MODAL] >= 0) {
+      for (int z = 0; z < nz; ++z) {
+        cd pn = {0.0, 0.0};
+        for (int j = 0; j < nz; ++j) {
+          const double c = pr[L.pinv() + z * nz + j];
+          pn.x += c * ph[j].x;
+          pn.y += c * ph[j].y;
+        }
+        if (A.slot[QGK_D_KESPEC_MODAL] >= 0) add(QGK_D_KESPEC_MODAL, z, wv2 * abs2(pn) * s);
+        if (z >= 1 && A.slot[QGK_D_PESPEC_MODAL] >= 0)
+          add(QGK_D_PESPEC_MODAL, z - 1, pr[L.kdi2() + z] * abs2(pn) * s);
+      }
+    }
+    if (A.slot[QGK_D_APESPEC] >= 0) {
+      double a = 0.0;
+      for (int z = 0; z + 1 < nz; ++z) a += pr[L.f2gpi() + z] * abs2(csub(ph[z], ph[z + 1]));
+      add(QGK_D_APESPEC, 0, a / H * s);
+    }
+    if (A.slot[QGK_D_KEFLUX_DIV] >= 0) {
+      double a = 0.0;
+      for (int z = 0; z < nz; ++z) a += Hi[z] * re_conj_mul(ph[z], A.Jpxi[base + z * plane]);
+      add(QGK_D_KEFLUX_DIV, 0, a / H * s);
+    }
+    if (A.slot[QGK_D_APEFLUX_DIV] >= 0) {
+      double a = 0.0;
+      for (int z = 0; z < nz; ++z) a += Hi[z] * re_conj_mul(ph[z], A.J3[base + z * plane]);
+      add(QGK_D_APEFLUX_DIV, 0, a / H * s);
+    }
+  }
+}
+
+// Real-space products for a Jacobian: P1 = u * x, P2 = v * x per plane.
+// mode 0: x, u, v have nzx = nz planes per member.  mode 1 (two layers):
+// one plane x per member, u -> del1 u0 + del2 u1, v likewise (qg_model.py:216-219).
+__global__ void k_diag_products(const double* __restrict__ x, const double* __restrict__ u,
+                                const double* __restrict__ v, const double* __restrict__ prm,
+                                double* __restrict__ P1, double* __restrict__ P2, int M, int nz,
+                                int npix, int mode, int plen) {
+  const int nzx = mode == 1 ? 1 : nz;
+  const size_t total = (size_t)M * nzx * npix;
+  const DiagLayout L{nz};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int p = (int)(i % npix);
+    const int z = (int)((i / npix) % nzx);
+    const int m = (int)(i / ((size_t)npix * nzx));
+    double uu, vv;
+    if (mode == 1) {
+      const double d1 = prm[(size_t)m * plen + L.del1()], d2 = prm[(size_t)m * plen + L.del2()];
+      const size_t o = (size_t)m * nz * npix + p;
+      uu = d1 * u[o] + d2 * u[o + npix];
+      vv = d1 * v[o] + d2 * v[o + npix];
+    } else {
+      uu = u[i];
+      vv = v[i];
+    }
+    P1[i] = uu * x[i];
+    P2[i] = vv * x[i];
+  }
+}
+
+// J = sign * (ik W1 + il W2) over nzx planes per member
+__global__ void k_diag_jacobian(const cd* __restrict__ W1, const cd* __restrict__ W2,
+                                const MemberParams* __restrict__ mp, cd* __restrict__ J, int M,
+                                int nzx, int nl, int ldk, double sign) {
+  const size_t plane = (size_t)nl * ldk;
+  const size_t total = (size_t)M * nzx * plane;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t pt = i % plane;
+    const int m = (int)(i / (plane * nzx));
+    const int l = (int)(pt / ldk), k = (int)(pt % ldk);
+    const double kk = mp[m].kk[k], ll = mp[m].ll[l];
+    const cd a = W1[i], b = W2[i];
+    J[i] = {sign * (-kk * a.y - ll * b.y), sign * (kk * a.x + ll * b.x)};
+  }
+}
+
+// Spectral inputs of the inverse transforms: mode 0 xih = -wv2 ph (nz planes),
+// mode 1 ph0 - ph1 (one plane), mode 2 S ph (nz planes).
+__global__ void k_diag_prepare(const cd* __restrict__ ph, const MemberParams* __restrict__ mp,
+                               const double* __restrict__ prm, cd* __restrict__ out, int M,
+                               int nz, int nl, int ldk, int mode, int plen) {
+  const size_t plane = (size_t)nl * ldk;
+  const int nzx = mode == 1 ? 1 : nz;
+  const size_t total = (size_t)M * nzx * plane;
+  const DiagLayout L{nz};
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t pt = i % plane;
+    const int z = (int)((i / plane) % nzx);
+    const int m = (int)(i / (plane * nzx));
+    const size_t base = (size_t)m * nz * plane + pt;
+    cd r;
+    if (mode == 0) {
+      const double w = mp[m].k2l2[pt];
+      const cd p = ph[base + z * plane];
+      r = {-w * p.x, -w * p.y};
+    } else if (mode == 1) {
+      r = csub(ph[base], ph[base + plane]);
+    } else {
+      r = {0.0, 0.0};
+      for (int j = 0; j < nz; ++j) {
+        const double Sij = prm[(size_t)m * plen + L.S() + z * nz + j];
+        const cd p = ph[base + j * plane];
+        r.x += Sij * p.x;
+        r.y += Sij * p.y;
+      }
+    }
+    out[i] = r;
+  }
+}
+
+// Deterministic block reduction of one plane per block: out[b] = sum(in plane b)
+// (mode 0), or sum(u^2 + v^2) over a real plane (mode 1).
+__global__ __launch_bounds__(256) void k_diag_plane_sum(const double* __restrict__ a,
+                                                        const double* __restrict__ b,
+                                                        double* __restrict__ out, size_t n,
+                                                        int mode) {
+  __shared__ double red[256];
+  const size_t off = (size_t)blockIdx.x * n;
+  double acc = 0.0;
+  for (size_t i = threadIdx.x; i < n; i += 256) {
+    const double x = a[off + i];
+    acc += mode == 1 ? x * x + b[off + i] * b[off + i] : x;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// Scalar diagnostics from the plane sums: EKE[m][z] += 0.5 sum_z / npix;
+// EKEdiss[m] += Hi[-1]/H rek sum_bottom / npix; APEgen[m] += U rd^-2 del1 del2
+// sum / M^2.  sums: [M][nz] (u^2+v^2) then [M] (APEgen points).
+__global__ void k_diag_scalars(const double* __restrict__ sums, const double* __restrict__ prm,
+                               const MemberParams* __restrict__ mp, double* __restrict__ accs,
+                               int M, int nz, int plen, double npix, double invM2, int s_eke,
+                               int s_ekediss, int s_apegen) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const DiagLayout L{nz};
+  const double* pr = prm + (size_t)m * plen;
+  if (s_eke >= 0)
+    for (int z = 0; z < nz; ++z)
+      accs[(size_t)(s_eke + z) * M + m] += 0.5 * (sums[(size_t)m * nz + z] / npix);
+  if (s_ekediss >= 0)
+    accs[(size_t)s_ekediss * M + m] +=
+        pr[L.Hi() + nz - 1] / pr[L.H()] * mp[m].rek * (sums[(size_t)m * nz + nz - 1] / npix);
+  if (s_apegen >= 0)
+    accs[(size_t)s_apegen * M + m] += pr[L.Ushear()] * pr[L.rd2inv()] * pr[L.del1()] *
+                                      pr[L.del2()] * sums[(size_t)M * nz + m] * invM2;
+}
+
+// out[j][pt] = sum over members of acc[slot + j][m][pt]
+__global__ void k_diag_member_sum(const double* __restrict__ acc, double* __restrict__ out,
+                                  int M, int nsl, size_t plane) {
+  const size_t total = (size_t)nsl * plane;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t j = i / plane, pt = i % plane;
+    double s = 0.0;
+    for (int m = 0; m < M; ++m) s += acc[(j * M + m) * plane + pt];
+    out[i] = s;
+  }
+}
+
 // Plain streaming copy (16 B per lane), the calibration kernel for HBM traffic
 // counters and the measured copy bandwidth.
+// Four independent 16-byte loads per lane are issued before their stores, so
+// enough bytes are in flight to reach the HBM ceiling.
 __global__ void k_stream_copy(const double2* __restrict__ in, double2* __restrict__ out,
                               size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    out[i] = in[i];
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const double2 a = in[i], b = in[i + stride], c = in[i + 2 * stride], d = in[i + 3 * stride];
+    out[i] = a;
+    out[i + stride] = b;
+    out[i + 2 * stride] = c;
+    out[i + 3 * stride] = d;
+  }
+  for (; i < n; i += stride) out[i] = in[i];
 }
 
 // ==========================================================================
@@ -978,6 +1375,19 @@ struct qgk_bucket {
   // register radix of the fused row and column kernels (autotuned)
   int erow = 8, ecol = 8;
   double* small = nullptr;  // 256 doubles of per-call constants (closure scales)
+  // diagnostics: per-member parameters, accumulators and work space
+  std::vector<double> dprm_h;  // [M][plen]
+  std::vector<char> dprm_set;  // per member: parameters given
+  double* dprm = nullptr;
+  bool dprm_dirty = true;
+  int dslot[QGK_NDIAG] = {};   // first slot (spectral or scalar set), -1 = invalid for nz
+  int dnsl[QGK_NDIAG] = {};    // slots per diagnostic
+  int nslot = 0, nscal = 0;
+  double *dacc = nullptr, *dsc = nullptr;      // running sums [slot][M][nl][ldk], [slot][M]
+  double *dacc_i = nullptr, *dsc_i = nullptr;  // one-shot evaluation
+  cd* dW = nullptr;            // 5 spectral sets of M*nz planes: W1, W2, Jq, Jpxi, J3
+  double* dR = nullptr;        // 3 real sets of M*nz planes: x, P1, P2
+  double* dsum = nullptr;      // M*nz + M plane sums; then nz planes of member sums
   hipStream_t lst = nullptr;  // stream override for fused-step launches (graph capture)
   long gen = 0;               // bumped whenever launch arguments change (graph cache key)
   // hipGraph of two fused steps of a set of buckets, replayed while the AB
@@ -1897,16 +2307,23 @@ int qgk_copy_probe(size_t nbytes, int iters, double* ms_per_copy) {
     rc = fail(QGK_E_HIP, "copy probe set-up failed");
   } else {
     hipMemsetAsync(a, 0, n * 16, st);
-    const int blocks = 256 * 8 * 4;
-    hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, st, a, c, n);
-    hipEventRecord(e0, st);
-    for (int i = 0; i < iters; ++i)
+    // best of a few grid sizes (workgroups per CU of a 256-CU device: 4..32)
+    double best = 1e30;
+    for (int blocks : {1024, 2048, 4096, 8192}) {
       hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, st, a, c, n);
-    hipEventRecord(e1, st);
-    float ms = 0;
-    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
-      rc = fail(QGK_E_HIP, "copy probe timing failed");
-    *ms_per_copy = ms / iters;
+      hipEventRecord(e0, st);
+      for (int i = 0; i < iters; ++i)
+        hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, st, a, c, n);
+      hipEventRecord(e1, st);
+      float ms = 0;
+      if (hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = fail(QGK_E_HIP, "copy probe timing failed");
+        break;
+      }
+      if (ms / iters < best) best = ms / iters;
+    }
+    *ms_per_copy = best;
   }
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
@@ -2164,6 +2581,297 @@ int qgk_add_forcing_device(qgk_bucket* b, const float* net_out, int coutp, const
 
 int qgk_step(qgk_bucket* b, int nsteps, const double* coeffs, int use_graph) {
   return qgk_step_buckets(&b, 1, nsteps, coeffs, use_graph);
+}
+
+}  // extern "C"
+
+// ==========================================================================
+// Diagnostics (host side)
+// ==========================================================================
+namespace {
+
+bool diag_is_scalar(int d) { return d == QGK_D_EKEDISS || d == QGK_D_EKE || d == QGK_D_APEGEN; }
+
+// Slot table for this bucket's nz; -1 marks diagnostics that do not exist for it.
+void diag_layout(qgk_bucket* b) {
+  if (b->nslot || b->nscal) return;
+  const int nz = b->nz;
+  for (int d = 0; d < QGK_NDIAG; ++d) {
+    int n = 1;
+    if (d == QGK_D_ENSSPEC || d == QGK_D_KESPEC || d == QGK_D_KESPEC_MODAL || d == QGK_D_EKE)
+      n = nz;
+    if (d == QGK_D_PESPEC_MODAL) n = nz - 1;
+    bool ok = true;
+    if ((d == QGK_D_APEFLUX || d == QGK_D_KEFLUX || d == QGK_D_APEGEN) && nz != 2) ok = false;
+    if ((d == QGK_D_PESPEC_MODAL || d == QGK_D_APESPEC) && nz < 2) ok = false;
+    if (!ok || n <= 0) {
+      b->dslot[d] = -1;
+      b->dnsl[d] = 0;
+      continue;
+    }
+    b->dnsl[d] = n;
+    if (diag_is_scalar(d)) {
+      b->dslot[d] = b->nscal;
+      b->nscal += n;
+    } else {
+      b->dslot[d] = b->nslot;
+      b->nslot += n;
+    }
+  }
+}
+
+int diag_plen(const qgk_bucket* b) { return DiagLayout{b->nz}.len(); }
+
+bool diag_needs_jacobian(unsigned mask) {
+  const unsigned j = (1u << QGK_D_ENSFLUX) | (1u << QGK_D_KEFLUX) | (1u << QGK_D_APEFLUX) |
+                     (1u << QGK_D_KEFLUX_DIV) | (1u << QGK_D_APEFLUX_DIV);
+  return (mask & j) != 0;
+}
+
+// Jacobian sign * (ik fft(u x) + il fft(v x)) of the real planes x (nzx per
+// member) into J; mode 1 combines the two layers' velocities.
+int diag_advect(qgk_bucket* b, const double* x, int nzx, int mode, double sign, cd* J) {
+  const size_t np = (size_t)b->M * b->nz;
+  cd* W1 = b->dW;
+  cd* W2 = b->dW + np * b->nplane_c;
+  double* P1 = b->dR + np * b->nplane_r;
+  double* P2 = b->dR + 2 * np * b->nplane_r;
+  const int nplanes = b->M * nzx;
+  const size_t nr = (size_t)nplanes * b->nplane_r;
+  hipLaunchKernelGGL(k_diag_products, dim3(grid_1d(nr)), dim3(256), 0, b->stream, x,
+                     b->field_r[QGK_U], b->field_r[QGK_V], b->dprm, P1, P2, b->M, b->nz,
+                     (int)b->nplane_r, mode, diag_plen(b));
+  HIPC(hipGetLastError());
+  int rc;
+  if ((rc = r2c2d(b, P1, W1, nplanes)) || (rc = r2c2d(b, P2, W2, nplanes))) return rc;
+  const size_t nc = (size_t)nplanes * b->nplane_c;
+  hipLaunchKernelGGL(k_diag_jacobian, dim3(grid_1d(nc)), dim3(256), 0, b->stream, W1, W2,
+                     b->dp, J, b->M, nzx, b->nl, b->ldk, sign);
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+// inverse transform of a prepared spectral input (k_diag_prepare mode) into x
+int diag_prepare_real(qgk_bucket* b, int mode, double* x) {
+  const int nzx = mode == 1 ? 1 : b->nz;
+  const size_t nc = (size_t)b->M * nzx * b->nplane_c;
+  hipLaunchKernelGGL(k_diag_prepare, dim3(grid_1d(nc)), dim3(256), 0, b->stream,
+                     b->field_c[QGK_PH], b->dp, b->dprm, b->dW, b->M, b->nz, b->nl, b->ldk, mode,
+                     diag_plen(b));
+  HIPC(hipGetLastError());
+  return c2r2d(b, b->dW, x, b->M * nzx);
+}
+
+hipError_t copy_real_rows(qgk_bucket* b, double* dst, const double* src, size_t rows) {
+  return hipMemcpy2DAsync(dst, (size_t)b->nk * sizeof(double), src,
+                          (size_t)b->ldk * sizeof(double), (size_t)b->nk * sizeof(double), rows,
+                          hipMemcpyDeviceToHost, b->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int qgk_diag_set_params(qgk_bucket* b, int member, const double* params) {
+  if (!b || !params) return fail(QGK_E_ARG, "NULL argument");
+  if (member < 0 || member >= b->M) return fail(QGK_E_ARG, "member %d out of range", member);
+  if (b->nz > kDiagNzMax) return fail(QGK_E_ARG, "diagnostics support nz <= %d", kDiagNzMax);
+  const int plen = diag_plen(b);
+  if (b->dprm_h.empty()) {
+    b->dprm_h.assign((size_t)b->M * plen, 0.0);
+    b->dprm_set.assign(b->M, 0);
+  }
+  std::memcpy(b->dprm_h.data() + (size_t)member * plen, params, plen * sizeof(double));
+  b->dprm_set[member] = 1;
+  b->dprm_dirty = true;
+  return 0;
+}
+
+int qgk_diag_reset(qgk_bucket* b) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  const size_t plane = (size_t)b->nl * b->ldk;
+  if (b->dacc)
+    HIPC(hipMemsetAsync(b->dacc, 0, (size_t)b->nslot * b->M * plane * sizeof(double), b->stream));
+  if (b->dsc) HIPC(hipMemsetAsync(b->dsc, 0, (size_t)b->nscal * b->M * sizeof(double), b->stream));
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_diag_accumulate(qgk_bucket* b, unsigned int mask, int instant, double dt, double dt1,
+                        double dt2, double dt3) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  int rc = need_full(b, "diagnostics");
+  if (rc) return rc;
+  if (b->nz > kDiagNzMax) return fail(QGK_E_ARG, "diagnostics support nz <= %d", kDiagNzMax);
+  diag_layout(b);
+  if (mask >> QGK_NDIAG) return fail(QGK_E_ARG, "unknown diagnostic bits in mask 0x%x", mask);
+  for (int d = 0; d < QGK_NDIAG; ++d)
+    if ((mask >> d & 1u) && b->dslot[d] < 0)
+      return fail(QGK_E_ARG, "diagnostic %d does not exist for nz=%d", d, b->nz);
+  if ((mask >> QGK_D_APEFLUX & 1u) && (mask >> QGK_D_APEFLUX_DIV & 1u))
+    return fail(QGK_E_ARG, "APEflux and APEflux_div cannot be accumulated in one call");
+  if (dt == 0.0) return fail(QGK_E_ARG, "dt must be nonzero");
+  if (b->dprm_set.empty()) return fail(QGK_E_STATE, "diagnostic parameters not set");
+  for (int m = 0; m < b->M; ++m)
+    if (!b->dprm_set[m])
+      return fail(QGK_E_STATE, "diagnostic parameters not set for member %d", m);
+  if (mask == 0) return 0;
+  const size_t plane = (size_t)b->nl * b->ldk;
+  const size_t np = (size_t)b->M * b->nz;
+  const int plen = diag_plen(b);
+  // buffers
+  if (!b->dprm && (rc = dev_alloc(b, (void**)&b->dprm, b->dprm_h.size() * sizeof(double))))
+    return rc;
+  if (!b->dacc) {
+    if ((rc = dev_alloc(b, (void**)&b->dacc, (size_t)b->nslot * b->M * plane * sizeof(double))) ||
+        (rc = dev_alloc(b, (void**)&b->dsc, (size_t)(b->nscal + 1) * b->M * sizeof(double))) ||
+        (rc = dev_alloc(b, (void**)&b->dsum,
+                        ((size_t)b->M * (b->nz + 1) + (size_t)b->nz * plane) * sizeof(double))))
+      return rc;
+  }
+  if (instant && !b->dacc_i) {
+    if ((rc = dev_alloc(b, (void**)&b->dacc_i, (size_t)b->nslot * b->M * plane * sizeof(double))) ||
+        (rc = dev_alloc(b, (void**)&b->dsc_i, (size_t)(b->nscal + 1) * b->M * sizeof(double))))
+      return rc;
+  }
+  const bool apegen = mask >> QGK_D_APEGEN & 1u;
+  if ((diag_needs_jacobian(mask) || apegen) && !b->dW) {
+    if ((rc = dev_alloc(b, (void**)&b->dW, 5 * np * b->nplane_c * sizeof(cd))) ||
+        (rc = dev_alloc(b, (void**)&b->dR, 3 * np * b->nplane_r * sizeof(double))))
+      return rc;
+  }
+  if (b->dprm_dirty) {
+    HIPC(hipMemcpyAsync(b->dprm, b->dprm_h.data(), b->dprm_h.size() * sizeof(double),
+                        hipMemcpyHostToDevice, b->stream));
+    b->dprm_dirty = false;
+  }
+  if ((rc = upload_params(b)) || (rc = refresh_q(b))) return rc;
+  double* acc = instant ? b->dacc_i : b->dacc;
+  double* accs = instant ? b->dsc_i : b->dsc;
+  if (instant) {
+    HIPC(hipMemsetAsync(acc, 0, (size_t)b->nslot * b->M * plane * sizeof(double), b->stream));
+    HIPC(hipMemsetAsync(accs, 0, (size_t)b->nscal * b->M * sizeof(double), b->stream));
+  }
+  // Jacobians of the derived fields (qg_model.py:213-224, layered_model.py:242-254)
+  cd* Jq = b->dW + 2 * np * b->nplane_c;
+  cd* Jpxi = b->dW + 3 * np * b->nplane_c;
+  cd* J3 = b->dW + 4 * np * b->nplane_c;
+  double* X = b->dR;
+  if (mask >> QGK_D_ENSFLUX & 1u)
+    if ((rc = diag_advect(b, b->field_r[QGK_Q], b->nz, 0, 1.0, Jq))) return rc;
+  if ((mask >> QGK_D_KEFLUX & 1u) || (mask >> QGK_D_KEFLUX_DIV & 1u))
+    if ((rc = diag_prepare_real(b, 0, X)) || (rc = diag_advect(b, X, b->nz, 0, 1.0, Jpxi)))
+      return rc;
+  if (mask >> QGK_D_APEFLUX & 1u)
+    if ((rc = diag_prepare_real(b, 1, X)) || (rc = diag_advect(b, X, 1, 1, -1.0, J3))) return rc;
+  if (mask >> QGK_D_APEFLUX_DIV & 1u)
+    if ((rc = diag_prepare_real(b, 2, X)) || (rc = diag_advect(b, X, b->nz, 0, 1.0, J3)))
+      return rc;
+  DiagArgs A;
+  A.qh = b->field_c[QGK_QH];
+  A.ph = b->field_c[QGK_PH];
+  A.D = D(b);
+  A.P = Pp(b);
+  A.PP = PP(b);
+  A.dqh = (b->flags & QGK_HAS_Q_PARAM) ? b->field_c[QGK_DQH] : nullptr;
+  A.duh = (b->flags & QGK_HAS_UV_PARAM) ? b->field_c[QGK_DUH] : nullptr;
+  A.dvh = (b->flags & QGK_HAS_UV_PARAM) ? b->field_c[QGK_DVH] : nullptr;
+  A.Jq = Jq;
+  A.Jpxi = Jpxi;
+  A.J3 = J3;
+  A.prm = b->dprm;
+  A.acc = acc;
+  A.pts = apegen ? b->dR + np * b->nplane_r : nullptr;
+  A.M = b->M;
+  A.nz = b->nz;
+  A.nl = b->nl;
+  A.nk = b->nk;
+  A.ldk = b->ldk;
+  A.plen = plen;
+  const double M2 = (double)b->nx * (double)b->ny;
+  A.invM2 = 1.0 / (M2 * M2);
+  A.dt = dt;
+  A.dt1 = dt1;
+  A.dt2 = dt2;
+  A.dt3 = dt3;
+  for (int d = 0; d < QGK_NDIAG; ++d)
+    A.slot[d] = (!diag_is_scalar(d) && (mask >> d & 1u)) ? b->dslot[d] : -1;
+  const size_t total = (size_t)b->M * plane;
+  hipLaunchKernelGGL(k_diag_spectral, dim3(grid_1d(total)), dim3(256), 0, b->stream, A, b->dp);
+  HIPC(hipGetLastError());
+  const bool eke = (mask >> QGK_D_EKE & 1u) || (mask >> QGK_D_EKEDISS & 1u);
+  if (eke) {
+    hipLaunchKernelGGL(k_diag_plane_sum, dim3((unsigned)np), dim3(256), 0, b->stream,
+                       b->field_r[QGK_U], b->field_r[QGK_V], b->dsum, b->nplane_r, 1);
+    HIPC(hipGetLastError());
+  }
+  if (apegen) {
+    hipLaunchKernelGGL(k_diag_plane_sum, dim3((unsigned)b->M), dim3(256), 0, b->stream, A.pts,
+                       (const double*)nullptr, b->dsum + np, plane, 0);
+    HIPC(hipGetLastError());
+  }
+  if (eke || apegen) {
+    hipLaunchKernelGGL(k_diag_scalars, dim3((b->M + 63) / 64), dim3(64), 0, b->stream, b->dsum,
+                       b->dprm, b->dp, accs, b->M, b->nz, plen, (double)b->nplane_r, A.invM2,
+                       (mask >> QGK_D_EKE & 1u) ? b->dslot[QGK_D_EKE] : -1,
+                       (mask >> QGK_D_EKEDISS & 1u) ? b->dslot[QGK_D_EKEDISS] : -1,
+                       apegen ? b->dslot[QGK_D_APEGEN] : -1);
+    HIPC(hipGetLastError());
+  }
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_diag_read(qgk_bucket* b, int diag, int member, int instant, double* host) {
+  if (!b || !host) return fail(QGK_E_ARG, "NULL argument");
+  if (diag < 0 || diag >= QGK_NDIAG) return fail(QGK_E_ARG, "bad diagnostic id %d", diag);
+  diag_layout(b);
+  if (b->dslot[diag] < 0)
+    return fail(QGK_E_ARG, "diagnostic %d does not exist for nz=%d", diag, b->nz);
+  if (member < -2 || member >= b->M) return fail(QGK_E_ARG, "member %d out of range", member);
+  double* acc = instant ? b->dacc_i : b->dacc;
+  double* accs = instant ? b->dsc_i : b->dsc;
+  if (!acc) return fail(QGK_E_STATE, "no diagnostics accumulated");
+  const int nsl = b->dnsl[diag], s0 = b->dslot[diag];
+  const size_t plane = (size_t)b->nl * b->ldk;
+  if (diag_is_scalar(diag)) {
+    std::vector<double> all((size_t)nsl * b->M);
+    HIPC(hipMemcpyAsync(all.data(), accs + (size_t)s0 * b->M, all.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, b->stream));
+    HIPC(hipStreamSynchronize(b->stream));
+    for (int j = 0; j < nsl; ++j) {
+      if (member >= 0) {
+        host[j] = all[(size_t)j * b->M + member];
+      } else if (member == -1) {
+        for (int m = 0; m < b->M; ++m) host[(size_t)m * nsl + j] = all[(size_t)j * b->M + m];
+      } else {
+        double s = 0.0;
+        for (int m = 0; m < b->M; ++m) s += all[(size_t)j * b->M + m];
+        host[j] = s;
+      }
+    }
+    return 0;
+  }
+  const size_t dense = (size_t)b->nl * b->nk;
+  if (member >= 0) {
+    for (int j = 0; j < nsl; ++j)
+      HIPC(copy_real_rows(b, host + j * dense, acc + ((size_t)(s0 + j) * b->M + member) * plane,
+                          b->nl));
+  } else if (member == -1) {
+    for (int m = 0; m < b->M; ++m)
+      for (int j = 0; j < nsl; ++j)
+        HIPC(copy_real_rows(b, host + ((size_t)m * nsl + j) * dense,
+                            acc + ((size_t)(s0 + j) * b->M + m) * plane, b->nl));
+  } else {
+    double* tmp = b->dsum + (size_t)b->M * (b->nz + 1);
+    const size_t n = (size_t)nsl * plane;
+    hipLaunchKernelGGL(k_diag_member_sum, dim3(grid_1d(n)), dim3(256), 0, b->stream,
+                       acc + (size_t)s0 * b->M * plane, tmp, b->M, nsl, plane);
+    HIPC(hipGetLastError());
+    HIPC(copy_real_rows(b, host, tmp, (size_t)nsl * b->nl));
+  }
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
 }
 
 }  // extern "C"

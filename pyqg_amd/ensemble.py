@@ -18,6 +18,8 @@ import numpy as np
 from . import _lib
 from . import constants as C
 from ._lib import Bucket
+from .diagnostics import param_block
+from .errors import DiagnosticNotFilledError
 
 
 def _as_array(v, n):
@@ -36,6 +38,8 @@ class Ensemble:
         self.buckets = []      # Bucket handles, one per (nz, nx)
         self.members = []      # member index -> (bucket index, position in bucket)
         self.member_nx = []
+        self.diag_params = []  # per bucket: (M, plen) constants of the device diagnostics
+        self.diag_count = {}
 
     @property
     def dt(self):
@@ -81,6 +85,7 @@ class Ensemble:
             filt = np.empty((len(idx), size, nk))
             Ubg = np.empty((len(idx), 2))
             Qy = np.empty((len(idx), 2))
+            dparams = []
             for j, m in enumerate(idx):
                 g = C.grid(size, size, p["L"][m], p["L"][m])
                 kk[j], ll[j], k2l2[j] = g["kk"], g["ll"], g["k2l2"]
@@ -89,6 +94,11 @@ class Ensemble:
                                                         p["U1"][m], p["U2"][m])
                 b.set_inversion_2layer(F1, F2, member=j)
                 b.set_rek(p["rek"][m], member=j)
+                d = p["delta"][m]
+                Hi = np.array([p["H1"][m], p["H1"][m] / d])
+                dparams.append(param_block(2, Hi, Hi.sum(), S=[[-F1, F1], [F2, -F2]],
+                                           del1=d / (d + 1.), del2=(d + 1.) ** -1,
+                                           rd=p["rd"][m], U=p["U1"][m] - p["U2"][m]))
                 ens.members[m] = (len(ens.buckets), j)
             ens._set_table(b, "kk", kk)
             ens._set_table(b, "ll", ll)
@@ -97,12 +107,13 @@ class Ensemble:
             ens._set_table(b, "Ubg", Ubg)
             ens._set_table(b, "Qy", Qy)
             ens.buckets.append(b)
+            ens.diag_params.append(np.array(dparams))
         ens.params = p
         return ens
 
     @classmethod
     def layered(cls, nmembers, nx, nz, H, rho, U, L=1e6, beta=1.5e-11, rek=5.787e-7,
-                f=0.0001236812857687059, g=9.81, dt=3600., filterfac=23.6):
+                f=0.0001236812857687059, g=9.81, dt=3600., filterfac=23.6, full_state=False):
         """nmembers LayeredModels with one shared stratification
         (layered_model.py:72-229); the inversion table is shared by all."""
         n, nz = int(nmembers), int(nz)
@@ -112,7 +123,7 @@ class Ensemble:
         Qy = beta - np.dot(S, Ubg)
         a = C.layered_inversion(S, gr["wv2"])
         ens = cls(dt)
-        b = Bucket(n, nz, nx, nx, 0)
+        b = Bucket(n, nz, nx, nx, _lib.FULL_STATE if full_state else 0)
         b.set_const("kk", gr["kk"])
         b.set_const("ll", gr["ll"])
         b.set_const("k2l2", gr["k2l2"])
@@ -125,6 +136,10 @@ class Ensemble:
         ens.members = [(0, j) for j in range(n)]
         ens.member_nx = [nx] * n
         ens.S, ens.Qy, ens.a = S, Qy, a
+        gpi = g * (rhoi[1:] - rhoi[:-1]) / rhoi[:-1]
+        kdi2, pmodes = C.vertical_modes(S, Hi)
+        blk = param_block(nz, Hi, Hi.sum(), S=S, f2gpi=f * f / gpi, pmodes=pmodes, kdi2=kdi2)
+        ens.diag_params = [np.tile(blk, (n, 1))]
         return ens
 
     @staticmethod
@@ -207,6 +222,49 @@ class Ensemble:
             except OSError:
                 pass
         return report
+
+    # ------------------------------------------------------------ diagnostics
+    def accumulate_diagnostics(self, names=("KEspec",)):
+        """Add the named diagnostics (pyqg's registry names) of the current
+        state of every member, on the device.  As in the reference they are
+        evaluated inside the step, after the inversion, advection and friction
+        of the current state and before the time step (model.py:406-436); the
+        ensemble's state is unchanged.  Needs full_state=True."""
+        names = list(names)
+        w, _ = C.ab_weights(self.ablevel, self._dt)
+        for b, prm in zip(self.buckets, self.diag_params):
+            if not b.flags & _lib.FULL_STATE:
+                raise ValueError("diagnostics need an ensemble built with full_state=True")
+            if not getattr(b, "_diag_params_set", False):
+                for j in range(b.M):
+                    b.diag_set_params(j, prm[j])
+                b._diag_params_set = True
+            for fn in (_lib.lib.qgk_invert, _lib.lib.qgk_advect, _lib.lib.qgk_friction):
+                _lib.check(fn(b.h))
+            b.diag_accumulate(names, self._dt, w)
+        for n in names:
+            self.diag_count[n] = self.diag_count.get(n, 0) + 1
+
+    def get_diagnostic(self, name, member=None, mean=False):
+        """Time mean of one diagnostic: of one member, of every member (a list),
+        or (mean=True) the mean over members of equal grid size (one array per
+        bucket, in bucket order)."""
+        cnt = self.diag_count.get(name, 0)
+        if cnt == 0:
+            raise DiagnosticNotFilledError(name)
+        if member is not None:
+            bi, j = self.members[member]
+            return self.buckets[bi].diag_read(name, member=j) / cnt
+        if mean:
+            return [b.diag_read(name, member=-2) / (cnt * b.M) for b in self.buckets]
+        per = [b.diag_read(name, member=-1) / cnt for b in self.buckets]
+        return [per[bi][j] for (bi, j) in self.members]
+
+    def reset_diagnostics(self):
+        for b in self.buckets:
+            if b.flags & _lib.FULL_STATE and self.diag_count:
+                b.diag_reset()
+        self.diag_count = {}
 
     def model_days(self, nsteps):
         """Simulated model-days of nsteps steps summed over members."""

@@ -5,8 +5,9 @@ order of pyqg's Model / QGModel / LayeredModel / BTModel / SQGModel
 (pyqg/model.py:101-258, 406-436; qg_model.py; layered_model.py; bt_model.py;
 sqg_model.py).  They only build the constants the kernel consumes (wavenumbers,
 filter, background, inversion matrix) and orchestrate the kernel's phases;
-all per-step arithmetic happens in libqgk.  Time-averaged diagnostics, xarray
-output and linear stability analysis are not part of this package.
+all per-step arithmetic happens in libqgk, including the time-averaged
+diagnostics (pyqg_amd.diagnostics).  xarray output and linear stability
+analysis are not part of this package.
 """
 import logging
 import warnings
@@ -15,10 +16,12 @@ import numpy as np
 from numpy import pi
 
 from . import constants as C
+from . import diagnostics as Dg
+from .diagnostics import DiagnosticsMixin
 from .kernel import PseudoSpectralKernel
 
 
-class Model(PseudoSpectralKernel):
+class Model(DiagnosticsMixin, PseudoSpectralKernel):
     """Generic pseudo-spectral model (pyqg/model.py:22-258)."""
 
     def __init__(self, nz=1, nx=64, ny=None, L=1e6, W=None, dt=7200., twrite=1000.,
@@ -64,7 +67,6 @@ class Model(PseudoSpectralKernel):
         self.q_parameterization = q_parameterization
         self.uv_parameterization = uv_parameterization
         self.diagnostics_list = diagnostics_list
-        self._diag_warned = False
 
         self._initialize_logger()
         self._initialize_grid()
@@ -73,6 +75,7 @@ class Model(PseudoSpectralKernel):
         self._initialize_filter()
         self._initialize_time()
         self._initialize_inversion_matrix()
+        self._initialize_diagnostics(diagnostics_list)
 
     # ------------------------------------------------------------ running
     def run_with_snapshots(self, tsnapstart=0., tsnapint=432000.):
@@ -86,9 +89,11 @@ class Model(PseudoSpectralKernel):
     def run(self):
         """Run until t >= tmax (model.py:280-283).
 
-        Without parameterizations the steps between two status reports run in
-        the fused two-kernel form; the step that a status report observes runs
-        phase by phase, so KE / CFL see the same fields as the reference's."""
+        Without parameterizations the steps between two status reports or
+        diagnostic accumulations run in the fused two-kernel form; the step
+        that a status report observes runs phase by phase, so KE / CFL see the
+        same fields as the reference's, and a step that accumulates
+        diagnostics runs as _step_forward."""
         if self.q_parameterization is not None or self.uv_parameterization is not None:
             while self.t < self.tmax:
                 self._step_forward()
@@ -103,11 +108,26 @@ class Model(PseudoSpectralKernel):
             if self.log_level:
                 to_status = int(self.twrite - (self.tc % self.twrite))
                 chunk = min(n, max(to_status, 1))
-            t_before = self.t
+            chunk = min(chunk, self._steps_before_diagnostics(chunk))
+            if chunk == 0:
+                self._step_forward()
+                n -= 1
+                continue
             self._fused_steps(chunk)
-            self._warn_diagnostics(t_before)
             self._print_status()
             n -= chunk
+
+    def _steps_before_diagnostics(self, limit):
+        """Steps (at most limit) that start before the next diagnostic step."""
+        if not any(d['active'] for d in self.diagnostics.values()):
+            return limit
+        t, tc = self.t, self.tc
+        for j in range(limit):
+            if self._diagnostics_due(t, tc):
+                return j
+            t += self.dt
+            tc += 1
+        return limit
 
     def _step_forward(self):
         """One step in the reference's order (model.py:406-436)."""
@@ -125,16 +145,6 @@ class Model(PseudoSpectralKernel):
 
     def _do_external_forcing(self):
         pass
-
-    def _calc_diagnostics(self):
-        if (self.t >= self.dt) and (self.t >= self.tavestart):
-            self._warn_diagnostics(self.t)
-
-    def _warn_diagnostics(self, t):
-        if t >= self.tavestart and not self._diag_warned:
-            warnings.warn("pyqg_amd does not accumulate time-averaged diagnostics; "
-                          "the state is stepped without them", RuntimeWarning)
-            self._diag_warned = True
 
     def _print_status(self):
         """Log KE and CFL every twrite steps; assert CFL < 1 (model.py:564-574)."""
@@ -221,15 +231,10 @@ class Model(PseudoSpectralKernel):
     def vertical_modes(self):
         """Deformation radii and pressure modes from the stretching matrix
         (model.py:286-305)."""
-        evals, evecs = np.linalg.eig(-self.S)
-        asort = evals.argsort()
-        self.kdi2 = evals[asort]
+        self.kdi2, self.pmodes = C.vertical_modes(self.S, self.Hi)
         self.radii = np.zeros_like(self.kdi2)
         self.radii[0] = np.sqrt(self.g * self.H) / np.abs(self.f)
         self.radii[1:] = 1. / np.sqrt(self.kdi2[1:])
-        self.pmodes = evecs[:, asort]
-        Ai = (self.H / (self.Hi[:, np.newaxis] * (self.pmodes ** 2)).sum(axis=0)) ** 0.5
-        self.pmodes = Ai[np.newaxis, :] * self.pmodes
 
     @property
     def parameterization(self):
@@ -289,6 +294,24 @@ class QGModel(Model):
         """Closed-form 2x2 inverse of (S - kappa^2 I), zero at kappa = 0
         (qg_model.py:146-161)."""
         self.a = C.qg_inversion(self.wv2, self.F1, self.F2)
+
+    def _initialize_model_diagnostics(self):
+        """qg_diagnostics.py:6-70 and qg_model.py:226-263"""
+        self._add_device_diagnostics(Dg.QGDIAG)
+        self._add_device_diagnostics(Dg.QG_ONLY)
+
+    def _calc_derived_fields(self):
+        """Host fields of pyqg's diagnostic functions (qg_model.py:213-224), for
+        diagnostics added with add_diagnostic; the built-in ones compute their
+        own on the device."""
+        self.p = self.ifft(self.ph)
+        self.xi = self.ifft(-self.wv2 * self.ph)
+        self.Jptpc = -self._advect((self.p[0] - self.p[1]),
+                                   (self.del1 * self.u[0] + self.del2 * self.u[1]),
+                                   (self.del1 * self.v[0] + self.del2 * self.v[1]))
+        self.Jpxi = self._advect(self.xi, self.u, self.v)
+        self.Jq = self._advect(self.q, self.u, self.v)
+        self.Sph = np.einsum("ij,jkl->ikl", self.S, self.ph)
 
     def set_q1q2(self, q1, q2, check=False):
         self.q = np.vstack([q1[np.newaxis, :, :], q2[np.newaxis, :, :]])
@@ -363,6 +386,29 @@ class LayeredModel(Model):
         (layered_model.py:211-229)."""
         self.a = C.layered_inversion(self.S, self.wv2)
 
+    def modal_projection(self, p, forward=True):
+        """Modal amplitudes of p in the basis pmodes, or the inverse
+        (model.py:307-317), as a solve at every wavenumber."""
+        if forward:
+            return np.linalg.solve(self.pmodes, p.reshape(self.nz, -1)).reshape(p.shape)
+        return np.einsum("ik,k...->i...", self.pmodes, p)
+
+    def _initialize_model_diagnostics(self):
+        """qg_diagnostics.py:6-70 and layered_model.py:256-301"""
+        self._add_device_diagnostics(Dg.QGDIAG)
+        self._add_device_diagnostics(Dg.LAYERED_ONLY)
+
+    def _calc_derived_fields(self):
+        """Host fields of pyqg's diagnostic functions (layered_model.py:242-254)."""
+        self.p = self.ifft(self.ph)
+        self.xi = self.ifft(-self.wv2 * self.ph)
+        self.Jpxi = self._advect(self.xi, self.u, self.v)
+        self.Jq = self._advect(self.q, self.u, self.v)
+        self.Sph = np.einsum("ij,jkl->ikl", self.S, self.ph)
+        self.Sp = self.ifft(self.Sph)
+        self.JSp = self._advect(self.Sp, self.u, self.v)
+        self.phn = self.modal_projection(self.ph)
+
 
 class BTModel(Model):
     """Single-layer (equivalent) barotropic model (pyqg/bt_model.py:6-104)."""
@@ -393,9 +439,9 @@ class BTModel(Model):
     def set_U(self, U):
         self.Ubg = np.asarray(U)[np.newaxis, ...]
 
-    def _calc_diagnostics(self):
-        if (self.t >= self.dt) and (self.tc % self.taveints == 0):
-            self._warn_diagnostics(self.tavestart)
+    def _diagnostics_due(self, t, tc):
+        """No tavestart test in this model (bt_model.py:106-109, sqg_model.py:75-78)."""
+        return (t >= self.dt) and (tc % self.taveints == 0)
 
     def _calc_cfl(self):
         return np.abs(np.hstack([self.u + self.Ubg, self.v])).max() * self.dt / self.dx
@@ -430,9 +476,9 @@ class SQGModel(Model):
     def set_U(self, U):
         self.Ubg = np.asarray(U)[np.newaxis, ...]
 
-    def _calc_diagnostics(self):
-        if (self.t >= self.dt) and (self.tc % self.taveints == 0):
-            self._warn_diagnostics(self.tavestart)
+    def _diagnostics_due(self, t, tc):
+        """No tavestart test in this model (bt_model.py:106-109, sqg_model.py:75-78)."""
+        return (t >= self.dt) and (tc % self.taveints == 0)
 
     def _calc_cfl(self):
         return np.abs(np.hstack([self.u + self.Ubg, self.v])).max() * self.dt / self.dx

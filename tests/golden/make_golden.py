@@ -241,9 +241,74 @@ def long_runs(pyqg):
     np.savez(os.path.join(HERE, "long_runs.npz"), **out)
 
 
+DIAG_CASES = {
+    # name: (model class, kwargs, IC seed, amplitude, with q param, with uv param, steps)
+    "qg": ("QGModel", dict(nx=32, dt=7200.0, tavestart=0.0, taveint=7200.0, **QG_PHYS),
+           5, 1e-6, True, True, 12),
+    "qg_sparse": ("QGModel", dict(nx=32, dt=3600.0, tavestart=5 * 3600.0, taveint=3 * 3600.0,
+                                  **QG_PHYS), 7, 1e-6, False, False, 20),
+    "layered3": ("LayeredModel", dict(nx=32, tavestart=0.0, taveint=3600.0, **LAYERED3),
+                 6, 1e-6, False, True, 12),
+    "layered2": ("LayeredModel", dict(nx=32, nz=2, rd=15000.0, delta=0.25, H=[500., 2000.],
+                                      U=[0.025, 0.0], V=[0.0, 0.0], rho=[1025.0, 1025.3],
+                                      dt=3600.0, tavestart=0.0, taveint=3600.0),
+                 8, 1e-6, True, False, 8),
+    "bt": ("BTModel", dict(nx=32, beta=1.5e-11, rd=15000.0, H=1.0, U=0.01, dt=3600.0,
+                           tavestart=0.0, taveint=3600.0), 9, 1e-6, False, False, 8),
+    "sqg": ("SQGModel", dict(nx=32, beta=0.0, H=1.0, U=0.0, dt=3600.0, tavestart=0.0,
+                             taveint=3600.0), 10, 1e-6, False, False, 8),
+}
+
+
+def uv_param_curl(m):
+    """Velocity parameterization with a nonzero curl for the diagnostics runs
+    (paramspec sees only -il duh + ik dvh)."""
+    return 1e-6 * m.v, 1e-6 * m.u + 1e-9 * np.cos(2 * np.pi * m.x / m.L)[np.newaxis]
+
+
+def _modal_projection_np1(m):
+    def proj(p, forward=True):
+        if forward:
+            return np.linalg.solve(m.pmodes, p.reshape(m.nz, -1)).reshape(p.shape)
+        return np.einsum("ik,k...->i...", m.pmodes, p)
+    return proj
+
+
+def diagnostics_runs(pyqg):
+    """Time-averaged diagnostics of the reference registry (model.py:644-822,
+    qg_diagnostics.py, qg_model.py:213-263, layered_model.py:242-301) after a
+    few steps of each model class, with and without parameterizations."""
+    out = {}
+    for case, (cls, kw, seed, amp, qp, uvp, steps) in DIAG_CASES.items():
+        extra = {}
+        if qp:
+            extra["q_parameterization"] = q_param
+        if uvp:
+            extra["uv_parameterization"] = uv_param_curl
+        m = getattr(pyqg, cls)(log_level=0, **kw, **extra)
+        if cls == "LayeredModel":
+            # model.py:307-317 passes a stack of vectors to np.linalg.solve, which
+            # numpy >= 2 reads as a stack of matrices and rejects; substitute the
+            # per-wavenumber solve that numpy 1.x performed
+            m.modal_projection = _modal_projection_np1(m)
+        q0 = amp * np.random.default_rng(seed).standard_normal((m.nz, m.ny, m.nx))
+        m.q = q0
+        for _ in range(steps):
+            m._step_forward()
+        for name, d in m.diagnostics.items():
+            if d["count"]:
+                out[f"{case}_{name}"] = np.asarray(m.get_diagnostic(name), dtype=np.float64)
+                out[f"{case}_count_{name}"] = np.int64(d["count"])
+        out[f"{case}_q_end"] = np.array(m.q).copy()
+    np.savez(os.path.join(HERE, "diagnostics.npz"), **out)
+
+
 if __name__ == "__main__":
     ref = import_reference()
+    only = sys.argv[1:]
     for fn in (qg_startup, qg_config1, layered_nz3, layered_nz10, bt_sqg, qg_param,
-               long_runs):
+               long_runs, diagnostics_runs):
+        if only and fn.__name__ not in only:
+            continue
         fn(ref)
         print("wrote", fn.__name__)

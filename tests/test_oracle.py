@@ -109,6 +109,10 @@ def q_param(m):
     return -2e-7 * m.q + 1e-13 * np.cos(2 * np.pi * m.x / m.L)[np.newaxis]
 
 
+def uv_param_curl(m):
+    return 1e-6 * m.v, 1e-6 * m.u + 1e-9 * np.cos(2 * np.pi * m.x / m.L)[np.newaxis]
+
+
 def uv_param(m):
     return 1e-6 * m.v, -1e-6 * m.u + 1e-9 * np.sin(2 * np.pi * m.y / m.W)[np.newaxis]
 
@@ -321,3 +325,102 @@ def test_planted_mistake_is_caught():
     m.rek = -m.rek
     m._step_forward()
     assert np.abs(m.dqhdt - g["s1_dqhdt"]).max() > 1e-10 * np.abs(g["s1_dqhdt"]).max()
+
+
+# ------------------------------------------------------------------ diagnostics
+DIAG_CASES = {
+    # the configurations of tests/golden/make_golden.py DIAG_CASES
+    "qg": (OracleQG, dict(nx=32, dt=7200.0, **QG_PHYS), 0.0, 7200.0, 5, True, True, 12),
+    "qg_sparse": (OracleQG, dict(nx=32, dt=3600.0, **QG_PHYS), 5 * 3600.0, 3 * 3600.0, 7,
+                  False, False, 20),
+    "layered3": (OracleLayered, dict(nx=32, **LAYERED3), 0.0, 3600.0, 6, False, True, 12),
+    "layered2": (OracleLayered, dict(nx=32, nz=2, rd=15000.0, delta=0.25, H=[500., 2000.],
+                                     U=[0.025, 0.0], V=[0.0, 0.0], rho=[1025.0, 1025.3],
+                                     dt=3600.0), 0.0, 3600.0, 8, True, False, 8),
+    "bt": (OracleBT, dict(nx=32, beta=1.5e-11, rd=15000.0, H=1.0, U=0.01, dt=3600.0),
+           0.0, 3600.0, 9, False, False, 8),
+    "sqg": (OracleSQG, dict(nx=32, beta=0.0, H=1.0, U=0.0, dt=3600.0), 0.0, 3600.0, 10,
+            False, False, 8),
+}
+
+
+def diag_case(case):
+    """Oracle model of one golden diagnostics case, stepped with its accumulator."""
+    from oracle.qg_diagnostics import Accumulator, step_with_diagnostics
+    cls, kw, tavestart, taveint, seed, qp, uvp, steps = DIAG_CASES[case]
+    extra = {}
+    if qp:
+        extra["q_parameterization"] = q_param
+    if uvp:
+        extra["uv_parameterization"] = uv_param_curl
+    m = cls(**kw, **extra)
+    m.q = 1e-6 * np.random.default_rng(seed).standard_normal((m.nz, m.ny, m.nx))
+    acc = Accumulator(m, tavestart=tavestart, taveint=taveint)
+    for _ in range(steps):
+        step_with_diagnostics(m, acc)
+    return m, acc
+
+
+def diag_close(a, b, tol):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    scale = np.abs(b).max()
+    if scale == 0.0:
+        return np.abs(a).max() == 0.0
+    return np.abs(a - b).max() <= tol * scale
+
+
+@pytest.mark.parametrize("case", list(DIAG_CASES))
+def test_diagnostics_match_reference(case):
+    """Every registered diagnostic's time mean and count equal the reference
+    build's (tests/golden/diagnostics.npz)."""
+    from oracle.qg_diagnostics import names
+    g = golden("diagnostics.npz")
+    m, acc = diag_case(case)
+    assert rel_l2(m.q, g[f"{case}_q_end"]) < 1e-12
+    expected = sorted(k[len(case) + 7:] for k in g.files if k.startswith(f"{case}_count_"))
+    assert sorted(names(m)) == expected
+    for n in expected:
+        assert acc.count == int(g[f"{case}_count_{n}"]), n
+        assert diag_close(acc.get(n), g[f"{case}_{n}"], 1e-11), n
+
+
+def test_diagnostics_planted_mistake_is_caught():
+    """Dissspec with the AB history terms dropped (forward-Euler weights at
+    every step) breaks the golden comparison."""
+    import oracle.qg_diagnostics as D
+    g = golden("diagnostics.npz")
+    m, acc = diag_case("qg")
+    good = acc.get("Dissspec")
+    assert diag_close(good, g["qg_Dissspec"], 1e-11)
+    orig = D._dissipation_spectrum
+
+    def euler_only(mm):
+        return np.stack([(mm.filtr - 1.0) * (mm.qh[k] + mm.dt * mm.dqhdt[k])
+                         for k in range(mm.nz)])
+    try:
+        D._dissipation_spectrum = euler_only
+        m, acc = diag_case("qg")
+        assert not diag_close(acc.get("Dissspec"), g["qg_Dissspec"], 1e-6)
+    finally:
+        D._dissipation_spectrum = orig
+
+
+def test_dissspec_identity():
+    """test_diagnostics.py:101-154: Dissspec equals the filtered minus the
+    unfiltered contribution of the AB update to the energy."""
+    from oracle.qg_diagnostics import evaluate, attach_constants, derived_fields
+    m = OracleQG(nx=32, dt=3600.0, **QG_PHYS)
+    m.q = 1e-6 * np.random.default_rng(0).standard_normal((2, 32, 32))
+    for _ in range(30):
+        m._step_forward()
+    m._invert()
+    m._do_advection()
+    m._do_friction()
+    attach_constants(m)
+    derived_fields(m)
+    dt1, dt2, dt3 = ab_coefficients(m.ablevel, m.dt)
+    rhs = m.qh + dt1 * m.dqhdt + dt2 * m.dqhdt_p + dt3 * m.dqhdt_pp
+    filt = -np.real(np.tensordot(m.Hi, np.conj(m.ph) * (m.filtr * rhs), axes=(0, 0))) \
+        / m.H / m.dt / m.M ** 2
+    unf = -np.real(np.tensordot(m.Hi, np.conj(m.ph) * rhs, axes=(0, 0))) / m.H / m.dt / m.M ** 2
+    np.testing.assert_allclose(evaluate(m, "Dissspec"), filt - unf, atol=1e-20)
