@@ -208,6 +208,29 @@ int qgk_diag_accumulate(qgk_bucket* b, unsigned int mask, int instant, double dt
 int qgk_diag_read(qgk_bucket* b, int diag, int member, int instant, double* host);
 int qgk_diag_reset(qgk_bucket* b);
 
+/* Built-in closures evaluated on the device from the current u, v, ph
+ * (pyqg/parameterizations.py:132-289: Smagorinsky, ZannaBolton2020,
+ * BackscatterBiharmonic).  The forcing times weight replaces (accumulate = 0)
+ * or is added to the handle's du, dv (velocity closures) or dq (PV closure),
+ * so that sums and multiples of closures (parameterizations.py:83-117) build
+ * up in place; qgk_apply_uv_param / qgk_apply_q_param then add the fields to
+ * dqhdt as kernel.pyx:397-442 does with host arrays.  With du_host/dv_host
+ * (dq_host) given, the forcing is copied there instead and no field changes.
+ * dx: grid spacing per member [M]; Hi: layer thicknesses [M][nz]. */
+int qgk_param_smagorinsky(qgk_bucket* b, const double* dx, double constant, double weight,
+                          int accumulate, double* du_host, double* dv_host);
+int qgk_smagorinsky_viscosity(qgk_bucket* b, const double* dx, double constant,
+                              double* nu_host);
+int qgk_param_zb2020(qgk_bucket* b, double constant, double weight, int accumulate,
+                     double* du_host, double* dv_host);
+int qgk_param_backscatter(qgk_bucket* b, const double* dx, const double* Hi,
+                          double smag_constant, double back_constant, double eps, double weight,
+                          int accumulate, double* dq_host);
+/* dqhdt += rfft2 of the handle's dq, or -il rfft2(du) + ik rfft2(dv)
+ * (the device half of qgk_add_q_param / qgk_add_uv_param). */
+int qgk_apply_q_param(qgk_bucket* b);
+int qgk_apply_uv_param(qgk_bucket* b);
+
 #ifdef __cplusplus
 }
 #endif

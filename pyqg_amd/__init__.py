@@ -8,6 +8,13 @@ from .kernel import (PseudoSpectralKernel, tendency_forward_euler, tendency_ab2,
                      tendency_ab3)
 from .model import Model, QGModel, LayeredModel, BTModel, SQGModel
 from .ensemble import Ensemble
+from .parameterizations import (Parameterization, CompositeParameterization,
+                                WeightedParameterization, UVParameterization, QParameterization,
+                                Smagorinsky, BackscatterBiharmonic, ZannaBolton2020, RingForcing)
+from . import diagnostic_tools
 
 __all__ = ["PseudoSpectralKernel", "tendency_forward_euler", "tendency_ab2", "tendency_ab3",
-           "Model", "QGModel", "LayeredModel", "BTModel", "SQGModel", "Ensemble"]
+           "Model", "QGModel", "LayeredModel", "BTModel", "SQGModel", "Ensemble",
+           "Parameterization", "CompositeParameterization", "WeightedParameterization",
+           "UVParameterization", "QParameterization", "Smagorinsky", "BackscatterBiharmonic",
+           "ZannaBolton2020", "RingForcing", "diagnostic_tools"]

@@ -32,7 +32,9 @@ EXPORTS = ["qgk_create", "qgk_destroy", "qgk_last_error", "qgk_set_const", "qgk_
            "qgk_copy_probe", "qgk_set_variant", "qgk_get_variant", "qgk_autotune",
            "qgk_device_name", "qgk_put_device", "qgk_get_device", "qgk_coarsen",
            "qgk_subgrid_forcing", "qgk_pack_inputs", "qgk_add_forcing_device",
-           "qgk_diag_set_params", "qgk_diag_accumulate", "qgk_diag_read", "qgk_diag_reset"]
+           "qgk_diag_set_params", "qgk_diag_accumulate", "qgk_diag_read", "qgk_diag_reset",
+           "qgk_param_smagorinsky", "qgk_smagorinsky_viscosity", "qgk_param_zb2020",
+           "qgk_param_backscatter", "qgk_apply_q_param", "qgk_apply_uv_param"]
 
 # device diagnostics, in the order of the qgk_diag enum
 DIAGS = ["Ensspec", "KEspec", "EKEdiss", "KEfrictionspec", "EKE", "Dissspec", "ENSDissspec",
@@ -117,6 +119,12 @@ def _load():
         "qgk_diag_accumulate": (I, [P, ctypes.c_uint, I, D, D, D, D]),
         "qgk_diag_read": (I, [P, I, I, I, P]),
         "qgk_diag_reset": (I, [P]),
+        "qgk_param_smagorinsky": (I, [P, P, D, D, I, P, P]),
+        "qgk_smagorinsky_viscosity": (I, [P, P, D, P]),
+        "qgk_param_zb2020": (I, [P, D, D, I, P, P]),
+        "qgk_param_backscatter": (I, [P, P, P, D, D, D, D, I, P]),
+        "qgk_apply_q_param": (I, [P]),
+        "qgk_apply_uv_param": (I, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

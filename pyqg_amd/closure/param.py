@@ -5,8 +5,9 @@ Two ways in:
     parameterization_type = "q_parameterization" (pyqg/parameterizations.py:
     4-130): m -> dq (nz, ny, nx), usable by pyqg_amd's models and by pyqg's own
     (the host-callback path of kernel.pyx:424-442);
-  * `ClosedEnsemble` steps an ensemble of coarse models with the network
-    inside the step on the GPU: per step the inversion, the packing of q, u, v
+  * `ClosedEnsemble` steps an ensemble of coarse models with the network (or
+    one of the built-in closures of pyqg_amd.parameterizations) inside the
+    step on the GPU: per step the inversion, the packing of q, u, v
     into the network input, the network, and dqhdt += rfft2(S) run as device
     kernels with no host round trip, in the reference's step order
     (pyqg/model.py:406-436: invert, advection, friction, q parameterization,
@@ -68,14 +69,17 @@ class ClosedEnsemble:
 
     def __init__(self, nmembers, n, dt, phys, param=None, seed=0):
         from ..ensemble import Ensemble
+        from ..parameterizations import Parameterization
+        self.builtin = isinstance(param, Parameterization)
+        uv = self.builtin and param.parameterization_type == 'uv_parameterization'
         self.ens = Ensemble.qg(np.full(nmembers, n), dt=dt, full_state=True,
-                               q_param=param is not None, **phys)
+                               q_param=param is not None and not uv, uv_param=uv, **phys)
         self.b = self.ens.buckets[0]
         self.param = param
         self.n = n
         rng = np.random.default_rng(seed)
         self.ens.set_q([1e-7 * rng.standard_normal((2, n, n)) for _ in range(nmembers)])
-        if param is not None:
+        if param is not None and not self.builtin:
             self.cinp = param.net.chp[0]
             self.xin = torch.empty((nmembers, n, n, self.cinp), dtype=torch.bfloat16,
                                    device="cuda")
@@ -89,6 +93,9 @@ class ClosedEnsemble:
     def step(self, nsteps=1):
         if self.param is None:
             self.ens.step(nsteps)
+            return
+        if self.builtin:
+            self.ens.step_parameterized(nsteps, self.param)
             return
         h = self.b.h
         L = _lib.lib

@@ -1297,6 +1297,159 @@ __global__ void k_diag_member_sum(const double* __restrict__ acc, double* __rest
   }
 }
 
+// ==========================================================================
+// Built-in closures (pyqg/parameterizations.py:132-289): spectral derivatives,
+// transforms and pointwise products over all members at once
+// ==========================================================================
+
+// out = cx * (ik X) + cy * (il Y) over nzx planes per member (X or Y may be null)
+__global__ void k_spec_ikil(const cd* __restrict__ X, const cd* __restrict__ Y,
+                            cd* __restrict__ out, const MemberParams* __restrict__ mp, int M,
+                            int nzx, int nl, int ldk, double cx, double cy) {
+  const size_t plane = (size_t)nl * ldk;
+  const size_t total = (size_t)M * nzx * plane;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t pt = i % plane;
+    const int m = (int)(i / (plane * nzx));
+    const int l = (int)(pt / ldk), k = (int)(pt % ldk);
+    cd r = {0.0, 0.0};
+    if (X) {  // ik X
+      const double kk = mp[m].kk[k];
+      const cd x = X[i];
+      r = {-kk * x.y, kk * x.x};
+      r = {cx * r.x, cx * r.y};
+    }
+    if (Y) {
+      const double ll = mp[m].ll[l];
+      const cd y = Y[i];
+      r = {r.x + cy * (-ll * y.y), r.y + cy * (ll * y.x)};
+    }
+    out[i] = r;
+  }
+}
+
+// out = a X + b Y (complex planes)
+__global__ void k_caxpby(const cd* __restrict__ X, const cd* __restrict__ Y, cd* __restrict__ out,
+                         double a, double b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = {a * X[i].x + b * Y[i].x, a * X[i].y + b * Y[i].y};
+}
+
+// out = sign * wv2^power * X (wv2 = k^2 + l^2, so lap = -wv2)
+__global__ void k_spec_wv2(const cd* __restrict__ X, cd* __restrict__ out,
+                           const MemberParams* __restrict__ mp, int M, int nz, int nl, int ldk,
+                           int power, double sign) {
+  const size_t plane = (size_t)nl * ldk;
+  const size_t total = (size_t)M * nz * plane;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t pt = i % plane;
+    const int m = (int)(i / (plane * nz));
+    double w = mp[m].k2l2[pt];
+    double f = sign;
+    for (int p = 0; p < power; ++p) f *= w;
+    const cd x = X[i];
+    out[i] = {f * x.x, f * x.y};
+  }
+}
+
+// Smagorinsky eddy viscosity (parameterizations.py:174-181): from Sxx, Syy and
+// 2 Sxy (R0, R1, R2) computes nu into R3 and nu*Sxx, nu*Syy, nu*Sxy in place.
+__global__ void k_smag_nu(double* __restrict__ R0, double* __restrict__ R1,
+                          double* __restrict__ R2, double* __restrict__ R3,
+                          const double* __restrict__ dx, double cs, int M, int nz, int npix) {
+  const size_t total = (size_t)M * nz * npix;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int m = (int)(i / ((size_t)nz * npix));
+    const double sxx = R0[i], syy = R1[i], sxy = 0.5 * R2[i];
+    const double c = cs * dx[m];
+    const double nu = (c * c) * sqrt(2 * (sxx * sxx + syy * syy + 2 * (sxy * sxy)));
+    R3[i] = nu;
+    R0[i] = nu * sxx;
+    R1[i] = nu * syy;
+    R2[i] = nu * sxy;
+  }
+}
+
+// ZannaBolton2020 products (parameterizations.py:270-280): from vx, vy, ux, uy
+// (R0..R3): R0 = vort*stretch, R1 = vort*shear, R2 = vort^2 + shear^2 + stretch^2.
+__global__ void k_zb_products(double* __restrict__ R0, double* __restrict__ R1,
+                              double* __restrict__ R2, const double* __restrict__ R3,
+                              size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double vx = R0[i], vy = R1[i], ux = R2[i], uy = R3[i];
+    const double rv = vx - uy, sh = vx + uy, st = ux - vy;
+    R0[i] = rv * st;
+    R1[i] = rv * sh;
+    R2[i] = rv * rv + sh * sh + st * st;
+  }
+}
+
+// out = (a * dx^2) * b per member (backscatter's lap_lap_psi * dx^2 * nu)
+__global__ void k_bs_product(const double* __restrict__ a, const double* __restrict__ b,
+                             const double* __restrict__ dx, double* __restrict__ out, int M,
+                             int nz, int npix) {
+  const size_t total = (size_t)M * nz * npix;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int m = (int)(i / ((size_t)nz * npix));
+    out[i] = (a[i] * (dx[m] * dx[m])) * b[i];
+  }
+}
+
+// Deterministic per-plane sums of a*b (one block per plane)
+__global__ __launch_bounds__(256) void k_plane_dot(const double* __restrict__ a,
+                                                   const double* __restrict__ b,
+                                                   double* __restrict__ out, size_t n) {
+  __shared__ double red[256];
+  const size_t off = (size_t)blockIdx.x * n;
+  double acc = 0.0;
+  for (size_t i = threadIdx.x; i < n; i += 256) acc += a[off + i] * b[off + i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// BackscatterBiharmonic forcing (parameterizations.py:226-236): dissipation =
+// -R2, backscatter = -Cb llpsi * sum_z Hi <psi diss> / (sum_z Hi <psi llpsi> + eps)
+// per member, from plane sums s_pd[m][z] = sum psi R2 and s_pl[m][z] = sum psi
+// llpsi; out = dissipation + backscatter.
+__global__ void k_bs_final(const double* __restrict__ R2, const double* __restrict__ llpsi,
+                           const double* __restrict__ s_pd, const double* __restrict__ s_pl,
+                           const double* __restrict__ Hi, double cb, double eps,
+                           double* __restrict__ out, int M, int nz, int npix) {
+  const size_t total = (size_t)M * nz * npix;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int m = (int)(i / ((size_t)nz * npix));
+    double num = 0.0, den = 0.0;
+    for (int z = 0; z < nz; ++z) {
+      num += Hi[(size_t)m * nz + z] * (-s_pd[(size_t)m * nz + z] / npix);
+      den += Hi[(size_t)m * nz + z] * (s_pl[(size_t)m * nz + z] / npix);
+    }
+    const double diss = -R2[i];
+    out[i] = diss + (-cb * llpsi[i]) * (num / (den + eps));
+  }
+}
+
+// out = (accumulate ? out : 0) + weight * (scale * a)
+__global__ void k_param_out(const double* __restrict__ a, double* __restrict__ out, double scale,
+                            double weight, int accumulate, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double v = (scale * a[i]) * weight;
+    out[i] = accumulate ? out[i] + v : v;
+  }
+}
+
 // Plain streaming copy (16 B per lane), the calibration kernel for HBM traffic
 // counters and the measured copy bandwidth.
 // Four independent 16-byte loads per lane are issued before their stores, so
@@ -1388,6 +1541,8 @@ struct qgk_bucket {
   cd* dW = nullptr;            // 5 spectral sets of M*nz planes: W1, W2, Jq, Jpxi, J3
   double* dR = nullptr;        // 3 real sets of M*nz planes: x, P1, P2
   double* dsum = nullptr;      // M*nz + M plane sums; then nz planes of member sums
+  // built-in closures: per-member grid spacing and layer thicknesses, plane sums
+  double *pdx = nullptr, *pHi = nullptr, *psum = nullptr;
   hipStream_t lst = nullptr;  // stream override for fused-step launches (graph capture)
   long gen = 0;               // bumped whenever launch arguments change (graph cache key)
   // hipGraph of two fused steps of a set of buckets, replayed while the AB
@@ -2043,12 +2198,10 @@ int qgk_friction(qgk_bucket* b) {
   return 0;
 }
 
-int qgk_add_q_param(qgk_bucket* b, const double* dq_host) {
-  if (!b || !dq_host) return fail(QGK_E_ARG, "NULL argument");
+int qgk_apply_q_param(qgk_bucket* b) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
   if (!b->field_r[QGK_DQ]) return fail(QGK_E_STATE, "dq is not allocated (no q parameterization)");
   const int np = b->M * b->nz;
-  HIPC(hipMemcpyAsync(b->field_r[QGK_DQ], dq_host, (size_t)np * b->nplane_r * sizeof(double),
-                      hipMemcpyHostToDevice, b->stream));
   int rc = r2c2d(b, b->field_r[QGK_DQ], b->field_c[QGK_DQH], np);
   if (rc) return rc;
   const size_t cpts = (size_t)np * b->nplane_c;
@@ -2059,16 +2212,22 @@ int qgk_add_q_param(qgk_bucket* b, const double* dq_host) {
   return 0;
 }
 
-int qgk_add_uv_param(qgk_bucket* b, const double* du_host, const double* dv_host) {
-  if (!b || !du_host || !dv_host) return fail(QGK_E_ARG, "NULL argument");
+int qgk_add_q_param(qgk_bucket* b, const double* dq_host) {
+  if (!b || !dq_host) return fail(QGK_E_ARG, "NULL argument");
+  if (!b->field_r[QGK_DQ]) return fail(QGK_E_STATE, "dq is not allocated (no q parameterization)");
+  const int np = b->M * b->nz;
+  HIPC(hipMemcpyAsync(b->field_r[QGK_DQ], dq_host, (size_t)np * b->nplane_r * sizeof(double),
+                      hipMemcpyHostToDevice, b->stream));
+  return qgk_apply_q_param(b);
+}
+
+int qgk_apply_uv_param(qgk_bucket* b) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
   if (!b->field_r[QGK_DU])
     return fail(QGK_E_STATE, "du/dv are not allocated (no uv parameterization)");
   int rc = upload_params(b);
   if (rc) return rc;
   const int np = b->M * b->nz;
-  const size_t bytes = (size_t)np * b->nplane_r * sizeof(double);
-  HIPC(hipMemcpyAsync(b->field_r[QGK_DU], du_host, bytes, hipMemcpyHostToDevice, b->stream));
-  HIPC(hipMemcpyAsync(b->field_r[QGK_DV], dv_host, bytes, hipMemcpyHostToDevice, b->stream));
   if ((rc = r2c2d(b, b->field_r[QGK_DU], b->field_c[QGK_DUH], np))) return rc;
   if ((rc = r2c2d(b, b->field_r[QGK_DV], b->field_c[QGK_DVH], np))) return rc;
   const size_t cpts = (size_t)np * b->nplane_c;
@@ -2077,6 +2236,17 @@ int qgk_add_uv_param(qgk_bucket* b, const double* du_host, const double* dv_host
   HIPC(hipGetLastError());
   HIPC(hipStreamSynchronize(b->stream));
   return 0;
+}
+
+int qgk_add_uv_param(qgk_bucket* b, const double* du_host, const double* dv_host) {
+  if (!b || !du_host || !dv_host) return fail(QGK_E_ARG, "NULL argument");
+  if (!b->field_r[QGK_DU])
+    return fail(QGK_E_STATE, "du/dv are not allocated (no uv parameterization)");
+  const int np = b->M * b->nz;
+  const size_t bytes = (size_t)np * b->nplane_r * sizeof(double);
+  HIPC(hipMemcpyAsync(b->field_r[QGK_DU], du_host, bytes, hipMemcpyHostToDevice, b->stream));
+  HIPC(hipMemcpyAsync(b->field_r[QGK_DV], dv_host, bytes, hipMemcpyHostToDevice, b->stream));
+  return qgk_apply_uv_param(b);
 }
 
 int qgk_timestep(qgk_bucket* b, double dt1, double dt2, double dt3) {
@@ -2622,6 +2792,18 @@ void diag_layout(qgk_bucket* b) {
 
 int diag_plen(const qgk_bucket* b) { return DiagLayout{b->nz}.len(); }
 
+// Work space shared by the diagnostics and the built-in closures: 5 spectral
+// and 4 real sets of M*nz planes.
+int ensure_work(qgk_bucket* b) {
+  if (b->dW) return 0;
+  const size_t np = (size_t)b->M * b->nz;
+  int rc;
+  if ((rc = dev_alloc(b, (void**)&b->dW, 5 * np * b->nplane_c * sizeof(cd))) ||
+      (rc = dev_alloc(b, (void**)&b->dR, 4 * np * b->nplane_r * sizeof(double))))
+    return rc;
+  return 0;
+}
+
 bool diag_needs_jacobian(unsigned mask) {
   const unsigned j = (1u << QGK_D_ENSFLUX) | (1u << QGK_D_KEFLUX) | (1u << QGK_D_APEFLUX) |
                      (1u << QGK_D_KEFLUX_DIV) | (1u << QGK_D_APEFLUX_DIV);
@@ -2735,11 +2917,7 @@ int qgk_diag_accumulate(qgk_bucket* b, unsigned int mask, int instant, double dt
       return rc;
   }
   const bool apegen = mask >> QGK_D_APEGEN & 1u;
-  if ((diag_needs_jacobian(mask) || apegen) && !b->dW) {
-    if ((rc = dev_alloc(b, (void**)&b->dW, 5 * np * b->nplane_c * sizeof(cd))) ||
-        (rc = dev_alloc(b, (void**)&b->dR, 3 * np * b->nplane_r * sizeof(double))))
-      return rc;
-  }
+  if ((diag_needs_jacobian(mask) || apegen) && (rc = ensure_work(b))) return rc;
   if (b->dprm_dirty) {
     HIPC(hipMemcpyAsync(b->dprm, b->dprm_h.data(), b->dprm_h.size() * sizeof(double),
                         hipMemcpyHostToDevice, b->stream));
@@ -2870,6 +3048,208 @@ int qgk_diag_read(qgk_bucket* b, int diag, int member, int instant, double* host
     HIPC(hipGetLastError());
     HIPC(copy_real_rows(b, host, tmp, (size_t)nsl * b->nl));
   }
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+}  // extern "C"
+
+// ==========================================================================
+// Built-in closures on the device (host side)
+// ==========================================================================
+namespace {
+
+const size_t kGrid = 256;
+
+int param_setup(qgk_bucket* b, const double* dx, const double* Hi, bool need_uv) {
+  int rc = need_full(b, "built-in closures");
+  if (rc) return rc;
+  if (need_uv && !b->field_r[QGK_U]) return fail(QGK_E_STATE, "u/v not allocated");
+  if ((rc = ensure_work(b)) || (rc = upload_params(b)) || (rc = refresh_q(b))) return rc;
+  const size_t np = (size_t)b->M * b->nz;
+  if (!b->pdx && ((rc = dev_alloc(b, (void**)&b->pdx, b->M * sizeof(double))) ||
+                  (rc = dev_alloc(b, (void**)&b->pHi, np * sizeof(double))) ||
+                  (rc = dev_alloc(b, (void**)&b->psum, 2 * np * sizeof(double)))))
+    return rc;
+  if (dx)
+    HIPC(hipMemcpyAsync(b->pdx, dx, b->M * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  if (Hi)
+    HIPC(hipMemcpyAsync(b->pHi, Hi, np * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  return 0;
+}
+
+cd* Wset(qgk_bucket* b, int i) { return b->dW + (size_t)i * b->M * b->nz * b->nplane_c; }
+double* Rset(qgk_bucket* b, int i) { return b->dR + (size_t)i * b->M * b->nz * b->nplane_r; }
+
+int spec_ikil(qgk_bucket* b, const cd* X, const cd* Y, cd* out, double cx, double cy) {
+  const size_t n = (size_t)b->M * b->nz * b->nplane_c;
+  hipLaunchKernelGGL(k_spec_ikil, dim3(grid_1d(n)), dim3(kGrid), 0, b->stream, X, Y, out, b->dp,
+                     b->M, b->nz, b->nl, b->ldk, cx, cy);
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+int spec_wv2(qgk_bucket* b, const cd* X, cd* out, int power, double sign) {
+  const size_t n = (size_t)b->M * b->nz * b->nplane_c;
+  hipLaunchKernelGGL(k_spec_wv2, dim3(grid_1d(n)), dim3(kGrid), 0, b->stream, X, out, b->dp,
+                     b->M, b->nz, b->nl, b->ldk, power, sign);
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+// weight * scale * src into the field dst (or added to it), or to host memory
+int param_emit(qgk_bucket* b, double* src, double* dst, double* host, double scale,
+               double weight, int accumulate) {
+  const size_t n = (size_t)b->M * b->nz * b->nplane_r;
+  if (host) {
+    hipLaunchKernelGGL(k_param_out, dim3(grid_1d(n)), dim3(kGrid), 0, b->stream, src, src, scale,
+                       weight, 0, n);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(host, src, n * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    return 0;
+  }
+  hipLaunchKernelGGL(k_param_out, dim3(grid_1d(n)), dim3(kGrid), 0, b->stream, src, dst, scale,
+                     weight, accumulate, n);
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+// Smagorinsky first stage (parameterizations.py:174-181): R3 = nu and
+// R0, R1, R2 = nu Sxx, nu Syy, nu Sxy
+int smag_viscosity(qgk_bucket* b, double cs) {
+  const int np = b->M * b->nz;
+  cd *W0 = Wset(b, 0), *W1 = Wset(b, 1), *W2 = Wset(b, 2);
+  double *R0 = Rset(b, 0), *R1 = Rset(b, 1), *R2 = Rset(b, 2), *R3 = Rset(b, 3);
+  int rc;
+  if ((rc = r2c2d(b, b->field_r[QGK_U], W0, np)) || (rc = r2c2d(b, b->field_r[QGK_V], W1, np)) ||
+      (rc = spec_ikil(b, W0, nullptr, W2, 1.0, 0.0)) || (rc = c2r2d(b, W2, R0, np)) ||
+      (rc = spec_ikil(b, nullptr, W1, W2, 0.0, 1.0)) || (rc = c2r2d(b, W2, R1, np)) ||
+      (rc = spec_ikil(b, W1, W0, W2, 1.0, 1.0)) || (rc = c2r2d(b, W2, R2, np)))
+    return rc;
+  const size_t n = (size_t)np * b->nplane_r;
+  hipLaunchKernelGGL(k_smag_nu, dim3(grid_1d(n)), dim3(kGrid), 0, b->stream, R0, R1, R2, R3,
+                     b->pdx, cs, b->M, b->nz, (int)b->nplane_r);
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+int check_uv_target(qgk_bucket* b, const double* du_host, const double* dv_host) {
+  if ((du_host == nullptr) != (dv_host == nullptr))
+    return fail(QGK_E_ARG, "give both du_host and dv_host or neither");
+  if (!du_host && !b->field_r[QGK_DU])
+    return fail(QGK_E_STATE, "du/dv are not allocated (no uv parameterization)");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qgk_param_smagorinsky(qgk_bucket* b, const double* dx, double constant, double weight,
+                          int accumulate, double* du_host, double* dv_host) {
+  if (!b || !dx) return fail(QGK_E_ARG, "NULL argument");
+  int rc;
+  if ((rc = check_uv_target(b, du_host, dv_host)) || (rc = param_setup(b, dx, nullptr, true)) ||
+      (rc = smag_viscosity(b, constant)))
+    return rc;
+  const int np = b->M * b->nz;
+  cd *W0 = Wset(b, 0), *W1 = Wset(b, 1), *W2 = Wset(b, 2), *W3 = Wset(b, 3);
+  double *R0 = Rset(b, 0), *R1 = Rset(b, 1), *R2 = Rset(b, 2);
+  // du = 2 ifft(ik F(nu Sxx) + il F(nu Sxy)), dv = 2 ifft(ik F(nu Sxy) + il F(nu Syy))
+  if ((rc = r2c2d(b, R0, W0, np)) || (rc = r2c2d(b, R2, W1, np)) || (rc = r2c2d(b, R1, W2, np)) ||
+      (rc = spec_ikil(b, W0, W1, W3, 1.0, 1.0)) || (rc = c2r2d(b, W3, R0, np)) ||
+      (rc = spec_ikil(b, W1, W2, W3, 1.0, 1.0)) || (rc = c2r2d(b, W3, R1, np)) ||
+      (rc = param_emit(b, R0, b->field_r[QGK_DU], du_host, 2.0, weight, accumulate)) ||
+      (rc = param_emit(b, R1, b->field_r[QGK_DV], dv_host, 2.0, weight, accumulate)))
+    return rc;
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_smagorinsky_viscosity(qgk_bucket* b, const double* dx, double constant,
+                              double* nu_host) {
+  if (!b || !dx || !nu_host) return fail(QGK_E_ARG, "NULL argument");
+  int rc;
+  if ((rc = param_setup(b, dx, nullptr, true)) || (rc = smag_viscosity(b, constant))) return rc;
+  const size_t n = (size_t)b->M * b->nz * b->nplane_r;
+  HIPC(hipMemcpyAsync(nu_host, Rset(b, 3), n * sizeof(double), hipMemcpyDeviceToHost,
+                      b->stream));
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_param_zb2020(qgk_bucket* b, double constant, double weight, int accumulate,
+                     double* du_host, double* dv_host) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  int rc;
+  if ((rc = check_uv_target(b, du_host, dv_host)) ||
+      (rc = param_setup(b, nullptr, nullptr, true)))
+    return rc;
+  const int np = b->M * b->nz;
+  cd *W0 = Wset(b, 0), *W1 = Wset(b, 1), *W2 = Wset(b, 2), *W3 = Wset(b, 3);
+  double *R0 = Rset(b, 0), *R1 = Rset(b, 1), *R2 = Rset(b, 2), *R3 = Rset(b, 3);
+  // vx, vy, ux, uy (parameterizations.py:264-269)
+  if ((rc = r2c2d(b, b->field_r[QGK_U], W0, np)) || (rc = r2c2d(b, b->field_r[QGK_V], W1, np)) ||
+      (rc = spec_ikil(b, W1, nullptr, W2, 1.0, 0.0)) || (rc = c2r2d(b, W2, R0, np)) ||
+      (rc = spec_ikil(b, nullptr, W1, W2, 0.0, 1.0)) || (rc = c2r2d(b, W2, R1, np)) ||
+      (rc = spec_ikil(b, W0, nullptr, W2, 1.0, 0.0)) || (rc = c2r2d(b, W2, R2, np)) ||
+      (rc = spec_ikil(b, nullptr, W0, W2, 0.0, 1.0)) || (rc = c2r2d(b, W2, R3, np)))
+    return rc;
+  const size_t nr = (size_t)np * b->nplane_r, nc = (size_t)np * b->nplane_c;
+  hipLaunchKernelGGL(k_zb_products, dim3(grid_1d(nr)), dim3(kGrid), 0, b->stream, R0, R1, R2, R3,
+                     nr);
+  HIPC(hipGetLastError());
+  // W0 = F(vort stretch), W1 = F(vort shear), W2 = F(sum of squares)
+  if ((rc = r2c2d(b, R0, W0, np)) || (rc = r2c2d(b, R1, W1, np)) || (rc = r2c2d(b, R2, W2, np)))
+    return rc;
+  // du = k ifft(ik (S/2 - shear) + il stretch); dv = k ifft(il (S/2 + shear) + ik stretch)
+  hipLaunchKernelGGL(k_caxpby, dim3(grid_1d(nc)), dim3(kGrid), 0, b->stream, W2, W1, W3, 0.5,
+                     -1.0, nc);
+  HIPC(hipGetLastError());
+  if ((rc = spec_ikil(b, W3, W0, W3, 1.0, 1.0)) || (rc = c2r2d(b, W3, R0, np))) return rc;
+  hipLaunchKernelGGL(k_caxpby, dim3(grid_1d(nc)), dim3(kGrid), 0, b->stream, W2, W1, W3, 0.5, 1.0,
+                     nc);
+  HIPC(hipGetLastError());
+  if ((rc = spec_ikil(b, W0, W3, W3, 1.0, 1.0)) || (rc = c2r2d(b, W3, R1, np)) ||
+      (rc = param_emit(b, R0, b->field_r[QGK_DU], du_host, constant, weight, accumulate)) ||
+      (rc = param_emit(b, R1, b->field_r[QGK_DV], dv_host, constant, weight, accumulate)))
+    return rc;
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int qgk_param_backscatter(qgk_bucket* b, const double* dx, const double* Hi,
+                          double smag_constant, double back_constant, double eps, double weight,
+                          int accumulate, double* dq_host) {
+  if (!b || !dx || !Hi) return fail(QGK_E_ARG, "NULL argument");
+  if (!dq_host && !b->field_r[QGK_DQ])
+    return fail(QGK_E_STATE, "dq is not allocated (no q parameterization)");
+  int rc;
+  if ((rc = param_setup(b, dx, Hi, true)) || (rc = smag_viscosity(b, smag_constant))) return rc;
+  const int np = b->M * b->nz;
+  cd* W0 = Wset(b, 0);
+  double *R0 = Rset(b, 0), *R1 = Rset(b, 1), *R2 = Rset(b, 2), *R3 = Rset(b, 3);
+  const size_t nr = (size_t)np * b->nplane_r;
+  // psi = ifft(ph), lap_lap_psi = ifft(lap^2 ph) (parameterizations.py:226-228)
+  if ((rc = c2r2d(b, b->field_c[QGK_PH], R0, np)) ||
+      (rc = spec_wv2(b, b->field_c[QGK_PH], W0, 2, 1.0)) || (rc = c2r2d(b, W0, R1, np)))
+    return rc;
+  // dissipation = -ifft(lap fft(lap_lap_psi dx^2 nu))
+  hipLaunchKernelGGL(k_bs_product, dim3(grid_1d(nr)), dim3(kGrid), 0, b->stream, R1, R3, b->pdx,
+                     R2, b->M, b->nz, (int)b->nplane_r);
+  HIPC(hipGetLastError());
+  if ((rc = r2c2d(b, R2, W0, np)) || (rc = spec_wv2(b, W0, W0, 1, -1.0)) ||
+      (rc = c2r2d(b, W0, R2, np)))
+    return rc;
+  hipLaunchKernelGGL(k_plane_dot, dim3((unsigned)np), dim3(256), 0, b->stream, R0, R2, b->psum,
+                     b->nplane_r);
+  hipLaunchKernelGGL(k_plane_dot, dim3((unsigned)np), dim3(256), 0, b->stream, R0, R1,
+                     b->psum + np, b->nplane_r);
+  hipLaunchKernelGGL(k_bs_final, dim3(grid_1d(nr)), dim3(kGrid), 0, b->stream, R2, R1, b->psum,
+                     b->psum + np, b->pHi, back_constant, eps, R3, b->M, b->nz,
+                     (int)b->nplane_r);
+  HIPC(hipGetLastError());
+  if ((rc = param_emit(b, R3, b->field_r[QGK_DQ], dq_host, 1.0, weight, accumulate))) return rc;
   HIPC(hipStreamSynchronize(b->stream));
   return 0;
 }

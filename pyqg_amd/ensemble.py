@@ -40,6 +40,8 @@ class Ensemble:
         self.member_nx = []
         self.diag_params = []  # per bucket: (M, plen) constants of the device diagnostics
         self.diag_count = {}
+        self.closure_dx = []   # per bucket: grid spacing [M] and layer thicknesses [M][nz]
+        self.closure_Hi = []
 
     @property
     def dt(self):
@@ -58,11 +60,12 @@ class Ensemble:
     @classmethod
     def qg(cls, nx, beta=1.5e-11, rd=15000.0, delta=0.25, rek=5.787e-7, L=1e6, H1=500.,
            U1=0.025, U2=0.0, dt=3600., filterfac=23.6, nmembers=None, full_state=False,
-           q_param=False):
+           q_param=False, uv_param=False):
         """Two-layer QGModels (qg_model.py:60-161).  Every argument may be a
         scalar or one value per member; members are bucketed by nx.  full_state
         keeps every intermediate field (needed by the per-phase operations and
-        the closure pipeline); q_param allocates dq / dqh."""
+        the closure pipeline); q_param allocates dq / dqh, uv_param du, dv,
+        duh, dvh."""
         if nmembers is None:
             sizes = [np.size(v) for v in (nx, beta, rd, delta, rek, L, U1, U2)]
             nmembers = max(sizes)
@@ -76,7 +79,8 @@ class Ensemble:
         ens.members = [None] * n
         for size in sorted(set(nxs.tolist())):
             idx = np.nonzero(nxs == size)[0]
-            flags = (_lib.FULL_STATE if full_state else 0) | (_lib.HAS_Q_PARAM if q_param else 0)
+            flags = ((_lib.FULL_STATE if full_state else 0) | (_lib.HAS_Q_PARAM if q_param else 0)
+                     | (_lib.HAS_UV_PARAM if uv_param else 0))
             b = Bucket(len(idx), 2, size, size, flags)
             nk = size // 2 + 1
             kk = np.empty((len(idx), nk))
@@ -108,6 +112,9 @@ class Ensemble:
             ens._set_table(b, "Qy", Qy)
             ens.buckets.append(b)
             ens.diag_params.append(np.array(dparams))
+            ens.closure_dx.append(np.array([p["L"][m] / size for m in idx]))
+            ens.closure_Hi.append(np.array([[p["H1"][m], p["H1"][m] / p["delta"][m]]
+                                            for m in idx]))
         ens.params = p
         return ens
 
@@ -140,6 +147,8 @@ class Ensemble:
         kdi2, pmodes = C.vertical_modes(S, Hi)
         blk = param_block(nz, Hi, Hi.sum(), S=S, f2gpi=f * f / gpi, pmodes=pmodes, kdi2=kdi2)
         ens.diag_params = [np.tile(blk, (n, 1))]
+        ens.closure_dx = [np.full(n, L / nx)]
+        ens.closure_Hi = [np.tile(Hi, (n, 1))]
         return ens
 
     @staticmethod
@@ -222,6 +231,29 @@ class Ensemble:
             except OSError:
                 pass
         return report
+
+    def step_parameterized(self, nsteps, parameterization):
+        """nsteps steps with a built-in closure (pyqg_amd.parameterizations) in
+        every member, in the reference's phase order (model.py:406-436):
+        inversion, advection, friction, the closure evaluated and added on the
+        device, time step.  The ensemble needs full_state and q_param or
+        uv_param matching the closure."""
+        p = parameterization
+        if not p._device_capable():
+            raise ValueError(f"{p!r} has no device evaluation")
+        uv = p.parameterization_type == 'uv_parameterization'
+        L = _lib.lib
+        for _ in range(int(nsteps)):
+            w, self.ablevel = C.ab_weights(self.ablevel, self._dt)
+            for b, dx, Hi in zip(self.buckets, self.closure_dx, self.closure_Hi):
+                for fn in (L.qgk_invert, L.qgk_advect, L.qgk_friction):
+                    _lib.check(fn(b.h))
+                p._device_bucket(b, np.ascontiguousarray(dx), np.ascontiguousarray(Hi), 1.0,
+                                 False)
+                _lib.check((L.qgk_apply_uv_param if uv else L.qgk_apply_q_param)(b.h))
+                _lib.check(L.qgk_timestep(b.h, *w))
+            self.t += self._dt
+            self.tc += 1
 
     # ------------------------------------------------------------ diagnostics
     def accumulate_diagnostics(self, names=("KEspec",)):

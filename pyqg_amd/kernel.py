@@ -204,8 +204,23 @@ class PseudoSpectralKernel:
         _lib.check(_lib.lib.qgk_friction(self._bucket.h))
         self._wrote("friction")
 
+    def _device_parameterization(self, p):
+        """A built-in closure (pyqg_amd.parameterizations) that writes its
+        forcing into the device fields itself, or None."""
+        dev = getattr(p, "_device_capable", None)
+        return p if dev is not None and dev() and hasattr(self, "dx") else None
+
     def _do_q_subgrid_parameterization(self):
-        """dqhdt += rfft2(q_parameterization(self)) (kernel.pyx:424-442)."""
+        """dqhdt += rfft2(q_parameterization(self)) (kernel.pyx:424-442).
+        Built-in closures evaluate on the device into dq directly."""
+        p = self._device_parameterization(self.q_parameterization)
+        if p is not None:
+            self._check_alloc("dq")
+            self._flush()
+            p._device(self)
+            _lib.check(_lib.lib.qgk_apply_q_param(self._bucket.h))
+            self._wrote("q_param")
+            return
         dq = self.q_parameterization(self)
         self._check_alloc("dq")
         dq = np.ascontiguousarray(np.broadcast_to(np.asarray(dq, dtype=DTYPE_real),
@@ -215,7 +230,16 @@ class PseudoSpectralKernel:
         self._wrote("q_param")
 
     def _do_uv_subgrid_parameterization(self):
-        """dqhdt += -il rfft2(du) + ik rfft2(dv) (kernel.pyx:397-422)."""
+        """dqhdt += -il rfft2(du) + ik rfft2(dv) (kernel.pyx:397-422).
+        Built-in closures evaluate on the device into du, dv directly."""
+        p = self._device_parameterization(self.uv_parameterization)
+        if p is not None:
+            self._check_alloc("du")
+            self._flush()
+            p._device(self)
+            _lib.check(_lib.lib.qgk_apply_uv_param(self._bucket.h))
+            self._wrote("uv_param")
+            return
         du, dv = self.uv_parameterization(self)
         self._check_alloc("du")
         shape = (self.nz, self.ny, self.nx)

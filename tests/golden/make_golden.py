@@ -303,11 +303,56 @@ def diagnostics_runs(pyqg):
     np.savez(os.path.join(HERE, "diagnostics.npz"), **out)
 
 
+PARAM_MODELS = {
+    # name: (model class, kwargs, IC seed)
+    "qg": ("QGModel", dict(nx=32, dt=7200.0, **QG_PHYS), 21),
+    "layered3": ("LayeredModel", dict(nx=32, **LAYERED3), 22),
+    "bt": ("BTModel", dict(nx=32, beta=1.5e-11, rd=15000.0, H=1.0, U=0.01, dt=3600.0), 23),
+    "sqg": ("SQGModel", dict(nx=32, beta=0.0, H=1.0, U=0.0, dt=3600.0), 24),
+}
+PARAM_RUNS = {
+    # name: (kind, constructor of the closure, steps)
+    "smag": ("uv_parameterization", lambda p: p.Smagorinsky(0.15), 10),
+    "back": ("q_parameterization", lambda p: p.BackscatterBiharmonic(), 10),
+    "zb": ("uv_parameterization", lambda p: p.ZannaBolton2020(), 10),
+    "combo": ("uv_parameterization", lambda p: 0.5 * p.Smagorinsky() + 0.75 * p.ZannaBolton2020(),
+              6),
+}
+
+
+def parameterizations(pyqg):
+    """The built-in closures (parameterizations.py:132-289) evaluated on model
+    states after three steps, and short QGModel runs that use them."""
+    out = {}
+    for name, (cls, kw, seed) in PARAM_MODELS.items():
+        m = getattr(pyqg, cls)(log_level=0, **kw)
+        m.q = 1e-6 * np.random.default_rng(seed).standard_normal((m.nz, m.ny, m.nx))
+        for _ in range(3):
+            m._step_forward()
+        m._invert()
+        out[f"{name}_q"] = np.array(m.q).copy()
+        du, dv = pyqg.Smagorinsky()(m)
+        out[f"{name}_smag_du"], out[f"{name}_smag_dv"] = du, dv
+        out[f"{name}_smag_nu"] = pyqg.Smagorinsky(0.2)(m, just_viscosity=True)
+        out[f"{name}_back_dq"] = pyqg.BackscatterBiharmonic()(m)
+        du, dv = pyqg.ZannaBolton2020()(m)
+        out[f"{name}_zb_du"], out[f"{name}_zb_dv"] = du, dv
+    for name, (kind, make, steps) in PARAM_RUNS.items():
+        m = pyqg.QGModel(nx=32, dt=7200.0, log_level=0, **{kind: make(pyqg)}, **QG_PHYS)
+        q0 = 1e-6 * np.random.default_rng(30).standard_normal((2, 32, 32))
+        m.q = q0
+        for _ in range(steps):
+            m._step_forward()
+        out[f"run_{name}_q"] = np.array(m.q).copy()
+    out["run_q0"] = q0
+    np.savez(os.path.join(HERE, "parameterizations.npz"), **out)
+
+
 if __name__ == "__main__":
     ref = import_reference()
     only = sys.argv[1:]
     for fn in (qg_startup, qg_config1, layered_nz3, layered_nz10, bt_sqg, qg_param,
-               long_runs, diagnostics_runs):
+               long_runs, diagnostics_runs, parameterizations):
         if only and fn.__name__ not in only:
             continue
         fn(ref)

@@ -424,3 +424,75 @@ def test_dissspec_identity():
         / m.H / m.dt / m.M ** 2
     unf = -np.real(np.tensordot(m.Hi, np.conj(m.ph) * rhs, axes=(0, 0))) / m.H / m.dt / m.M ** 2
     np.testing.assert_allclose(evaluate(m, "Dissspec"), filt - unf, atol=1e-20)
+
+
+# ------------------------------------------------------------------ closures
+PARAM_MODELS = {
+    "qg": (OracleQG, dict(nx=32, dt=7200.0, **QG_PHYS), 21),
+    "layered3": (OracleLayered, dict(nx=32, **LAYERED3), 22),
+    "bt": (OracleBT, dict(nx=32, beta=1.5e-11, rd=15000.0, H=1.0, U=0.01, dt=3600.0), 23),
+    "sqg": (OracleSQG, dict(nx=32, beta=0.0, H=1.0, U=0.0, dt=3600.0), 24),
+}
+
+
+def closure_state(name):
+    """Oracle model of one golden closure case after three steps and an inversion."""
+    from oracle.qg_diagnostics import attach_constants
+    cls, kw, seed = PARAM_MODELS[name]
+    m = cls(**kw)
+    m.q = 1e-6 * np.random.default_rng(seed).standard_normal((m.nz, m.ny, m.nx))
+    for _ in range(3):
+        m._step_forward()
+    m._invert()
+    return attach_constants(m)
+
+
+@pytest.mark.parametrize("name", list(PARAM_MODELS))
+def test_closures_match_reference(name):
+    """Smagorinsky, BackscatterBiharmonic and ZannaBolton2020 of the oracle
+    equal the reference's on the same states (tests/golden/parameterizations.npz)."""
+    import oracle.qg_parameterizations as P
+    g = golden("parameterizations.npz")
+    m = closure_state(name)
+    assert rel_l2(m.q, g[f"{name}_q"]) < 1e-13
+    du, dv = P.smagorinsky(m)
+    assert rel_l2(du, g[f"{name}_smag_du"]) < 1e-12 and rel_l2(dv, g[f"{name}_smag_dv"]) < 1e-12
+    assert rel_l2(P.smagorinsky(m, 0.2, just_viscosity=True), g[f"{name}_smag_nu"]) < 1e-12
+    assert rel_l2(P.backscatter_biharmonic(m), g[f"{name}_back_dq"]) < 1e-12
+    du, dv = P.zanna_bolton_2020(m)
+    assert rel_l2(du, g[f"{name}_zb_du"]) < 1e-12 and rel_l2(dv, g[f"{name}_zb_dv"]) < 1e-12
+
+
+@pytest.mark.parametrize("run", ["smag", "back", "zb", "combo"])
+def test_closure_runs_match_reference(run):
+    """Ten QGModel steps with a built-in closure as the model's parameterization."""
+    import oracle.qg_parameterizations as P
+    g = golden("parameterizations.npz")
+    kinds = {
+        "smag": ("uv_parameterization", lambda m: P.smagorinsky(m, 0.15), 10),
+        "back": ("q_parameterization", lambda m: P.backscatter_biharmonic(m), 10),
+        "zb": ("uv_parameterization", lambda m: P.zanna_bolton_2020(m), 10),
+        "combo": ("uv_parameterization",
+                  lambda m: np.sum([np.array(P.smagorinsky(m)) * 0.5,
+                                    np.array(P.zanna_bolton_2020(m)) * 0.75], axis=0), 6),
+    }
+    kind, fn, steps = kinds[run]
+    m = OracleQG(nx=32, dt=7200.0, **{kind: fn}, **QG_PHYS)
+    from oracle.qg_diagnostics import attach_constants
+    attach_constants(m)
+    m.q = g["run_q0"]
+    for _ in range(steps):
+        m._step_forward()
+    assert rel_l2(m.q, g[f"run_{run}_q"]) < 1e-12
+
+
+def test_closure_planted_mistake_is_caught():
+    """Smagorinsky with Sxy missing its factor 1/2 differs from the reference."""
+    g = golden("parameterizations.npz")
+    m = closure_state("qg")
+    ik, il = 1j * m.k, 1j * m.l
+    uh, vh = m.fft(m.u), m.fft(m.v)
+    Sxx, Syy = m.ifft(uh * ik), m.ifft(vh * il)
+    Sxy = m.ifft(uh * il + vh * ik)  # planted: no 0.5
+    nu = (0.2 * m.dx) ** 2 * np.sqrt(2 * (Sxx ** 2 + Syy ** 2 + 2 * Sxy ** 2))
+    assert rel_l2(nu, g["qg_smag_nu"]) > 1e-3
