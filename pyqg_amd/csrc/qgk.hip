@@ -435,6 +435,51 @@ __device__ __forceinline__ void split_store_t(const cd* lds, int t, int rp, bool
   }
 }
 
+// Hermitian extension of a row pair for one complex c2r transform:
+// z[s] = X0(p) + i X1(p), p = t + s*T, where X_r(p) = h_r[p] for p <= NX/2 and
+// conj(h_r[NX - p]) above, with the imaginary parts of bins 0 and NX/2
+// dropped (numpy's c2r).  h holds the thread's own slots p < NX/2 of both rows;
+// they are exchanged through LDS (rows at [0, NX/2) and [NX/2, NX) of each
+// lane) for the mirrored positions; the Nyquist bin is read by thread 0.
+template <int NX, int E, int T, int RP>
+__device__ __forceinline__ void row_pair_extend(const cd (&h)[2][E / 2],
+                                                const cd* __restrict__ row, int NK,
+                                                cd (&z)[E], cd* lds, int t, int rp) {
+  constexpr int HALF = NX / 2;
+  __syncthreads();  // the previous transform's last LDS reads are done
+#pragma unroll
+  for (int s = 0; s < E / 2; ++s) {
+    const int p = t + s * T;
+    lds[p * RP + rp] = h[0][s];
+    lds[(HALF + p) * RP + rp] = h[1][s];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const int p = t + s * T;
+    cd a, b;
+    if (s < E / 2) {
+      a = h[0][s];
+      b = h[1][s];
+      if (p == 0) {
+        a.y = 0.0;
+        b.y = 0.0;
+      }
+    } else if (p == HALF) {
+      a = row[HALF];
+      b = row[NK + HALF];
+      a.y = 0.0;
+      b.y = 0.0;
+    } else {
+      a = lds[(NX - p) * RP + rp];
+      b = lds[(HALF + NX - p) * RP + rp];
+      a.y = -a.y;
+      b.y = -b.y;
+    }
+    z[s] = {a.x - b.y, a.y + b.x};
+  }
+}
+
 // Row phase: per (member, layer, row pair): x-c2r of q, u, v, flux, x-r2c.
 // In:  Qm, Um, Vm  [M][nz][ny][ldk] (y already inverse transformed)
 // Out: UQm, VQm    [M][nz][ny][ldk] (x already forward transformed)
@@ -459,21 +504,27 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP)) void k_row_flux
   const double ubg = params[m].Ubg[z];
   const size_t row0 = ((size_t)plane_id * ny + j0) * NK;
 
-  cd zq[G::E], zu[G::E], zv[G::E];
+  // Every half-spectrum element of the six rows is read from memory once: the
+  // positions p < NX/2 a thread owns come straight from global memory, the
+  // mirrored ones (p > NX/2, conj of NX - p) from the LDS copy of the pair.
+  constexpr int H = G::E / 2;  // slots with p = t + s*T < NX/2
+  cd hq[2][H], hu[2][H], hv[2][H];
 #pragma unroll
-  for (int s = 0; s < G::E; ++s) {
+  for (int s = 0; s < H; ++s) {
     const int p = t + s * G::T;
-    cd a = hext<NX>(Qm + row0, p), b = hext<NX>(Qm + row0 + NK, p);
-    zq[s] = {a.x - b.y, a.y + b.x};
-    a = hext<NX>(Um + row0, p);
-    b = hext<NX>(Um + row0 + NK, p);
-    zu[s] = {a.x - b.y, a.y + b.x};
-    a = hext<NX>(Vm + row0, p);
-    b = hext<NX>(Vm + row0 + NK, p);
-    zv[s] = {a.x - b.y, a.y + b.x};
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      hq[r][s] = Qm[row0 + r * NK + p];
+      hu[r][s] = Um[row0 + r * NK + p];
+      hv[r][s] = Vm[row0 + r * NK + p];
+    }
   }
+  cd zq[G::E], zu[G::E], zv[G::E];
+  row_pair_extend<NX, G::E, G::T, G::RP>(hq, Qm + row0, NK, zq, lds, t, rp);
   qgk::fft_block<NX, G::E, +1, G::RP>(zq, lds, t, rp, tw);
+  row_pair_extend<NX, G::E, G::T, G::RP>(hu, Um + row0, NK, zu, lds, t, rp);
   qgk::fft_block<NX, G::E, +1, G::RP>(zu, lds, t, rp, tw);
+  row_pair_extend<NX, G::E, G::T, G::RP>(hv, Vm + row0, NK, zv, lds, t, rp);
   qgk::fft_block<NX, G::E, +1, G::RP>(zv, lds, t, rp, tw);
   // fluxes of both rows: uq = (u + Ubg) q, vq = v q (kernel.pyx:372-378)
 #pragma unroll
