@@ -122,6 +122,103 @@ def cpu_baseline(workload="qg256", nsteps=None):
                        f"cores; {wall:.1f} s wall")}
 
 
+# ------------------------------------------------------------------ CNN closure
+CNN_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
+
+
+def cnn_flops(channels, h=64, w=64):
+    """Multiply-adds x 2 of one sample's forward pass over the real channels."""
+    return 2.0 * h * w * 25 * sum(channels[l] * channels[l + 1] for l in range(len(channels) - 1))
+
+
+def cnn_cpu_baseline(channels, batch=4, reps=3):
+    """The same network in FP32 on the host with torch's CPU convolutions
+    (circular padding): a port, timed on a small batch."""
+    import torch
+    torch.manual_seed(0)
+    layers = []
+    for l in range(len(channels) - 1):
+        layers.append(torch.nn.Conv2d(channels[l], channels[l + 1], 5, padding=2,
+                                      padding_mode="circular"))
+        if l < len(channels) - 2:
+            layers.append(torch.nn.ReLU())
+    net = torch.nn.Sequential(*layers).eval()
+    x = torch.randn(batch, channels[0], 64, 64)
+    with torch.no_grad():
+        net(x)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            net(x)
+        wall = time.perf_counter() - t0
+    return {"value": batch * reps / wall, "unit": "samples/s", "cores": torch.get_num_threads(),
+            "kind": "port",
+            "sample": f"{reps} forward passes of {batch} samples, torch CPU FP32 conv2d"}
+
+
+def bench_cnn(args):
+    """Inference and training throughput of the closure network on libqgcnn."""
+    import torch
+    from pyqg_amd.closure import cnn as C
+    cpu = None if args.no_cpu else cnn_cpu_baseline(C.DEFAULT_CHANNELS)
+    net = C.CNN(seed=0)
+    B = 128
+    xp = net.pad_input(torch.randn(B, 64, 64, net.channels[0], device="cuda"))
+    st = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        net.forward(xp)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(st)
+    for _ in range(args.steps):
+        net.forward(xp)
+    e1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_fwd = e0.elapsed_time(e1) / args.steps
+    # training step (batch 64): forward, backward (data and weights), Adam
+    Bt = 64
+    xt = xp[:Bt].contiguous()
+    yt = torch.randn(Bt, 64, 64, net.channels[-1], device="cuda")
+    for _ in range(3):
+        net.train_step(xt, yt, lr=1e-5)
+    torch.cuda.synchronize()
+    nt = max(args.steps // 4, 5)
+    e0.record(st)
+    for _ in range(nt):
+        net.train_step(xt, yt, lr=1e-5)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms_train = e0.elapsed_time(e1) / nt
+    fl = cnn_flops(net.channels)
+    achieved = B * fl / (ms_fwd * 1e-3) / 1e12
+    out = {
+        "metric": "CNN subgrid-closure inference throughput (64x64 inputs, 8 conv layers)",
+        "value": B * args.steps / wall,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (random inputs, seeded weights)",
+        "config": {"workload": f"CNN closure forward, batch {B}, channels "
+                               f"{'-'.join(map(str, net.channels))}, 5x5 circular, NHWC BF16",
+                   "batch": B},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": CNN_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / CNN_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_conv_fwd_mt (all layers of one forward pass)",
+                     "algorithmic_flops_per_launch": B * fl, "forward_ms": ms_fwd,
+                     "train_step_ms_batch64": ms_train,
+                     "train_tflops": 3 * Bt * fl / (ms_train * 1e-3) / 1e12},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 # ------------------------------------------------------------------ GPU leg
 def main():
     ap = argparse.ArgumentParser()
@@ -130,7 +227,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--members", type=int, default=MEMBERS)
-    ap.add_argument("--workload", choices=["qg256", "uneven", "layered10"], default="qg256")
+    ap.add_argument("--workload", choices=["qg256", "uneven", "layered10", "cnn"],
+                    default="qg256")
     ap.add_argument("--copy-probe", type=int, default=1 << 30,
                     help="bytes moved by the copy kernel that calibrates HBM counters (0: skip)")
     args = ap.parse_args()
@@ -139,6 +237,8 @@ def main():
               "(members shard across GPUs with no per-step communication).", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "cnn":
+        return bench_cnn(args)
     cpu = None if args.no_cpu else cpu_baseline(args.workload)
 
     import numpy as np

@@ -31,6 +31,13 @@ int qgcnn_conv_fwd(const void* x, int B, int H, int W, int cinp, const void* w,
 /* dw[co][tap][ci] += sum_px dy[px][co] x[px + tap][ci] (FP32 accumulation) */
 int qgcnn_conv_wgrad(const void* dy, const void* x, int B, int H, int W, int cinp, int coutp,
                      float* dw, void* stream);
+/* The same weight gradient with the matrix cores, deterministic: every block
+ * writes its partial sums to its own slice of `workspace` (at least
+ * qgcnn_wgrad_workspace_bytes bytes, device memory) and a second kernel adds
+ * them into dw.  Returns 0 bytes for unsupported shapes. */
+size_t qgcnn_wgrad_workspace_bytes(int B, int H, int W, int cinp, int coutp);
+int qgcnn_conv_wgrad_ws(const void* dy, const void* x, int B, int H, int W, int cinp, int coutp,
+                        float* dw, void* workspace, size_t ws_bytes, void* stream);
 /* wt[ci][24 - tap][co] = w[co][tap][ci] (weights of the data-gradient convolution) */
 int qgcnn_flip_transpose(const void* w, void* wt, int cinp, int coutp, void* stream);
 /* db[c] += sum_px dy[px][c] */

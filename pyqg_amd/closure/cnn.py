@@ -36,6 +36,8 @@ def _load():
         "qgcnn_f32_to_bf16": (I, [P, P, Z, P]),
         "qgcnn_bf16_to_f32": (I, [P, P, Z, P]),
         "qgcnn_adam": (I, [P, P, P, P, P, Z, F, F, F, F, I, F, P]),
+        "qgcnn_wgrad_workspace_bytes": (Z, [I, I, I, I, I]),
+        "qgcnn_conv_wgrad_ws": (I, [P, P, I, I, I, I, I, P, P, Z, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -47,7 +49,7 @@ def _load():
 lib = _load()
 EXPORTS = ["qgcnn_last_error", "qgcnn_conv_fwd", "qgcnn_conv_wgrad", "qgcnn_flip_transpose",
            "qgcnn_bias_grad", "qgcnn_mse", "qgcnn_f32_to_bf16", "qgcnn_bf16_to_f32",
-           "qgcnn_adam"]
+           "qgcnn_adam", "qgcnn_wgrad_workspace_bytes", "qgcnn_conv_wgrad_ws"]
 
 
 def check(rc):
@@ -79,9 +81,32 @@ def conv_fwd(x, w, bias, coutp, relu=True, mask=None, out_bf16=True, out_f32=Fal
     return y, y32
 
 
-def conv_wgrad(dy, x, dw):
+_WS = {}
+
+
+def _workspace(nbytes, device):
+    """A device buffer of at least nbytes, kept for later calls."""
+    ws = _WS.get(device)
+    if ws is None or ws.numel() < nbytes:
+        ws = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+        _WS[device] = ws
+    return ws
+
+
+def conv_wgrad(dy, x, dw, blocked=True):
+    """dw += weight gradient.  blocked: the matrix-core kernel with a workspace
+    (deterministic); otherwise the atomic kernel."""
     B, H, W, coutp = dy.shape
     cinp = x.shape[-1]
+    if blocked:
+        n = lib.qgcnn_wgrad_workspace_bytes(B, H, W, cinp, coutp)
+        if n == 0:
+            raise ValueError(f"unsupported shape for the weight gradient: {tuple(dy.shape)}, "
+                             f"cinp={cinp}")
+        ws = _workspace(n, dy.device)
+        check(lib.qgcnn_conv_wgrad_ws(_p(dy), _p(x), B, H, W, cinp, coutp, _p(dw), _p(ws),
+                                      ws.numel(), _stream()))
+        return
     check(lib.qgcnn_conv_wgrad(_p(dy), _p(x), B, H, W, cinp, coutp, _p(dw), _stream()))
 
 

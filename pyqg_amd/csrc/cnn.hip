@@ -2,11 +2,13 @@
 //
 // A fully convolutional network of 5x5 periodic ("circular") convolutions on
 // square images, activations NHWC in BF16 with channels padded to multiples of
-// 16, FP32 accumulation in the matrix cores (v_mfma_f32_16x16x16_bf16).
+// 16, FP32 accumulation in the matrix cores (v_mfma_f32_16x16x32_bf16, and the
+// K=16 form for 16-channel inputs).
 //
-//   forward        y = act(conv(x, W) + b)            k_conv_fwd
-//   backward data  dx = conv(dy, W') * [x_prev > 0]   k_conv_fwd on W' = flip/transpose(W)
-//   backward wts   dW = sum_px dy (x) x_shifted        k_conv_wgrad
+//   forward        y = act(conv(x, W) + b)            k_conv_fwd_mt
+//   backward data  dx = conv(dy, W') * [x_prev > 0]   k_conv_fwd_mt on W' = flip/transpose(W)
+//   backward wts   dW = sum_px dy (x) x_shifted        k_conv_wgrad_mt (+ k_wgrad_reduce),
+//                                                      or k_conv_wgrad with atomics
 //   bias grad, MSE loss / grad, Adam, casts           small element-wise kernels
 //
 // Weight layout W[cout][25][cin] (tap = dy*5 + dx, dy, dx in 0..4 for offsets
@@ -60,83 +62,125 @@ constexpr int KS = 5;       // kernel size
 constexpr int NTAP = KS * KS;
 
 // --------------------------------------------------------------------------
-// Forward convolution.  Block: one output row y of sample b, a group of up to
-// 64 output channels (one wave per 16), all W pixels (W/16 MFMA tiles per
-// wave).  The five input rows y-2..y+2 of one 16-channel chunk are staged in
-// LDS; every tap reads its shifted pixels from there.
+// Forward convolution, blocked for the gfx950 matrix cores.  A block of four
+// waves computes four consecutive output rows (one per wave) of one sample for
+// CO = 16*MT output channels and all 64 pixels: MT x 4 MFMA tiles per wave.
+// The K loop runs over chunks of KC input channels (KC = 32 uses the K=32
+// instruction v_mfma_f32_16x16x32_bf16, KC = 16 the K=16 one); per chunk the
+// eight input rows the four output rows need are staged in LDS once, with a
+// padded pixel stride against bank conflicts.  Every B operand read from LDS
+// feeds MT MFMAs; the A operands (weights) come from the L1/L2-resident tensor.
+// MFMA lane layout (A 16 x K, B K x 16): lane (i = lane%16, g = lane/16) holds
+// A[i][g*K/4 .. +K/4] and B[g*K/4 .. +K/4][i]; the accumulator lane holds
+// D[4g + j][i].  A and B use the same K slots, so the sum covers every channel.
 // --------------------------------------------------------------------------
-template <int W>
-__global__ __launch_bounds__(256) void k_conv_fwd(const bf16_t* __restrict__ x,
-                                                  const bf16_t* __restrict__ w,
-                                                  const float* __restrict__ bias, int H,
-                                                  int cinp, int coutp, int relu,
-                                                  const bf16_t* __restrict__ mask,
-                                                  bf16_t* __restrict__ y,
-                                                  float* __restrict__ y32) {
-  constexpr int NT = W / 16;  // pixel tiles per row
-  __shared__ bf16_t lds[KS * W * 16];
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+template <int KC>
+struct MfmaK;
+template <>
+struct MfmaK<32> {
+  typedef v8bf op;
+  __device__ __forceinline__ static v4f mma(op a, op b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct MfmaK<16> {
+  typedef v4s op;
+  __device__ __forceinline__ static v4f mma(op a, op b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+  }
+};
+
+template <int W, int MT, int KC>
+__global__ __launch_bounds__(256) void k_conv_fwd_mt(const bf16_t* __restrict__ x,
+                                                     const bf16_t* __restrict__ w,
+                                                     const float* __restrict__ bias, int H,
+                                                     int cinp, int coutp, int relu,
+                                                     const bf16_t* __restrict__ mask,
+                                                     bf16_t* __restrict__ y,
+                                                     float* __restrict__ y32) {
+  using M = MfmaK<KC>;
+  typedef typename M::op op;
+  constexpr int NT = W / 16;            // pixel tiles per row
+  constexpr int ROWS = 4;               // output rows per block, one per wave
+  constexpr int SR = ROWS + KS - 1;     // staged input rows
+  constexpr int PS = KC + 8;            // padded pixel stride (bf16 elements)
+  constexpr int KG = KC / 4;            // K elements per lane
+  constexpr int PIECES = KC / 8;        // 16-byte pieces per staged pixel
+  __shared__ __attribute__((aligned(16))) bf16_t lds[SR * W * PS];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int nwaves = blockDim.x / 64;
-  const int row = blockIdx.x, b = blockIdx.y;
-  const int co0 = blockIdx.z * 64 + wave * 16;  // this wave's 16 output channels
-  const bool active = co0 < coutp;
-  v4f acc[NT];
+  const int row0 = blockIdx.x * ROWS, b = blockIdx.y;
+  const int co0 = blockIdx.z * (16 * MT);
+  const int row = row0 + wave;
+  const bool row_ok = row < H;
+  const int li = lane % 16, lg = lane / 16;
+  v4f acc[MT][NT];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
-  const int li = lane % 16, lk = lane / 16;
-  for (int ci0 = 0; ci0 < cinp; ci0 += 16) {
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[m][t] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int ci0 = 0; ci0 < cinp; ci0 += KC) {
     __syncthreads();
-    // stage rows row-2..row+2 (periodic), W pixels, 16 channels: 16-byte pieces
-    for (int q = threadIdx.x; q < KS * W * 2; q += blockDim.x) {
-      const int half = q & 1, pix = (q >> 1) % W, r = (q >> 1) / W;
-      const int yy = (row + r - 2 + H) % H;
+    for (int q = threadIdx.x; q < SR * W * PIECES; q += 256) {
+      const int pc = q % PIECES, pix = (q / PIECES) % W, r = q / (PIECES * W);
+      const int yy = ((row0 + r - 2) % H + H) % H;
       const uint4 v = *reinterpret_cast<const uint4*>(
-          x + (((size_t)b * H + yy) * W + pix) * cinp + ci0 + half * 8);
-      *reinterpret_cast<uint4*>(lds + (r * W + pix) * 16 + half * 8) = v;
+          x + (((size_t)b * H + yy) * W + pix) * cinp + ci0 + pc * 8);
+      *reinterpret_cast<uint4*>(lds + (r * W + pix) * PS + pc * 8) = v;
     }
     __syncthreads();
-    if (active) {
+    if (row_ok) {
+      const bf16_t* wrow = w + (size_t)(co0 + li) * NTAP * cinp + ci0 + lg * KG;
+#pragma unroll 1
       for (int tap = 0; tap < NTAP; ++tap) {
         const int dy = tap / KS, dx = tap % KS;
-        const v4s a = *reinterpret_cast<const v4s*>(
-            w + ((size_t)(co0 + li) * NTAP + tap) * cinp + ci0 + lk * 4);
+        op a[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+          a[m] = *reinterpret_cast<const op*>(wrow + ((size_t)m * 16 * NTAP + tap) * cinp);
+        const bf16_t* lrow = lds + (wave + dy) * W * PS + lg * KG;
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
           const int pix = (t * 16 + li + dx - 2 + W) % W;
-          const v4s bv = *reinterpret_cast<const v4s*>(lds + (dy * W + pix) * 16 + lk * 4);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[t], 0, 0, 0);
+          const op bv = *reinterpret_cast<const op*>(lrow + pix * PS);
+#pragma unroll
+          for (int m = 0; m < MT; ++m) acc[m][t] = M::mma(a[m], bv, acc[m][t]);
         }
       }
     }
   }
-  (void)nwaves;
-  if (!active) return;
-  // epilogue: lane holds channels co0 + 4*lk + j at pixel 16 t + li
+  if (!row_ok) return;
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int pix = t * 16 + li;
-    const size_t o = (((size_t)b * H + row) * W + pix) * coutp + co0 + lk * 4;
-    float v[4] = {acc[t][0], acc[t][1], acc[t][2], acc[t][3]};
+  for (int m = 0; m < MT; ++m) {
+    const int cb = co0 + m * 16 + lg * 4;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      if (bias) v[j] += bias[co0 + lk * 4 + j];
-      if (relu) v[j] = v[j] > 0.f ? v[j] : 0.f;
-    }
-    if (mask) {
-      const v4s m = *reinterpret_cast<const v4s*>(mask + o);
+    for (int t = 0; t < NT; ++t) {
+      const int pix = t * 16 + li;
+      const size_t o = (((size_t)b * H + row) * W + pix) * coutp + cb;
+      float v[4] = {acc[m][t][0], acc[m][t][1], acc[m][t][2], acc[m][t][3]};
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (bf2f((bf16_t)m[j]) <= 0.f) v[j] = 0.f;
-    }
-    if (y) {
-      v4s out;
+      for (int j = 0; j < 4; ++j) {
+        if (bias) v[j] += bias[cb + j];
+        if (relu) v[j] = v[j] > 0.f ? v[j] : 0.f;
+      }
+      if (mask) {
+        const v4s mk = *reinterpret_cast<const v4s*>(mask + o);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) out[j] = (short)f2bf(v[j]);
-      *reinterpret_cast<v4s*>(y + o) = out;
-    }
-    if (y32) {
+        for (int j = 0; j < 4; ++j)
+          if (bf2f((bf16_t)mk[j]) <= 0.f) v[j] = 0.f;
+      }
+      if (y) {
+        v4s out;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) y32[o + j] = v[j];
+        for (int j = 0; j < 4; ++j) out[j] = (short)f2bf(v[j]);
+        *reinterpret_cast<v4s*>(y + o) = out;
+      }
+      if (y32) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) y32[o + j] = v[j];
+      }
     }
   }
 }
@@ -208,6 +252,133 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16_t* __restrict__ d
 #pragma unroll
     for (int j = 0; j < 4; ++j)
       atomicAdd(dw + ((size_t)(co0 + lk * 4 + j) * NTAP + tap) * cinp + t * 16 + li, acc[t][j]);
+  }
+}
+
+// --------------------------------------------------------------------------
+// Weight gradient, blocked for the matrix cores.  dW[co][tap][ci] = sum over
+// pixels p of dy[p][co] * x[p + shift(tap)][ci]: a GEMM per tap with the pixels
+// as K.  A block of WC x WI waves owns the tile co [16 WC) x ci [16 WI) for all
+// 25 taps (each wave one 16 x 16 tile, 25 accumulators) and the rows
+// [r_begin, r_end) of one sample.  Per output row it stages dy (transposed to
+// [co][px]) and one new x row (transposed to [ci][px] with two periodic pixels
+// of padding on each side) into a five-row LDS ring.  For each 32-pixel K step
+// one A read (dy) and two aligned 16-byte B reads per x row give the B
+// operands of all five horizontal shifts (byte-aligned in registers), so one
+// dy read feeds 25 MFMAs.  The block's sums go to its own slice of a workspace
+// ([split][tap][co][ci]) that k_wgrad_reduce adds into dW.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ v8bf window8(const uint4& c0, const uint4& c1, int dx) {
+  unsigned w[4];
+  const unsigned u[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+  const int d = dx >> 1;
+  if (dx & 1) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = __builtin_amdgcn_alignbyte(u[d + k + 1], u[d + k], 2);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = u[d + k];
+  }
+  v8bf r;
+  __builtin_memcpy(&r, w, sizeof(r));
+  return r;
+}
+
+template <int W, int WC, int WI>
+__global__ __launch_bounds__(256) void k_conv_wgrad_mt(const bf16_t* __restrict__ dy,
+                                                       const bf16_t* __restrict__ x, int H,
+                                                       int cinp, int coutp, int rsplit,
+                                                       float* __restrict__ ws) {
+  constexpr int CO = 16 * WC, CI = 16 * WI;
+  constexpr int LS = W + 8;  // LDS row stride (elements): 64 + 8 keeps 16-lane reads conflict-free
+  constexpr int XP = W + 8;  // padded x row: element e holds pixel (e - 2) mod W
+  __shared__ __attribute__((aligned(16))) bf16_t sdy[CO * LS];
+  __shared__ __attribute__((aligned(16))) bf16_t sx[5 * CI * LS];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int nthreads = blockDim.x;
+  const int tiles_ci = cinp / CI;
+  const int cot = blockIdx.x / tiles_ci, cit = blockIdx.x % tiles_ci;
+  const int co_base = cot * CO, ci_base = cit * CI;
+  const int split = blockIdx.y;
+  const int b = split / rsplit, rs = split % rsplit;
+  const int rows = H / rsplit;
+  const int r_begin = rs * rows, r_end = r_begin + rows;
+  const int wc = wave % WC, wi = wave / WC;
+  const int li = lane % 16, lg = lane / 16;
+  v4f acc[NTAP];
+#pragma unroll
+  for (int t = 0; t < NTAP; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+
+  // stage x row q (logical, periodic) into ring slot q mod 5
+  auto stage_x = [&](int q) {
+    const int yy = ((q % H) + H) % H;
+    const int slot = ((q % 5) + 5) % 5;
+    for (int pc = threadIdx.x; pc < XP * (CI / 8); pc += nthreads) {
+      const int e = pc / (CI / 8), c8 = (pc % (CI / 8)) * 8;
+      const int px = (e - 2 + W) % W;
+      const uint4 v = *reinterpret_cast<const uint4*>(
+          x + (((size_t)b * H + yy) * W + px) * cinp + ci_base + c8);
+      const bf16_t* hv = reinterpret_cast<const bf16_t*>(&v);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) sx[(slot * CI + c8 + c) * LS + e] = hv[c];
+    }
+  };
+  stage_x(r_begin - 2);
+  stage_x(r_begin - 1);
+  stage_x(r_begin);
+  stage_x(r_begin + 1);
+  for (int r = r_begin; r < r_end; ++r) {
+    __syncthreads();  // previous row's reads are done
+    stage_x(r + 2);
+    for (int pc = threadIdx.x; pc < W * (CO / 8); pc += nthreads) {
+      const int px = pc / (CO / 8), c8 = (pc % (CO / 8)) * 8;
+      const uint4 v = *reinterpret_cast<const uint4*>(
+          dy + (((size_t)b * H + r) * W + px) * coutp + co_base + c8);
+      const bf16_t* hv = reinterpret_cast<const bf16_t*>(&v);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) sdy[(c8 + c) * LS + px] = hv[c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < W / 32; ++kk) {
+      const int p0 = kk * 32 + lg * 8;
+      const v8bf a = *reinterpret_cast<const v8bf*>(sdy + (wc * 16 + li) * LS + p0);
+#pragma unroll
+      for (int t = 0; t < KS; ++t) {
+        const int slot = (((r + t - 2) % 5) + 5) % 5;
+        const bf16_t* xr = sx + (slot * CI + wi * 16 + li) * LS + p0;
+        const uint4 c0 = *reinterpret_cast<const uint4*>(xr);
+        const uint4 c1 = *reinterpret_cast<const uint4*>(xr + 8);
+#pragma unroll
+        for (int dx = 0; dx < KS; ++dx)
+          acc[t * KS + dx] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, window8(c0, c1, dx),
+                                                                     acc[t * KS + dx], 0, 0, 0);
+      }
+    }
+  }
+  // lane holds D[co = 4 lg + j][ci = li] of its wave's tile, for every tap
+  float* out = ws + (size_t)split * NTAP * coutp * cinp;
+#pragma unroll
+  for (int t = 0; t < NTAP; ++t)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int co = co_base + wc * 16 + lg * 4 + j, ci = ci_base + wi * 16 + li;
+      out[((size_t)t * coutp + co) * cinp + ci] = acc[t][j];
+    }
+}
+
+// dW[co][tap][ci] += sum over splits of ws[split][tap][co][ci]
+__global__ void k_wgrad_reduce(const float* __restrict__ ws, float* __restrict__ dw, int nsplit,
+                               int cinp, int coutp) {
+  const size_t n = (size_t)NTAP * coutp * cinp;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int ci = (int)(i % cinp);
+    const int co = (int)((i / cinp) % coutp);
+    const int tap = (int)(i / ((size_t)cinp * coutp));
+    float s = 0.f;
+    for (int k = 0; k < nsplit; ++k) s += ws[(size_t)k * n + i];
+    dw[((size_t)co * NTAP + tap) * cinp + ci] += s;
   }
 }
 
@@ -312,11 +483,18 @@ int qgcnn_conv_fwd(const void* x, int B, int H, int W, int cinp, const void* w,
   if (B < 1 || H < KS || W != 64)
     return fail(QGCNN_E_ARG, "images must be W=64 wide and at least 5 rows (got %dx%d)", H, W);
   hipStream_t st = (hipStream_t)stream;
-  dim3 grid(H, B, (coutp + 63) / 64);
-  const int waves = coutp >= 64 ? 4 : coutp / 16;
-  hipLaunchKernelGGL(k_conv_fwd<64>, grid, dim3(64 * waves), 0, st, (const bf16_t*)x,
-                     (const bf16_t*)w, bias, H, cinp, coutp, relu, (const bf16_t*)mask,
-                     (bf16_t*)y, y32);
+  const int mt = coutp % 64 == 0 ? 4 : (coutp % 32 == 0 ? 2 : 1);
+  dim3 grid((H + 3) / 4, B, coutp / (16 * mt));
+#define QGCNN_FWD(MT, KC)                                                                    \
+  hipLaunchKernelGGL((k_conv_fwd_mt<64, MT, KC>), grid, dim3(256), 0, st, (const bf16_t*)x, \
+                     (const bf16_t*)w, bias, H, cinp, coutp, relu, (const bf16_t*)mask,      \
+                     (bf16_t*)y, y32)
+  if (cinp % 32 == 0) {
+    if (mt == 4) QGCNN_FWD(4, 32); else if (mt == 2) QGCNN_FWD(2, 32); else QGCNN_FWD(1, 32);
+  } else {
+    if (mt == 4) QGCNN_FWD(4, 16); else if (mt == 2) QGCNN_FWD(2, 16); else QGCNN_FWD(1, 16);
+  }
+#undef QGCNN_FWD
   HIPC(hipGetLastError());
   return 0;
 }
@@ -334,6 +512,68 @@ int qgcnn_conv_wgrad(const void* dy, const void* x, int B, int H, int W, int cin
   const size_t lds = (size_t)(64 * 16 + cinp * 16) * sizeof(bf16_t);
   hipLaunchKernelGGL(k_conv_wgrad<64>, grid, dim3(64 * waves), lds, st, (const bf16_t*)dy,
                      (const bf16_t*)x, H, cinp, coutp, rows_per_block, dw);
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+namespace {
+
+// wave layout and K split of the blocked weight-gradient kernel
+struct WgradPlan {
+  int wc, wi, rsplit, nsplit, tiles;
+};
+
+WgradPlan wgrad_plan(int B, int H, int cinp, int coutp) {
+  WgradPlan p;
+  p.wc = coutp >= 32 ? 2 : 1;
+  p.wi = cinp >= 32 ? 2 : 1;
+  if (p.wi == 1 && coutp >= 64) p.wc = 4;
+  if (p.wc == 1 && cinp >= 64) p.wi = 4;
+  p.tiles = (coutp / (16 * p.wc)) * (cinp / (16 * p.wi));
+  p.rsplit = 1;
+  while (p.rsplit < 8 && H % (2 * p.rsplit) == 0 && H / (2 * p.rsplit) >= 4 &&
+         (size_t)p.tiles * B * p.rsplit < 1024)
+    p.rsplit *= 2;
+  p.nsplit = B * p.rsplit;
+  return p;
+}
+
+}  // namespace
+
+size_t qgcnn_wgrad_workspace_bytes(int B, int H, int W, int cinp, int coutp) {
+  if (!mult16(cinp) || !mult16(coutp) || B < 1 || H < KS || W != 64) return 0;
+  const WgradPlan p = wgrad_plan(B, H, cinp, coutp);
+  return (size_t)p.nsplit * NTAP * coutp * cinp * sizeof(float);
+}
+
+int qgcnn_conv_wgrad_ws(const void* dy, const void* x, int B, int H, int W, int cinp, int coutp,
+                        float* dw, void* workspace, size_t ws_bytes, void* stream) {
+  if (!dy || !x || !dw || !workspace) return fail(QGCNN_E_ARG, "NULL argument");
+  if (!mult16(cinp) || !mult16(coutp))
+    return fail(QGCNN_E_ARG, "channel counts must be multiples of 16 in [16, 128]");
+  if (B < 1 || H < KS || W != 64) return fail(QGCNN_E_ARG, "images must be W=64 wide");
+  const WgradPlan p = wgrad_plan(B, H, cinp, coutp);
+  const size_t need = (size_t)p.nsplit * NTAP * coutp * cinp * sizeof(float);
+  if (ws_bytes < need)
+    return fail(QGCNN_E_ARG, "workspace of %zu bytes is smaller than %zu", ws_bytes, need);
+  if (H % p.rsplit != 0) return fail(QGCNN_E_ARG, "internal row split does not divide H");
+  hipStream_t st = (hipStream_t)stream;
+  dim3 grid(p.tiles, p.nsplit);
+  dim3 block(64 * p.wc * p.wi);
+#define QGCNN_WG(WC, WI)                                                                  \
+  hipLaunchKernelGGL((k_conv_wgrad_mt<64, WC, WI>), grid, block, 0, st, (const bf16_t*)dy, \
+                     (const bf16_t*)x, H, cinp, coutp, p.rsplit, (float*)workspace)
+  if (p.wc == 2 && p.wi == 2) QGCNN_WG(2, 2);
+  else if (p.wc == 4 && p.wi == 1) QGCNN_WG(4, 1);
+  else if (p.wc == 1 && p.wi == 4) QGCNN_WG(1, 4);
+  else if (p.wc == 2 && p.wi == 1) QGCNN_WG(2, 1);
+  else if (p.wc == 1 && p.wi == 2) QGCNN_WG(1, 2);
+  else QGCNN_WG(1, 1);
+#undef QGCNN_WG
+  HIPC(hipGetLastError());
+  const size_t n = (size_t)NTAP * coutp * cinp;
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid1(n)), dim3(256), 0, st, (const float*)workspace,
+                     dw, p.nsplit, cinp, coutp);
   HIPC(hipGetLastError());
   return 0;
 }

@@ -16,7 +16,7 @@ CNN_LIB = os.path.join(ROOT, "pyqg_amd", "_qgcnn.so")
 
 def declared(header=HEADER, prefix="qgk_"):
     text = open(header).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(" + prefix + r"\w+)\s*\(",
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char\*)\s+(" + prefix + r"\w+)\s*\(",
                                  text, re.M)))
 
 
