@@ -367,9 +367,23 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_mt(const bf16_t* __restrict_
     }
 }
 
-// dW[co][tap][ci] += sum over splits of ws[split][tap][co][ci]
-__global__ void k_wgrad_reduce(const float* __restrict__ ws, float* __restrict__ dw, int nsplit,
-                               int cinp, int coutp) {
+// First stage of the workspace reduction: part[g][i] = sum of ws[k][i] over
+// the splits k of group g (blockIdx.y), in a fixed order.
+constexpr int kSplitGroup = 32;
+__global__ void k_wgrad_partial(const float* __restrict__ ws, float* __restrict__ part,
+                                int nsplit, size_t n) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int k0 = blockIdx.y * kSplitGroup;
+  const int k1 = k0 + kSplitGroup < nsplit ? k0 + kSplitGroup : nsplit;
+  float s = 0.f;
+  for (int k = k0; k < k1; ++k) s += ws[(size_t)k * n + i];
+  part[(size_t)blockIdx.y * n + i] = s;
+}
+
+// dW[co][tap][ci] += sum over groups of part[g][tap][co][ci]
+__global__ void k_wgrad_reduce(const float* __restrict__ part, float* __restrict__ dw,
+                               int ngroups, int cinp, int coutp) {
   const size_t n = (size_t)NTAP * coutp * cinp;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
@@ -377,7 +391,7 @@ __global__ void k_wgrad_reduce(const float* __restrict__ ws, float* __restrict__
     const int co = (int)((i / cinp) % coutp);
     const int tap = (int)(i / ((size_t)cinp * coutp));
     float s = 0.f;
-    for (int k = 0; k < nsplit; ++k) s += ws[(size_t)k * n + i];
+    for (int g = 0; g < ngroups; ++g) s += part[(size_t)g * n + i];
     dw[((size_t)co * NTAP + tap) * cinp + ci] += s;
   }
 }
@@ -395,9 +409,12 @@ __global__ void k_flip_transpose(const bf16_t* __restrict__ w, bf16_t* __restric
   }
 }
 
-// db[c] += sum over pixels of dy[px][c]
-__global__ void k_bias_grad(const bf16_t* __restrict__ dy, size_t npix, int coutp,
-                            float* __restrict__ db) {
+// db[c] += sum over pixels of dy[px][c]: each thread sums one channel over a
+// strided set of pixels, the block combines its threads in LDS, and one atomic
+// per channel and block adds the result (blockDim = 256, a multiple of coutp).
+__global__ __launch_bounds__(256) void k_bias_grad(const bf16_t* __restrict__ dy, size_t npix,
+                                                   int coutp, float* __restrict__ db) {
+  __shared__ float red[256];
   const int c = threadIdx.x % coutp;
   const int lanes_per_c = blockDim.x / coutp;
   const int sub = threadIdx.x / coutp;
@@ -405,7 +422,12 @@ __global__ void k_bias_grad(const bf16_t* __restrict__ dy, size_t npix, int cout
   for (size_t p = blockIdx.x * (size_t)lanes_per_c + sub; p < npix;
        p += (size_t)gridDim.x * lanes_per_c)
     s += bf2f(dy[p * coutp + c]);
-  if (sub < lanes_per_c) atomicAdd(db + c, s);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (sub == 0) {
+    for (int k = 1; k < lanes_per_c; ++k) s += red[k * coutp + c];
+    atomicAdd(db + c, s);
+  }
 }
 
 // MSE over the first `nch` of `coutp` channels: loss += sum (pred - target)^2 * inv_n;
@@ -426,9 +448,16 @@ __global__ void k_mse(const float* __restrict__ pred, const float* __restrict__ 
     }
     if (grad) grad[i] = f2bf(g);
   }
-  // wave reduction then one atomic per wave
+  // wave reduction, then the block's waves through LDS, one atomic per block
+  __shared__ float red[16];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if ((threadIdx.x & 63) == 0 && loss) atomicAdd(loss, s * inv_n);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0 && loss) {
+    float t = 0.f;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    atomicAdd(loss, t * inv_n);
+  }
 }
 
 __global__ void k_f2bf(const float* __restrict__ a, bf16_t* __restrict__ o, size_t n) {
@@ -543,7 +572,8 @@ WgradPlan wgrad_plan(int B, int H, int cinp, int coutp) {
 size_t qgcnn_wgrad_workspace_bytes(int B, int H, int W, int cinp, int coutp) {
   if (!mult16(cinp) || !mult16(coutp) || B < 1 || H < KS || W != 64) return 0;
   const WgradPlan p = wgrad_plan(B, H, cinp, coutp);
-  return (size_t)p.nsplit * NTAP * coutp * cinp * sizeof(float);
+  const size_t groups = (p.nsplit + kSplitGroup - 1) / kSplitGroup;
+  return ((size_t)p.nsplit + groups) * NTAP * coutp * cinp * sizeof(float);
 }
 
 int qgcnn_conv_wgrad_ws(const void* dy, const void* x, int B, int H, int W, int cinp, int coutp,
@@ -553,7 +583,7 @@ int qgcnn_conv_wgrad_ws(const void* dy, const void* x, int B, int H, int W, int 
     return fail(QGCNN_E_ARG, "channel counts must be multiples of 16 in [16, 128]");
   if (B < 1 || H < KS || W != 64) return fail(QGCNN_E_ARG, "images must be W=64 wide");
   const WgradPlan p = wgrad_plan(B, H, cinp, coutp);
-  const size_t need = (size_t)p.nsplit * NTAP * coutp * cinp * sizeof(float);
+  const size_t need = qgcnn_wgrad_workspace_bytes(B, H, W, cinp, coutp);
   if (ws_bytes < need)
     return fail(QGCNN_E_ARG, "workspace of %zu bytes is smaller than %zu", ws_bytes, need);
   if (H % p.rsplit != 0) return fail(QGCNN_E_ARG, "internal row split does not divide H");
@@ -572,8 +602,12 @@ int qgcnn_conv_wgrad_ws(const void* dy, const void* x, int B, int H, int W, int 
 #undef QGCNN_WG
   HIPC(hipGetLastError());
   const size_t n = (size_t)NTAP * coutp * cinp;
-  hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid1(n)), dim3(256), 0, st, (const float*)workspace,
-                     dw, p.nsplit, cinp, coutp);
+  const int ngroups = (p.nsplit + kSplitGroup - 1) / kSplitGroup;
+  float* part = (float*)workspace + (size_t)p.nsplit * n;
+  hipLaunchKernelGGL(k_wgrad_partial, dim3((unsigned)((n + 255) / 256), ngroups), dim3(256), 0,
+                     st, (const float*)workspace, part, p.nsplit, n);
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(grid1(n)), dim3(256), 0, st, (const float*)part, dw,
+                     ngroups, cinp, coutp);
   HIPC(hipGetLastError());
   return 0;
 }
@@ -589,7 +623,7 @@ int qgcnn_flip_transpose(const void* w, void* wt, int cinp, int coutp, void* str
 
 int qgcnn_bias_grad(const void* dy, size_t npix, int coutp, float* db, void* stream) {
   if (!dy || !db || !mult16(coutp)) return fail(QGCNN_E_ARG, "bad arguments");
-  hipLaunchKernelGGL(k_bias_grad, dim3(512), dim3(256 / coutp * coutp), 0, (hipStream_t)stream,
+  hipLaunchKernelGGL(k_bias_grad, dim3(256), dim3(256 / coutp * coutp), 0, (hipStream_t)stream,
                      (const bf16_t*)dy, npix, coutp, db);
   HIPC(hipGetLastError());
   return 0;
@@ -599,8 +633,9 @@ int qgcnn_mse(const float* pred, const float* target, size_t npix, int coutp, in
               float gscale, void* grad, float* loss, void* stream) {
   if (!pred || !target || nch < 1 || nch > coutp) return fail(QGCNN_E_ARG, "bad arguments");
   const float inv_n = 1.0f / (float)((double)npix * nch);
-  hipLaunchKernelGGL(k_mse, dim3(grid1(npix * coutp)), dim3(256), 0, (hipStream_t)stream, pred,
-                     target, npix, coutp, nch, inv_n, gscale, (bf16_t*)grad, loss);
+  const int blocks = grid1(npix * coutp) < 1024 ? grid1(npix * coutp) : 1024;
+  hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, (hipStream_t)stream, pred, target, npix,
+                     coutp, nch, inv_n, gscale, (bf16_t*)grad, loss);
   HIPC(hipGetLastError());
   return 0;
 }
