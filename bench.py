@@ -157,6 +157,7 @@ def cnn_cpu_baseline(channels, batch=4, reps=3):
 
 def bench_cnn(args):
     """Inference and training throughput of the closure network on libqgcnn."""
+    import numpy as np
     import torch
     from pyqg_amd.closure import cnn as C
     cpu = None if args.no_cpu else cnn_cpu_baseline(C.DEFAULT_CHANNELS)
@@ -176,22 +177,41 @@ def bench_cnn(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_fwd = e0.elapsed_time(e1) / args.steps
-    # training step (batch 64): forward, backward (data and weights), Adam
-    Bt = 64
-    xt = xp[:Bt].contiguous()
-    yt = torch.randn(Bt, 64, 64, net.channels[-1], device="cuda")
-    for _ in range(3):
-        net.train_step(xt, yt, lr=1e-5)
-    torch.cuda.synchronize()
-    nt = max(args.steps // 4, 5)
-    e0.record(st)
-    for _ in range(nt):
-        net.train_step(xt, yt, lr=1e-5)
-    e1.record(st)
-    torch.cuda.synchronize()
-    ms_train = e0.elapsed_time(e1) / nt
+    # per-layer times of the same forward pass (events on the launch stream)
+    layer_ms = []
+    h = xp
+    for l in range(net.nlayers):
+        last = l == net.nlayers - 1
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record(st)
+        for _ in range(args.steps):
+            y, _ = C.conv_fwd(h, net.wb[l], net.b32[l], net.chp[l + 1], relu=not last)
+        ev[1].record(st)
+        torch.cuda.synchronize()
+        layer_ms.append(ev[0].elapsed_time(ev[1]) / args.steps)
+        h = y
     fl = cnn_flops(net.channels)
-    achieved = B * fl / (ms_fwd * 1e-3) / 1e12
+    layer_fl = [B * 2.0 * 64 * 64 * 25 * net.channels[l] * net.channels[l + 1]
+                for l in range(net.nlayers)]
+    dom = int(np.argmax(layer_fl))
+    achieved = layer_fl[dom] / (layer_ms[dom] * 1e-3) / 1e12
+    ms_train, train_tflops = None, None
+    if not args.cnn_inference_only:
+        # training step (batch 64): forward, backward (data and weights), Adam
+        Bt = 64
+        xt = xp[:Bt].contiguous()
+        yt = torch.randn(Bt, 64, 64, net.channels[-1], device="cuda")
+        for _ in range(3):
+            net.train_step(xt, yt, lr=1e-5)
+        torch.cuda.synchronize()
+        nt = max(args.steps // 4, 5)
+        e0.record(st)
+        for _ in range(nt):
+            net.train_step(xt, yt, lr=1e-5)
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms_train = e0.elapsed_time(e1) / nt
+        train_tflops = 3 * Bt * fl / (ms_train * 1e-3) / 1e12
     out = {
         "metric": "CNN subgrid-closure inference throughput (64x64 inputs, 8 conv layers)",
         "value": B * args.steps / wall,
@@ -210,10 +230,14 @@ def bench_cnn(args):
                    "batch": B},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": CNN_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / CNN_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_conv_fwd_mt (all layers of one forward pass)",
-                     "algorithmic_flops_per_launch": B * fl, "forward_ms": ms_fwd,
-                     "train_step_ms_batch64": ms_train,
-                     "train_tflops": 3 * Bt * fl / (ms_train * 1e-3) / 1e12},
+                     "kernel": (f"k_conv_fwd_mt<64, 4, 32>: layer {dom + 1} "
+                                f"({net.channels[dom]}->{net.channels[dom + 1]} channels), "
+                                f"batch {B}"),
+                     "algorithmic_flops_per_launch": layer_fl[dom],
+                     "kernel_ms": layer_ms[dom], "layer_ms": layer_ms,
+                     "forward_ms": ms_fwd,
+                     "forward_tflops": B * fl / (ms_fwd * 1e-3) / 1e12,
+                     "train_step_ms_batch64": ms_train, "train_tflops": train_tflops},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -229,6 +253,8 @@ def main():
     ap.add_argument("--members", type=int, default=MEMBERS)
     ap.add_argument("--workload", choices=["qg256", "uneven", "layered10", "cnn"],
                     default="qg256")
+    ap.add_argument("--cnn-inference-only", action="store_true",
+                    help="cnn workload: skip the training-step timing (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=1 << 30,
                     help="bytes moved by the copy kernel that calibrates HBM counters (0: skip)")
     args = ap.parse_args()
