@@ -688,6 +688,187 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
     col_prologue_layer<NY, E>(qh_new, Qm, Um, Vm, P, nz, z, nk, ldk, t, c, k, ok, mbase, lds, tw);
 }
 
+// ---- 3 <= nz <= 16: ph kept in its own buffer.  The prologue that produces
+// the next step's Qm, Um, Vm also writes ph, so the tendency reads ph once; the
+// inversion reads each layer's qh once per point (in registers) right after
+// the block wrote it.  Every thread reads back only the points it wrote, so no
+// barrier separates the phases.
+constexpr int kTileNzMax = 16;
+
+// phm = a qh at this thread's points of the column tile, for every layer, in
+// invert_point's summation order.
+template <int NY, int E>
+__device__ __forceinline__ void col_invert_points(const cd* __restrict__ qh, cd* __restrict__ phm,
+                                                  const MemberParams& P, int nz, int t, int k,
+                                                  bool ok, size_t mbase, int ldk) {
+#pragma clang fp contract(off)
+  using G = FGeo<NY, E>;
+  if (!ok) return;
+  const size_t plane = (size_t)NY * ldk;
+#pragma unroll 1
+  for (int s = 0; s < G::E; ++s) {
+    const size_t pt = (size_t)(t + s * G::T) * ldk + k;
+    if (!P.a) {  // closed-form two-layer inverse (not used by layered buckets)
+      for (int z = 0; z < nz; ++z)
+        phm[mbase + z * plane + pt] = invert_point(P, nz, z, qh + mbase + pt, plane, pt);
+      continue;
+    }
+    cd q[kTileNzMax];
+#pragma unroll
+    for (int j = 0; j < kTileNzMax; ++j)
+      q[j] = j < nz ? qh[mbase + j * plane + pt] : cd{0.0, 0.0};
+#pragma unroll
+    for (int z = 0; z < kTileNzMax; ++z) {
+      if (z < nz) {
+        cd acc = {0.0, 0.0};
+#pragma unroll
+        for (int j = 0; j < kTileNzMax; ++j) {
+          if (j < nz) {
+            const double a = P.a[((size_t)z * nz + j) * plane + pt];
+            acc.x = acc.x + a * q[j].x;
+            acc.y = acc.y + a * q[j].y;
+          }
+        }
+        phm[mbase + z * plane + pt] = acc;
+      }
+    }
+  }
+}
+
+// y IFFT of q, u = -il ph, v = ik ph of layer z from qh and the ph buffer
+template <int NY, int E>
+__device__ __forceinline__ void col_prologue_layer_ph(const cd* __restrict__ qh,
+                                                      const cd* __restrict__ phm,
+                                                      cd* __restrict__ Qm, cd* __restrict__ Um,
+                                                      cd* __restrict__ Vm, const MemberParams& P,
+                                                      int z, int ldk, int t, int c, int k,
+                                                      bool ok, size_t mbase, cd* lds,
+                                                      const cd* __restrict__ tw) {
+  using G = FGeo<NY, E>;
+  const size_t plane = (size_t)NY * ldk;
+  const size_t zoff = mbase + (size_t)z * plane;
+  const int kc = ok ? k : 0;
+  cd x[G::E], ph[G::E];
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) {
+    const size_t pt = (size_t)(t + s * G::T) * ldk + kc;
+    x[s] = ok ? qh[zoff + pt] : cd{0.0, 0.0};
+    ph[s] = ok ? phm[zoff + pt] : cd{0.0, 0.0};
+  }
+  qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+  if (ok) {
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) Qm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+  }
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) {
+    const double ll = P.ll[t + s * G::T];
+    x[s] = {ll * ph[s].y, -ll * ph[s].x};
+  }
+  qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+  if (ok) {
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) Um[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+  }
+  const double kk = P.kk[kc];
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) x[s] = {-kk * ph[s].y, kk * ph[s].x};
+  qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+  if (ok) {
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) Vm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+  }
+}
+
+template <int NY, int E>
+__global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_prologue_ph(
+    const cd* __restrict__ qh, cd* __restrict__ phm, cd* __restrict__ Qm, cd* __restrict__ Um,
+    cd* __restrict__ Vm, const MemberParams* __restrict__ params, int nz, int nk, int ldk,
+    const cd* __restrict__ tw) {
+  using G = FGeo<NY, E>;
+  __shared__ cd lds[NY * G::TK];
+  const int c = threadIdx.x % G::TK;
+  const int t = threadIdx.x / G::TK;
+  const int k = blockIdx.x * G::TK + c;
+  const int m = blockIdx.y;
+  const bool ok = k < nk;
+  const MemberParams& P = params[m];
+  const size_t mbase = (size_t)m * nz * NY * ldk;
+  col_invert_points<NY, E>(qh, phm, P, nz, t, k, ok, mbase, ldk);
+  for (int z = 0; z < nz; ++z)
+    col_prologue_layer_ph<NY, E>(qh, phm, Qm, Um, Vm, P, z, ldk, t, c, k, ok, mbase, lds, tw);
+}
+
+// Column phase for 3 <= nz <= 16: as k_col_step, with ph_old read from the ph
+// buffer, then ph_new = a qh_new written there for the next step.
+template <int NY, int E>
+__global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step_ph(
+    const cd* __restrict__ qh_old, cd* __restrict__ qh_new, const cd* __restrict__ p,
+    cd* __restrict__ pp_to_newp, const cd* __restrict__ UQm, const cd* __restrict__ VQm,
+    cd* __restrict__ phm, cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
+    const MemberParams* __restrict__ params, StepCoeffs cf, int nz, int nk, int ldk,
+    const cd* __restrict__ tw) {
+#pragma clang fp contract(off)
+  using G = FGeo<NY, E>;
+  __shared__ cd lds[NY * G::TK];
+  const int c = threadIdx.x % G::TK;
+  const int t = threadIdx.x / G::TK;
+  const int k = blockIdx.x * G::TK + c;
+  const int m = blockIdx.y;
+  const bool ok = k < nk;
+  const int kc = ok ? k : 0;
+  const MemberParams& P = params[m];
+  const size_t plane = (size_t)NY * ldk;
+  const size_t mbase = (size_t)m * nz * plane;
+  const double kk = P.kk[kc];
+
+  for (int z = 0; z < nz; ++z) {
+    const size_t zoff = mbase + (size_t)z * plane;
+    cd xa[G::E], xb[G::E];
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) {
+      const size_t pt = (size_t)(t + s * G::T) * ldk + kc;
+      xa[s] = ok ? UQm[zoff + pt] : cd{0.0, 0.0};
+      xb[s] = ok ? VQm[zoff + pt] : cd{0.0, 0.0};
+    }
+    qgk::fft_block<NY, G::E, -1, G::TK>(xa, lds, t, c, tw);
+    qgk::fft_block<NY, G::E, -1, G::TK>(xb, lds, t, c, tw);
+    if (ok) {
+      const double kq = kk * P.Qy[z];
+      const bool drag = (z == nz - 1) && P.rek != 0.0;
+#pragma unroll
+      for (int s = 0; s < G::E; ++s) {
+        const int l = t + s * G::T;
+        const size_t pt = (size_t)l * ldk + k;
+        const size_t o = zoff + pt;
+        const double ll = P.ll[l];
+        const cd S = {-kk * xa[s].y + -ll * xb[s].y, kk * xa[s].x + ll * xb[s].x};
+        const cd ph = phm[o];
+        cd d = {S.x + -kq * ph.y, S.y + kq * ph.x};
+        d = {-d.x, -d.y};
+        if (drag) {
+          const double cr = P.rek * P.k2l2[pt];
+          d = {d.x + cr * ph.x, d.y + cr * ph.y};
+        }
+        const double f = P.filtr[pt];
+        const cd q = qh_old[o], pv = p[o], ppv = pp_to_newp[o];
+        double rx = q.x + cf.dt1 * d.x;
+        double ry = q.y + cf.dt1 * d.y;
+        rx = rx + cf.dt2 * pv.x;
+        ry = ry + cf.dt2 * pv.y;
+        rx = rx + cf.dt3 * ppv.x;
+        ry = ry + cf.dt3 * ppv.y;
+        qh_new[o] = {f * rx, f * ry};
+        pp_to_newp[o] = d;
+      }
+    }
+  }
+  col_invert_points<NY, E>(qh_new, phm, P, nz, t, k, ok, mbase, ldk);
+  for (int z = 0; z < nz; ++z)
+    col_prologue_layer_ph<NY, E>(qh_new, phm, Qm, Um, Vm, P, z, ldk, t, c, k, ok, mbase, lds,
+                                 tw);
+}
+
 // ---- nz <= 2: every field is read exactly once; the layers' qh and ph at the
 // thread's points stay in registers.
 
@@ -1550,6 +1731,7 @@ struct qgk_bucket {
   cd* qh_alt = nullptr;  // the fused step writes qh here, then the two swap
   int iD = 0, iP = 1, iPP = 2;
   cd *Qm = nullptr, *Um = nullptr, *Vm = nullptr, *UQm = nullptr, *VQm = nullptr;
+  cd* phm = nullptr;  // ph of the current state for the fused step, 3 <= nz <= 16
   cd* ctmp = nullptr;      // spectral scratch (M*nz planes)
   double* rtmp = nullptr;  // real scratch (M*nz planes)
   bool primed = false;     // Qm/Um/Vm match the current qh
@@ -1825,6 +2007,9 @@ struct ColPrologueL {
     else if (b->nz == 2)
       hipLaunchKernelGGL((k_col_prologue_reg<NY, 2, E>), grid, block, 0, b->stream, qh, b->Qm,
                          b->Um, b->Vm, b->dp, b->nk, b->ldk, b->tw_y);
+    else if (b->phm)
+      hipLaunchKernelGGL((k_col_prologue_ph<NY, E>), grid, block, 0, b->stream, qh, b->phm,
+                         b->Qm, b->Um, b->Vm, b->dp, b->nz, b->nk, b->ldk, b->tw_y);
     else
       hipLaunchKernelGGL((k_col_prologue<NY, E>), grid, block, 0, b->stream, qh, b->Qm, b->Um,
                          b->Vm, b->dp, b->nz, b->nk, b->ldk, b->tw_y);
@@ -1846,6 +2031,10 @@ struct ColStepL {
       hipLaunchKernelGGL((k_col_step_reg<NY, 2, E>), grid, block, 0, b->lst ? b->lst : b->stream, qo, qn, Pp(b),
                          PP(b), b->UQm, b->VQm, b->Qm, b->Um, b->Vm, b->dp, cf, b->nk, b->ldk,
                          b->tw_y);
+    else if (b->phm)
+      hipLaunchKernelGGL((k_col_step_ph<NY, E>), grid, block, 0, b->lst ? b->lst : b->stream, qo,
+                         qn, Pp(b), PP(b), b->UQm, b->VQm, b->phm, b->Qm, b->Um, b->Vm, b->dp, cf,
+                         b->nz, b->nk, b->ldk, b->tw_y);
     else
       hipLaunchKernelGGL((k_col_step<NY, E>), grid, block, 0, b->lst ? b->lst : b->stream, qo, qn, Pp(b), PP(b),
                          b->UQm, b->VQm, b->Qm, b->Um, b->Vm, b->dp, cf, b->nz, b->nk, b->ldk,
@@ -2009,6 +2198,7 @@ int qgk_create(int nmembers, int nz, int ny, int nx, int flags, qgk_bucket** out
       (rc = ac(&b->UQm)) || (rc = ac(&b->VQm)) || (rc = ac(&b->ctmp)) || (rc = ar(&b->rtmp)) ||
       (rc = ar(&b->field_r[QGK_Q])))
     return cleanup(rc);
+  if (nz >= 3 && nz <= kTileNzMax && (rc = ac(&b->phm))) return cleanup(rc);
   if (flags & QGK_FULL_STATE) {
     const int cf[] = {QGK_PH, QGK_UH, QGK_VH, QGK_UQH, QGK_VQH};
     const int rf[] = {QGK_U, QGK_V, QGK_UQ, QGK_VQ};

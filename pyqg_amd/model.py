@@ -79,10 +79,29 @@ class Model(DiagnosticsMixin, PseudoSpectralKernel):
 
     # ------------------------------------------------------------ running
     def run_with_snapshots(self, tsnapstart=0., tsnapint=432000.):
-        """Yield model time every tsnapint once t >= tsnapstart (model.py:260-278)."""
+        """Yield model time every tsnapint once t >= tsnapstart (model.py:260-278).
+
+        Between yields the steps run as in run(): fused stretches, with the
+        step before each yield, status report or diagnostic accumulation run
+        phase by phase, so the fields a caller reads at a yield are the
+        per-method path's."""
         tsnapints = np.ceil(tsnapint / self.dt)
+        if self.q_parameterization is not None or self.uv_parameterization is not None:
+            while self.t < self.tmax:
+                self._step_forward()
+                if self.t >= tsnapstart and (self.tc % tsnapints) == 0:
+                    yield self.t
+            return
         while self.t < self.tmax:
-            self._step_forward()
+            # steps up to and including the next one that yields (or the end)
+            t, tc, n = self.t, self.tc, 0
+            while t < self.tmax:
+                t += self.dt
+                tc += 1
+                n += 1
+                if t >= tsnapstart and (tc % tsnapints) == 0:
+                    break
+            self._advance(n)
             if self.t >= tsnapstart and (self.tc % tsnapints) == 0:
                 yield self.t
 
@@ -103,6 +122,11 @@ class Model(DiagnosticsMixin, PseudoSpectralKernel):
         while t < self.tmax:
             t += self.dt
             n += 1
+        self._advance(n)
+
+    def _advance(self, n):
+        """n unparameterized steps: fused stretches ending at status reports and
+        before diagnostic steps; those run phase by phase."""
         while n > 0:
             chunk = n
             if self.log_level:

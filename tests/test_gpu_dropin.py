@@ -355,3 +355,29 @@ def test_one_hip_runtime_with_torch(gpu, order):
     r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True,
                        timeout=180)
     assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stderr[-2000:]
+
+
+def test_run_with_snapshots_matches_oracle(pq):
+    """model.py:260-278: yields after the steps with t >= tsnapstart and
+    tc % ceil(tsnapint / dt) == 0; the yielded states are the per-step path's."""
+    from oracle.qg_oracle import OracleQG
+    dt = 3600.0
+    kw = dict(nx=64, dt=dt, **QG_PHYS)
+    q0 = 1e-7 * np.random.default_rng(8).standard_normal((2, 64, 64))
+    m = pq.QGModel(log_level=0, tmax=30 * dt, **kw)
+    m.q = q0
+    o = OracleQG(**kw)
+    o.q = q0
+    got = []
+    for t in m.run_with_snapshots(tsnapstart=10 * dt, tsnapint=7 * dt):
+        got.append((t, m.tc, m.q.copy(), m.ph.copy()))
+    want = []
+    while o.t < 30 * dt:
+        o._step_forward()
+        if o.t >= 10 * dt and o.tc % 7 == 0:
+            want.append((o.t, o.tc, o.q.copy(), o.ph.copy()))
+    assert [(t, tc) for t, tc, _, _ in got] == [(t, tc) for t, tc, _, _ in want]
+    assert len(got) == 3
+    for (_, _, q, ph), (_, _, qo, pho) in zip(got, want):
+        assert rel_l2(q, qo) < 1e-12 and rel_l2(ph, pho) < 1e-12
+    assert m.t == o.t and m.tc == o.tc
