@@ -427,11 +427,17 @@ template <int NX, int T, int RP>
 __device__ __forceinline__ void split_store_t(const cd* lds, int t, int rp, bool ok, cd* o0,
                                               cd* o1) {
   if (!ok) return;
-  for (int k = t; k <= NX / 2; k += T) {
-    cd A = lds[k * RP + rp];
-    cd B = lds[((NX - k) & (NX - 1)) * RP + rp];
-    o0[k] = {0.5 * (A.x + B.x), 0.5 * (A.y - B.y)};
-    o1[k] = {0.5 * (A.y + B.y), -0.5 * (A.x - B.x)};
+  constexpr int NK = ldk_of(NX);
+  for (int k = t; k < NK; k += T) {
+    if (k <= NX / 2) {
+      cd A = lds[k * RP + rp];
+      cd B = lds[((NX - k) & (NX - 1)) * RP + rp];
+      o0[k] = {0.5 * (A.x + B.x), 0.5 * (A.y - B.y)};
+      o1[k] = {0.5 * (A.y + B.y), -0.5 * (A.x - B.x)};
+    } else {  // padding: zeros, so the row is written as whole 128-byte lines
+      o0[k] = cd{0.0, 0.0};
+      o1[k] = cd{0.0, 0.0};
+    }
   }
 }
 
@@ -573,7 +579,7 @@ __device__ __forceinline__ void col_prologue_layer(const cd* __restrict__ qh,
     ph[s] = ok ? invert_point(P, nz, z, qh + mbase + pt, plane, pt) : cd{0.0, 0.0};
   }
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
-  if (ok) {
+  {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
     for (int s = 0; s < G::E; ++s) Qm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
   }
@@ -584,7 +590,7 @@ __device__ __forceinline__ void col_prologue_layer(const cd* __restrict__ qh,
     x[s] = {ll * ph[s].y, -ll * ph[s].x};
   }
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
-  if (ok) {
+  {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
     for (int s = 0; s < G::E; ++s) Um[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
   }
@@ -593,7 +599,7 @@ __device__ __forceinline__ void col_prologue_layer(const cd* __restrict__ qh,
 #pragma unroll
   for (int s = 0; s < G::E; ++s) x[s] = {-kk * ph[s].y, kk * ph[s].x};
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
-  if (ok) {
+  {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
     for (int s = 0; s < G::E; ++s) Vm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
   }
@@ -682,6 +688,14 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
         qh_new[o] = {f * rx, f * ry};
         pp_to_newp[o] = d;
       }
+    } else {
+      // lanes past nk write zeros so every tile row is a full 128-byte line
+#pragma unroll
+      for (int s = 0; s < G::E; ++s) {
+        const size_t o = zoff + (size_t)(t + s * G::T) * ldk + k;
+        qh_new[o] = cd{0.0, 0.0};
+        pp_to_newp[o] = cd{0.0, 0.0};
+      }
     }
   }
   for (int z = 0; z < nz; ++z)
@@ -756,7 +770,7 @@ __device__ __forceinline__ void col_prologue_layer_ph(const cd* __restrict__ qh,
     ph[s] = ok ? phm[zoff + pt] : cd{0.0, 0.0};
   }
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
-  if (ok) {
+  {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
     for (int s = 0; s < G::E; ++s) Qm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
   }
@@ -766,7 +780,7 @@ __device__ __forceinline__ void col_prologue_layer_ph(const cd* __restrict__ qh,
     x[s] = {ll * ph[s].y, -ll * ph[s].x};
   }
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
-  if (ok) {
+  {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
     for (int s = 0; s < G::E; ++s) Um[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
   }
@@ -774,7 +788,7 @@ __device__ __forceinline__ void col_prologue_layer_ph(const cd* __restrict__ qh,
 #pragma unroll
   for (int s = 0; s < G::E; ++s) x[s] = {-kk * ph[s].y, kk * ph[s].x};
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
-  if (ok) {
+  {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
     for (int s = 0; s < G::E; ++s) Vm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
   }
@@ -861,6 +875,14 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
         qh_new[o] = {f * rx, f * ry};
         pp_to_newp[o] = d;
       }
+    } else {
+      // lanes past nk write zeros so every tile row is a full 128-byte line
+#pragma unroll
+      for (int s = 0; s < G::E; ++s) {
+        const size_t o = zoff + (size_t)(t + s * G::T) * ldk + k;
+        qh_new[o] = cd{0.0, 0.0};
+        pp_to_newp[o] = cd{0.0, 0.0};
+      }
     }
   }
   col_invert_points<NY, E>(qh_new, phm, P, nz, t, k, ok, mbase, ldk);
@@ -931,7 +953,7 @@ __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
       x[s] = Q[z][s];
     }
     qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
-    if (ok) {
+    {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
       for (int s = 0; s < G::E; ++s) Qm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
     }
@@ -941,14 +963,14 @@ __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
       x[s] = {ll * ph[s].y, -ll * ph[s].x};
     }
     qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
-    if (ok) {
+    {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
       for (int s = 0; s < G::E; ++s) Um[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
     }
 #pragma unroll
     for (int s = 0; s < G::E; ++s) x[s] = {-kk * ph[s].y, kk * ph[s].x};
     qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
-    if (ok) {
+    {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
       for (int s = 0; s < G::E; ++s) Vm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
     }
@@ -1060,6 +1082,14 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
           if (z == 0) Q0old[s] = q;
         }
         Q[z][s] = qn;
+      }
+    } else {
+      // lanes past nk write zeros so every tile row is a full 128-byte line
+#pragma unroll
+      for (int s = 0; s < G::E; ++s) {
+        const size_t o = zoff + (size_t)(t + s * G::T) * ldk + k;
+        qh_new[o] = cd{0.0, 0.0};
+        pp_to_newp[o] = cd{0.0, 0.0};
       }
     }
   }
