@@ -4,9 +4,9 @@
 // complex values in registers, x[s] = element (t + s*T).  The transform is a
 // sequence of Stockham auto-sort passes of radix R <= E (largest radix first),
 // each pass a register-resident radix-R DFT; consecutive passes exchange data
-// through LDS.  Several independent transforms ("lanes": columns of a tile or
-// rows of a row group) share one LDS buffer interleaved with stride LANES, so a
-// wave touches consecutive 16-byte words.  On exit x[s] holds output element
+// through LDS.  Several independent transforms ("lanes") share one LDS buffer:
+// the columns of a tile interleaved with stride LANES, so a wave touches
+// consecutive 16-byte words; the rows of a row group one swizzled region each.  On exit x[s] holds output element
 // (t + s*T) in natural order, so a forward transform can feed an inverse one
 // (or the reverse) without any reordering.
 //
@@ -118,10 +118,41 @@ struct DFT<4, SIGN> {
   }
 };
 
+// LDS placement of element e of transform `lane`.
+//
+// Interleaved: transforms interleaved word by word (a column tile: the lanes of
+// a wave are consecutive columns, so every access is a run of whole lines).
+//
+// at(base, off, lane) places element base + off, where `off` is a constant once
+// the caller's loops are unrolled: layouts use it to fold the offset into the
+// instruction's immediate.
+template <int LANES>
+struct Interleaved {
+  __device__ __forceinline__ static int at(int e, int lane) { return e * LANES + lane; }
+  __device__ __forceinline__ static int at(int base, int off, int lane) {
+    return (base + off) * LANES + lane;
+  }
+};
+// RowSwizzled: one contiguous region of N words per transform (a row); inside
+// each block of 8 words the low three index bits are XORed with the block
+// number, so the power-of-two lane strides of the Stockham writes, which would
+// land on one bank group, spread over all of them, while runs of consecutive
+// elements stay on distinct banks.
+template <int N>
+struct RowSwizzled {
+  __device__ __forceinline__ static int swz(int e) { return e ^ ((e >> 3) & 7); }
+  __device__ __forceinline__ static int at(int e, int lane) { return lane * N + swz(e); }
+  // an offset that is a multiple of 64 leaves index bits 0-5 alone, so it
+  // passes through the swizzle unchanged
+  __device__ __forceinline__ static int at(int base, int off, int lane) {
+    return (off & 63) == 0 ? lane * N + swz(base) + off : lane * N + swz(base + off);
+  }
+};
+
 // One Stockham pass of radix R on sub-transforms of length L (L*R <= N).
 // Butterfly b (0 <= b < N/R) reads positions b + s*N/R, writes k*L*R + j + m*L
 // with j = b % L, k = b / L, after twiddling input s by w^(j*s*N/(L*R)).
-template <int N, int E, int R, int L, int SIGN, int LANES, bool FIRST, bool LAST>
+template <int N, int E, int R, int L, int SIGN, class LY, bool FIRST, bool LAST>
 __device__ __forceinline__ void stockham_pass(cd (&x)[E], cd* lds, int t, int lane,
                                               const cd* __restrict__ tw) {
   constexpr int T = N / E;
@@ -136,9 +167,8 @@ __device__ __forceinline__ void stockham_pass(cd (&x)[E], cd* lds, int t, int la
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
-      const int b = t + i * T;
 #pragma unroll
-      for (int s = 0; s < R; ++s) y[i][s] = lds[(b + s * (N / R)) * LANES + lane];
+      for (int s = 0; s < R; ++s) y[i][s] = lds[LY::at(t, i * T + s * (N / R), lane)];
     }
   }
 #pragma unroll
@@ -173,19 +203,19 @@ __device__ __forceinline__ void stockham_pass(cd (&x)[E], cd* lds, int t, int la
       const int j = b % L;
       const int k = b / L;
 #pragma unroll
-      for (int m = 0; m < R; ++m) lds[(k * L * R + j + m * L) * LANES + lane] = y[i][m];
+      for (int m = 0; m < R; ++m) lds[LY::at(k * L * R + j + m * L, lane)] = y[i][m];
     }
   }
 }
 
-template <int N, int E, int SIGN, int LANES, int L>
+template <int N, int E, int SIGN, class LY, int L>
 struct StockhamFrom {
   static constexpr int REM = N / L;
   static constexpr int R = REM < E ? REM : E;
   __device__ __forceinline__ static void run(cd (&x)[E], cd* lds, int t, int lane,
                                              const cd* __restrict__ tw) {
-    stockham_pass<N, E, R, L, SIGN, LANES, (L == 1), (L * R == N)>(x, lds, t, lane, tw);
-    if constexpr (L * R < N) StockhamFrom<N, E, SIGN, LANES, L * R>::run(x, lds, t, lane, tw);
+    stockham_pass<N, E, R, L, SIGN, LY, (L == 1), (L * R == N)>(x, lds, t, lane, tw);
+    if constexpr (L * R < N) StockhamFrom<N, E, SIGN, LY, L * R>::run(x, lds, t, lane, tw);
   }
 };
 
@@ -195,7 +225,16 @@ template <int N, int E, int SIGN, int LANES>
 __device__ __forceinline__ void fft_block(cd (&x)[E], cd* lds, int t, int lane,
                                           const cd* __restrict__ tw) {
   static_assert((N & (N - 1)) == 0 && N >= E, "N must be a power of two >= E");
-  StockhamFrom<N, E, SIGN, LANES, 1>::run(x, lds, t, lane, tw);
+  StockhamFrom<N, E, SIGN, Interleaved<LANES>, 1>::run(x, lds, t, lane, tw);
+}
+
+// As fft_block, for transforms along rows: transform `row` of the block keeps
+// its own swizzled LDS region (RowSwizzled<N>); lds must hold N*rows values.
+template <int N, int E, int SIGN>
+__device__ __forceinline__ void fft_block_rows(cd (&x)[E], cd* lds, int t, int row,
+                                               const cd* __restrict__ tw) {
+  static_assert((N & (N - 1)) == 0 && N >= E, "N must be a power of two >= E");
+  StockhamFrom<N, E, SIGN, RowSwizzled<N>, 1>::run(x, lds, t, row, tw);
 }
 
 }  // namespace qgk

@@ -198,7 +198,7 @@ __global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_c2r(
     cd a = hext<NX>(r0, p), b = hext<NX>(r1, p);
     x[s] = {a.x - b.y, a.y + b.x};
   }
-  qgk::fft_block<NX, G::E, +1, G::RP>(x, lds, t, rp, tw);
+  qgk::fft_block_rows<NX, G::E, +1>(x, lds, t, rp, tw);
   if (ok) {
     double* o0 = out + ((size_t)blockIdx.y * ny + j0) * NX;
     double* o1 = o0 + NX;
@@ -211,7 +211,8 @@ __global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_c2r(
   }
 }
 
-// Splits the transform Z of z = x0 + i x1 (held in LDS, interleave RP) into
+// Splits the transform Z of z = x0 + i x1 (held in LDS, one swizzled region
+// per row pair) into
 // the half spectra of x0 and x1 and writes them.
 template <int NX, int RP>
 __device__ __forceinline__ void split_store(const cd* lds, int t, int rp, bool ok, cd* o0,
@@ -219,8 +220,8 @@ __device__ __forceinline__ void split_store(const cd* lds, int t, int rp, bool o
   using G = Geo<NX>;
   if (!ok) return;
   for (int k = t; k <= NX / 2; k += G::T) {
-    cd A = lds[k * RP + rp];
-    cd B = lds[((NX - k) & (NX - 1)) * RP + rp];
+    cd A = lds[qgk::RowSwizzled<NX>::at(k, rp)];
+    cd B = lds[qgk::RowSwizzled<NX>::at((NX - k) & (NX - 1), rp)];
     o0[k] = {0.5 * (A.x + B.x), 0.5 * (A.y - B.y)};
     o1[k] = {0.5 * (A.y + B.y), -0.5 * (A.x - B.x)};
   }
@@ -246,10 +247,10 @@ __global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_r2c(
     const int p = t + s * G::T;
     x[s] = {i0[p], i1[p]};
   }
-  qgk::fft_block<NX, G::E, -1, G::RP>(x, lds, t, rp, tw);
+  qgk::fft_block_rows<NX, G::E, -1>(x, lds, t, rp, tw);
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < G::E; ++s) lds[(t + s * G::T) * G::RP + rp] = x[s];
+  for (int s = 0; s < G::E; ++s) lds[qgk::RowSwizzled<NX>::at(t, s * G::T, rp)] = x[s];
   __syncthreads();
   cd* o0 = out + ((size_t)blockIdx.y * ny + j0) * NK;
   split_store<NX, G::RP>(lds, t, rp, ok, o0, o0 + NK);
@@ -430,8 +431,8 @@ __device__ __forceinline__ void split_store_t(const cd* lds, int t, int rp, bool
   constexpr int NK = ldk_of(NX);
   for (int k = t; k < NK; k += T) {
     if (k <= NX / 2) {
-      cd A = lds[k * RP + rp];
-      cd B = lds[((NX - k) & (NX - 1)) * RP + rp];
+      cd A = lds[qgk::RowSwizzled<NX>::at(k, rp)];
+      cd B = lds[qgk::RowSwizzled<NX>::at((NX - k) & (NX - 1), rp)];
       o0[k] = {0.5 * (A.x + B.x), 0.5 * (A.y - B.y)};
       o1[k] = {0.5 * (A.y + B.y), -0.5 * (A.x - B.x)};
     } else {  // padding: zeros, so the row is written as whole 128-byte lines
@@ -455,9 +456,8 @@ __device__ __forceinline__ void row_pair_extend(const cd (&h)[2][E / 2],
   __syncthreads();  // the previous transform's last LDS reads are done
 #pragma unroll
   for (int s = 0; s < E / 2; ++s) {
-    const int p = t + s * T;
-    lds[p * RP + rp] = h[0][s];
-    lds[(HALF + p) * RP + rp] = h[1][s];
+    lds[qgk::RowSwizzled<NX>::at(t, s * T, rp)] = h[0][s];
+    lds[qgk::RowSwizzled<NX>::at(t, HALF + s * T, rp)] = h[1][s];
   }
   __syncthreads();
 #pragma unroll
@@ -477,8 +477,9 @@ __device__ __forceinline__ void row_pair_extend(const cd (&h)[2][E / 2],
       a.y = 0.0;
       b.y = 0.0;
     } else {
-      a = lds[(NX - p) * RP + rp];
-      b = lds[(HALF + NX - p) * RP + rp];
+      // element NX - p = -t + (NX - s*T)
+      a = lds[qgk::RowSwizzled<NX>::at(-t, NX - s * T, rp)];
+      b = lds[qgk::RowSwizzled<NX>::at(-t, HALF + NX - s * T, rp)];
       a.y = -a.y;
       b.y = -b.y;
     }
@@ -527,11 +528,11 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP)) void k_row_flux
   }
   cd zq[G::E], zu[G::E], zv[G::E];
   row_pair_extend<NX, G::E, G::T, G::RP>(hq, Qm + row0, NK, zq, lds, t, rp);
-  qgk::fft_block<NX, G::E, +1, G::RP>(zq, lds, t, rp, tw);
+  qgk::fft_block_rows<NX, G::E, +1>(zq, lds, t, rp, tw);
   row_pair_extend<NX, G::E, G::T, G::RP>(hu, Um + row0, NK, zu, lds, t, rp);
-  qgk::fft_block<NX, G::E, +1, G::RP>(zu, lds, t, rp, tw);
+  qgk::fft_block_rows<NX, G::E, +1>(zu, lds, t, rp, tw);
   row_pair_extend<NX, G::E, G::T, G::RP>(hv, Vm + row0, NK, zv, lds, t, rp);
-  qgk::fft_block<NX, G::E, +1, G::RP>(zv, lds, t, rp, tw);
+  qgk::fft_block_rows<NX, G::E, +1>(zv, lds, t, rp, tw);
   // fluxes of both rows: uq = (u + Ubg) q, vq = v q (kernel.pyx:372-378)
 #pragma unroll
   for (int s = 0; s < G::E; ++s) {
@@ -541,16 +542,16 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP)) void k_row_flux
     zu[s] = {(u0 + ubg) * q0, (u1 + ubg) * q1};
     zv[s] = {v0 * q0, v1 * q1};
   }
-  qgk::fft_block<NX, G::E, -1, G::RP>(zu, lds, t, rp, tw);
+  qgk::fft_block_rows<NX, G::E, -1>(zu, lds, t, rp, tw);
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < G::E; ++s) lds[(t + s * G::T) * G::RP + rp] = zu[s];
+  for (int s = 0; s < G::E; ++s) lds[qgk::RowSwizzled<NX>::at(t, s * G::T, rp)] = zu[s];
   __syncthreads();
   split_store_t<NX, G::T, G::RP>(lds, t, rp, ok, UQm + row0, UQm + row0 + NK);
-  qgk::fft_block<NX, G::E, -1, G::RP>(zv, lds, t, rp, tw);
+  qgk::fft_block_rows<NX, G::E, -1>(zv, lds, t, rp, tw);
   __syncthreads();
 #pragma unroll
-  for (int s = 0; s < G::E; ++s) lds[(t + s * G::T) * G::RP + rp] = zv[s];
+  for (int s = 0; s < G::E; ++s) lds[qgk::RowSwizzled<NX>::at(t, s * G::T, rp)] = zv[s];
   __syncthreads();
   split_store_t<NX, G::T, G::RP>(lds, t, rp, ok, VQm + row0, VQm + row0 + NK);
 }
