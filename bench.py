@@ -322,6 +322,16 @@ def main():
         ms = pyqg_amd._lib.copy_probe(args.copy_probe)
         copy = {"bytes": args.copy_probe, "ms": ms,
                 "GB/s": 2 * args.copy_probe / (ms * 1e-3) / 1e9}
+    pattern = None
+    if args.workload == "qg256" and nz == 2:
+        # memory-only stand-ins of the two kernels at this shape: the time the
+        # fused kernels would take if their transforms were free
+        b0 = ens.buckets[0]
+        ldk = ((b0.nx // 2 + 1) + 7) // 8 * 8
+        ms_r, ms_c = pyqg_amd._lib.pattern_probe(b0.ny, ldk, b0.M, nz)
+        pattern = {"row_ms": ms_r, "col_ms": ms_c,
+                   "row_kernel_over_bound": per_bucket[0]["row_kernel_ms"] / ms_r,
+                   "col_kernel_over_bound": per_bucket[0]["col_kernel_ms"] / ms_c}
 
     for m in (0, M - 1):
         if not np.isfinite(ens.get("qh", m)).all():
@@ -355,7 +365,7 @@ def main():
                      "kernel": "fused step = k_row_flux + k_col_step (per bucket, summed)",
                      "algorithmic_bytes_per_launch": bytes_step,
                      "kernel_ms_per_step": step_ms, "buckets": per_bucket,
-                     "copy_kernel": copy, "autotune": tune},
+                     "copy_kernel": copy, "access_pattern_bound": pattern, "autotune": tune},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -166,6 +166,15 @@ int qgk_device_name(char* buf, int len);
 /* Measurement aid: average time of a plain device-to-device copy kernel
  * (nbytes read + nbytes written) over iters launches. */
 int qgk_copy_probe(size_t nbytes, int iters, double* ms_per_copy);
+/* Measurement aid: memory-only stand-ins for the fused step's two kernels at
+ * the shape of a bucket (members x nz planes of ny rows with row stride ldk
+ * complex128): ms_rows times three fields read and two written as sequential
+ * streams (the row kernel's traffic), ms_cols five fields read and five
+ * written by (8-column tile, member) blocks over every row and layer (the
+ * column kernel's traffic and access pattern).  Average ms over iters
+ * launches; allocates and frees 9 fields. */
+int qgk_pattern_probe(int ny, int ldk, int members, int nz, int iters, double* ms_rows,
+                      double* ms_cols);
 
 /* Multi-bucket stepping for uneven ensembles: nsteps fused steps of every
  * bucket, all launched on the first bucket's stream (one graph when

@@ -424,6 +424,18 @@ struct FGeo {
   static constexpr int RP = RPraw < 1 ? 1 : RPraw;
 };
 
+// Streaming accesses.  The fused step touches every spectral field once per
+// step, so its loads and stores carry the non-temporal hint: the L2 is left to
+// the small tables every block re-reads (filter, kappa^2, wavenumbers,
+// twiddles) instead of being flushed through by the fields.
+__device__ __forceinline__ cd ld_stream(const cd* p) {
+  return {__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y)};
+}
+__device__ __forceinline__ void st_stream(cd* p, cd v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
+
 template <int NX, int T, int RP>
 __device__ __forceinline__ void split_store_t(const cd* lds, int t, int rp, bool ok, cd* o0,
                                               cd* o1) {
@@ -433,11 +445,11 @@ __device__ __forceinline__ void split_store_t(const cd* lds, int t, int rp, bool
     if (k <= NX / 2) {
       cd A = lds[qgk::RowSwizzled<NX>::at(k, rp)];
       cd B = lds[qgk::RowSwizzled<NX>::at((NX - k) & (NX - 1), rp)];
-      o0[k] = {0.5 * (A.x + B.x), 0.5 * (A.y - B.y)};
-      o1[k] = {0.5 * (A.y + B.y), -0.5 * (A.x - B.x)};
+      st_stream(o0 + k, {0.5 * (A.x + B.x), 0.5 * (A.y - B.y)});
+      st_stream(o1 + k, {0.5 * (A.y + B.y), -0.5 * (A.x - B.x)});
     } else {  // padding: zeros, so the row is written as whole 128-byte lines
-      o0[k] = cd{0.0, 0.0};
-      o1[k] = cd{0.0, 0.0};
+      st_stream(o0 + k, cd{0.0, 0.0});
+      st_stream(o1 + k, cd{0.0, 0.0});
     }
   }
 }
@@ -472,8 +484,8 @@ __device__ __forceinline__ void row_pair_extend(const cd (&h)[2][E / 2],
         b.y = 0.0;
       }
     } else if (p == HALF) {
-      a = row[HALF];
-      b = row[NK + HALF];
+      a = ld_stream(row + HALF);
+      b = ld_stream(row + NK + HALF);
       a.y = 0.0;
       b.y = 0.0;
     } else {
@@ -521,9 +533,9 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP)) void k_row_flux
     const int p = t + s * G::T;
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
-      hq[r][s] = Qm[row0 + r * NK + p];
-      hu[r][s] = Um[row0 + r * NK + p];
-      hv[r][s] = Vm[row0 + r * NK + p];
+      hq[r][s] = ld_stream(Qm + row0 + r * NK + p);
+      hu[r][s] = ld_stream(Um + row0 + r * NK + p);
+      hv[r][s] = ld_stream(Vm + row0 + r * NK + p);
     }
   }
   cd zq[G::E], zu[G::E], zv[G::E];
@@ -582,7 +594,7 @@ __device__ __forceinline__ void col_prologue_layer(const cd* __restrict__ qh,
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
   {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
-    for (int s = 0; s < G::E; ++s) Qm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+    for (int s = 0; s < G::E; ++s) st_stream(Qm + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
   }
   // uh = -(i ll) ph
 #pragma unroll
@@ -593,7 +605,7 @@ __device__ __forceinline__ void col_prologue_layer(const cd* __restrict__ qh,
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
   {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
-    for (int s = 0; s < G::E; ++s) Um[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+    for (int s = 0; s < G::E; ++s) st_stream(Um + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
   }
   // vh = (i kk) ph
   const double kk = P.kk[kc];
@@ -602,7 +614,7 @@ __device__ __forceinline__ void col_prologue_layer(const cd* __restrict__ qh,
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
   {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
-    for (int s = 0; s < G::E; ++s) Vm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+    for (int s = 0; s < G::E; ++s) st_stream(Vm + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
   }
 }
 
@@ -655,8 +667,8 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
 #pragma unroll
     for (int s = 0; s < G::E; ++s) {
       const size_t pt = (size_t)(t + s * G::T) * ldk + kc;
-      xa[s] = ok ? UQm[zoff + pt] : cd{0.0, 0.0};
-      xb[s] = ok ? VQm[zoff + pt] : cd{0.0, 0.0};
+      xa[s] = ok ? ld_stream(UQm + zoff + pt) : cd{0.0, 0.0};
+      xb[s] = ok ? ld_stream(VQm + zoff + pt) : cd{0.0, 0.0};
     }
     qgk::fft_block<NY, G::E, -1, G::TK>(xa, lds, t, c, tw);
     qgk::fft_block<NY, G::E, -1, G::TK>(xb, lds, t, c, tw);
@@ -679,7 +691,7 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
           d = {d.x + cr * ph.x, d.y + cr * ph.y};
         }
         const double f = P.filtr[pt];
-        const cd q = qh_old[o], pv = p[o], ppv = pp_to_newp[o];
+        const cd q = qh_old[o], pv = ld_stream(p + o), ppv = ld_stream(pp_to_newp + o);
         double rx = q.x + cf.dt1 * d.x;
         double ry = q.y + cf.dt1 * d.y;
         rx = rx + cf.dt2 * pv.x;
@@ -687,7 +699,7 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
         rx = rx + cf.dt3 * ppv.x;
         ry = ry + cf.dt3 * ppv.y;
         qh_new[o] = {f * rx, f * ry};
-        pp_to_newp[o] = d;
+        st_stream(pp_to_newp + o, d);
       }
     } else {
       // lanes past nk write zeros so every tile row is a full 128-byte line
@@ -695,7 +707,7 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
       for (int s = 0; s < G::E; ++s) {
         const size_t o = zoff + (size_t)(t + s * G::T) * ldk + k;
         qh_new[o] = cd{0.0, 0.0};
-        pp_to_newp[o] = cd{0.0, 0.0};
+        st_stream(pp_to_newp + o, cd{0.0, 0.0});
       }
     }
   }
@@ -773,7 +785,7 @@ __device__ __forceinline__ void col_prologue_layer_ph(const cd* __restrict__ qh,
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
   {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
-    for (int s = 0; s < G::E; ++s) Qm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+    for (int s = 0; s < G::E; ++s) st_stream(Qm + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
   }
 #pragma unroll
   for (int s = 0; s < G::E; ++s) {
@@ -783,7 +795,7 @@ __device__ __forceinline__ void col_prologue_layer_ph(const cd* __restrict__ qh,
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
   {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
-    for (int s = 0; s < G::E; ++s) Um[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+    for (int s = 0; s < G::E; ++s) st_stream(Um + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
   }
   const double kk = P.kk[kc];
 #pragma unroll
@@ -791,7 +803,7 @@ __device__ __forceinline__ void col_prologue_layer_ph(const cd* __restrict__ qh,
   qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
   {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
-    for (int s = 0; s < G::E; ++s) Vm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+    for (int s = 0; s < G::E; ++s) st_stream(Vm + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
   }
 }
 
@@ -843,8 +855,8 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
 #pragma unroll
     for (int s = 0; s < G::E; ++s) {
       const size_t pt = (size_t)(t + s * G::T) * ldk + kc;
-      xa[s] = ok ? UQm[zoff + pt] : cd{0.0, 0.0};
-      xb[s] = ok ? VQm[zoff + pt] : cd{0.0, 0.0};
+      xa[s] = ok ? ld_stream(UQm + zoff + pt) : cd{0.0, 0.0};
+      xb[s] = ok ? ld_stream(VQm + zoff + pt) : cd{0.0, 0.0};
     }
     qgk::fft_block<NY, G::E, -1, G::TK>(xa, lds, t, c, tw);
     qgk::fft_block<NY, G::E, -1, G::TK>(xb, lds, t, c, tw);
@@ -858,7 +870,7 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
         const size_t o = zoff + pt;
         const double ll = P.ll[l];
         const cd S = {-kk * xa[s].y + -ll * xb[s].y, kk * xa[s].x + ll * xb[s].x};
-        const cd ph = phm[o];
+        const cd ph = ld_stream(phm + o);
         cd d = {S.x + -kq * ph.y, S.y + kq * ph.x};
         d = {-d.x, -d.y};
         if (drag) {
@@ -866,15 +878,15 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
           d = {d.x + cr * ph.x, d.y + cr * ph.y};
         }
         const double f = P.filtr[pt];
-        const cd q = qh_old[o], pv = p[o], ppv = pp_to_newp[o];
+        const cd q = ld_stream(qh_old + o), pv = ld_stream(p + o), ppv = ld_stream(pp_to_newp + o);
         double rx = q.x + cf.dt1 * d.x;
         double ry = q.y + cf.dt1 * d.y;
         rx = rx + cf.dt2 * pv.x;
         ry = ry + cf.dt2 * pv.y;
         rx = rx + cf.dt3 * ppv.x;
         ry = ry + cf.dt3 * ppv.y;
-        qh_new[o] = {f * rx, f * ry};
-        pp_to_newp[o] = d;
+        qh_new[o] = {f * rx, f * ry};  // read back by the inversion below
+        st_stream(pp_to_newp + o, d);
       }
     } else {
       // lanes past nk write zeros so every tile row is a full 128-byte line
@@ -882,7 +894,7 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
       for (int s = 0; s < G::E; ++s) {
         const size_t o = zoff + (size_t)(t + s * G::T) * ldk + k;
         qh_new[o] = cd{0.0, 0.0};
-        pp_to_newp[o] = cd{0.0, 0.0};
+        st_stream(pp_to_newp + o, cd{0.0, 0.0});
       }
     }
   }
@@ -956,7 +968,7 @@ __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
     qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
     {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
-      for (int s = 0; s < G::E; ++s) Qm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+      for (int s = 0; s < G::E; ++s) st_stream(Qm + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
     }
 #pragma unroll
     for (int s = 0; s < G::E; ++s) {
@@ -966,14 +978,14 @@ __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
     qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
     {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
-      for (int s = 0; s < G::E; ++s) Um[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+      for (int s = 0; s < G::E; ++s) st_stream(Um + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
     }
 #pragma unroll
     for (int s = 0; s < G::E; ++s) x[s] = {-kk * ph[s].y, kk * ph[s].x};
     qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
     {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
-      for (int s = 0; s < G::E; ++s) Vm[zoff + (size_t)(t + s * G::T) * ldk + k] = x[s];
+      for (int s = 0; s < G::E; ++s) st_stream(Vm + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
     }
   }
 }
@@ -998,7 +1010,8 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_prol
   for (int z = 0; z < NZ; ++z)
 #pragma unroll
     for (int s = 0; s < G::E; ++s)
-      Q[z][s] = ok ? qh[mbase + z * plane + (size_t)(t + s * G::T) * ldk + kc] : cd{0.0, 0.0};
+      Q[z][s] = ok ? ld_stream(qh + mbase + z * plane + (size_t)(t + s * G::T) * ldk + kc)
+                   : cd{0.0, 0.0};
   col_prologue_regs<NY, NZ, G::E>(Q, Qm, Um, Vm, P, nk, ldk, t, c, k, ok, mbase, lds, tw);
 }
 
@@ -1028,7 +1041,7 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
   for (int z = 0; z < NZ; ++z)
 #pragma unroll
     for (int s = 0; s < G::E; ++s)
-      Q[z][s] = ok ? qh_old[mbase + z * plane + (size_t)(t + s * G::T) * ldk + kc]
+      Q[z][s] = ok ? ld_stream(qh_old + mbase + z * plane + (size_t)(t + s * G::T) * ldk + kc)
                    : cd{0.0, 0.0};
 
 #pragma unroll
@@ -1038,8 +1051,8 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
 #pragma unroll
     for (int s = 0; s < G::E; ++s) {
       const size_t pt = (size_t)(t + s * G::T) * ldk + kc;
-      xa[s] = ok ? UQm[zoff + pt] : cd{0.0, 0.0};
-      xb[s] = ok ? VQm[zoff + pt] : cd{0.0, 0.0};
+      xa[s] = ok ? ld_stream(UQm + zoff + pt) : cd{0.0, 0.0};
+      xb[s] = ok ? ld_stream(VQm + zoff + pt) : cd{0.0, 0.0};
     }
     qgk::fft_block<NY, G::E, -1, G::TK>(xa, lds, t, c, tw);
     qgk::fft_block<NY, G::E, -1, G::TK>(xb, lds, t, c, tw);
@@ -1069,7 +1082,7 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
           d = {d.x + cr * ph.x, d.y + cr * ph.y};
         }
         const double f = P.filtr[pt];
-        const cd q = Q[z][s], pv = p[o], ppv = pp_to_newp[o];
+        const cd q = Q[z][s], pv = ld_stream(p + o), ppv = ld_stream(pp_to_newp + o);
         double rx = q.x + cf.dt1 * d.x;
         double ry = q.y + cf.dt1 * d.y;
         rx = rx + cf.dt2 * pv.x;
@@ -1077,8 +1090,8 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
         rx = rx + cf.dt3 * ppv.x;
         ry = ry + cf.dt3 * ppv.y;
         const cd qn = {f * rx, f * ry};
-        qh_new[o] = qn;
-        pp_to_newp[o] = d;
+        st_stream(qh_new + o, qn);
+        st_stream(pp_to_newp + o, d);
         if constexpr (NZ == 2) {
           if (z == 0) Q0old[s] = q;
         }
@@ -1089,8 +1102,8 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
 #pragma unroll
       for (int s = 0; s < G::E; ++s) {
         const size_t o = zoff + (size_t)(t + s * G::T) * ldk + k;
-        qh_new[o] = cd{0.0, 0.0};
-        pp_to_newp[o] = cd{0.0, 0.0};
+        st_stream(qh_new + o, cd{0.0, 0.0});
+        st_stream(pp_to_newp + o, cd{0.0, 0.0});
       }
     }
   }
@@ -1713,22 +1726,72 @@ __global__ void k_param_out(const double* __restrict__ a, double* __restrict__ o
   }
 }
 
-// Plain streaming copy (16 B per lane), the calibration kernel for HBM traffic
-// counters and the measured copy bandwidth.
-// Four independent 16-byte loads per lane are issued before their stores, so
-// enough bytes are in flight to reach the HBM ceiling.
+// Plain streaming copy, one 16-byte element per lane over a grid that covers
+// the buffer: the calibration kernel for HBM traffic counters and the measured
+// copy bandwidth.  Workgroups are dispatched in address order, so the lines in
+// flight at any moment form one moving window; this reaches a higher rate on
+// HBM3E than a grid-stride loop, whose lanes touch pages across the whole
+// buffer at once.
 __global__ void k_stream_copy(const double2* __restrict__ in, double2* __restrict__ out,
                               size_t n) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  for (; i + 3 * stride < n; i += 4 * stride) {
-    const double2 a = in[i], b = in[i + stride], c = in[i + 2 * stride], d = in[i + 3 * stride];
-    out[i] = a;
-    out[i + stride] = b;
-    out[i + 2 * stride] = c;
-    out[i + 3 * stride] = d;
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
+// Memory-only stand-ins for the two fused kernels: the same fields, the same
+// access patterns and the same bytes, no transforms.  Their times bound what
+// the fused kernels can reach on this memory system.
+//
+// Row phase: three fields read and two written, each a sequential stream.
+__global__ void k_probe_rows(const cd* __restrict__ a, const cd* __restrict__ b,
+                             const cd* __restrict__ c, cd* __restrict__ d, cd* __restrict__ e,
+                             size_t n) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) {
+    const cd x = a[i], y = b[i], z = c[i];
+    d[i] = {x.x + y.x, x.y + y.y};
+    e[i] = {z.x + x.x, z.y + x.y};
   }
-  for (; i < n; i += stride) out[i] = in[i];
+}
+
+// Column phase: a block per (8-column tile, member), lanes (column c = tid % 8,
+// row t = tid / 8) over rows t + 32 s, every layer: UQm, VQm, qh_old, dqhdt_p
+// and dqhdt_pp read, qh_new, dqhdt (over dqhdt_pp), Qm, Um and Vm written.
+__global__ __launch_bounds__(256) void k_probe_cols(
+    const cd* __restrict__ uq, const cd* __restrict__ vq, const cd* __restrict__ qo,
+    const cd* __restrict__ p, cd* __restrict__ pp, cd* __restrict__ qn, cd* __restrict__ qm,
+    cd* __restrict__ um, cd* __restrict__ vm, int ny, int ldk, int nz) {
+  const int c = threadIdx.x % 8, t = threadIdx.x / 8;
+  const int k = blockIdx.x * 8 + c;
+  const size_t plane = (size_t)ny * ldk;
+  for (int z = 0; z < nz; ++z) {
+    const size_t base = ((size_t)blockIdx.y * nz + z) * plane + k;
+    for (int r0 = 0; r0 < ny; r0 += 256) {
+      cd v[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const int row = r0 + t + 32 * s;
+        v[s] = cd{0.0, 0.0};
+        if (row < ny) {
+          const size_t o = base + (size_t)row * ldk;
+          const cd a = uq[o], b = vq[o], q = qo[o], d = p[o], e = pp[o];
+          v[s] = {a.x + b.x + q.x + d.x + e.x, a.y + b.y + q.y + d.y + e.y};
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const int row = r0 + t + 32 * s;
+        if (row < ny) {
+          const size_t o = base + (size_t)row * ldk;
+          qn[o] = v[s];
+          pp[o] = v[s];
+          qm[o] = v[s];
+          um[o] = v[s];
+          vm[o] = v[s];
+        }
+      }
+    }
+  }
 }
 
 // ==========================================================================
@@ -2749,21 +2812,24 @@ int qgk_copy_probe(size_t nbytes, int iters, double* ms_per_copy) {
     rc = fail(QGK_E_HIP, "copy probe set-up failed");
   } else {
     hipMemsetAsync(a, 0, n * 16, st);
-    // best of a few grid sizes (workgroups per CU of a 256-CU device: 4..32)
-    double best = 1e30;
-    for (int blocks : {1024, 2048, 4096, 8192}) {
-      hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, st, a, c, n);
+    double best = 0.0;
+    const size_t nblk = (n + 255) / 256;
+    if (nblk > 0x7fffffffu) {
+      rc = fail(QGK_E_ARG, "copy probe size too large");
+    } else {
+      const dim3 grid((unsigned)nblk);
+      hipLaunchKernelGGL(k_stream_copy, grid, dim3(256), 0, st, a, c, n);
       hipEventRecord(e0, st);
       for (int i = 0; i < iters; ++i)
-        hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, st, a, c, n);
+        hipLaunchKernelGGL(k_stream_copy, grid, dim3(256), 0, st, a, c, n);
       hipEventRecord(e1, st);
       float ms = 0;
       if (hipEventSynchronize(e1) != hipSuccess ||
           hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
         rc = fail(QGK_E_HIP, "copy probe timing failed");
-        break;
+      } else {
+        best = ms / iters;
       }
-      if (ms / iters < best) best = ms / iters;
     }
     *ms_per_copy = best;
   }
@@ -2772,6 +2838,53 @@ int qgk_copy_probe(size_t nbytes, int iters, double* ms_per_copy) {
   if (st) hipStreamDestroy(st);
   if (a) hipFree(a);
   if (c) hipFree(c);
+  return rc;
+}
+
+int qgk_pattern_probe(int ny, int ldk, int members, int nz, int iters, double* ms_rows,
+                      double* ms_cols) {
+  if (ny < 32 || ny % 32 || ldk < 8 || ldk % 8 || members < 1 || nz < 1 || iters < 1 ||
+      !ms_rows || !ms_cols)
+    return fail(QGK_E_ARG, "pattern probe: bad arguments");
+  const size_t n = (size_t)members * nz * ny * ldk;
+  if ((n + 255) / 256 > 0x7fffffffu || members > 65535)
+    return fail(QGK_E_ARG, "pattern probe: size too large");
+  cd* f[9] = {};
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = 0;
+  bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+  for (int i = 0; ok && i < 9; ++i)
+    ok = hipMalloc((void**)&f[i], n * sizeof(cd)) == hipSuccess &&
+         hipMemsetAsync(f[i], 0, n * sizeof(cd), st) == hipSuccess;
+  if (!ok) rc = fail(QGK_E_HIP, "pattern probe set-up failed");
+  auto timed = [&](auto launch, double* out) {
+    if (rc) return;
+    launch();
+    hipEventRecord(e0, st);
+    for (int i = 0; i < iters; ++i) launch();
+    hipEventRecord(e1, st);
+    float ms = 0;
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = fail(QGK_E_HIP, "pattern probe timing failed");
+    else
+      *out = ms / iters;
+  };
+  timed([&] {
+    hipLaunchKernelGGL(k_probe_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, f[0],
+                       f[1], f[2], f[3], f[4], n);
+  }, ms_rows);
+  timed([&] {
+    hipLaunchKernelGGL(k_probe_cols, dim3(ldk / 8, members), dim3(256), 0, st, f[0], f[1], f[2],
+                       f[3], f[4], f[5], f[6], f[7], f[8], ny, ldk, nz);
+  }, ms_cols);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (st) hipStreamDestroy(st);
+  for (int i = 0; i < 9; ++i)
+    if (f[i]) hipFree(f[i]);
   return rc;
 }
 
