@@ -97,6 +97,19 @@ int qgk_add_uv_param(qgk_bucket* b, const double* du_host, const double* dv_host
 /* AB update + filter + history rotation + q = irfft2(qh) (kernel.pyx:463-526).
  * dt1..dt3 are the Adams-Bashforth weights for this step (kernel.pyx:482-500). */
 int qgk_timestep(qgk_bucket* b, double dt1, double dt2, double dt3);
+/* Step graphs (no counterpart in the reference, whose steps are host loops):
+ * between qgk_capture_begin and qgk_capture_end the phase entry points above
+ * (qgk_invert, qgk_advect, qgk_friction, qgk_apply_*_param, qgk_timestep) and
+ * the built-in closures (qgk_param_*, device targets only) are recorded into
+ * a HIP graph instead of run; qgk_graph_replay(b, n) then runs the recorded
+ * steps n times with one synchronization.  A graph must hold a multiple of 2
+ * steps (the period of the tendency history's buffer swap) and is refused
+ * after a constants
+ * change; the closures' dx/Hi must not change while capturing.
+ * qgk_capture_end(b, 0) discards the capture. */
+int qgk_capture_begin(qgk_bucket* b);
+int qgk_capture_end(qgk_bucket* b, int keep);
+int qgk_graph_replay(qgk_bucket* b, int n);
 /* The q setter (rfft2 of q into qh, kernel.pyx:576-582) and the qh setter's
  * inverse transform (kernel.pyx:583-591). */
 int qgk_q_to_qh(qgk_bucket* b);

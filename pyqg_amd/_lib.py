@@ -29,7 +29,8 @@ EXPORTS = ["qgk_create", "qgk_destroy", "qgk_last_error", "qgk_set_const", "qgk_
            "qgk_advect", "qgk_friction", "qgk_add_q_param", "qgk_add_uv_param",
            "qgk_timestep", "qgk_q_to_qh", "qgk_qh_to_q", "qgk_fft", "qgk_ifft", "qgk_step",
            "qgk_timing_enable", "qgk_timing", "qgk_sync", "qgk_step_buckets",
-           "qgk_copy_probe", "qgk_pattern_probe", "qgk_set_variant", "qgk_get_variant", "qgk_autotune",
+           "qgk_copy_probe", "qgk_pattern_probe", "qgk_capture_begin", "qgk_capture_end",
+           "qgk_graph_replay", "qgk_set_variant", "qgk_get_variant", "qgk_autotune",
            "qgk_device_name", "qgk_put_device", "qgk_get_device", "qgk_coarsen",
            "qgk_subgrid_forcing", "qgk_pack_inputs", "qgk_add_forcing_device",
            "qgk_diag_set_params", "qgk_diag_accumulate", "qgk_diag_read", "qgk_diag_reset",
@@ -106,6 +107,9 @@ def _load():
         "qgk_step_buckets": (I, [ctypes.POINTER(P), I, I, P, I]),
         "qgk_copy_probe": (I, [ctypes.c_size_t, I, DP]),
         "qgk_pattern_probe": (I, [I, I, I, I, I, DP, DP]),
+        "qgk_capture_begin": (I, [P]),
+        "qgk_capture_end": (I, [P, I]),
+        "qgk_graph_replay": (I, [P, I]),
         "qgk_set_variant": (I, [P, I, I]),
         "qgk_get_variant": (I, [P, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "qgk_autotune": (I, [P, I, ctypes.POINTER(I), ctypes.POINTER(I), DP, DP]),

@@ -18,6 +18,7 @@
 //    HBM; ph, u, v, uq, vq and the real q never leave the chip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -507,8 +508,13 @@ __device__ __forceinline__ void row_pair_extend(const cd (&h)[2][E / 2],
 // Each complex transform carries the two rows of the pair (row j in the real
 // part, row j+1 in the imaginary part), so the six input rows are loaded at
 // once and the five transforms run back to back.
+// Waves per SIMD the row kernel is compiled for: radix 8 at 3 (a few spilled
+// registers) rather than the 2 its natural 172 VGPRs allow.
+template <int E>
+constexpr int row_flux_waves() { return E == 8 ? 3 : 1; }
+
 template <int NX, int E>
-__global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP)) void k_row_flux(
+__global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP), row_flux_waves<E>()) void k_row_flux(
     const cd* __restrict__ Qm, const cd* __restrict__ Um, const cd* __restrict__ Vm,
     cd* __restrict__ UQm, cd* __restrict__ VQm, const MemberParams* __restrict__ params,
     int nz, int ny, double scale, const cd* __restrict__ tw) {
@@ -1873,6 +1879,12 @@ struct qgk_bucket {
   double* dsum = nullptr;      // M*nz + M plane sums; then nz planes of member sums
   // built-in closures: per-member grid spacing and layer thicknesses, plane sums
   double *pdx = nullptr, *pHi = nullptr, *psum = nullptr;
+  std::vector<double> pdx_h, pHi_h;  // last values uploaded to pdx, pHi
+  // user step graph (qgk_capture_begin / qgk_capture_end / qgk_graph_replay)
+  bool capturing = false;
+  hipGraphExec_t ugexec = nullptr;
+  int ug_phase = -1;  // history rotation (iP) at capture
+  long ug_gen = -1;   // gen at capture
   hipStream_t lst = nullptr;  // stream override for fused-step launches (graph capture)
   long gen = 0;               // bumped whenever launch arguments change (graph cache key)
   // hipGraph of two fused steps of a set of buckets, replayed while the AB
@@ -2359,6 +2371,7 @@ int qgk_destroy(qgk_bucket* b) {
   for (auto& g : b->gexec)
     if (g) hipGraphExecDestroy(g);
   if (b->graph.exec) hipGraphExecDestroy(b->graph.exec);
+  if (b->ugexec) hipGraphExecDestroy(b->ugexec);
   for (auto& p : b->pending) {
     hipEventDestroy(p.a);
     hipEventDestroy(p.b);
@@ -2487,6 +2500,12 @@ int qgk_get(qgk_bucket* b, int field, void* host) {
   return 0;
 }
 
+// End of an entry point that only enqueues device work: wait for it, unless
+// the stream is being captured into a graph (then nothing runs yet).
+hipError_t stream_done(qgk_bucket* b) {
+  return b->capturing ? hipSuccess : hipStreamSynchronize(b->stream);
+}
+
 int qgk_invert(qgk_bucket* b) {
   if (!b) return fail(QGK_E_ARG, "NULL handle");
   int rc = need_full(b, "_invert");
@@ -2499,7 +2518,7 @@ int qgk_invert(qgk_bucket* b) {
   const int np = b->M * b->nz;
   if ((rc = c2r2d(b, b->field_c[QGK_UH], b->field_r[QGK_U], np))) return rc;
   if ((rc = c2r2d(b, b->field_c[QGK_VH], b->field_r[QGK_V], np))) return rc;
-  HIPC(hipStreamSynchronize(b->stream));
+  HIPC(stream_done(b));
   return 0;
 }
 
@@ -2520,7 +2539,7 @@ int qgk_advect(qgk_bucket* b) {
                      b->field_c[QGK_UQH], b->field_c[QGK_VQH], b->field_c[QGK_PH], D(b), b->dp,
                      b->nz, b->nl, b->ldk, b->M);
   HIPC(hipGetLastError());
-  HIPC(hipStreamSynchronize(b->stream));
+  HIPC(stream_done(b));
   return 0;
 }
 
@@ -2532,7 +2551,7 @@ int qgk_friction(qgk_bucket* b) {
   hipLaunchKernelGGL(k_friction, dim3(grid_1d(pts)), dim3(256), 0, b->stream,
                      b->field_c[QGK_PH], D(b), b->dp, b->nz, b->nl, b->ldk, b->M);
   HIPC(hipGetLastError());
-  HIPC(hipStreamSynchronize(b->stream));
+  HIPC(stream_done(b));
   return 0;
 }
 
@@ -2546,7 +2565,7 @@ int qgk_apply_q_param(qgk_bucket* b) {
   hipLaunchKernelGGL(k_add, dim3(grid_1d(cpts)), dim3(256), 0, b->stream, D(b),
                      b->field_c[QGK_DQH], cpts);
   HIPC(hipGetLastError());
-  HIPC(hipStreamSynchronize(b->stream));
+  HIPC(stream_done(b));
   return 0;
 }
 
@@ -2572,7 +2591,7 @@ int qgk_apply_uv_param(qgk_bucket* b) {
   hipLaunchKernelGGL(k_add_curl, dim3(grid_1d(cpts)), dim3(256), 0, b->stream, D(b),
                      b->field_c[QGK_DUH], b->field_c[QGK_DVH], b->dp, b->nz, b->nl, b->ldk, b->M);
   HIPC(hipGetLastError());
-  HIPC(hipStreamSynchronize(b->stream));
+  HIPC(stream_done(b));
   return 0;
 }
 
@@ -2602,6 +2621,64 @@ int qgk_timestep(qgk_bucket* b, double dt1, double dt2, double dt3) {
   b->primed = false;
   b->q_stale = false;
   if ((rc = c2r2d(b, b->field_c[QGK_QH], b->field_r[QGK_Q], np))) return rc;
+  HIPC(stream_done(b));
+  return 0;
+}
+
+int qgk_capture_begin(qgk_bucket* b) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (b->capturing) return fail(QGK_E_STATE, "already capturing");
+  int rc = need_full(b, "graph capture");
+  if (rc || (rc = upload_params(b)) || (rc = refresh_q(b))) return rc;
+  HIPC(hipStreamSynchronize(b->stream));
+  HIPC(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  b->capturing = true;
+  b->ug_phase = b->iP;
+  b->ug_gen = b->gen;
+  return 0;
+}
+
+int qgk_capture_end(qgk_bucket* b, int keep) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (!b->capturing) return fail(QGK_E_STATE, "not capturing");
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->stream, &g);
+  b->capturing = false;
+  if (b->ugexec) {
+    hipGraphExecDestroy(b->ugexec);
+    b->ugexec = nullptr;
+  }
+  int rc = 0;
+  if (e != hipSuccess || !g) {
+    rc = fail(QGK_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+  } else if (keep) {
+    if (b->iP != b->ug_phase)
+      rc = fail(QGK_E_STATE, "a step graph must cover whole cycles of the history rotation "
+                             "(a multiple of 2 steps)");
+    else if (b->gen != b->ug_gen)
+      rc = fail(QGK_E_STATE, "constants changed during graph capture");
+    else if (hipGraphInstantiate(&b->ugexec, g, nullptr, nullptr, 0) != hipSuccess)
+      rc = fail(QGK_E_HIP, "graph instantiation failed");
+  }
+  if (g) hipGraphDestroy(g);
+  if (rc) b->ugexec = nullptr;
+  return rc;
+}
+
+int qgk_graph_replay(qgk_bucket* b, int n) {
+  if (!b || n < 0) return fail(QGK_E_ARG, "bad arguments");
+  if (b->capturing) return fail(QGK_E_STATE, "capture in progress");
+  if (!b->ugexec) return fail(QGK_E_STATE, "no captured graph");
+  if (b->iP != b->ug_phase || b->gen != b->ug_gen)
+    return fail(QGK_E_STATE, "the captured graph is stale (history phase or constants changed)");
+  int rc = upload_params(b);
+  if (rc) return rc;
+  for (int i = 0; i < n; ++i) HIPC(hipGraphLaunch(b->ugexec, b->stream));
+  // the graph ends with a time step, as qgk_timestep leaves the bucket
+  if (n > 0) {
+    b->primed = false;
+    b->q_stale = false;
+  }
   HIPC(hipStreamSynchronize(b->stream));
   return 0;
 }
@@ -3459,10 +3536,18 @@ int param_setup(qgk_bucket* b, const double* dx, const double* Hi, bool need_uv)
                   (rc = dev_alloc(b, (void**)&b->pHi, np * sizeof(double))) ||
                   (rc = dev_alloc(b, (void**)&b->psum, 2 * np * sizeof(double)))))
     return rc;
-  if (dx)
-    HIPC(hipMemcpyAsync(b->pdx, dx, b->M * sizeof(double), hipMemcpyHostToDevice, b->stream));
-  if (Hi)
-    HIPC(hipMemcpyAsync(b->pHi, Hi, np * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  // upload dx and Hi only when they change; under graph capture an upload
+  // would become a copy from a host buffer that is gone at replay
+  auto upload = [&](const double* host, std::vector<double>& cache, double* dev,
+                    size_t n) -> int {
+    if (!host || (cache.size() == n && std::equal(host, host + n, cache.begin()))) return 0;
+    if (b->capturing) return fail(QGK_E_STATE, "closure constants changed during graph capture");
+    cache.assign(host, host + n);
+    HIPC(hipMemcpyAsync(dev, cache.data(), n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    return 0;
+  };
+  if ((rc = upload(dx, b->pdx_h, b->pdx, b->M)) || (rc = upload(Hi, b->pHi_h, b->pHi, np)))
+    return rc;
   return 0;
 }
 
@@ -3550,7 +3635,7 @@ int qgk_param_smagorinsky(qgk_bucket* b, const double* dx, double constant, doub
       (rc = param_emit(b, R0, b->field_r[QGK_DU], du_host, 2.0, weight, accumulate)) ||
       (rc = param_emit(b, R1, b->field_r[QGK_DV], dv_host, 2.0, weight, accumulate)))
     return rc;
-  HIPC(hipStreamSynchronize(b->stream));
+  HIPC(stream_done(b));
   return 0;
 }
 
@@ -3602,7 +3687,7 @@ int qgk_param_zb2020(qgk_bucket* b, double constant, double weight, int accumula
       (rc = param_emit(b, R0, b->field_r[QGK_DU], du_host, constant, weight, accumulate)) ||
       (rc = param_emit(b, R1, b->field_r[QGK_DV], dv_host, constant, weight, accumulate)))
     return rc;
-  HIPC(hipStreamSynchronize(b->stream));
+  HIPC(stream_done(b));
   return 0;
 }
 
@@ -3638,7 +3723,7 @@ int qgk_param_backscatter(qgk_bucket* b, const double* dx, const double* Hi,
                      (int)b->nplane_r);
   HIPC(hipGetLastError());
   if ((rc = param_emit(b, R3, b->field_r[QGK_DQ], dq_host, 1.0, weight, accumulate))) return rc;
-  HIPC(hipStreamSynchronize(b->stream));
+  HIPC(stream_done(b));
   return 0;
 }
 

@@ -15,6 +15,7 @@ import warnings
 import numpy as np
 from numpy import pi
 
+from . import _lib
 from . import constants as C
 from . import diagnostics as Dg
 from .diagnostics import DiagnosticsMixin
@@ -112,8 +113,11 @@ class Model(DiagnosticsMixin, PseudoSpectralKernel):
         diagnostic accumulations run in the fused two-kernel form; the step
         that a status report observes runs phase by phase, so KE / CFL see the
         same fields as the reference's, and a step that accumulates
-        diagnostics runs as _step_forward."""
-        if self.q_parameterization is not None or self.uv_parameterization is not None:
+        diagnostics runs as _step_forward.  With built-in closures (evaluated
+        on the device) those stretches replay a HIP graph of the phase-by-phase
+        step; other parameterizations step one _step_forward at a time."""
+        if ((self.q_parameterization is not None or self.uv_parameterization is not None)
+                and not self._graphable()):
             while self.t < self.tmax:
                 self._step_forward()
             return
@@ -125,8 +129,16 @@ class Model(DiagnosticsMixin, PseudoSpectralKernel):
         self._advance(n)
 
     def _advance(self, n):
-        """n unparameterized steps: fused stretches ending at status reports and
-        before diagnostic steps; those run phase by phase."""
+        """n steps: stretches ending at status reports and before diagnostic
+        steps run fused (no parameterization) or as graph replays (built-in
+        closures); the diagnostic steps run as _step_forward."""
+        stretch = self._fused_steps
+        if self.q_parameterization is not None or self.uv_parameterization is not None:
+            if not self._graphable():
+                for _ in range(n):
+                    self._step_forward()
+                return
+            stretch = self._graph_steps
         while n > 0:
             chunk = n
             if self.log_level:
@@ -137,9 +149,80 @@ class Model(DiagnosticsMixin, PseudoSpectralKernel):
                 self._step_forward()
                 n -= 1
                 continue
-            self._fused_steps(chunk)
+            stretch(chunk)
             self._print_status()
             n -= chunk
+
+    def _graphable(self):
+        """Every parameterization present is a built-in closure evaluated on the
+        device, and the model adds no external forcing of its own."""
+        for p in (self.q_parameterization, self.uv_parameterization):
+            if p is not None and self._device_parameterization(p) is None:
+                return False
+        return type(self)._do_external_forcing is Model._do_external_forcing
+
+    def _graph_steps(self, nsteps):
+        """nsteps parameterized steps with no diagnostic or status step among
+        them.  Once the Adams-Bashforth weights are constant the steps replay a
+        HIP graph of two phase-by-phase steps (invert, advection, friction,
+        closures written into du/dv or dq on the device, time step; two steps
+        are one period of the tendency history's buffer swap), so every kernel and
+        every field is the one _step_forward produces; the rest run as
+        _step_forward.  The graph is kept while the closures, dt and the
+        kernel's constants stay the same."""
+        nsteps = int(nsteps)
+        first = True
+        while nsteps > 0 and (first or self._ablevel < 2):
+            self._step_forward()      # allocations, uploads and AB start-up
+            nsteps -= 1
+            first = False
+        if nsteps >= 2:
+            from .parameterizations import _device_target
+            b, dx, Hi = _device_target(self)
+            qp, uvp = self.q_parameterization, self.uv_parameterization
+            w, self._ablevel = C.ab_weights(self._ablevel, self._dt)
+            key = (id(qp), repr(qp), id(uvp), repr(uvp), tuple(w), dx.tobytes(), Hi.tobytes())
+            self._flush()
+            ok = getattr(self, "_graph_key", None) == key
+            if ok:
+                # one ordinary step brings the history buffers back to the
+                # phase the graph was captured at
+                if nsteps >= 3 and _lib.lib.qgk_graph_replay(b.h, 0) != 0:
+                    self._step_forward()
+                    nsteps -= 1
+                ok = _lib.lib.qgk_graph_replay(b.h, 0) == 0
+            if not ok:
+                self._graph_key = None
+                self._capture_steps(b, dx, Hi, qp, uvp, w)
+                self._graph_key = key
+        reps = nsteps // 2
+        if reps > 0:
+            _lib.check(_lib.lib.qgk_graph_replay(b.h, reps))
+            self._invalidate_all()
+            for _ in range(2 * reps):
+                self._t += self._dt
+            self._tc += 2 * reps
+        for _ in range(nsteps - 2 * reps):
+            self._step_forward()
+
+    def _capture_steps(self, b, dx, Hi, qp, uvp, w):
+        L = _lib.lib
+        _lib.check(L.qgk_capture_begin(b.h))
+        try:
+            for _ in range(2):
+                for fn in (L.qgk_invert, L.qgk_advect, L.qgk_friction):
+                    _lib.check(fn(b.h))
+                if uvp is not None:
+                    uvp._device_bucket(b, dx, Hi, 1.0, False)
+                    _lib.check(L.qgk_apply_uv_param(b.h))
+                if qp is not None:
+                    qp._device_bucket(b, dx, Hi, 1.0, False)
+                    _lib.check(L.qgk_apply_q_param(b.h))
+                _lib.check(L.qgk_timestep(b.h, *w))
+        except Exception:
+            L.qgk_capture_end(b.h, 0)
+            raise
+        _lib.check(L.qgk_capture_end(b.h, 1))
 
     def _steps_before_diagnostics(self, limit):
         """Steps (at most limit) that start before the next diagnostic step."""
