@@ -290,6 +290,7 @@ def bench_dropin(args):
                      "speedup_vs_reference": ref_wall / (t2 - t1)})
         print(f"{name}: {runs[-1]['run_s']:.2f} s ({runs[-1]['model_days_per_s']:.0f} "
               f"model-days/s), reference {ref_wall} s", file=sys.stderr, flush=True)
+    config2 = bench_config2(args)
     heavy = runs[1]
     print(json.dumps({
         "metric": "model-days/s of the reference's example runs through the drop-in QGModel",
@@ -298,7 +299,46 @@ def bench_dropin(args):
         "dtype": "f64", "data": "the reference's default initial condition",
         "config": {"workload": "docs/examples/parameterizations.ipynb runs: dt=3600 s, "
                                "tmax=5 years, tavestart=2.5 years, daily diagnostics",
-                   "runs": runs}}))
+                   "runs": runs, "config2": config2}}))
+
+
+def bench_config2(args, nsteps=2000, cpu_steps=20):
+    """One two-layer member at nx=512, dt=1800 (seeded 1e-7 N(0,1) PV, diagnostics
+    off): run() (fused stretches), a plain _step_forward loop (phase by phase), and
+    the oracle on one host core for a bounded sample of the same run."""
+    import numpy as np
+    import pyqg_amd
+    phys = dict(PHYS)
+    q0 = 1e-7 * np.random.default_rng(0).standard_normal((2, 512, 512))
+    out = {"nx": 512, "dt": 1800.0, "steps": nsteps}
+    m = pyqg_amd.QGModel(nx=512, dt=1800.0, tmax=nsteps * 1800.0, log_level=0, **phys)
+    m.q = q0
+    t0 = time.perf_counter()
+    m.run()
+    np.asarray(m.q)
+    out["run_model_days_per_s"] = nsteps * 1800.0 / 86400. / (time.perf_counter() - t0)
+    m = pyqg_amd.QGModel(nx=512, dt=1800.0, log_level=0, **phys)
+    m.q = q0
+    n2 = nsteps // 4
+    t0 = time.perf_counter()
+    for _ in range(n2):
+        m._step_forward()
+    np.asarray(m.q)
+    out["step_forward_model_days_per_s"] = n2 * 1800.0 / 86400. / (time.perf_counter() - t0)
+    if not args.no_cpu:
+        from oracle.qg_oracle import OracleQG
+        o = OracleQG(nx=512, dt=1800.0, **phys)
+        o.q = q0
+        o._step_forward()
+        t0 = time.perf_counter()
+        for _ in range(cpu_steps):
+            o._step_forward()
+        w = time.perf_counter() - t0
+        out["cpu_oracle"] = {"model_days_per_s": cpu_steps * 1800.0 / 86400. / w,
+                             "cores": 1, "kind": "port",
+                             "sample": f"{cpu_steps} steps of the same run, one process"}
+    print(f"config 2 (nx=512): {out}", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
