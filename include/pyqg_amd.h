@@ -14,7 +14,7 @@
  *   real field     double [M][nz][ny][nx]
  *   spectral field double complex (interleaved re,im) [M][nz][nl][nk],
  *                  nl = ny, nk = nx/2 + 1
- * nx and ny must be powers of two in [8, 1024].
+ * nx and ny are 2^a in [8, 1024] or 3 * 2^a in [24, 768].
  *
  * Every function returns 0 on success or a negative QGK_E* code; the message
  * of the last failure on the calling thread is qgk_last_error().  Calls are

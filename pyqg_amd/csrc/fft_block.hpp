@@ -2,13 +2,15 @@
 //
 // One transform of length N is carried by T = N/E threads; thread t holds E
 // complex values in registers, x[s] = element (t + s*T).  The transform is a
-// sequence of Stockham auto-sort passes of radix R <= E (largest radix first),
+// sequence of Stockham auto-sort passes of radix R | E (largest radix first),
 // each pass a register-resident radix-R DFT; consecutive passes exchange data
 // through LDS.  Several independent transforms ("lanes") share one LDS buffer:
 // the columns of a tile interleaved with stride LANES, so a wave touches
-// consecutive 16-byte words; the rows of a row group one swizzled region each.  On exit x[s] holds output element
-// (t + s*T) in natural order, so a forward transform can feed an inverse one
-// (or the reverse) without any reordering.
+// consecutive 16-byte words; the rows of a row group one swizzled region each.
+// On exit x[s] holds output element (t + s*T) in natural order, so a forward
+// transform can feed an inverse one (or the reverse) without any reordering.
+// N is 2^a or 3 * 2^a; for the latter E is 6 or 12 and the first pass is the
+// radix-3 one (radix 6 or 12).
 //
 // Twiddles come from a per-length table tw[m] = exp(-2*pi*i*m/N), built on the
 // host in extended precision (correctly rounded to FP64).
@@ -41,6 +43,14 @@ __device__ __forceinline__ cd twiddle_const(cd v) {
     return {-v.x, -v.y};
   } else if constexpr (4 * m == 3 * R) {  // -SIGN * i
     return SIGN < 0 ? cd{-v.y, v.x} : cd{v.y, -v.x};
+  } else if constexpr ((12 * m) % R == 0) {  // multiples of 30 degrees (R = 3, 6, 12, 24)
+    constexpr int k = 12 * m / R;            // 1..11, not 3, 6, 9 (handled above)
+    constexpr double h = 0.86602540378443864676;  // cos(30 deg) = sqrt(3)/2
+    constexpr double C12[12] = {1.0, h, 0.5, 0.0, -0.5, -h, -1.0, -h, -0.5, 0.0, 0.5, h};
+    constexpr double S12[12] = {0.0, 0.5, h, 1.0, h, 0.5, 0.0, -0.5, -h, -1.0, -h, -0.5};
+    constexpr double c = C12[k];
+    constexpr double s = SIGN * S12[k];
+    return {v.x * c - v.y * s, v.x * s + v.y * c};
   } else if constexpr (8 * m == R || 8 * m == 3 * R || 8 * m == 5 * R || 8 * m == 7 * R) {
     constexpr double h = 0.70710678118654752440;
     constexpr double c = (8 * m == R || 8 * m == 7 * R) ? h : -h;
@@ -103,6 +113,22 @@ struct DFT<2, SIGN> {
     cd a = x[0];
     x[0] = cadd(a, x[1]);
     x[1] = csub(a, x[1]);
+  }
+};
+// Radix 3: x1' = x0 + w x1 + conj(w) x2, x2' = x0 + conj(w) x1 + w x2 with
+// w = exp(SIGN 2 pi i / 3) = -1/2 + SIGN i sqrt(3)/2.  Radix 6, 12 and 24 are
+// the even/odd recursion above over this one.
+template <int SIGN>
+struct DFT<3, SIGN> {
+  __device__ __forceinline__ static void run(cd* x) {
+    constexpr double s = SIGN * 0.86602540378443864676;
+    const cd a = cadd(x[1], x[2]);
+    const cd d = csub(x[1], x[2]);
+    const cd m = {x[0].x - 0.5 * a.x, x[0].y - 0.5 * a.y};
+    const cd r = {-s * d.y, s * d.x};  // i s (x1 - x2)
+    x[0] = cadd(x[0], a);
+    x[1] = cadd(m, r);
+    x[2] = csub(m, r);
   }
 };
 template <int SIGN>
@@ -208,10 +234,15 @@ __device__ __forceinline__ void stockham_pass(cd (&x)[E], cd* lds, int t, int la
   }
 }
 
+constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+
 template <int N, int E, int SIGN, class LY, int L>
 struct StockhamFrom {
   static constexpr int REM = N / L;
-  static constexpr int R = REM < E ? REM : E;
+  // the largest radix that divides both what is left of N and E (E itself
+  // while it can; N = 3 * 2^a needs E divisible by 3)
+  static constexpr int R = gcd_c(REM, E);
+  static_assert(R > 1, "E must hold every prime factor of N");
   __device__ __forceinline__ static void run(cd (&x)[E], cd* lds, int t, int lane,
                                              const cd* __restrict__ tw) {
     stockham_pass<N, E, R, L, SIGN, LY, (L == 1), (L * R == N)>(x, lds, t, lane, tw);
@@ -224,7 +255,7 @@ struct StockhamFrom {
 template <int N, int E, int SIGN, int LANES>
 __device__ __forceinline__ void fft_block(cd (&x)[E], cd* lds, int t, int lane,
                                           const cd* __restrict__ tw) {
-  static_assert((N & (N - 1)) == 0 && N >= E, "N must be a power of two >= E");
+  static_assert(N % E == 0, "E must divide N");
   StockhamFrom<N, E, SIGN, Interleaved<LANES>, 1>::run(x, lds, t, lane, tw);
 }
 
@@ -233,7 +264,7 @@ __device__ __forceinline__ void fft_block(cd (&x)[E], cd* lds, int t, int lane,
 template <int N, int E, int SIGN>
 __device__ __forceinline__ void fft_block_rows(cd (&x)[E], cd* lds, int t, int row,
                                                const cd* __restrict__ tw) {
-  static_assert((N & (N - 1)) == 0 && N >= E, "N must be a power of two >= E");
+  static_assert(N % E == 0 && N % 8 == 0, "E must divide N; rows are multiples of 8");
   StockhamFrom<N, E, SIGN, RowSwizzled<N>, 1>::run(x, lds, t, row, tw);
 }
 

@@ -81,9 +81,19 @@ struct StepCoeffs {
 // --------------------------------------------------------------------------
 // launch geometry per transform length
 // --------------------------------------------------------------------------
+// Register radix of a transform of length N = 2^a or 3 * 2^a: lengths with the
+// factor 3 need E divisible by 3 (6 or 12; a radix-24 pass would need 15-degree
+// constants).
+template <int N, int E2>
+struct RadixFor {
+  static constexpr int value =
+      N % 3 == 0 ? (E2 <= 4 || N <= 48 ? 6 : 12) : (E2 < N ? E2 : N);
+};
+
 template <int N>
 struct Geo {
-  static constexpr int E = N <= 256 ? (N < 8 ? N : 8) : 16;  // elements per thread
+  static constexpr int E = N % 3 == 0 ? RadixFor<N, 8>::value
+                                      : (N <= 256 ? (N < 8 ? N : 8) : 16);  // elements per thread
   static constexpr int T = N / E;                            // threads per transform
   static constexpr int TK = N >= 1024 ? 4 : 8;               // columns per column tile
   static constexpr int RPraw = 128 / T;
@@ -222,7 +232,7 @@ __device__ __forceinline__ void split_store(const cd* lds, int t, int rp, bool o
   if (!ok) return;
   for (int k = t; k <= NX / 2; k += G::T) {
     cd A = lds[qgk::RowSwizzled<NX>::at(k, rp)];
-    cd B = lds[qgk::RowSwizzled<NX>::at((NX - k) & (NX - 1), rp)];
+    cd B = lds[qgk::RowSwizzled<NX>::at(k == 0 ? 0 : NX - k, rp)];
     o0[k] = {0.5 * (A.x + B.x), 0.5 * (A.y - B.y)};
     o1[k] = {0.5 * (A.y + B.y), -0.5 * (A.x - B.x)};
   }
@@ -447,7 +457,7 @@ __device__ __forceinline__ void split_store_t(const cd* lds, int t, int rp, bool
   for (int k = t; k < NK; k += T) {
     if (k <= NX / 2) {
       cd A = lds[qgk::RowSwizzled<NX>::at(k, rp)];
-      cd B = lds[qgk::RowSwizzled<NX>::at((NX - k) & (NX - 1), rp)];
+      cd B = lds[qgk::RowSwizzled<NX>::at(k == 0 ? 0 : NX - k, rp)];
       o0[k] = {0.5 * (A.x + B.x), 0.5 * (A.y - B.y)};
       o1[k] = {0.5 * (A.y + B.y), -0.5 * (A.x - B.x)};
     } else {  // padding: zeros, so the row is written as whole 128-byte lines
@@ -1817,7 +1827,12 @@ std::vector<cd> make_twiddles(int n) {
   return tw;
 }
 
-bool pow2_in_range(int n) { return n >= 8 && n <= 1024 && (n & (n - 1)) == 0; }
+// Grid sizes: 2^a in [8, 1024] or 3 * 2^a in [24, 768].
+bool size_supported(int n) {
+  const int m = n % 3 == 0 ? n / 3 : n;
+  const bool pow2 = m > 0 && (m & (m - 1)) == 0;
+  return pow2 && (n % 3 == 0 ? (n >= 24 && n <= 768) : (n >= 8 && n <= 1024));
+}
 
 }  // namespace
 
@@ -2008,7 +2023,7 @@ int launch_col_fft(qgk_bucket* b, const cd* in, cd* out, int nplanes) {
   }
   switch (ny) {
     QGK_COL(8) QGK_COL(16) QGK_COL(32) QGK_COL(64) QGK_COL(128) QGK_COL(256) QGK_COL(512)
-    QGK_COL(1024)
+    QGK_COL(1024) QGK_COL(24) QGK_COL(48) QGK_COL(96) QGK_COL(192) QGK_COL(384) QGK_COL(768)
     default: return fail(QGK_E_ARG, "unsupported ny=%d", ny);
   }
 #undef QGK_COL
@@ -2029,7 +2044,7 @@ int launch_row_c2r(qgk_bucket* b, const cd* in, double* out, int nplanes) {
   }
   switch (nx) {
     QGK_ROW(8) QGK_ROW(16) QGK_ROW(32) QGK_ROW(64) QGK_ROW(128) QGK_ROW(256) QGK_ROW(512)
-    QGK_ROW(1024)
+    QGK_ROW(1024) QGK_ROW(24) QGK_ROW(48) QGK_ROW(96) QGK_ROW(192) QGK_ROW(384) QGK_ROW(768)
     default: return fail(QGK_E_ARG, "unsupported nx=%d", nx);
   }
 #undef QGK_ROW
@@ -2049,7 +2064,7 @@ int launch_row_r2c(qgk_bucket* b, const double* in, cd* out, int nplanes) {
   }
   switch (nx) {
     QGK_ROW(8) QGK_ROW(16) QGK_ROW(32) QGK_ROW(64) QGK_ROW(128) QGK_ROW(256) QGK_ROW(512)
-    QGK_ROW(1024)
+    QGK_ROW(1024) QGK_ROW(24) QGK_ROW(48) QGK_ROW(96) QGK_ROW(192) QGK_ROW(384) QGK_ROW(768)
     default: return fail(QGK_E_ARG, "unsupported nx=%d", nx);
   }
 #undef QGK_ROW
@@ -2072,21 +2087,22 @@ int c2r2d(qgk_bucket* b, const cd* in, double* out, int nplanes) {
   return launch_row_c2r(b, b->ctmp, out, nplanes);
 }
 
-// Calls L::go<N, E>(args...) for a power-of-two length n and register radix e
-// (4, 8 or 16, clamped to n).
+// Calls L::go<N, E>(args...) for a transform length n (2^a or 3 * 2^a) and
+// register radix choice e (4, 8 or 16: clamped to n for powers of two, 6 or 12
+// for lengths with the factor 3).
 template <class L, class... A>
 int with_ne(int n, int e, A... a) {
-#define QGK_NE(N)                                                     \
-  case N:                                                             \
-    switch (e) {                                                      \
-      case 4: L::template go<N, (4 < N ? 4 : N)>(a...); return 0;     \
-      case 8: L::template go<N, (8 < N ? 8 : N)>(a...); return 0;     \
-      case 16: L::template go<N, (16 < N ? 16 : N)>(a...); return 0;  \
-      default: return fail(QGK_E_ARG, "unsupported radix %d", e);     \
+#define QGK_NE(N)                                                               \
+  case N:                                                                       \
+    switch (e) {                                                                \
+      case 4: L::template go<N, RadixFor<N, 4>::value>(a...); return 0;         \
+      case 8: L::template go<N, RadixFor<N, 8>::value>(a...); return 0;         \
+      case 16: L::template go<N, RadixFor<N, 16>::value>(a...); return 0;       \
+      default: return fail(QGK_E_ARG, "unsupported radix %d", e);               \
     }
   switch (n) {
     QGK_NE(8) QGK_NE(16) QGK_NE(32) QGK_NE(64) QGK_NE(128) QGK_NE(256) QGK_NE(512)
-    QGK_NE(1024)
+    QGK_NE(1024) QGK_NE(24) QGK_NE(48) QGK_NE(96) QGK_NE(192) QGK_NE(384) QGK_NE(768)
     default: return fail(QGK_E_ARG, "unsupported transform length %d", n);
   }
 #undef QGK_NE
@@ -2274,8 +2290,10 @@ int qgk_create(int nmembers, int nz, int ny, int nx, int flags, qgk_bucket** out
   if (!out) return fail(QGK_E_ARG, "out is NULL");
   *out = nullptr;
   if (nmembers < 1 || nz < 1) return fail(QGK_E_ARG, "nmembers and nz must be >= 1");
-  if (!pow2_in_range(nx) || !pow2_in_range(ny))
-    return fail(QGK_E_ARG, "nx=%d, ny=%d: only powers of two in [8, 1024] are supported", nx, ny);
+  if (!size_supported(nx) || !size_supported(ny))
+    return fail(QGK_E_ARG,
+                "nx=%d, ny=%d: supported sizes are 2^a in [8, 1024] and 3*2^a in [24, 768]",
+                nx, ny);
   int ndev = 0;
   hipError_t he = hipGetDeviceCount(&ndev);
   if (he != hipSuccess || ndev == 0)

@@ -126,6 +126,23 @@ def layered_nz3(pyqg):
     np.savez(os.path.join(HERE, "layered_nz3_nx64.npz"), **out)
 
 
+def layered_nz3_nx96(pyqg):
+    """LayeredModel nz=3 at nx=96 (a 3 * 2^5 grid) from the reference's other
+    spun-up fixture state: q, ph and dqhdt after 1 step, q and ph after 20."""
+    out = {}
+    m = pyqg.LayeredModel(nx=96, log_level=0, **LAYERED3)
+    q0 = np.load(os.path.join(REF_ROOT, "pyqg/tests/fixtures/LayeredModel_nx96_q.npy"))
+    m.q = q0
+    out["q0"] = q0
+    for step in range(1, 21):
+        m._step_forward()
+        if step == 1:
+            snap(m, ["q", "ph", "dqhdt"], "s1_", out)
+        if step == 20:
+            snap(m, ["q", "ph"], "s20_", out)
+    np.savez(os.path.join(HERE, "layered_nz3_nx96.npz"), **out)
+
+
 def layered_nz10(pyqg):
     """LayeredModel nz=10 (exponential stratification, 20 km first radius) at nx=32:
     inversion-matrix samples and q after 10 steps."""
@@ -351,7 +368,7 @@ def parameterizations(pyqg):
 if __name__ == "__main__":
     ref = import_reference()
     only = sys.argv[1:]
-    for fn in (qg_startup, qg_config1, layered_nz3, layered_nz10, bt_sqg, qg_param,
+    for fn in (qg_startup, qg_config1, layered_nz3, layered_nz3_nx96, layered_nz10, bt_sqg, qg_param,
                long_runs, diagnostics_runs, parameterizations):
         if only and fn.__name__ not in only:
             continue

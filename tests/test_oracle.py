@@ -74,6 +74,20 @@ def test_layered_nz3_from_spun_up_fixture():
     assert rel_l2(m.ph, g["s20_ph"]) < 1e-12
 
 
+def test_layered_nz3_nx96_from_spun_up_fixture():
+    """The reference's nx=96 fixture state (a 3 * 2^5 grid)."""
+    g = golden("layered_nz3_nx96.npz")
+    m = OracleLayered(nx=96, **LAYERED3)
+    m.q = g["q0"]
+    m._step_forward()
+    for f in ("q", "ph", "dqhdt"):
+        close(getattr(m, f), g["s1_" + f], 1e-12)
+    for _ in range(19):
+        m._step_forward()
+    assert rel_l2(m.q, g["s20_q"]) < 1e-12
+    assert rel_l2(m.ph, g["s20_ph"]) < 1e-12
+
+
 def test_layered_nz10():
     g = golden("layered_nz10_nx32.npz")
     m = OracleLayered(nz=10, nx=32, H=g["in_H"], rho=g["in_rho"], U=g["in_U"],
