@@ -54,8 +54,10 @@ def test_create_fails_loudly_without_device():
 
 def test_bad_sizes_rejected_before_device_use():
     from pyqg_amd import _lib
-    with pytest.raises(ValueError):
-        _lib.Bucket(1, 2, 48, 48, 0)
+    # sizes are 2^a in [8, 1024] or 3 * 2^a in [24, 768]
+    for n in (40, 12, 1536, 2048, 100):
+        with pytest.raises(ValueError):
+            _lib.Bucket(1, 2, n, n, 0)
     with pytest.raises(ValueError):
         _lib.Bucket(0, 2, 32, 32, 0)
 
