@@ -104,7 +104,7 @@ int qgk_timestep(qgk_bucket* b, double dt1, double dt2, double dt3);
  * a HIP graph instead of run; qgk_graph_replay(b, n) then runs the recorded
  * steps n times with one synchronization.  A graph must hold a multiple of 2
  * steps (the period of the tendency history's buffer swap) and is refused
- * after a constants
+ * after fused steps (which swap the qh buffers) or a constants
  * change; the closures' dx/Hi must not change while capturing.
  * qgk_capture_end(b, 0) discards the capture. */
 int qgk_capture_begin(qgk_bucket* b);

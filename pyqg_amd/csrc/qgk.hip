@@ -1900,6 +1900,7 @@ struct qgk_bucket {
   hipGraphExec_t ugexec = nullptr;
   int ug_phase = -1;  // history rotation (iP) at capture
   long ug_gen = -1;   // gen at capture
+  const cd* ug_qh = nullptr;  // qh buffer at capture (the fused step swaps qh buffers)
   hipStream_t lst = nullptr;  // stream override for fused-step launches (graph capture)
   long gen = 0;               // bumped whenever launch arguments change (graph cache key)
   // hipGraph of two fused steps of a set of buckets, replayed while the AB
@@ -2653,6 +2654,7 @@ int qgk_capture_begin(qgk_bucket* b) {
   b->capturing = true;
   b->ug_phase = b->iP;
   b->ug_gen = b->gen;
+  b->ug_qh = b->field_c[QGK_QH];
   return 0;
 }
 
@@ -2687,8 +2689,9 @@ int qgk_graph_replay(qgk_bucket* b, int n) {
   if (!b || n < 0) return fail(QGK_E_ARG, "bad arguments");
   if (b->capturing) return fail(QGK_E_STATE, "capture in progress");
   if (!b->ugexec) return fail(QGK_E_STATE, "no captured graph");
-  if (b->iP != b->ug_phase || b->gen != b->ug_gen)
-    return fail(QGK_E_STATE, "the captured graph is stale (history phase or constants changed)");
+  if (b->iP != b->ug_phase || b->gen != b->ug_gen || b->field_c[QGK_QH] != b->ug_qh)
+    return fail(QGK_E_STATE,
+                "the captured graph is stale (history phase, qh buffer or constants changed)");
   int rc = upload_params(b);
   if (rc) return rc;
   for (int i = 0; i < n; ++i) HIPC(hipGraphLaunch(b->ugexec, b->stream));

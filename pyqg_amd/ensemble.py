@@ -241,19 +241,55 @@ class Ensemble:
         p = parameterization
         if not p._device_capable():
             raise ValueError(f"{p!r} has no device evaluation")
+        nsteps = int(nsteps)
+        first = True
+        while nsteps > 0 and (first or self.ablevel < 2):
+            self._param_step(p)       # allocations, uploads and AB start-up
+            nsteps -= 1
+            first = False
+        if nsteps >= 2:
+            # once the AB weights are constant, every bucket replays a HIP graph
+            # of two of these steps (the same kernels, bit for bit)
+            from .parameterizations import _capture_steps
+            uv = p.parameterization_type == 'uv_parameterization'
+            w, self.ablevel = C.ab_weights(self.ablevel, self._dt)
+            key = (id(p), repr(p), tuple(w))
+            L = _lib.lib
+            ok = getattr(self, "_graph_key", None) == key
+            if ok and nsteps >= 3 and any(L.qgk_graph_replay(b.h, 0) != 0
+                                          for b in self.buckets):
+                self._param_step(p)   # back to the history phase of the graphs
+                nsteps -= 1
+            ok = ok and all(L.qgk_graph_replay(b.h, 0) == 0 for b in self.buckets)
+            if not ok:
+                self._graph_key = None
+                for b, dx, Hi in zip(self.buckets, self.closure_dx, self.closure_Hi):
+                    _capture_steps(b, np.ascontiguousarray(dx), np.ascontiguousarray(Hi),
+                                   None if uv else p, p if uv else None, w)
+                self._graph_key = key
+            reps = nsteps // 2
+            for b in self.buckets:
+                _lib.check(L.qgk_graph_replay(b.h, reps))
+            for _ in range(2 * reps):
+                self.t += self._dt
+            self.tc += 2 * reps
+            nsteps -= 2 * reps
+        for _ in range(nsteps):
+            self._param_step(p)
+
+    def _param_step(self, p):
+        """One step of every bucket, phase by phase, with the device closure p."""
         uv = p.parameterization_type == 'uv_parameterization'
         L = _lib.lib
-        for _ in range(int(nsteps)):
-            w, self.ablevel = C.ab_weights(self.ablevel, self._dt)
-            for b, dx, Hi in zip(self.buckets, self.closure_dx, self.closure_Hi):
-                for fn in (L.qgk_invert, L.qgk_advect, L.qgk_friction):
-                    _lib.check(fn(b.h))
-                p._device_bucket(b, np.ascontiguousarray(dx), np.ascontiguousarray(Hi), 1.0,
-                                 False)
-                _lib.check((L.qgk_apply_uv_param if uv else L.qgk_apply_q_param)(b.h))
-                _lib.check(L.qgk_timestep(b.h, *w))
-            self.t += self._dt
-            self.tc += 1
+        w, self.ablevel = C.ab_weights(self.ablevel, self._dt)
+        for b, dx, Hi in zip(self.buckets, self.closure_dx, self.closure_Hi):
+            for fn in (L.qgk_invert, L.qgk_advect, L.qgk_friction):
+                _lib.check(fn(b.h))
+            p._device_bucket(b, np.ascontiguousarray(dx), np.ascontiguousarray(Hi), 1.0, False)
+            _lib.check((L.qgk_apply_uv_param if uv else L.qgk_apply_q_param)(b.h))
+            _lib.check(L.qgk_timestep(b.h, *w))
+        self.t += self._dt
+        self.tc += 1
 
     # ------------------------------------------------------------ diagnostics
     def accumulate_diagnostics(self, names=("KEspec",)):

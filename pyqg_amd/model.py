@@ -192,8 +192,9 @@ class Model(DiagnosticsMixin, PseudoSpectralKernel):
                     nsteps -= 1
                 ok = _lib.lib.qgk_graph_replay(b.h, 0) == 0
             if not ok:
+                from .parameterizations import _capture_steps
                 self._graph_key = None
-                self._capture_steps(b, dx, Hi, qp, uvp, w)
+                _capture_steps(b, dx, Hi, qp, uvp, w)
                 self._graph_key = key
         reps = nsteps // 2
         if reps > 0:
@@ -204,25 +205,6 @@ class Model(DiagnosticsMixin, PseudoSpectralKernel):
             self._tc += 2 * reps
         for _ in range(nsteps - 2 * reps):
             self._step_forward()
-
-    def _capture_steps(self, b, dx, Hi, qp, uvp, w):
-        L = _lib.lib
-        _lib.check(L.qgk_capture_begin(b.h))
-        try:
-            for _ in range(2):
-                for fn in (L.qgk_invert, L.qgk_advect, L.qgk_friction):
-                    _lib.check(fn(b.h))
-                if uvp is not None:
-                    uvp._device_bucket(b, dx, Hi, 1.0, False)
-                    _lib.check(L.qgk_apply_uv_param(b.h))
-                if qp is not None:
-                    qp._device_bucket(b, dx, Hi, 1.0, False)
-                    _lib.check(L.qgk_apply_q_param(b.h))
-                _lib.check(L.qgk_timestep(b.h, *w))
-        except Exception:
-            L.qgk_capture_end(b.h, 0)
-            raise
-        _lib.check(L.qgk_capture_end(b.h, 1))
 
     def _steps_before_diagnostics(self, limit):
         """Steps (at most limit) that start before the next diagnostic step."""

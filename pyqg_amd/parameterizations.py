@@ -36,6 +36,30 @@ def _require_device(m, name):
     return t
 
 
+def _capture_steps(b, dx, Hi, qp, uvp, w):
+    """Record two phase-by-phase steps of bucket b with the closures qp (into dq)
+    and uvp (into du, dv) as a HIP graph (qgk_capture_begin/end): invert,
+    advection, friction, closures, time step with the AB weights w.  Two steps
+    are one period of the tendency history's buffer swap."""
+    L = _lib.lib
+    _lib.check(L.qgk_capture_begin(b.h))
+    try:
+        for _ in range(2):
+            for fn in (L.qgk_invert, L.qgk_advect, L.qgk_friction):
+                _lib.check(fn(b.h))
+            if uvp is not None:
+                uvp._device_bucket(b, dx, Hi, 1.0, False)
+                _lib.check(L.qgk_apply_uv_param(b.h))
+            if qp is not None:
+                qp._device_bucket(b, dx, Hi, 1.0, False)
+                _lib.check(L.qgk_apply_q_param(b.h))
+            _lib.check(L.qgk_timestep(b.h, *w))
+    except Exception:
+        L.qgk_capture_end(b.h, 0)
+        raise
+    _lib.check(L.qgk_capture_end(b.h, 1))
+
+
 class Parameterization(ABC):
     """A subgrid parameterization (parameterizations.py:4-81)."""
 

@@ -106,3 +106,47 @@ def test_capture_rejects_partial_history_cycle(pq):
     assert L.qgk_capture_end(b.h, 1) != 0
     assert "multiple of 2" in _lib.lib.qgk_last_error().decode()
     assert L.qgk_graph_replay(b.h, 1) != 0
+
+
+@pytest.mark.parametrize("which", ["smag", "back"])
+def test_ensemble_graph_steps_match_phase_by_phase(pq, which):
+    """Ensemble.step_parameterized replays per-bucket graphs; in uneven chunks
+    (odd lengths re-align the history phase) it matches an ensemble stepped
+    phase by phase bit for bit."""
+    from conftest import QG_PHYS
+    print(f"ensemble graph {which}", flush=True)
+    p = CLOSURES[which](pq)
+    uv = p.parameterization_type == "uv_parameterization"
+    phys = {k: v for k, v in QG_PHYS.items() if k != "beta"}
+    betas = np.array([1.0e-11, 1.5e-11, 2.0e-11])
+    ens = [pq.Ensemble.qg(32, beta=betas, dt=DT, nmembers=3, full_state=True,
+                          q_param=not uv, uv_param=uv, **phys) for _ in range(2)]
+    q0 = [1e-6 * np.random.default_rng(50 + i).standard_normal((2, 32, 32)) for i in range(3)]
+    for e in ens:
+        e.set_q(q0)
+    for chunk in (7, 5, 6):
+        ens[0].step_parameterized(chunk, p)
+        for _ in range(chunk):
+            ens[1]._param_step(p)
+        assert ens[0].tc == ens[1].tc and ens[0].t == ens[1].t
+        for i in range(3):
+            np.testing.assert_array_equal(ens[0].get("q", i), ens[1].get("q", i))
+
+
+def test_graph_recaptured_after_fused_steps(pq):
+    """Fused steps swap the qh buffers: a graph captured before them is refused
+    and rebuilt, so a model that drops its closure for a while and takes it
+    back still steps exactly as _step_forward does."""
+    a, b = make(pq, "smag", nsteps=12), make(pq, "smag", nsteps=12)
+    a.run()
+    stepped(b)
+    smag = a.uv_parameterization
+    for m in (a, b):
+        m.uv_parameterization = None
+        m.tmax += 8 * DT
+        m.run()                 # fused stretches (an odd number of qh swaps) in both
+        m.uv_parameterization = smag
+        m.tmax += 10 * DT
+    a.run()
+    stepped(b)
+    np.testing.assert_array_equal(a.q, b.q)
