@@ -63,11 +63,11 @@ constexpr int NTAP = KS * KS;
 
 // --------------------------------------------------------------------------
 // Forward convolution, blocked for the gfx950 matrix cores.  A block of four
-// waves computes four consecutive output rows (one per wave) of one sample for
-// CO = 16*MT output channels and all 64 pixels: MT x 4 MFMA tiles per wave.
+// waves computes 4*RW consecutive output rows (RW per wave) of one sample for
+// CO = 16*MT output channels and all 64 pixels: MT x 4*RW MFMA tiles per wave.
 // The K loop runs over chunks of KC input channels (KC = 32 uses the K=32
 // instruction v_mfma_f32_16x16x32_bf16, KC = 16 the K=16 one); per chunk the
-// eight input rows the four output rows need are staged in LDS once, with a
+// 4*RW + 4 input rows the block's output rows need are staged in LDS once, with a
 // padded pixel stride against bank conflicts.  Every B operand read from LDS
 // feeds MT MFMAs; the A operands (weights) come from the L1/L2-resident tensor.
 // MFMA lane layout (A 16 x K, B K x 16): lane (i = lane%16, g = lane/16) holds
@@ -93,8 +93,8 @@ struct MfmaK<16> {
   }
 };
 
-template <int W, int MT, int KC>
-__global__ __launch_bounds__(256) void k_conv_fwd_mt(const bf16_t* __restrict__ x,
+template <int W, int MT, int KC, int RW>
+__global__ __launch_bounds__(256, RW) void k_conv_fwd_mt(const bf16_t* __restrict__ x,
                                                      const bf16_t* __restrict__ w,
                                                      const float* __restrict__ bias, int H,
                                                      int cinp, int coutp, int relu,
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_mt(const bf16_t* __restrict__ 
   using M = MfmaK<KC>;
   typedef typename M::op op;
   constexpr int NT = W / 16;            // pixel tiles per row
-  constexpr int ROWS = 4;               // output rows per block, one per wave
+  constexpr int ROWS = 4 * RW;          // output rows per block, RW per wave
   constexpr int SR = ROWS + KS - 1;     // staged input rows
   constexpr int PS = KC + 8;            // padded pixel stride (bf16 elements)
   constexpr int KG = KC / 4;            // K elements per lane
@@ -113,14 +113,16 @@ __global__ __launch_bounds__(256) void k_conv_fwd_mt(const bf16_t* __restrict__ 
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int row0 = blockIdx.x * ROWS, b = blockIdx.y;
   const int co0 = blockIdx.z * (16 * MT);
-  const int row = row0 + wave;
-  const bool row_ok = row < H;
+  const int wrow0 = row0 + wave * RW;   // the wave's first output row
+  const bool wave_ok = wrow0 < H;
   const int li = lane % 16, lg = lane / 16;
-  v4f acc[MT][NT];
+  v4f acc[MT][RW][NT];
 #pragma unroll
   for (int m = 0; m < MT; ++m)
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[m][t] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int rr = 0; rr < RW; ++rr)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[m][rr][t] = v4f{0.f, 0.f, 0.f, 0.f};
   for (int ci0 = 0; ci0 < cinp; ci0 += KC) {
     __syncthreads();
     for (int q = threadIdx.x; q < SR * W * PIECES; q += 256) {
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_mt(const bf16_t* __restrict__ 
       *reinterpret_cast<uint4*>(lds + (r * W + pix) * PS + pc * 8) = v;
     }
     __syncthreads();
-    if (row_ok) {
+    if (wave_ok) {
       const bf16_t* wrow = w + (size_t)(co0 + li) * NTAP * cinp + ci0 + lg * KG;
 #pragma unroll 1
       for (int tap = 0; tap < NTAP; ++tap) {
@@ -140,46 +142,54 @@ __global__ __launch_bounds__(256) void k_conv_fwd_mt(const bf16_t* __restrict__ 
 #pragma unroll
         for (int m = 0; m < MT; ++m)
           a[m] = *reinterpret_cast<const op*>(wrow + ((size_t)m * 16 * NTAP + tap) * cinp);
-        const bf16_t* lrow = lds + (wave + dy) * W * PS + lg * KG;
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const int pix = (t * 16 + li + dx - 2 + W) % W;
-          const op bv = *reinterpret_cast<const op*>(lrow + pix * PS);
+        for (int rr = 0; rr < RW; ++rr) {
+          const bf16_t* lrow = lds + (wave * RW + rr + dy) * W * PS + lg * KG;
 #pragma unroll
-          for (int m = 0; m < MT; ++m) acc[m][t] = M::mma(a[m], bv, acc[m][t]);
+          for (int t = 0; t < NT; ++t) {
+            const int pix = (t * 16 + li + dx - 2 + W) % W;
+            const op bv = *reinterpret_cast<const op*>(lrow + pix * PS);
+#pragma unroll
+            for (int m = 0; m < MT; ++m) acc[m][rr][t] = M::mma(a[m], bv, acc[m][rr][t]);
+          }
         }
       }
     }
   }
-  if (!row_ok) return;
+  if (!wave_ok) return;
 #pragma unroll
-  for (int m = 0; m < MT; ++m) {
-    const int cb = co0 + m * 16 + lg * 4;
+  for (int rr = 0; rr < RW; ++rr) {
+    const int row = wrow0 + rr;
+    if (row >= H) break;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const int pix = t * 16 + li;
-      const size_t o = (((size_t)b * H + row) * W + pix) * coutp + cb;
-      float v[4] = {acc[m][t][0], acc[m][t][1], acc[m][t][2], acc[m][t][3]};
+    for (int m = 0; m < MT; ++m) {
+      const int cb = co0 + m * 16 + lg * 4;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        if (bias) v[j] += bias[cb + j];
-        if (relu) v[j] = v[j] > 0.f ? v[j] : 0.f;
-      }
-      if (mask) {
-        const v4s mk = *reinterpret_cast<const v4s*>(mask + o);
+      for (int t = 0; t < NT; ++t) {
+        const int pix = t * 16 + li;
+        const size_t o = (((size_t)b * H + row) * W + pix) * coutp + cb;
+        float v[4] = {acc[m][rr][t][0], acc[m][rr][t][1], acc[m][rr][t][2], acc[m][rr][t][3]};
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (bf2f((bf16_t)mk[j]) <= 0.f) v[j] = 0.f;
-      }
-      if (y) {
-        v4s out;
+        for (int j = 0; j < 4; ++j) {
+          if (bias) v[j] += bias[cb + j];
+          if (relu) v[j] = v[j] > 0.f ? v[j] : 0.f;
+        }
+        if (mask) {
+          const v4s mk = *reinterpret_cast<const v4s*>(mask + o);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) out[j] = (short)f2bf(v[j]);
-        *reinterpret_cast<v4s*>(y + o) = out;
-      }
-      if (y32) {
+          for (int j = 0; j < 4; ++j)
+            if (bf2f((bf16_t)mk[j]) <= 0.f) v[j] = 0.f;
+        }
+        if (y) {
+          v4s out;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) y32[o + j] = v[j];
+          for (int j = 0; j < 4; ++j) out[j] = (short)f2bf(v[j]);
+          *reinterpret_cast<v4s*>(y + o) = out;
+        }
+        if (y32) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) y32[o + j] = v[j];
+        }
       }
     }
   }
@@ -513,16 +523,23 @@ int qgcnn_conv_fwd(const void* x, int B, int H, int W, int cinp, const void* w,
     return fail(QGCNN_E_ARG, "images must be W=64 wide and at least 5 rows (got %dx%d)", H, W);
   hipStream_t st = (hipStream_t)stream;
   const int mt = coutp % 64 == 0 ? 4 : (coutp % 32 == 0 ? 2 : 1);
-  dim3 grid((H + 3) / 4, B, coutp / (16 * mt));
-#define QGCNN_FWD(MT, KC)                                                                    \
-  hipLaunchKernelGGL((k_conv_fwd_mt<64, MT, KC>), grid, dim3(256), 0, st, (const bf16_t*)x, \
-                     (const bf16_t*)w, bias, H, cinp, coutp, relu, (const bf16_t*)mask,      \
+  // two output rows per wave (every weight operand then feeds twice the MFMAs)
+  // where the weights dominate: 64 output channels per block or at least 64
+  // input channels; smaller layers keep more, shorter blocks
+  const int rw = H % 8 == 0 && (mt == 4 || cinp >= 64) ? 2 : 1;
+  dim3 grid((H + 4 * rw - 1) / (4 * rw), B, coutp / (16 * mt));
+#define QGCNN_FWD(MT, KC, RW)                                                                  \
+  hipLaunchKernelGGL((k_conv_fwd_mt<64, MT, KC, RW>), grid, dim3(256), 0, st, (const bf16_t*)x, \
+                     (const bf16_t*)w, bias, H, cinp, coutp, relu, (const bf16_t*)mask,        \
                      (bf16_t*)y, y32)
+#define QGCNN_FWD_RW(MT, KC) \
+  do { if (rw == 2) QGCNN_FWD(MT, KC, 2); else QGCNN_FWD(MT, KC, 1); } while (0)
   if (cinp % 32 == 0) {
-    if (mt == 4) QGCNN_FWD(4, 32); else if (mt == 2) QGCNN_FWD(2, 32); else QGCNN_FWD(1, 32);
+    if (mt == 4) QGCNN_FWD_RW(4, 32); else if (mt == 2) QGCNN_FWD_RW(2, 32); else QGCNN_FWD_RW(1, 32);
   } else {
-    if (mt == 4) QGCNN_FWD(4, 16); else if (mt == 2) QGCNN_FWD(2, 16); else QGCNN_FWD(1, 16);
+    if (mt == 4) QGCNN_FWD_RW(4, 16); else if (mt == 2) QGCNN_FWD_RW(2, 16); else QGCNN_FWD_RW(1, 16);
   }
+#undef QGCNN_FWD_RW
 #undef QGCNN_FWD
   HIPC(hipGetLastError());
   return 0;
