@@ -110,6 +110,10 @@ int qgk_timestep(qgk_bucket* b, double dt1, double dt2, double dt3);
 int qgk_capture_begin(qgk_bucket* b);
 int qgk_capture_end(qgk_bucket* b, int keep);
 int qgk_graph_replay(qgk_bucket* b, int n);
+/* The handle's HIP stream (hipStream_t), for callers that enqueue their own
+ * kernels between the phases (e.g. a network's forward pass inside a captured
+ * step).  No counterpart in the reference. */
+void* qgk_stream(qgk_bucket* b);
 /* The q setter (rfft2 of q into qh, kernel.pyx:576-582) and the qh setter's
  * inverse transform (kernel.pyx:583-591). */
 int qgk_q_to_qh(qgk_bucket* b);

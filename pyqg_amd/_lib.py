@@ -30,7 +30,7 @@ EXPORTS = ["qgk_create", "qgk_destroy", "qgk_last_error", "qgk_set_const", "qgk_
            "qgk_timestep", "qgk_q_to_qh", "qgk_qh_to_q", "qgk_fft", "qgk_ifft", "qgk_step",
            "qgk_timing_enable", "qgk_timing", "qgk_sync", "qgk_step_buckets",
            "qgk_copy_probe", "qgk_pattern_probe", "qgk_capture_begin", "qgk_capture_end",
-           "qgk_graph_replay", "qgk_set_variant", "qgk_get_variant", "qgk_autotune",
+           "qgk_graph_replay", "qgk_stream", "qgk_set_variant", "qgk_get_variant", "qgk_autotune",
            "qgk_device_name", "qgk_put_device", "qgk_get_device", "qgk_coarsen",
            "qgk_subgrid_forcing", "qgk_pack_inputs", "qgk_add_forcing_device",
            "qgk_diag_set_params", "qgk_diag_accumulate", "qgk_diag_read", "qgk_diag_reset",
@@ -110,6 +110,7 @@ def _load():
         "qgk_capture_begin": (I, [P]),
         "qgk_capture_end": (I, [P, I]),
         "qgk_graph_replay": (I, [P, I]),
+        "qgk_stream": (P, [P]),
         "qgk_set_variant": (I, [P, I, I]),
         "qgk_get_variant": (I, [P, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "qgk_autotune": (I, [P, I, ctypes.POINTER(I), ctypes.POINTER(I), DP, DP]),

@@ -71,13 +71,20 @@ def pad16(c):
 
 
 # ---------------------------------------------------------------- primitives
-def conv_fwd(x, w, bias, coutp, relu=True, mask=None, out_bf16=True, out_f32=False):
-    """x: (B, H, 64, cinp) bf16; w: (coutp, 25, cinp) bf16; returns (y_bf16, y_f32)."""
+def conv_fwd(x, w, bias, coutp, relu=True, mask=None, out_bf16=True, out_f32=False, y=None,
+             y32=None, stream=None):
+    """x: (B, H, 64, cinp) bf16; w: (coutp, 25, cinp) bf16; returns (y_bf16, y_f32).
+    Outputs are allocated unless given (y, y32); stream: a hipStream_t handle
+    (default: torch's current stream)."""
     B, H, W, cinp = x.shape
-    y = torch.empty((B, H, W, coutp), dtype=torch.bfloat16, device=x.device) if out_bf16 else None
-    y32 = torch.empty((B, H, W, coutp), dtype=torch.float32, device=x.device) if out_f32 else None
+    if out_bf16 and y is None:
+        y = torch.empty((B, H, W, coutp), dtype=torch.bfloat16, device=x.device)
+    if out_f32 and y32 is None:
+        y32 = torch.empty((B, H, W, coutp), dtype=torch.float32, device=x.device)
+    y = y if out_bf16 else None
+    y32 = y32 if out_f32 else None
     check(lib.qgcnn_conv_fwd(_p(x), B, H, W, cinp, _p(w), _p(bias), coutp, int(relu), _p(mask),
-                             _p(y), _p(y32), _stream()))
+                             _p(y), _p(y32), stream if stream is not None else _stream()))
     return y, y32
 
 
@@ -192,6 +199,25 @@ class CNN:
                 if keep:
                     acts.append(y)
         return (out, acts) if keep else out
+
+    def activation_buffers(self, B, H, W):
+        """Persistent outputs of every layer for forward_into: BF16 for the
+        hidden layers, FP32 for the last."""
+        return [torch.empty((B, H, W, self.chp[l + 1]),
+                            dtype=torch.float32 if l == self.nlayers - 1 else torch.bfloat16,
+                            device=self.device) for l in range(self.nlayers)]
+
+    def forward_into(self, xp, bufs, stream):
+        """The forward pass into preallocated buffers (activation_buffers) on a
+        given stream: nothing is allocated, so it can be recorded into a graph."""
+        h = xp
+        for l in range(self.nlayers):
+            last = l == self.nlayers - 1
+            conv_fwd(h, self.wb[l], self.b32[l], self.chp[l + 1], relu=not last,
+                     out_bf16=not last, out_f32=last, y=None if last else bufs[l],
+                     y32=bufs[l] if last else None, stream=stream)
+            h = bufs[l]
+        return bufs[-1]
 
     def backward(self, acts, dout):
         """dout: bf16 gradient of the loss w.r.t. the last (pre-activation) output.

@@ -13,6 +13,8 @@ Two ways in:
     (pyqg/model.py:406-436: invert, advection, friction, q parameterization,
     timestep).
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -83,8 +85,11 @@ class ClosedEnsemble:
             self.cinp = param.net.chp[0]
             self.xin = torch.empty((nmembers, n, n, self.cinp), dtype=torch.bfloat16,
                                    device="cuda")
+            self.acts = param.net.activation_buffers(nmembers, n, n)
             self.inv_in = np.ascontiguousarray(1.0 / param.scales.in_scale)
             self.out_scale = np.ascontiguousarray(param.scales.out_scale)
+            self.stream = ctypes.c_void_p(_lib.lib.qgk_stream(self.b.h))
+            self._graph_key = None
 
     @property
     def t(self):
@@ -97,23 +102,65 @@ class ClosedEnsemble:
         if self.builtin:
             self.ens.step_parameterized(nsteps, self.param)
             return
-        h = self.b.h
+        # the network runs on the bucket's stream into persistent buffers, so
+        # once the AB weights are constant two steps are recorded as a HIP
+        # graph and replayed (the same kernels as the step-by-step form)
         L = _lib.lib
-        for _ in range(int(nsteps)):
-            _lib.check(L.qgk_invert(h))
-            _lib.check(L.qgk_advect(h))
-            _lib.check(L.qgk_friction(h))
-            if self.param is not None:
-                _lib.check(L.qgk_pack_inputs(h, _lib.ptr(self.inv_in), self.cinp,
-                                             ctypes_p(self.xin)))
-                out = self.param.net.forward(self.xin)
-                torch.cuda.current_stream().synchronize()
-                _lib.check(L.qgk_add_forcing_device(h, ctypes_p(out), out.shape[-1],
-                                                    _lib.ptr(self.out_scale)))
+        h = self.b.h
+        nsteps = int(nsteps)
+        first = True
+        while nsteps > 0 and (first or self.ens.ablevel < 2):
+            self._cnn_step()
+            nsteps -= 1
+            first = False
+        if nsteps >= 2:
             w, self.ens.ablevel = C.ab_weights(self.ens.ablevel, self.ens.dt)
-            _lib.check(L.qgk_timestep(h, *w))
-            self.ens.t += self.ens.dt
-            self.ens.tc += 1
+            net = self.param.net
+            key = (tuple(t.data_ptr() for t in net.wb), tuple(t.data_ptr() for t in net.b32),
+                   tuple(w))
+            ok = self._graph_key == key
+            if ok and nsteps >= 3 and L.qgk_graph_replay(h, 0) != 0:
+                self._cnn_step()      # back to the history phase of the graph
+                nsteps -= 1
+            ok = ok and L.qgk_graph_replay(h, 0) == 0
+            if not ok:
+                self._graph_key = None
+                _lib.check(L.qgk_capture_begin(h))
+                try:
+                    for _ in range(2):
+                        self._cnn_phases(w)
+                except Exception:
+                    L.qgk_capture_end(h, 0)
+                    raise
+                _lib.check(L.qgk_capture_end(h, 1))
+                self._graph_key = key
+            reps = nsteps // 2
+            _lib.check(L.qgk_graph_replay(h, reps))
+            for _ in range(2 * reps):
+                self.ens.t += self.ens.dt
+            self.ens.tc += 2 * reps
+            nsteps -= 2 * reps
+        for _ in range(nsteps):
+            self._cnn_step()
+
+    def _cnn_phases(self, w):
+        """Invert, advect, friction, the network's forcing, time step (weights w)."""
+        L = _lib.lib
+        h = self.b.h
+        _lib.check(L.qgk_invert(h))
+        _lib.check(L.qgk_advect(h))
+        _lib.check(L.qgk_friction(h))
+        _lib.check(L.qgk_pack_inputs(h, _lib.ptr(self.inv_in), self.cinp, ctypes_p(self.xin)))
+        out = self.param.net.forward_into(self.xin, self.acts, self.stream)
+        _lib.check(L.qgk_add_forcing_device(h, ctypes_p(out), out.shape[-1],
+                                            _lib.ptr(self.out_scale)))
+        _lib.check(L.qgk_timestep(h, *w))
+
+    def _cnn_step(self):
+        w, self.ens.ablevel = C.ab_weights(self.ens.ablevel, self.ens.dt)
+        self._cnn_phases(w)
+        self.ens.t += self.ens.dt
+        self.ens.tc += 1
 
     def ph(self):
         return self.b.get("ph")

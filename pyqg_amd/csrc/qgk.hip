@@ -1878,7 +1878,8 @@ struct qgk_bucket {
   bool gvalid = false;
   // register radix of the fused row and column kernels (autotuned)
   int erow = 8, ecol = 8;
-  double* small = nullptr;  // 256 doubles of per-call constants (closure scales)
+  double* small = nullptr;  // 256 doubles: learned-closure scales (input [0,128), output [128,256))
+  std::vector<double> small_in_h, small_out_h;  // values last uploaded there
   // diagnostics: per-member parameters, accumulators and work space
   std::vector<double> dprm_h;  // [M][plen]
   std::vector<char> dprm_set;  // per member: parameters given
@@ -3193,23 +3194,35 @@ int qgk_subgrid_forcing(qgk_bucket* hi, qgk_bucket* lo, const double* filter_lo)
   return 0;
 }
 
+// Upload n constants to dev only when they differ from the cached copy; under
+// graph capture an upload would read a host buffer at replay, so it is refused.
+int upload_cached(qgk_bucket* b, const double* host, int n, std::vector<double>& cache,
+                  double* dev) {
+  if (cache.size() == (size_t)n && std::equal(host, host + n, cache.begin())) return 0;
+  if (b->capturing) return fail(QGK_E_STATE, "closure scales changed during graph capture");
+  cache.assign(host, host + n);
+  HIPC(hipMemcpyAsync(dev, cache.data(), n * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  return 0;
+}
+
+void* qgk_stream(qgk_bucket* b) { return b ? (void*)b->stream : nullptr; }
+
 int qgk_pack_inputs(qgk_bucket* b, const double* inv_scale, int cinp, void* dst) {
   if (!b || !inv_scale || !dst) return fail(QGK_E_ARG, "NULL argument");
   if (!(b->flags & QGK_FULL_STATE)) return fail(QGK_E_STATE, "needs a full-state handle");
   if (cinp < 3 * b->nz) return fail(QGK_E_ARG, "cinp too small");
   int rc = refresh_q(b);
   if (rc) return rc;
-  if (3 * b->nz > 256) return fail(QGK_E_ARG, "too many layers for the packing constants");
+  if (3 * b->nz > 128) return fail(QGK_E_ARG, "too many layers for the packing constants");
   double* sc = b->small;
-  HIPC(hipMemcpyAsync(sc, inv_scale, 3 * b->nz * sizeof(double), hipMemcpyHostToDevice,
-                      b->stream));
+  if ((rc = upload_cached(b, inv_scale, 3 * b->nz, b->small_in_h, sc))) return rc;
   const int npix = (int)b->nplane_r;
   const size_t n = (size_t)b->M * npix * cinp;
   hipLaunchKernelGGL(k_pack_nhwc, dim3(grid_1d(n)), dim3(256), 0, b->stream, b->field_r[QGK_Q],
                      b->field_r[QGK_U], b->field_r[QGK_V], sc, (unsigned short*)dst, b->M, b->nz,
                      npix, cinp);
   HIPC(hipGetLastError());
-  HIPC(hipStreamSynchronize(b->stream));
+  HIPC(stream_done(b));
   return 0;
 }
 
@@ -3217,21 +3230,22 @@ int qgk_add_forcing_device(qgk_bucket* b, const float* net_out, int coutp, const
   if (!b || !net_out || !scale) return fail(QGK_E_ARG, "NULL argument");
   if (!b->field_r[QGK_DQ]) return fail(QGK_E_STATE, "dq is not allocated (no q parameterization)");
   if (coutp < b->nz) return fail(QGK_E_ARG, "coutp too small");
-  if (b->nz > 256) return fail(QGK_E_ARG, "too many layers for the forcing scales");
-  double* sc = b->small;
-  HIPC(hipMemcpyAsync(sc, scale, b->nz * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  if (b->nz > 128) return fail(QGK_E_ARG, "too many layers for the forcing scales");
+  double* sc = b->small + 128;
+  int rc = upload_cached(b, scale, b->nz, b->small_out_h, sc);
+  if (rc) return rc;
   const int np = b->M * b->nz;
   const size_t n = (size_t)np * b->nplane_r;
   hipLaunchKernelGGL(k_unpack_forcing, dim3(grid_1d(n)), dim3(256), 0, b->stream, net_out, sc,
                      b->field_r[QGK_DQ], b->M, b->nz, (int)b->nplane_r, coutp);
   HIPC(hipGetLastError());
-  int rc = r2c2d(b, b->field_r[QGK_DQ], b->field_c[QGK_DQH], np);
+  rc = r2c2d(b, b->field_r[QGK_DQ], b->field_c[QGK_DQH], np);
   if (rc) return rc;
   const size_t cpts = (size_t)np * b->nplane_c;
   hipLaunchKernelGGL(k_add, dim3(grid_1d(cpts)), dim3(256), 0, b->stream, D(b),
                      b->field_c[QGK_DQH], cpts);
   HIPC(hipGetLastError());
-  HIPC(hipStreamSynchronize(b->stream));
+  HIPC(stream_done(b));
   return 0;
 }
 

@@ -108,6 +108,26 @@ def test_device_closure_step_equals_host_callback(mods):
     assert ce.ens.t == m.t
 
 
+def test_closed_ensemble_graph_matches_step_by_step(mods):
+    """ClosedEnsemble.step replays a graph of two steps (network on the
+    bucket's stream into persistent buffers); in uneven chunks it matches the
+    same ensemble stepped one step at a time bit for bit."""
+    pq, cnn, _, param, _ = mods
+    net = cnn.CNN(channels=(6, 32, 16, 2), seed=2)
+    sc = param.Scales(in_scale=[1e-6, 1e-6, 1e-2, 1e-2, 1e-2, 1e-2], out_scale=[1e-12, 1e-12])
+    p = param.CNNParameterization(net, sc)
+    phys = dict(QG_PHYS)
+    a = param.ClosedEnsemble(3, 64, 3600., phys, param=p, seed=5)
+    b = param.ClosedEnsemble(3, 64, 3600., phys, param=p, seed=5)
+    for chunk in (7, 6, 5):
+        a.step(chunk)
+        for _ in range(chunk):
+            b._cnn_step()
+        assert a.ens.tc == b.ens.tc and a.ens.t == b.ens.t
+        for i in range(3):
+            np.testing.assert_array_equal(a.ens.get("q", i), b.ens.get("q", i))
+
+
 def test_trained_network_beats_no_closure(mods):
     """Offline: on held-out pairs the trained network's forcing error is well
     below the no-closure prediction (zero forcing), i.e. R^2 > 0.3."""

@@ -16,8 +16,8 @@ CNN_LIB = os.path.join(ROOT, "pyqg_amd", "_qgcnn.so")
 
 def declared(header=HEADER, prefix="qgk_"):
     text = open(header).read()
-    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char\*)\s+(" + prefix + r"\w+)\s*\(",
-                                 text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char\*|void\*)\s+(" + prefix +
+                                 r"\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_entry_points():
