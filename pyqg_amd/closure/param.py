@@ -108,22 +108,22 @@ class ClosedEnsemble:
         L = _lib.lib
         h = self.b.h
         nsteps = int(nsteps)
-        first = True
-        while nsteps > 0 and (first or self.ens.ablevel < 2):
-            self._cnn_step()
+        while nsteps > 0 and self.ens.ablevel < 2:
+            self._cnn_step()          # AB start-up
             nsteps -= 1
-            first = False
         if nsteps >= 2:
             w, self.ens.ablevel = C.ab_weights(self.ens.ablevel, self.ens.dt)
             net = self.param.net
             key = (tuple(t.data_ptr() for t in net.wb), tuple(t.data_ptr() for t in net.b32),
                    tuple(w))
             ok = self._graph_key == key
-            if ok and nsteps >= 3 and L.qgk_graph_replay(h, 0) != 0:
-                self._cnn_step()      # back to the history phase of the graph
+            if not ok or L.qgk_graph_replay(h, 0) != 0:
+                # allocations and uploads before a capture; the history phase
+                # of a kept graph
+                self._cnn_step()
                 nsteps -= 1
-            ok = ok and L.qgk_graph_replay(h, 0) == 0
-            if not ok:
+                ok = ok and L.qgk_graph_replay(h, 0) == 0
+            if not ok and nsteps >= 2:
                 self._graph_key = None
                 _lib.check(L.qgk_capture_begin(h))
                 try:
@@ -134,12 +134,14 @@ class ClosedEnsemble:
                     raise
                 _lib.check(L.qgk_capture_end(h, 1))
                 self._graph_key = key
-            reps = nsteps // 2
-            _lib.check(L.qgk_graph_replay(h, reps))
-            for _ in range(2 * reps):
-                self.ens.t += self.ens.dt
-            self.ens.tc += 2 * reps
-            nsteps -= 2 * reps
+                ok = True
+            if ok and nsteps >= 2:
+                reps = nsteps // 2
+                _lib.check(L.qgk_graph_replay(h, reps))
+                for _ in range(2 * reps):
+                    self.ens.t += self.ens.dt
+                self.ens.tc += 2 * reps
+                nsteps -= 2 * reps
         for _ in range(nsteps):
             self._cnn_step()
 

@@ -242,11 +242,9 @@ class Ensemble:
         if not p._device_capable():
             raise ValueError(f"{p!r} has no device evaluation")
         nsteps = int(nsteps)
-        first = True
-        while nsteps > 0 and (first or self.ablevel < 2):
-            self._param_step(p)       # allocations, uploads and AB start-up
+        while nsteps > 0 and self.ablevel < 2:
+            self._param_step(p)       # AB start-up
             nsteps -= 1
-            first = False
         if nsteps >= 2:
             # once the AB weights are constant, every bucket replays a HIP graph
             # of two of these steps (the same kernels, bit for bit)
@@ -255,25 +253,32 @@ class Ensemble:
             w, self.ablevel = C.ab_weights(self.ablevel, self._dt)
             key = (id(p), repr(p), tuple(w))
             L = _lib.lib
+
+            def replayable():
+                return all(L.qgk_graph_replay(b.h, 0) == 0 for b in self.buckets)
+
             ok = getattr(self, "_graph_key", None) == key
-            if ok and nsteps >= 3 and any(L.qgk_graph_replay(b.h, 0) != 0
-                                          for b in self.buckets):
-                self._param_step(p)   # back to the history phase of the graphs
+            if not ok or not replayable():
+                # allocations and uploads before a capture; the history phase
+                # of a kept graph
+                self._param_step(p)
                 nsteps -= 1
-            ok = ok and all(L.qgk_graph_replay(b.h, 0) == 0 for b in self.buckets)
-            if not ok:
+                ok = ok and replayable()
+            if not ok and nsteps >= 2:
                 self._graph_key = None
                 for b, dx, Hi in zip(self.buckets, self.closure_dx, self.closure_Hi):
                     _capture_steps(b, np.ascontiguousarray(dx), np.ascontiguousarray(Hi),
                                    None if uv else p, p if uv else None, w)
                 self._graph_key = key
-            reps = nsteps // 2
-            for b in self.buckets:
-                _lib.check(L.qgk_graph_replay(b.h, reps))
-            for _ in range(2 * reps):
-                self.t += self._dt
-            self.tc += 2 * reps
-            nsteps -= 2 * reps
+                ok = True
+            if ok and nsteps >= 2:
+                reps = nsteps // 2
+                for b in self.buckets:
+                    _lib.check(L.qgk_graph_replay(b.h, reps))
+                for _ in range(2 * reps):
+                    self.t += self._dt
+                self.tc += 2 * reps
+                nsteps -= 2 * reps
         for _ in range(nsteps):
             self._param_step(p)
 

@@ -171,39 +171,39 @@ class Model(DiagnosticsMixin, PseudoSpectralKernel):
         _step_forward.  The graph is kept while the closures, dt and the
         kernel's constants stay the same."""
         nsteps = int(nsteps)
-        first = True
-        while nsteps > 0 and (first or self._ablevel < 2):
-            self._step_forward()      # allocations, uploads and AB start-up
+        while nsteps > 0 and self._ablevel < 2:
+            self._step_forward()      # AB start-up
             nsteps -= 1
-            first = False
         if nsteps >= 2:
-            from .parameterizations import _device_target
+            from .parameterizations import _capture_steps, _device_target
+            L = _lib.lib
             b, dx, Hi = _device_target(self)
             qp, uvp = self.q_parameterization, self.uv_parameterization
             w, self._ablevel = C.ab_weights(self._ablevel, self._dt)
             key = (id(qp), repr(qp), id(uvp), repr(uvp), tuple(w), dx.tobytes(), Hi.tobytes())
             self._flush()
             ok = getattr(self, "_graph_key", None) == key
-            if ok:
-                # one ordinary step brings the history buffers back to the
-                # phase the graph was captured at
-                if nsteps >= 3 and _lib.lib.qgk_graph_replay(b.h, 0) != 0:
-                    self._step_forward()
-                    nsteps -= 1
-                ok = _lib.lib.qgk_graph_replay(b.h, 0) == 0
-            if not ok:
-                from .parameterizations import _capture_steps
+            if not ok or L.qgk_graph_replay(b.h, 0) != 0:
+                # an ordinary step first: before a capture it makes the
+                # allocations and uploads; for a kept graph it brings the
+                # history buffers back to the phase the graph was captured at
+                self._step_forward()
+                nsteps -= 1
+                ok = ok and L.qgk_graph_replay(b.h, 0) == 0
+            if not ok and nsteps >= 2:
                 self._graph_key = None
                 _capture_steps(b, dx, Hi, qp, uvp, w)
                 self._graph_key = key
-        reps = nsteps // 2
-        if reps > 0:
-            _lib.check(_lib.lib.qgk_graph_replay(b.h, reps))
-            self._invalidate_all()
-            for _ in range(2 * reps):
-                self._t += self._dt
-            self._tc += 2 * reps
-        for _ in range(nsteps - 2 * reps):
+                ok = True
+            if ok and nsteps >= 2:
+                reps = nsteps // 2
+                _lib.check(L.qgk_graph_replay(b.h, reps))
+                self._invalidate_all()
+                for _ in range(2 * reps):
+                    self._t += self._dt
+                self._tc += 2 * reps
+                nsteps -= 2 * reps
+        for _ in range(nsteps):
             self._step_forward()
 
     def _steps_before_diagnostics(self, limit):
