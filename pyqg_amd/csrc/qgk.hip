@@ -597,6 +597,7 @@ __device__ __forceinline__ void col_prologue_layer(const cd* __restrict__ qh,
                                                    int nz, int z, int nk, int ldk, int t, int c, int k,
                                                    bool ok, size_t mbase, cd* lds,
                                                    const cd* __restrict__ tw) {
+#pragma clang fp contract(off)
   using G = FGeo<NY, E>;
   const size_t plane = (size_t)NY * ldk;
   const size_t zoff = mbase + (size_t)z * plane;
@@ -640,6 +641,7 @@ template <int NY, int E>
 __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_prologue(
     const cd* __restrict__ qh, cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
     const MemberParams* __restrict__ params, int nz, int nk, int ldk, const cd* __restrict__ tw) {
+#pragma clang fp contract(off)
   using G = FGeo<NY, E>;
   __shared__ cd lds[NY * G::TK];
   const int c = threadIdx.x % G::TK;
@@ -666,6 +668,7 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
     cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
     const MemberParams* __restrict__ params, StepCoeffs cf, int nz, int nk, int ldk,
     const cd* __restrict__ tw) {
+#pragma clang fp contract(off)
   using G = FGeo<NY, E>;
   __shared__ cd lds[NY * G::TK];
   const int c = threadIdx.x % G::TK;
@@ -966,6 +969,7 @@ __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
                                                   int nk, int ldk, int t, int c, int k, bool ok,
                                                   size_t mbase, cd* lds,
                                                   const cd* __restrict__ tw) {
+#pragma clang fp contract(off)
   using G = FGeo<NY, E>;
   const size_t plane = (size_t)NY * ldk;
   const int kc = ok ? k : 0;
@@ -1012,6 +1016,7 @@ template <int NY, int NZ, int E>
 __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_prologue_reg(
     const cd* __restrict__ qh, cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
     const MemberParams* __restrict__ params, int nk, int ldk, const cd* __restrict__ tw) {
+#pragma clang fp contract(off)
   using G = FGeo<NY, E>;
   __shared__ cd lds[NY * G::TK];
   const int c = threadIdx.x % G::TK;
@@ -1040,6 +1045,7 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
     cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
     const MemberParams* __restrict__ params, StepCoeffs cf, int nk, int ldk,
     const cd* __restrict__ tw) {
+#pragma clang fp contract(off)
   using G = FGeo<NY, E>;
   __shared__ cd lds[NY * G::TK];
   const int c = threadIdx.x % G::TK;
