@@ -3369,9 +3369,12 @@ int qgk_diag_set_params(qgk_bucket* b, int member, const double* params) {
     b->dprm_h.assign((size_t)b->M * plen, 0.0);
     b->dprm_set.assign(b->M, 0);
   }
-  std::memcpy(b->dprm_h.data() + (size_t)member * plen, params, plen * sizeof(double));
+  double* dst = b->dprm_h.data() + (size_t)member * plen;
+  if (!b->dprm_set[member] || std::memcmp(dst, params, plen * sizeof(double)) != 0) {
+    std::memcpy(dst, params, plen * sizeof(double));
+    b->dprm_dirty = true;  // uploaded with the next accumulation
+  }
   b->dprm_set[member] = 1;
-  b->dprm_dirty = true;
   return 0;
 }
 

@@ -201,16 +201,28 @@ class DiagnosticsMixin:
         pass
 
     # ------------------------------------------------------------- device side
+    _DIAG_PARAM_INPUTS = ('Hi', 'H', 'S', 'del1', 'del2', 'rd', 'U', 'Vbg', 'Qx', 'f2gpi',
+                          'pmodes', 'kdi2')
+
     def _diagnostic_params(self):
-        """The per-member constant block of qgk_diag_set_params."""
+        """The per-member constant block of qgk_diag_set_params, rebuilt only
+        when one of the model attributes it is made of changes."""
+        key = tuple(np.asarray(v).tobytes() if v is not None else None
+                    for v in (getattr(self, a, None) for a in self._DIAG_PARAM_INPUTS))
+        cached = self.__dict__.get('_diag_param_cache')
+        if cached is not None and cached[0] == key:
+            return cached[1]
         nz = self.nz
-        return param_block(nz, self.Hi, self.H, S=getattr(self, 'S', None),
-                           del1=getattr(self, 'del1', 0.0), del2=getattr(self, 'del2', 0.0),
-                           rd=getattr(self, 'rd', None),
-                           U=float(getattr(self, 'U', 0.0)) if nz == 2 else 0.0,
-                           Vbg=getattr(self, 'Vbg', None), Qx=getattr(self, 'Qx', None),
-                           f2gpi=getattr(self, 'f2gpi', None),
-                           pmodes=getattr(self, 'pmodes', None), kdi2=getattr(self, 'kdi2', None))
+        block = param_block(nz, self.Hi, self.H, S=getattr(self, 'S', None),
+                            del1=getattr(self, 'del1', 0.0), del2=getattr(self, 'del2', 0.0),
+                            rd=getattr(self, 'rd', None),
+                            U=float(getattr(self, 'U', 0.0)) if nz == 2 else 0.0,
+                            Vbg=getattr(self, 'Vbg', None), Qx=getattr(self, 'Qx', None),
+                            f2gpi=getattr(self, 'f2gpi', None),
+                            pmodes=getattr(self, 'pmodes', None),
+                            kdi2=getattr(self, 'kdi2', None))
+        self.__dict__['_diag_param_cache'] = (key, block)
+        return block
 
     def _device_accumulate(self, names, instant):
         self._flush()
