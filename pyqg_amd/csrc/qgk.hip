@@ -1213,7 +1213,8 @@ __global__ void k_unpack_forcing(const float* __restrict__ net, const double* __
 // Diagnostics (pyqg/model.py:644-736, qg_diagnostics.py:6-70,
 // qg_model.py:213-263, layered_model.py:242-301)
 // ==========================================================================
-constexpr int kDiagNzMax = 16;
+constexpr int kDiagNzSmall = 16;  // per-point arrays of the common case
+constexpr int kDiagNzMax = 64;
 
 // Offsets into the per-member diagnostic parameter block (doubles):
 // Hi[nz], H, S[nz][nz], del1, del2, rd^-2, U1-U2, Vbg[nz], Qx[nz],
@@ -1275,6 +1276,7 @@ __device__ __forceinline__ double abs2(cd a) { return a.x * a.x + a.y * a.y; }
 
 // Every requested spectral diagnostic at every (member, l, k), added into its
 // accumulator; the per-point APEgen terms go to pts for a per-member sum.
+template <int NZMAX>
 __global__ void k_diag_spectral(DiagArgs A, const MemberParams* __restrict__ mp) {
   const size_t plane = (size_t)A.nl * A.ldk;
   const size_t total = (size_t)A.M * plane;
@@ -1292,7 +1294,7 @@ __global__ void k_diag_spectral(DiagArgs A, const MemberParams* __restrict__ mp)
     const MemberParams& P = mp[m];
     const double* pr = A.prm + (size_t)m * A.plen;
     const size_t base = (size_t)m * nz * plane + pt;
-    cd qh[kDiagNzMax], ph[kDiagNzMax];
+    cd qh[NZMAX], ph[NZMAX];
     for (int z = 0; z < nz; ++z) {
       qh[z] = A.qh[base + z * plane];
       ph[z] = A.ph[base + z * plane];
@@ -1330,7 +1332,7 @@ __global__ void k_diag_spectral(DiagArgs A, const MemberParams* __restrict__ mp)
                             A.slot[QGK_D_PARAMSPEC_APEFLUX] >= 0 ||
                             A.slot[QGK_D_PARAMSPEC_KEFLUX] >= 0;
     if (want_param) {
-      cd dp[kDiagNzMax];
+      cd dp[NZMAX];
       for (int z = 0; z < nz; ++z) {
         const size_t o = base + z * plane;
         cd v = {0.0, 0.0};
@@ -1347,7 +1349,7 @@ __global__ void k_diag_spectral(DiagArgs A, const MemberParams* __restrict__ mp)
         dp[z] = v;
       }
       double ps = 0.0, es = 0.0, ape = 0.0, ke = 0.0;
-      cd adp[kDiagNzMax];
+      cd adp[NZMAX];
       const bool want_a = A.slot[QGK_D_PARAMSPEC_APEFLUX] >= 0 ||
                           A.slot[QGK_D_PARAMSPEC_KEFLUX] >= 0;
       if (want_a)
@@ -3484,7 +3486,12 @@ int qgk_diag_accumulate(qgk_bucket* b, unsigned int mask, int instant, double dt
   for (int d = 0; d < QGK_NDIAG; ++d)
     A.slot[d] = (!diag_is_scalar(d) && (mask >> d & 1u)) ? b->dslot[d] : -1;
   const size_t total = (size_t)b->M * plane;
-  hipLaunchKernelGGL(k_diag_spectral, dim3(grid_1d(total)), dim3(256), 0, b->stream, A, b->dp);
+  if (b->nz <= kDiagNzSmall)
+    hipLaunchKernelGGL(k_diag_spectral<kDiagNzSmall>, dim3(grid_1d(total)), dim3(256), 0,
+                       b->stream, A, b->dp);
+  else
+    hipLaunchKernelGGL(k_diag_spectral<kDiagNzMax>, dim3(grid_1d(total)), dim3(256), 0,
+                       b->stream, A, b->dp);
   HIPC(hipGetLastError());
   const bool eke = (mask >> QGK_D_EKE & 1u) || (mask >> QGK_D_EKEDISS & 1u);
   if (eke) {

@@ -11,7 +11,7 @@
 import numpy as np
 import pytest
 
-from conftest import QG_PHYS, golden
+from conftest import QG_PHYS, golden, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -228,3 +228,32 @@ def test_ensemble_diagnostics_per_member(pq):
             assert close(ens.get_diagnostic(n, member=i), acc.get(n), 1e-9), (n, i)
     mean = ens.get_diagnostic("KEspec", mean=True)[0]
     assert close(mean, np.mean([acc.get("KEspec") for _, acc in orc], axis=0), 1e-9)
+
+
+@pytest.mark.parametrize("nz", [16, 17])
+def test_many_layers_run_and_diagnostics_match_oracle(pq, nz):
+    """nz = 16 is the last layer count of the column kernel that keeps ph in
+    its own buffer, nz = 17 runs the kernel for any nz (and the diagnostics'
+    larger per-point arrays): fused stretches plus accumulation steps against
+    the oracle stepping every step."""
+    from oracle.qg_oracle import OracleLayered
+    from oracle.qg_diagnostics import Accumulator, step_with_diagnostics
+    from pyqg_amd.constants import exponential_stratification
+    print(f"layered nz={nz} run + diagnostics", flush=True)
+    dt = 3600.0
+    H, rho, U = exponential_stratification(nz=nz)
+    kw = dict(nz=nz, nx=32, H=H, rho=rho, U=U, V=np.zeros(nz), L=1e6, beta=1.5e-11,
+              rek=5.787e-7, dt=dt, f=0.0001236812857687059, g=9.81)
+    q0 = 1e-7 * np.random.default_rng(nz).standard_normal((nz, 32, 32))
+    m = pq.LayeredModel(log_level=0, tmax=24 * dt, tavestart=12 * dt, taveint=4 * dt, **kw)
+    m.q = q0
+    m.run()
+    o = OracleLayered(**kw)
+    o.q = q0
+    acc = Accumulator(o, tavestart=12 * dt, taveint=4 * dt)
+    while o.t < 24 * dt:
+        step_with_diagnostics(o, acc)
+    assert rel_l2(m.q, o.q) < 1e-10
+    assert m.diagnostics["KEspec"]["count"] == acc.count > 0
+    for n in acc.active:
+        assert close(m.get_diagnostic(n), acc.get(n), 1e-9), n
