@@ -381,3 +381,79 @@ def test_run_with_snapshots_matches_oracle(pq):
     for (_, _, q, ph), (_, _, qo, pho) in zip(got, want):
         assert rel_l2(q, qo) < 1e-12 and rel_l2(ph, pho) < 1e-12
     assert m.t == o.t and m.tc == o.tc
+
+
+# ------------------------------------------------------- rectangular grids
+def _rect_constants(nz, ny, nx, L=1e6):
+    """Wavenumbers, inversion and background of an nz-layer channel on an
+    ny x nx grid of square cells.  The model classes build ll from nx
+    (model.py:461-462) and are square-only; the kernel sizes every array from
+    nz, ny and nx (kernel.pyx:120-206), so a caller with its own constants can
+    step rectangular grids.  a = (S - kappa^2 I)^-1 per wavenumber, zero at
+    kappa = 0, for the stretching S of nz layers with interface couplings F."""
+    W = L * ny / nx
+    kk = 2 * np.pi / L * np.arange(nx // 2 + 1)
+    ll = 2 * np.pi / W * np.fft.fftfreq(ny, 1.0 / ny)
+    k2l2 = kk[np.newaxis, :] ** 2 + ll[:, np.newaxis] ** 2
+    S = np.zeros((nz, nz))
+    if nz == 1:
+        S[0, 0] = -(1 / 15e3) ** 2
+    for i, F in enumerate([4.4e-9, 2.1e-9][:nz - 1]):
+        S[i, i] -= F
+        S[i, i + 1] += F
+        S[i + 1, i] += 0.6 * F
+        S[i + 1, i + 1] -= 0.6 * F
+    M = S[:, :, np.newaxis, np.newaxis] - k2l2 * np.eye(nz)[:, :, np.newaxis, np.newaxis]
+    M[:, :, k2l2 == 0] = np.eye(nz)[:, :, np.newaxis]   # S is singular at kappa = 0
+    a = np.linalg.inv(M.transpose(2, 3, 0, 1)).transpose(2, 3, 0, 1)
+    a[:, :, k2l2 == 0] = 0.0
+    dx = L / nx
+    wvx = np.sqrt(k2l2) * dx
+    filtr = np.where(wvx <= 0.65 * np.pi, 1.0, np.exp(-23.6 * (wvx - 0.65 * np.pi) ** 4))
+    Ubg = np.linspace(0.025, 0.0, nz)
+    Qy = 1.5e-11 - S @ Ubg
+    return dict(kk=kk, ll=ll, a=a, filtr=filtr, Ubg=Ubg, Qy=Qy)
+
+
+@pytest.mark.parametrize("nz,ny,nx", [(2, 64, 128), (2, 128, 32), (2, 96, 64), (1, 32, 48),
+                                      (3, 64, 32)])
+def test_rectangular_kernel_matches_oracle_kernel(pq, nz, ny, nx):
+    """The bare kernel on an ny != nx grid against the oracle's kernel with the
+    same constants: 10 steps method by method (every intermediate field), then
+    30 fused steps (the row transforms at length nx, the column transforms at
+    length ny)."""
+    from oracle.qg_oracle import OracleKernel
+    print(f"rectangular kernel nz={nz} ny={ny} nx={nx}", flush=True)
+    k = pq.PseudoSpectralKernel(nz, ny, nx)
+    o = OracleKernel(nz, ny, nx)
+    c = _rect_constants(nz, ny, nx)
+    q0 = 1e-6 * np.random.default_rng(ny + nx + nz).standard_normal((nz, ny, nx))
+    for m in (k, o):
+        m.kk = c["kk"]
+        m.ll = c["ll"]
+        m.a = c["a"]
+        m.filtr = c["filtr"]
+        m.Ubg = c["Ubg"]
+        m.Qy = c["Qy"]
+        m.rek = 5.787e-7
+        m.dt = 3600.0
+        m.q = q0
+    np.testing.assert_array_equal(k._k2l2, o._k2l2)
+    for _ in range(10):
+        for m in (k, o):
+            m._invert()
+            m._do_advection()
+            m._do_friction()
+            m._forward_timestep()
+    for f in ("q", "qh", "ph", "u", "v", "uq", "vq", "uqh", "vqh", "dqhdt", "dqhdt_p"):
+        assert rel_l2(getattr(k, f), getattr(o, "_" + f)) < 1e-12, f
+    k._fused_steps(30)
+    for _ in range(30):
+        o._invert()
+        o._do_advection()
+        o._do_friction()
+        o._forward_timestep()
+    assert k.tc == o.tc == 40 and k.t == o.t
+    assert rel_l2(k.q, o._q) < 1e-11
+    assert rel_l2(k.qh, o._qh) < 1e-11
+    assert rel_l2(k.ph, o._ph) < 1e-11

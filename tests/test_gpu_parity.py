@@ -203,6 +203,25 @@ def test_fused_radix_variants(pq, erow, ecol):
     assert rel_l2(lm.q, ol.q) < TOL_TRAJ
 
 
+@pytest.mark.parametrize("n", [128, 256, 1024, 384, 768])
+def test_single_member_default_radix(pq, n):
+    """A one-member model steps its fused stretches with radix 4 (radix 6 for
+    3 * 2^a): the large power-of-two and mixed-radix grids against the oracle."""
+    print(f"single-member fused n={n}", flush=True)
+    nsteps = 10
+    m = pq.QGModel(nx=n, dt=3600., tmax=nsteps * 3600., log_level=0, **QG_PHYS)
+    assert m._bucket.variant() == (4, 4)
+    o = OracleQG(nx=n, dt=3600., **QG_PHYS)
+    q0 = seeded_q(2, n, 60)
+    m.q = q0
+    o.q = q0
+    m.run()
+    for _ in range(nsteps):
+        o._step_forward()
+    assert m.tc == nsteps
+    assert rel_l2(m.q, o.q) < 1e-12
+
+
 def test_autotune_keeps_state_and_parity(pq, tmp_path):
     ens = pq.Ensemble.qg(np.array([64, 64, 128]), dt=3600.)
     q0 = [seeded_q(2, n, 60 + i) for i, n in enumerate([64, 64, 128])]
