@@ -268,6 +268,127 @@ __global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_r2c(
 }
 
 // ==========================================================================
+// One-launch 2-D transforms of small square planes (n <= 64).  A small model
+// steps through dozens of short launches per step, each bound by launch
+// latency rather than by its work, so here one workgroup transforms a whole
+// plane: the row and column passes of k_row_r2c / k_row_c2r and k_col_fft (the
+// same Stockham passes, radix and twiddles) with the plane exchanged through
+// LDS instead of through global memory.  One LDS region of N x (N/2 + 1)
+// values serves in turn as the row transforms' exchange, the half-spectrum
+// tile and the column transforms' exchange: every transform's first LDS write
+// follows a barrier, after the reads of the previous use.  A launch carries up
+// to kPlaneJobs fields of nplanes planes each.
+// ==========================================================================
+constexpr int kPlaneMax = 64;
+constexpr int kPlaneJobs = 4;
+
+struct PlaneJobs {
+  const void* in[kPlaneJobs];
+  void* out[kPlaneJobs];
+};
+
+// T threads per transform; N/2 + 1 groups of them: the columns of the half
+// spectrum in the column pass, the N/2 row pairs (and one idle group) in the
+// row pass.
+template <int N>
+struct PlaneGeo {
+  static constexpr int E = RadixFor<N, 4>::value;
+  static constexpr int T = N / E;
+  static constexpr int NK = N / 2 + 1;
+  static constexpr int THREADS = T * NK;
+};
+
+// rfftn of real planes [N][N] -> half spectra [N][ldk] (padding columns zeroed)
+template <int N>
+__global__ __launch_bounds__(PlaneGeo<N>::THREADS) void k_plane_r2c(
+    PlaneJobs jobs, int nplanes, int ldk, const cd* __restrict__ twx,
+    const cd* __restrict__ twy) {
+  using G = PlaneGeo<N>;
+  using RS = qgk::RowSwizzled<N>;
+  __shared__ cd lds[N * G::NK];
+  const int f = blockIdx.x / nplanes, p = blockIdx.x - f * nplanes;
+  const double* in = static_cast<const double*>(jobs.in[f]) + (size_t)p * N * N;
+  cd* out = static_cast<cd*>(jobs.out[f]) + (size_t)p * N * ldk;
+  const int t = threadIdx.x % G::T;
+  const int g = threadIdx.x / G::T;
+  // row pass: group g carries rows 2g and 2g + 1 as z = x_2g + i x_2g+1
+  const int j0 = g < N / 2 ? 2 * g : 0;
+  cd x[G::E];
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) {
+    const int q = t + s * G::T;
+    x[s] = {in[j0 * N + q], in[(j0 + 1) * N + q]};
+  }
+  qgk::fft_block_rows<N, G::E, -1>(x, lds, t, g, twx);
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) lds[RS::at(t, s * G::T, g)] = x[s];
+  __syncthreads();
+  // column pass: group g is column k = g; row r is split out of pair r / 2
+  const int k = g, km = k == 0 ? 0 : N - k;
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) {
+    const int r = t + s * G::T;
+    const cd A = lds[RS::at(k, r >> 1)], B = lds[RS::at(km, r >> 1)];
+    x[s] = (r & 1) ? cd{0.5 * (A.y + B.y), -0.5 * (A.x - B.x)}
+                   : cd{0.5 * (A.x + B.x), 0.5 * (A.y - B.y)};
+  }
+  qgk::fft_block<N, G::E, -1, G::NK>(x, lds, t, k, twy);
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) out[(size_t)(t + s * G::T) * ldk + k] = x[s];
+  if (G::NK + g < ldk) {
+#pragma unroll
+    for (int s = 0; s < G::E; ++s)
+      out[(size_t)(t + s * G::T) * ldk + G::NK + g] = cd{0.0, 0.0};
+  }
+}
+
+// irfftn of half spectra [N][ldk] -> real planes [N][N], scaled by `scale`
+// (numpy c2r: the imaginary parts of bins kx = 0 and N/2 are dropped)
+template <int N>
+__global__ __launch_bounds__(PlaneGeo<N>::THREADS) void k_plane_c2r(
+    PlaneJobs jobs, int nplanes, int ldk, double scale, const cd* __restrict__ twx,
+    const cd* __restrict__ twy) {
+  using G = PlaneGeo<N>;
+  __shared__ cd lds[N * G::NK];
+  const int f = blockIdx.x / nplanes, p = blockIdx.x - f * nplanes;
+  const cd* in = static_cast<const cd*>(jobs.in[f]) + (size_t)p * N * ldk;
+  double* out = static_cast<double*>(jobs.out[f]) + (size_t)p * N * N;
+  const int t = threadIdx.x % G::T;
+  const int g = threadIdx.x / G::T;
+  // column pass: group g is column k = g
+  cd x[G::E];
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) x[s] = in[(size_t)(t + s * G::T) * ldk + g];
+  qgk::fft_block<N, G::E, +1, G::NK>(x, lds, t, g, twy);
+  __syncthreads();
+  // the y-transformed half spectra, row r at lds + r * NK
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) lds[(t + s * G::T) * G::NK + g] = x[s];
+  __syncthreads();
+  // row pass: group g carries rows 2g and 2g + 1 (group N/2 idles)
+  const bool ok = g < N / 2;
+  const int j0 = ok ? 2 * g : 0;
+  const cd* r0 = lds + j0 * G::NK;
+  const cd* r1 = r0 + G::NK;
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) {
+    const int q = t + s * G::T;
+    const cd a = hext<N>(r0, q), c = hext<N>(r1, q);
+    x[s] = {a.x - c.y, a.y + c.x};
+  }
+  qgk::fft_block_rows<N, G::E, +1>(x, lds, t, g, twx);
+  if (ok) {
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) {
+      const int q = t + s * G::T;
+      out[j0 * N + q] = x[s].x * scale;
+      out[(j0 + 1) * N + q] = x[s].y * scale;
+    }
+  }
+}
+
+// ==========================================================================
 // Pointwise phases (drop-in path).  Contraction is disabled so every product
 // and sum rounds exactly as the reference's loops do.
 // ==========================================================================
@@ -2082,19 +2203,108 @@ int launch_row_r2c(qgk_bucket* b, const double* in, cd* out, int nplanes) {
   return 0;
 }
 
-// 2-D transforms of nplanes planes: rfftn / irfftn over the last two axes
+// one-launch plane transforms: square grids of n <= kPlaneMax
+bool plane_ok(const qgk_bucket* b) { return b->nx == b->ny && b->nx <= kPlaneMax; }
+
+struct PlaneR2CL {
+  template <int N>
+  static void go(qgk_bucket* b, const PlaneJobs& j, int nf, int nplanes) {
+    hipLaunchKernelGGL((k_plane_r2c<N>), dim3(nf * nplanes), dim3(PlaneGeo<N>::THREADS), 0,
+                       b->stream, j, nplanes, b->ldk, b->tw_x, b->tw_y);
+  }
+};
+
+struct PlaneC2RL {
+  template <int N>
+  static void go(qgk_bucket* b, const PlaneJobs& j, int nf, int nplanes) {
+    const double scale = 1.0 / ((double)N * (double)N);
+    hipLaunchKernelGGL((k_plane_c2r<N>), dim3(nf * nplanes), dim3(PlaneGeo<N>::THREADS), 0,
+                       b->stream, j, nplanes, b->ldk, scale, b->tw_x, b->tw_y);
+  }
+};
+
+template <class L>
+int launch_plane(qgk_bucket* b, const PlaneJobs& j, int nf, int nplanes) {
+  switch (b->nx) {
+    case 8: L::template go<8>(b, j, nf, nplanes); break;
+    case 16: L::template go<16>(b, j, nf, nplanes); break;
+    case 24: L::template go<24>(b, j, nf, nplanes); break;
+    case 32: L::template go<32>(b, j, nf, nplanes); break;
+    case 48: L::template go<48>(b, j, nf, nplanes); break;
+    case 64: L::template go<64>(b, j, nf, nplanes); break;
+    default: return fail(QGK_E_ARG, "no one-launch transform for n=%d", b->nx);
+  }
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+// true when some field's output bytes overlap some field's input bytes
+bool jobs_overlap(int nf, const void* const* in, size_t in_bytes, void* const* out,
+                  size_t out_bytes) {
+  for (int i = 0; i < nf; ++i)
+    for (int k = 0; k < nf; ++k) {
+      const char* a = static_cast<const char*>(out[i]);
+      const char* c = static_cast<const char*>(in[k]);
+      if (a < c + in_bytes && c < a + out_bytes) return true;
+    }
+  return false;
+}
+
+// 2-D transforms of nf fields of nplanes planes each: rfftn / irfftn over the
+// last two axes.  Small square grids take one launch for all fields; the rest
+// a row launch and a column launch per field.
+int r2c2d_n(qgk_bucket* b, int nf, const double* const* in, cd* const* out, int nplanes) {
+  if (nf < 1 || nf > kPlaneJobs) return fail(QGK_E_ARG, "1 to %d fields per transform", kPlaneJobs);
+  const size_t ib = (size_t)nplanes * b->nplane_r * sizeof(double);
+  const size_t ob = (size_t)nplanes * b->nplane_c * sizeof(cd);
+  if (plane_ok(b) &&
+      !jobs_overlap(nf, reinterpret_cast<const void* const*>(in), ib,
+                    reinterpret_cast<void* const*>(out), ob)) {
+    PlaneJobs j{};
+    for (int i = 0; i < nf; ++i) {
+      j.in[i] = in[i];
+      j.out[i] = out[i];
+    }
+    return launch_plane<PlaneR2CL>(b, j, nf, nplanes);
+  }
+  for (int i = 0; i < nf; ++i) {
+    int rc = launch_row_r2c(b, in[i], out[i], nplanes);
+    if (rc) return rc;
+    if ((rc = launch_col_fft<-1>(b, out[i], out[i], nplanes))) return rc;
+  }
+  return 0;
+}
+
+int c2r2d_n(qgk_bucket* b, int nf, const cd* const* in, double* const* out, int nplanes) {
+  if (nf < 1 || nf > kPlaneJobs) return fail(QGK_E_ARG, "1 to %d fields per transform", kPlaneJobs);
+  const size_t ib = (size_t)nplanes * b->nplane_c * sizeof(cd);
+  const size_t ob = (size_t)nplanes * b->nplane_r * sizeof(double);
+  if (plane_ok(b) &&
+      !jobs_overlap(nf, reinterpret_cast<const void* const*>(in), ib,
+                    reinterpret_cast<void* const*>(out), ob)) {
+    PlaneJobs j{};
+    for (int i = 0; i < nf; ++i) {
+      j.in[i] = in[i];
+      j.out[i] = out[i];
+    }
+    return launch_plane<PlaneC2RL>(b, j, nf, nplanes);
+  }
+  // y pass into the spectral scratch, x pass into out
+  if (nplanes > b->M * b->nz) return fail(QGK_E_ARG, "too many planes for scratch");
+  for (int i = 0; i < nf; ++i) {
+    int rc = launch_col_fft<+1>(b, in[i], b->ctmp, nplanes);
+    if (rc) return rc;
+    if ((rc = launch_row_c2r(b, b->ctmp, out[i], nplanes))) return rc;
+  }
+  return 0;
+}
+
 int r2c2d(qgk_bucket* b, const double* in, cd* out, int nplanes) {
-  int rc = launch_row_r2c(b, in, out, nplanes);
-  if (rc) return rc;
-  return launch_col_fft<-1>(b, out, out, nplanes);
+  return r2c2d_n(b, 1, &in, &out, nplanes);
 }
 
 int c2r2d(qgk_bucket* b, const cd* in, double* out, int nplanes) {
-  // y pass into the spectral scratch, x pass into out
-  if (nplanes > b->M * b->nz) return fail(QGK_E_ARG, "too many planes for scratch");
-  int rc = launch_col_fft<+1>(b, in, b->ctmp, nplanes);
-  if (rc) return rc;
-  return launch_row_c2r(b, b->ctmp, out, nplanes);
+  return c2r2d_n(b, 1, &in, &out, nplanes);
 }
 
 // Calls L::go<N, E>(args...) for a transform length n (2^a or 3 * 2^a) and
@@ -2552,8 +2762,11 @@ int qgk_invert(qgk_bucket* b) {
                      b->nl, b->ldk, b->M);
   HIPC(hipGetLastError());
   const int np = b->M * b->nz;
-  if ((rc = c2r2d(b, b->field_c[QGK_UH], b->field_r[QGK_U], np))) return rc;
-  if ((rc = c2r2d(b, b->field_c[QGK_VH], b->field_r[QGK_V], np))) return rc;
+  {
+    const cd* in[2] = {b->field_c[QGK_UH], b->field_c[QGK_VH]};
+    double* out[2] = {b->field_r[QGK_U], b->field_r[QGK_V]};
+    if ((rc = c2r2d_n(b, 2, in, out, np))) return rc;
+  }
   HIPC(stream_done(b));
   return 0;
 }
@@ -2568,8 +2781,11 @@ int qgk_advect(qgk_bucket* b) {
                      b->field_r[QGK_V], b->field_r[QGK_Q], b->field_r[QGK_UQ],
                      b->field_r[QGK_VQ], b->dp, b->nz, b->nplane_r, b->M);
   HIPC(hipGetLastError());
-  if ((rc = r2c2d(b, b->field_r[QGK_UQ], b->field_c[QGK_UQH], np))) return rc;
-  if ((rc = r2c2d(b, b->field_r[QGK_VQ], b->field_c[QGK_VQH], np))) return rc;
+  {
+    const double* in[2] = {b->field_r[QGK_UQ], b->field_r[QGK_VQ]};
+    cd* out[2] = {b->field_c[QGK_UQH], b->field_c[QGK_VQH]};
+    if ((rc = r2c2d_n(b, 2, in, out, np))) return rc;
+  }
   const size_t cpts = (size_t)np * b->nplane_c;
   hipLaunchKernelGGL(k_divergence, dim3(grid_1d(cpts)), dim3(256), 0, b->stream,
                      b->field_c[QGK_UQH], b->field_c[QGK_VQH], b->field_c[QGK_PH], D(b), b->dp,
@@ -2621,8 +2837,11 @@ int qgk_apply_uv_param(qgk_bucket* b) {
   int rc = upload_params(b);
   if (rc) return rc;
   const int np = b->M * b->nz;
-  if ((rc = r2c2d(b, b->field_r[QGK_DU], b->field_c[QGK_DUH], np))) return rc;
-  if ((rc = r2c2d(b, b->field_r[QGK_DV], b->field_c[QGK_DVH], np))) return rc;
+  {
+    const double* in[2] = {b->field_r[QGK_DU], b->field_r[QGK_DV]};
+    cd* out[2] = {b->field_c[QGK_DUH], b->field_c[QGK_DVH]};
+    if ((rc = r2c2d_n(b, 2, in, out, np))) return rc;
+  }
   const size_t cpts = (size_t)np * b->nplane_c;
   hipLaunchKernelGGL(k_add_curl, dim3(grid_1d(cpts)), dim3(256), 0, b->stream, D(b),
                      b->field_c[QGK_DUH], b->field_c[QGK_DVH], b->dp, b->nz, b->nl, b->ldk, b->M);
@@ -3341,7 +3560,11 @@ int diag_advect(qgk_bucket* b, const double* x, int nzx, int mode, double sign, 
                      (int)b->nplane_r, mode, diag_plen(b));
   HIPC(hipGetLastError());
   int rc;
-  if ((rc = r2c2d(b, P1, W1, nplanes)) || (rc = r2c2d(b, P2, W2, nplanes))) return rc;
+  {
+    const double* in[2] = {P1, P2};
+    cd* out[2] = {W1, W2};
+    if ((rc = r2c2d_n(b, 2, in, out, nplanes))) return rc;
+  }
   const size_t nc = (size_t)nplanes * b->nplane_c;
   hipLaunchKernelGGL(k_diag_jacobian, dim3(grid_1d(nc)), dim3(256), 0, b->stream, W1, W2,
                      b->dp, J, b->M, nzx, b->nl, b->ldk, sign);
@@ -3650,13 +3873,16 @@ int param_emit(qgk_bucket* b, double* src, double* dst, double* host, double sca
 // R0, R1, R2 = nu Sxx, nu Syy, nu Sxy
 int smag_viscosity(qgk_bucket* b, double cs) {
   const int np = b->M * b->nz;
-  cd *W0 = Wset(b, 0), *W1 = Wset(b, 1), *W2 = Wset(b, 2);
+  cd *W0 = Wset(b, 0), *W1 = Wset(b, 1), *W2 = Wset(b, 2), *W3 = Wset(b, 3), *W4 = Wset(b, 4);
   double *R0 = Rset(b, 0), *R1 = Rset(b, 1), *R2 = Rset(b, 2), *R3 = Rset(b, 3);
+  const double* uv[2] = {b->field_r[QGK_U], b->field_r[QGK_V]};
+  cd* uvh[2] = {W0, W1};
+  const cd* S[3] = {W2, W3, W4};
+  double* Sr[3] = {R0, R1, R2};
   int rc;
-  if ((rc = r2c2d(b, b->field_r[QGK_U], W0, np)) || (rc = r2c2d(b, b->field_r[QGK_V], W1, np)) ||
-      (rc = spec_ikil(b, W0, nullptr, W2, 1.0, 0.0)) || (rc = c2r2d(b, W2, R0, np)) ||
-      (rc = spec_ikil(b, nullptr, W1, W2, 0.0, 1.0)) || (rc = c2r2d(b, W2, R1, np)) ||
-      (rc = spec_ikil(b, W1, W0, W2, 1.0, 1.0)) || (rc = c2r2d(b, W2, R2, np)))
+  if ((rc = r2c2d_n(b, 2, uv, uvh, np)) || (rc = spec_ikil(b, W0, nullptr, W2, 1.0, 0.0)) ||
+      (rc = spec_ikil(b, nullptr, W1, W3, 0.0, 1.0)) ||
+      (rc = spec_ikil(b, W1, W0, W4, 1.0, 1.0)) || (rc = c2r2d_n(b, 3, S, Sr, np)))
     return rc;
   const size_t n = (size_t)np * b->nplane_r;
   hipLaunchKernelGGL(k_smag_nu, dim3(grid_1d(n)), dim3(kGrid), 0, b->stream, R0, R1, R2, R3,
@@ -3685,12 +3911,15 @@ int qgk_param_smagorinsky(qgk_bucket* b, const double* dx, double constant, doub
       (rc = smag_viscosity(b, constant)))
     return rc;
   const int np = b->M * b->nz;
-  cd *W0 = Wset(b, 0), *W1 = Wset(b, 1), *W2 = Wset(b, 2), *W3 = Wset(b, 3);
+  cd *W0 = Wset(b, 0), *W1 = Wset(b, 1), *W2 = Wset(b, 2), *W3 = Wset(b, 3), *W4 = Wset(b, 4);
   double *R0 = Rset(b, 0), *R1 = Rset(b, 1), *R2 = Rset(b, 2);
+  const double* st[3] = {R0, R2, R1};
+  cd* sth[3] = {W0, W1, W2};
+  const cd* duvh[2] = {W3, W4};
+  double* duv[2] = {R0, R1};
   // du = 2 ifft(ik F(nu Sxx) + il F(nu Sxy)), dv = 2 ifft(ik F(nu Sxy) + il F(nu Syy))
-  if ((rc = r2c2d(b, R0, W0, np)) || (rc = r2c2d(b, R2, W1, np)) || (rc = r2c2d(b, R1, W2, np)) ||
-      (rc = spec_ikil(b, W0, W1, W3, 1.0, 1.0)) || (rc = c2r2d(b, W3, R0, np)) ||
-      (rc = spec_ikil(b, W1, W2, W3, 1.0, 1.0)) || (rc = c2r2d(b, W3, R1, np)) ||
+  if ((rc = r2c2d_n(b, 3, st, sth, np)) || (rc = spec_ikil(b, W0, W1, W3, 1.0, 1.0)) ||
+      (rc = spec_ikil(b, W1, W2, W4, 1.0, 1.0)) || (rc = c2r2d_n(b, 2, duvh, duv, np)) ||
       (rc = param_emit(b, R0, b->field_r[QGK_DU], du_host, 2.0, weight, accumulate)) ||
       (rc = param_emit(b, R1, b->field_r[QGK_DV], dv_host, 2.0, weight, accumulate)))
     return rc;
@@ -3718,13 +3947,16 @@ int qgk_param_zb2020(qgk_bucket* b, double constant, double weight, int accumula
       (rc = param_setup(b, nullptr, nullptr, true)))
     return rc;
   const int np = b->M * b->nz;
-  cd *W0 = Wset(b, 0), *W1 = Wset(b, 1), *W2 = Wset(b, 2), *W3 = Wset(b, 3);
+  cd *W0 = Wset(b, 0), *W1 = Wset(b, 1), *W2 = Wset(b, 2), *W3 = Wset(b, 3), *W4 = Wset(b, 4);
   double *R0 = Rset(b, 0), *R1 = Rset(b, 1), *R2 = Rset(b, 2), *R3 = Rset(b, 3);
+  const double* uv[2] = {b->field_r[QGK_U], b->field_r[QGK_V]};
+  cd* uvh[2] = {W0, W1};
+  const cd* gh[3] = {W2, W3, W4};
+  double* g3[3] = {R0, R1, R2};
   // vx, vy, ux, uy (parameterizations.py:264-269)
-  if ((rc = r2c2d(b, b->field_r[QGK_U], W0, np)) || (rc = r2c2d(b, b->field_r[QGK_V], W1, np)) ||
-      (rc = spec_ikil(b, W1, nullptr, W2, 1.0, 0.0)) || (rc = c2r2d(b, W2, R0, np)) ||
-      (rc = spec_ikil(b, nullptr, W1, W2, 0.0, 1.0)) || (rc = c2r2d(b, W2, R1, np)) ||
-      (rc = spec_ikil(b, W0, nullptr, W2, 1.0, 0.0)) || (rc = c2r2d(b, W2, R2, np)) ||
+  if ((rc = r2c2d_n(b, 2, uv, uvh, np)) || (rc = spec_ikil(b, W1, nullptr, W2, 1.0, 0.0)) ||
+      (rc = spec_ikil(b, nullptr, W1, W3, 0.0, 1.0)) ||
+      (rc = spec_ikil(b, W0, nullptr, W4, 1.0, 0.0)) || (rc = c2r2d_n(b, 3, gh, g3, np)) ||
       (rc = spec_ikil(b, nullptr, W0, W2, 0.0, 1.0)) || (rc = c2r2d(b, W2, R3, np)))
     return rc;
   const size_t nr = (size_t)np * b->nplane_r, nc = (size_t)np * b->nplane_c;
@@ -3732,17 +3964,22 @@ int qgk_param_zb2020(qgk_bucket* b, double constant, double weight, int accumula
                      nr);
   HIPC(hipGetLastError());
   // W0 = F(vort stretch), W1 = F(vort shear), W2 = F(sum of squares)
-  if ((rc = r2c2d(b, R0, W0, np)) || (rc = r2c2d(b, R1, W1, np)) || (rc = r2c2d(b, R2, W2, np)))
-    return rc;
+  {
+    const double* pr[3] = {R0, R1, R2};
+    cd* ph3[3] = {W0, W1, W2};
+    if ((rc = r2c2d_n(b, 3, pr, ph3, np))) return rc;
+  }
   // du = k ifft(ik (S/2 - shear) + il stretch); dv = k ifft(il (S/2 + shear) + ik stretch)
   hipLaunchKernelGGL(k_caxpby, dim3(grid_1d(nc)), dim3(kGrid), 0, b->stream, W2, W1, W3, 0.5,
                      -1.0, nc);
   HIPC(hipGetLastError());
-  if ((rc = spec_ikil(b, W3, W0, W3, 1.0, 1.0)) || (rc = c2r2d(b, W3, R0, np))) return rc;
-  hipLaunchKernelGGL(k_caxpby, dim3(grid_1d(nc)), dim3(kGrid), 0, b->stream, W2, W1, W3, 0.5, 1.0,
+  if ((rc = spec_ikil(b, W3, W0, W3, 1.0, 1.0))) return rc;
+  hipLaunchKernelGGL(k_caxpby, dim3(grid_1d(nc)), dim3(kGrid), 0, b->stream, W2, W1, W4, 0.5, 1.0,
                      nc);
   HIPC(hipGetLastError());
-  if ((rc = spec_ikil(b, W0, W3, W3, 1.0, 1.0)) || (rc = c2r2d(b, W3, R1, np)) ||
+  const cd* duvh[2] = {W3, W4};
+  double* duv[2] = {R0, R1};
+  if ((rc = spec_ikil(b, W0, W4, W4, 1.0, 1.0)) || (rc = c2r2d_n(b, 2, duvh, duv, np)) ||
       (rc = param_emit(b, R0, b->field_r[QGK_DU], du_host, constant, weight, accumulate)) ||
       (rc = param_emit(b, R1, b->field_r[QGK_DV], dv_host, constant, weight, accumulate)))
     return rc;
@@ -3763,8 +4000,9 @@ int qgk_param_backscatter(qgk_bucket* b, const double* dx, const double* Hi,
   double *R0 = Rset(b, 0), *R1 = Rset(b, 1), *R2 = Rset(b, 2), *R3 = Rset(b, 3);
   const size_t nr = (size_t)np * b->nplane_r;
   // psi = ifft(ph), lap_lap_psi = ifft(lap^2 ph) (parameterizations.py:226-228)
-  if ((rc = c2r2d(b, b->field_c[QGK_PH], R0, np)) ||
-      (rc = spec_wv2(b, b->field_c[QGK_PH], W0, 2, 1.0)) || (rc = c2r2d(b, W0, R1, np)))
+  const cd* psih[2] = {b->field_c[QGK_PH], W0};
+  double* psi[2] = {R0, R1};
+  if ((rc = spec_wv2(b, b->field_c[QGK_PH], W0, 2, 1.0)) || (rc = c2r2d_n(b, 2, psih, psi, np)))
     return rc;
   // dissipation = -ifft(lap fft(lap_lap_psi dx^2 nu))
   hipLaunchKernelGGL(k_bs_product, dim3(grid_1d(nr)), dim3(kGrid), 0, b->stream, R1, R3, b->pdx,
