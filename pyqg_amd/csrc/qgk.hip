@@ -570,6 +570,17 @@ __device__ __forceinline__ void st_stream(cd* p, cd v) {
   __builtin_nontemporal_store(v.y, &p->y);
 }
 
+// The twiddle table of a length-N transform copied into LDS at block start.
+// Every Stockham pass after the first reads one twiddle right after a
+// barrier; from LDS that read costs an LDS round trip instead of an L2 one on
+// the pass's critical path.  No barrier is needed here: the first pass reads
+// no twiddle and each later pass begins with a barrier.
+template <int N>
+__device__ __forceinline__ const cd* stage_twiddles(cd* twl, const cd* __restrict__ tw) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) twl[i] = tw[i];
+  return twl;
+}
+
 template <int NX, int T, int RP>
 __device__ __forceinline__ void split_store_t(const cd* lds, int t, int rp, bool ok, cd* o0,
                                               cd* o1) {
@@ -652,6 +663,8 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP), row_flux_waves<
   using G = FGeo<NX, E>;
   constexpr int NK = ldk_of(NX);  // row stride
   __shared__ cd lds[NX * G::RP];
+  __shared__ cd twl[NX];
+  tw = stage_twiddles<NX>(twl, tw);
   const int t = threadIdx.x % G::T;
   const int rp = threadIdx.x / G::T;
   const int jp = blockIdx.x * G::RP + rp;
@@ -1169,6 +1182,8 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
 #pragma clang fp contract(off)
   using G = FGeo<NY, E>;
   __shared__ cd lds[NY * G::TK];
+  __shared__ cd twl[NY];
+  tw = stage_twiddles<NY>(twl, tw);
   const int c = threadIdx.x % G::TK;
   const int t = threadIdx.x / G::TK;
   const int k = blockIdx.x * G::TK + c;
