@@ -325,6 +325,14 @@ class Model(DiagnosticsMixin, PseudoSpectralKernel):
         self.radii[0] = np.sqrt(self.g * self.H) / np.abs(self.f)
         self.radii[1:] = 1. / np.sqrt(self.kdi2[1:])
 
+    def to_dataset(self):
+        """The state, constants and accumulated diagnostics as an
+        xarray.Dataset (model.py:840-848); ImportError without xarray.
+        `pyqg_amd.dataset.dataset_parts(model)` gives the same contents as
+        plain (dims, data, attrs) tuples."""
+        from .dataset import model_to_dataset
+        return model_to_dataset(self)
+
     @property
     def parameterization(self):
         if self.q_parameterization and self.uv_parameterization:
