@@ -343,23 +343,76 @@ __global__ __launch_bounds__(PlaneGeo<N>::THREADS) void k_plane_r2c(
   }
 }
 
+// Spectral operators applied on the way into an inverse transform, per field:
+//   kSpecPlain: X itself;
+//   kSpecIkil:  cx (i k) X + cy (i l) Y, X or Y possibly absent, the products
+//               and sums in k_spec_ikil's order;
+//   kSpecWv2:   cx (kappa^2)^power X, as k_spec_wv2 forms it (cx the sign).
+enum { kSpecPlain = 0, kSpecIkil = 1, kSpecWv2 = 2 };
+struct SpecJobs {
+  const cd* X[kPlaneJobs];
+  const cd* Y[kPlaneJobs];
+  double cx[kPlaneJobs], cy[kPlaneJobs];
+  int kind[kPlaneJobs], power[kPlaneJobs];
+};
+
 // irfftn of half spectra [N][ldk] -> real planes [N][N], scaled by `scale`
-// (numpy c2r: the imaginary parts of bins kx = 0 and N/2 are dropped)
-template <int N>
+// (numpy c2r: the imaginary parts of bins kx = 0 and N/2 are dropped).  With
+// SPEC the half spectra are the operator results of `dj` (planes of member
+// p / nz) instead of jobs.in.
+template <int N, bool SPEC>
 __global__ __launch_bounds__(PlaneGeo<N>::THREADS) void k_plane_c2r(
-    PlaneJobs jobs, int nplanes, int ldk, double scale, const cd* __restrict__ twx,
-    const cd* __restrict__ twy) {
+    PlaneJobs jobs, SpecJobs dj, const MemberParams* __restrict__ mp, int nz, int nplanes,
+    int ldk, double scale, const cd* __restrict__ twx, const cd* __restrict__ twy) {
   using G = PlaneGeo<N>;
   __shared__ cd lds[N * G::NK];
   const int f = blockIdx.x / nplanes, p = blockIdx.x - f * nplanes;
-  const cd* in = static_cast<const cd*>(jobs.in[f]) + (size_t)p * N * ldk;
   double* out = static_cast<double*>(jobs.out[f]) + (size_t)p * N * N;
   const int t = threadIdx.x % G::T;
   const int g = threadIdx.x / G::T;
   // column pass: group g is column k = g
   cd x[G::E];
+  if constexpr (SPEC) {
+    const size_t base = (size_t)p * N * ldk;
+    const MemberParams& P = mp[p / nz];
+    const cd* X = dj.X[f];
+    const cd* Y = dj.Y[f];
+    const double cx = dj.cx[f], cy = dj.cy[f];
+    const int kind = dj.kind[f], power = dj.power[f];
+    const double kk = P.kk[g];
 #pragma unroll
-  for (int s = 0; s < G::E; ++s) x[s] = in[(size_t)(t + s * G::T) * ldk + g];
+    for (int s = 0; s < G::E; ++s) {
+      const int l = t + s * G::T;
+      const size_t pt = (size_t)l * ldk + g;
+      const size_t i = base + pt;
+      cd r = {0.0, 0.0};
+      if (kind == kSpecPlain) {
+        r = X[i];
+      } else if (kind == kSpecIkil) {
+        if (X) {
+          const cd v = X[i];
+          r = {-kk * v.y, kk * v.x};
+          r = {cx * r.x, cx * r.y};
+        }
+        if (Y) {
+          const double ll = P.ll[l];
+          const cd v = Y[i];
+          r = {r.x + cy * (-ll * v.y), r.y + cy * (ll * v.x)};
+        }
+      } else {
+        const double w = P.k2l2[pt];
+        double fw = cx;
+        for (int q = 0; q < power; ++q) fw *= w;
+        const cd v = X[i];
+        r = {fw * v.x, fw * v.y};
+      }
+      x[s] = r;
+    }
+  } else {
+    const cd* in = static_cast<const cd*>(jobs.in[f]) + (size_t)p * N * ldk;
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) x[s] = in[(size_t)(t + s * G::T) * ldk + g];
+  }
   qgk::fft_block<N, G::E, +1, G::NK>(x, lds, t, g, twy);
   __syncthreads();
   // the y-transformed half spectra, row r at lds + r * NK
@@ -2233,20 +2286,31 @@ struct PlaneC2RL {
   template <int N>
   static void go(qgk_bucket* b, const PlaneJobs& j, int nf, int nplanes) {
     const double scale = 1.0 / ((double)N * (double)N);
-    hipLaunchKernelGGL((k_plane_c2r<N>), dim3(nf * nplanes), dim3(PlaneGeo<N>::THREADS), 0,
-                       b->stream, j, nplanes, b->ldk, scale, b->tw_x, b->tw_y);
+    hipLaunchKernelGGL((k_plane_c2r<N, false>), dim3(nf * nplanes), dim3(PlaneGeo<N>::THREADS),
+                       0, b->stream, j, SpecJobs{}, b->dp, b->nz, nplanes, b->ldk, scale,
+                       b->tw_x, b->tw_y);
   }
 };
 
-template <class L>
-int launch_plane(qgk_bucket* b, const PlaneJobs& j, int nf, int nplanes) {
+struct PlaneSpecL {
+  template <int N>
+  static void go(qgk_bucket* b, const PlaneJobs& j, int nf, int nplanes, const SpecJobs& dj) {
+    const double scale = 1.0 / ((double)N * (double)N);
+    hipLaunchKernelGGL((k_plane_c2r<N, true>), dim3(nf * nplanes), dim3(PlaneGeo<N>::THREADS),
+                       0, b->stream, j, dj, b->dp, b->nz, nplanes, b->ldk, scale, b->tw_x,
+                       b->tw_y);
+  }
+};
+
+template <class L, class... A>
+int launch_plane(qgk_bucket* b, const PlaneJobs& j, int nf, int nplanes, A... a) {
   switch (b->nx) {
-    case 8: L::template go<8>(b, j, nf, nplanes); break;
-    case 16: L::template go<16>(b, j, nf, nplanes); break;
-    case 24: L::template go<24>(b, j, nf, nplanes); break;
-    case 32: L::template go<32>(b, j, nf, nplanes); break;
-    case 48: L::template go<48>(b, j, nf, nplanes); break;
-    case 64: L::template go<64>(b, j, nf, nplanes); break;
+    case 8: L::template go<8>(b, j, nf, nplanes, a...); break;
+    case 16: L::template go<16>(b, j, nf, nplanes, a...); break;
+    case 24: L::template go<24>(b, j, nf, nplanes, a...); break;
+    case 32: L::template go<32>(b, j, nf, nplanes, a...); break;
+    case 48: L::template go<48>(b, j, nf, nplanes, a...); break;
+    case 64: L::template go<64>(b, j, nf, nplanes, a...); break;
     default: return fail(QGK_E_ARG, "no one-launch transform for n=%d", b->nx);
   }
   HIPC(hipGetLastError());
@@ -3867,6 +3931,65 @@ int spec_wv2(qgk_bucket* b, const cd* X, cd* out, int power, double sign) {
   return 0;
 }
 
+// Inverse transforms of nf spectral operator results (SpecJobs kinds; M*nz
+// planes each) into out[f]: formed inside the one-launch plane transform where
+// that applies, else by spec_ikil / spec_wv2 into work[f] and the row and
+// column kernels.
+int c2r2d_spec(qgk_bucket* b, int nf, const SpecJobs& ops, double* const* out,
+               cd* const* work) {
+  if (nf < 1 || nf > kPlaneJobs) return fail(QGK_E_ARG, "1 to %d fields per transform", kPlaneJobs);
+  const int np = b->M * b->nz;
+  const size_t ib = (size_t)np * b->nplane_c * sizeof(cd);
+  const size_t ob = (size_t)np * b->nplane_r * sizeof(double);
+  bool plane = plane_ok(b);
+  for (int f = 0; plane && f < nf; ++f)
+    for (int g = 0; g < nf; ++g) {
+      const void* srcs[2] = {ops.X[g], ops.Y[g]};
+      for (const void* src : srcs)
+        if (src && jobs_overlap(1, &src, ib, reinterpret_cast<void* const*>(&out[f]), ob))
+          plane = false;
+    }
+  if (plane) {
+    PlaneJobs j{};
+    for (int f = 0; f < nf; ++f) j.out[f] = out[f];
+    return launch_plane<PlaneSpecL>(b, j, nf, np, ops);
+  }
+  const cd* in[kPlaneJobs];
+  for (int f = 0; f < nf; ++f) {
+    int rc = 0;
+    if (ops.kind[f] == kSpecPlain) {
+      in[f] = ops.X[f];
+      continue;
+    }
+    if (ops.kind[f] == kSpecIkil)
+      rc = spec_ikil(b, ops.X[f], ops.Y[f], work[f], ops.cx[f], ops.cy[f]);
+    else
+      rc = spec_wv2(b, ops.X[f], work[f], ops.power[f], ops.cx[f]);
+    if (rc) return rc;
+    in[f] = work[f];
+  }
+  return c2r2d_n(b, nf, in, out, np);
+}
+
+// SpecJobs of nf ik/il derivative fields
+SpecJobs ikil_jobs(int nf, const cd* const* X, const cd* const* Y, const double* cx,
+                   const double* cy) {
+  SpecJobs o{};
+  for (int f = 0; f < nf; ++f) {
+    o.X[f] = X[f];
+    o.Y[f] = Y[f];
+    o.cx[f] = cx[f];
+    o.cy[f] = cy[f];
+    o.kind[f] = kSpecIkil;
+  }
+  return o;
+}
+
+int c2r2d_ikil(qgk_bucket* b, int nf, const cd* const* X, const cd* const* Y, const double* cx,
+               const double* cy, double* const* out, cd* const* work) {
+  return c2r2d_spec(b, nf, ikil_jobs(nf, X, Y, cx, cy), out, work);
+}
+
 // weight * scale * src into the field dst (or added to it), or to host memory
 int param_emit(qgk_bucket* b, double* src, double* dst, double* host, double scale,
                double weight, int accumulate) {
@@ -3892,12 +4015,14 @@ int smag_viscosity(qgk_bucket* b, double cs) {
   double *R0 = Rset(b, 0), *R1 = Rset(b, 1), *R2 = Rset(b, 2), *R3 = Rset(b, 3);
   const double* uv[2] = {b->field_r[QGK_U], b->field_r[QGK_V]};
   cd* uvh[2] = {W0, W1};
-  const cd* S[3] = {W2, W3, W4};
+  // ux = ifft(ik uh), vy = ifft(il vh), ifft(ik vh + il uh)
+  const cd* X[3] = {W0, nullptr, W1};
+  const cd* Y[3] = {nullptr, W1, W0};
+  const double cx[3] = {1.0, 0.0, 1.0}, cy[3] = {0.0, 1.0, 1.0};
   double* Sr[3] = {R0, R1, R2};
+  cd* work[3] = {W2, W3, W4};
   int rc;
-  if ((rc = r2c2d_n(b, 2, uv, uvh, np)) || (rc = spec_ikil(b, W0, nullptr, W2, 1.0, 0.0)) ||
-      (rc = spec_ikil(b, nullptr, W1, W3, 0.0, 1.0)) ||
-      (rc = spec_ikil(b, W1, W0, W4, 1.0, 1.0)) || (rc = c2r2d_n(b, 3, S, Sr, np)))
+  if ((rc = r2c2d_n(b, 2, uv, uvh, np)) || (rc = c2r2d_ikil(b, 3, X, Y, cx, cy, Sr, work)))
     return rc;
   const size_t n = (size_t)np * b->nplane_r;
   hipLaunchKernelGGL(k_smag_nu, dim3(grid_1d(n)), dim3(kGrid), 0, b->stream, R0, R1, R2, R3,
@@ -3930,11 +4055,13 @@ int qgk_param_smagorinsky(qgk_bucket* b, const double* dx, double constant, doub
   double *R0 = Rset(b, 0), *R1 = Rset(b, 1), *R2 = Rset(b, 2);
   const double* st[3] = {R0, R2, R1};
   cd* sth[3] = {W0, W1, W2};
-  const cd* duvh[2] = {W3, W4};
+  const cd* X[2] = {W0, W1};
+  const cd* Y[2] = {W1, W2};
+  const double one[2] = {1.0, 1.0};
   double* duv[2] = {R0, R1};
+  cd* work[2] = {W3, W4};
   // du = 2 ifft(ik F(nu Sxx) + il F(nu Sxy)), dv = 2 ifft(ik F(nu Sxy) + il F(nu Syy))
-  if ((rc = r2c2d_n(b, 3, st, sth, np)) || (rc = spec_ikil(b, W0, W1, W3, 1.0, 1.0)) ||
-      (rc = spec_ikil(b, W1, W2, W4, 1.0, 1.0)) || (rc = c2r2d_n(b, 2, duvh, duv, np)) ||
+  if ((rc = r2c2d_n(b, 3, st, sth, np)) || (rc = c2r2d_ikil(b, 2, X, Y, one, one, duv, work)) ||
       (rc = param_emit(b, R0, b->field_r[QGK_DU], du_host, 2.0, weight, accumulate)) ||
       (rc = param_emit(b, R1, b->field_r[QGK_DV], dv_host, 2.0, weight, accumulate)))
     return rc;
@@ -3966,13 +4093,14 @@ int qgk_param_zb2020(qgk_bucket* b, double constant, double weight, int accumula
   double *R0 = Rset(b, 0), *R1 = Rset(b, 1), *R2 = Rset(b, 2), *R3 = Rset(b, 3);
   const double* uv[2] = {b->field_r[QGK_U], b->field_r[QGK_V]};
   cd* uvh[2] = {W0, W1};
-  const cd* gh[3] = {W2, W3, W4};
-  double* g3[3] = {R0, R1, R2};
   // vx, vy, ux, uy (parameterizations.py:264-269)
-  if ((rc = r2c2d_n(b, 2, uv, uvh, np)) || (rc = spec_ikil(b, W1, nullptr, W2, 1.0, 0.0)) ||
-      (rc = spec_ikil(b, nullptr, W1, W3, 0.0, 1.0)) ||
-      (rc = spec_ikil(b, W0, nullptr, W4, 1.0, 0.0)) || (rc = c2r2d_n(b, 3, gh, g3, np)) ||
-      (rc = spec_ikil(b, nullptr, W0, W2, 0.0, 1.0)) || (rc = c2r2d(b, W2, R3, np)))
+  const cd* X[4] = {W1, nullptr, W0, nullptr};
+  const cd* Y[4] = {nullptr, W1, nullptr, W0};
+  const double cx[4] = {1.0, 0.0, 1.0, 0.0}, cy[4] = {0.0, 1.0, 0.0, 1.0};
+  double* g4[4] = {R0, R1, R2, R3};
+  cd* work[3] = {W2, W3, W4};
+  if ((rc = r2c2d_n(b, 2, uv, uvh, np)) || (rc = c2r2d_ikil(b, 3, X, Y, cx, cy, g4, work)) ||
+      (rc = c2r2d_ikil(b, 1, X + 3, Y + 3, cx + 3, cy + 3, g4 + 3, work)))
     return rc;
   const size_t nr = (size_t)np * b->nplane_r, nc = (size_t)np * b->nplane_c;
   hipLaunchKernelGGL(k_zb_products, dim3(grid_1d(nr)), dim3(kGrid), 0, b->stream, R0, R1, R2, R3,
@@ -3988,13 +4116,15 @@ int qgk_param_zb2020(qgk_bucket* b, double constant, double weight, int accumula
   hipLaunchKernelGGL(k_caxpby, dim3(grid_1d(nc)), dim3(kGrid), 0, b->stream, W2, W1, W3, 0.5,
                      -1.0, nc);
   HIPC(hipGetLastError());
-  if ((rc = spec_ikil(b, W3, W0, W3, 1.0, 1.0))) return rc;
   hipLaunchKernelGGL(k_caxpby, dim3(grid_1d(nc)), dim3(kGrid), 0, b->stream, W2, W1, W4, 0.5, 1.0,
                      nc);
   HIPC(hipGetLastError());
-  const cd* duvh[2] = {W3, W4};
+  const cd* dX[2] = {W3, W0};
+  const cd* dY[2] = {W0, W4};
+  const double one[2] = {1.0, 1.0};
   double* duv[2] = {R0, R1};
-  if ((rc = spec_ikil(b, W0, W4, W4, 1.0, 1.0)) || (rc = c2r2d_n(b, 2, duvh, duv, np)) ||
+  cd* dwork[2] = {W3, W4};
+  if ((rc = c2r2d_ikil(b, 2, dX, dY, one, one, duv, dwork)) ||
       (rc = param_emit(b, R0, b->field_r[QGK_DU], du_host, constant, weight, accumulate)) ||
       (rc = param_emit(b, R1, b->field_r[QGK_DV], dv_host, constant, weight, accumulate)))
     return rc;
@@ -4015,17 +4145,25 @@ int qgk_param_backscatter(qgk_bucket* b, const double* dx, const double* Hi,
   double *R0 = Rset(b, 0), *R1 = Rset(b, 1), *R2 = Rset(b, 2), *R3 = Rset(b, 3);
   const size_t nr = (size_t)np * b->nplane_r;
   // psi = ifft(ph), lap_lap_psi = ifft(lap^2 ph) (parameterizations.py:226-228)
-  const cd* psih[2] = {b->field_c[QGK_PH], W0};
-  double* psi[2] = {R0, R1};
-  if ((rc = spec_wv2(b, b->field_c[QGK_PH], W0, 2, 1.0)) || (rc = c2r2d_n(b, 2, psih, psi, np)))
-    return rc;
+  SpecJobs psi{};
+  psi.X[0] = psi.X[1] = b->field_c[QGK_PH];
+  psi.kind[0] = kSpecPlain;
+  psi.kind[1] = kSpecWv2;
+  psi.power[1] = 2;
+  psi.cx[1] = 1.0;
+  double* psir[2] = {R0, R1};
+  cd* psiw[2] = {nullptr, W0};
+  if ((rc = c2r2d_spec(b, 2, psi, psir, psiw))) return rc;
   // dissipation = -ifft(lap fft(lap_lap_psi dx^2 nu))
   hipLaunchKernelGGL(k_bs_product, dim3(grid_1d(nr)), dim3(kGrid), 0, b->stream, R1, R3, b->pdx,
                      R2, b->M, b->nz, (int)b->nplane_r);
   HIPC(hipGetLastError());
-  if ((rc = r2c2d(b, R2, W0, np)) || (rc = spec_wv2(b, W0, W0, 1, -1.0)) ||
-      (rc = c2r2d(b, W0, R2, np)))
-    return rc;
+  SpecJobs lap{};
+  lap.X[0] = W0;
+  lap.kind[0] = kSpecWv2;
+  lap.power[0] = 1;
+  lap.cx[0] = -1.0;
+  if ((rc = r2c2d(b, R2, W0, np)) || (rc = c2r2d_spec(b, 1, lap, &R2, &W0))) return rc;
   hipLaunchKernelGGL(k_plane_dot, dim3((unsigned)np), dim3(256), 0, b->stream, R0, R2, b->psum,
                      b->nplane_r);
   hipLaunchKernelGGL(k_plane_dot, dim3((unsigned)np), dim3(256), 0, b->stream, R0, R1,
