@@ -41,7 +41,7 @@ def test_transforms_match_numpy(pq, n):
 @pytest.mark.parametrize("n", [48, 96, 192])
 def test_qg_matches_oracle(pq, n):
     """100 steps through run() (fused stretches) against the oracle: the
-    north-star tolerance 1e-10 on q."""
+    parity tolerance 1e-10 on q after 100 steps."""
     print(f"mixed-radix QG n={n} 100 steps", flush=True)
     q0 = 1e-7 * np.random.default_rng(7).standard_normal((2, n, n))
     m = pq.QGModel(nx=n, dt=3600.0, tmax=100 * 3600.0, log_level=0, **QG_PHYS)
