@@ -80,6 +80,19 @@ int qgk_set_const_all(qgk_bucket* b, int which, const double* host);
 /* Two-layer closed-form inversion (qg_model.py:146-161) evaluated in the kernel
  * from F1, F2 instead of a table; used by ensembles of QGModels. */
 int qgk_set_inversion_2layer(qgk_bucket* b, int member, double F1, double F2);
+/* N-layer inversion by a tridiagonal solve (no counterpart in the reference,
+ * which multiplies by the table a = (S - kappa^2 I)^-1, layered_model.py:211-229).
+ * bands holds the [3][nz] bands of the stretching matrix S (below, on and above
+ * the diagonal; bands[0][0] = bands[2][nz-1] = 0), which must be tridiagonal with
+ * a non-positive diagonal, non-negative off-diagonals and diagonal dominance, as
+ * the layered model's S is.  The fused steps of buckets with 3 <= nz <= 16 then
+ * solve (S - kappa^2 I) ph = qh per wavenumber instead of reading a, which stays
+ * in use for the per-method phases and must be the inverse of the same S.
+ * Writing a (qgk_set_const / qgk_set_const_all) switches the solve off for the
+ * members written; bands == NULL switches it off.  member < 0: all members. */
+int qgk_set_inversion_tridiagonal(qgk_bucket* b, int member, const double* bands);
+/* *on = 1 when member uses the tridiagonal solve (no counterpart in the reference). */
+int qgk_get_inversion_tridiagonal(qgk_bucket* b, int member, int* on);
 /* Bottom drag rek (public attribute, kernel.pyx:101). member < 0: all. */
 int qgk_set_rek(qgk_bucket* b, int member, double rek);
 

@@ -25,7 +25,8 @@ CONSTS = ["kk", "ll", "k2l2", "filtr", "a", "Ubg", "Qy"]
 CONST_ID = {name: i for i, name in enumerate(CONSTS)}
 
 EXPORTS = ["qgk_create", "qgk_destroy", "qgk_last_error", "qgk_set_const", "qgk_set_const_all",
-           "qgk_set_inversion_2layer", "qgk_set_rek", "qgk_put", "qgk_get", "qgk_invert",
+           "qgk_set_inversion_2layer", "qgk_set_inversion_tridiagonal",
+           "qgk_get_inversion_tridiagonal", "qgk_set_rek", "qgk_put", "qgk_get", "qgk_invert",
            "qgk_advect", "qgk_friction", "qgk_add_q_param", "qgk_add_uv_param",
            "qgk_timestep", "qgk_q_to_qh", "qgk_qh_to_q", "qgk_fft", "qgk_ifft", "qgk_step",
            "qgk_timing_enable", "qgk_timing", "qgk_sync", "qgk_step_buckets",
@@ -87,6 +88,8 @@ def _load():
         "qgk_set_const": (I, [P, I, I, P]),
         "qgk_set_const_all": (I, [P, I, P]),
         "qgk_set_inversion_2layer": (I, [P, I, D, D]),
+        "qgk_set_inversion_tridiagonal": (I, [P, I, DP]),
+        "qgk_get_inversion_tridiagonal": (I, [P, I, ctypes.POINTER(I)]),
         "qgk_set_rek": (I, [P, I, D]),
         "qgk_put": (I, [P, I, P]),
         "qgk_get": (I, [P, I, P]),
@@ -212,6 +215,30 @@ class Bucket:
 
     def set_inversion_2layer(self, F1, F2, member=-1):
         check(lib.qgk_set_inversion_2layer(self.h, int(member), float(F1), float(F2)))
+
+    def set_inversion_tridiagonal(self, S, member=-1):
+        """Fused steps solve (S - kappa^2 I) ph = qh with the tridiagonal
+        stretching matrix S (nz x nz) instead of reading the table a; None
+        switches the solve off."""
+        if S is None:
+            check(lib.qgk_set_inversion_tridiagonal(self.h, int(member), None))
+            return
+        S = np.asarray(S, dtype=np.float64)
+        nz = S.shape[0]
+        if S.shape != (nz, nz) or np.any(np.triu(S, 2)) or np.any(np.tril(S, -2)):
+            raise ValueError("S must be a square tridiagonal matrix")
+        bands = np.zeros((3, nz))
+        bands[0, 1:] = np.diag(S, -1)
+        bands[1] = np.diag(S)
+        bands[2, :-1] = np.diag(S, 1)
+        bands = np.ascontiguousarray(bands)
+        check(lib.qgk_set_inversion_tridiagonal(
+            self.h, int(member), bands.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+    def inversion_tridiagonal(self, member=0):
+        on = ctypes.c_int()
+        check(lib.qgk_get_inversion_tridiagonal(self.h, int(member), ctypes.byref(on)))
+        return bool(on.value)
 
     def set_rek(self, rek, member=-1):
         check(lib.qgk_set_rek(self.h, int(member), float(rek)))

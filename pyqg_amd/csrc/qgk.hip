@@ -72,6 +72,10 @@ struct MemberParams {
   double F1, F2;        // closed-form two-layer inversion
   double rek;
   double pad;
+  // [3][nz] bands of a tridiagonal stretching matrix S (below, on and above
+  // the diagonal) or nullptr: the fused nz >= 3 steps then solve
+  // (S - kappa^2 I) ph = qh instead of reading the table a
+  const double* tri;
 };
 
 struct StepCoeffs {
@@ -943,6 +947,45 @@ __device__ __forceinline__ void col_invert_points(const cd* __restrict__ qh, cd*
 #pragma unroll 1
   for (int s = 0; s < G::E; ++s) {
     const size_t pt = (size_t)(t + s * G::T) * ldk + k;
+    if (P.tri) {
+      // Elimination down the layers and back-substitution: S - kappa^2 I is
+      // tridiagonal and, for kappa > 0, strictly diagonally dominant (S has a
+      // negative diagonal and rows that sum to zero), so no pivoting is
+      // needed.  ph = 0 at kappa = 0, where the reference zeroes a.
+      const double w = P.k2l2[pt];
+      if (w == 0.0) {
+        for (int z = 0; z < nz; ++z) phm[mbase + z * plane + pt] = cd{0.0, 0.0};
+        continue;
+      }
+      const double* lo = P.tri;
+      const double* dg = P.tri + nz;
+      const double* hi = P.tri + 2 * nz;
+      double cp[kTileNzMax];
+      cd dp[kTileNzMax];
+      double c_prev = 0.0;
+      cd d_prev = {0.0, 0.0};
+#pragma unroll
+      for (int j = 0; j < kTileNzMax; ++j) {
+        if (j < nz) {
+          const cd qj = qh[mbase + j * plane + pt];
+          const double l = lo[j];
+          const double r = 1.0 / ((dg[j] - w) - l * c_prev);
+          c_prev = hi[j] * r;
+          d_prev = {(qj.x - l * d_prev.x) * r, (qj.y - l * d_prev.y) * r};
+          cp[j] = c_prev;
+          dp[j] = d_prev;
+        }
+      }
+      cd xn = {0.0, 0.0};
+#pragma unroll
+      for (int j = kTileNzMax - 1; j >= 0; --j) {
+        if (j < nz) {
+          xn = {dp[j].x - cp[j] * xn.x, dp[j].y - cp[j] * xn.y};
+          phm[mbase + j * plane + pt] = xn;
+        }
+      }
+      continue;
+    }
     if (!P.a) {  // closed-form two-layer inverse (not used by layered buckets)
       for (int z = 0; z < nz; ++z)
         phm[mbase + z * plane + pt] = invert_point(P, nz, z, qh + mbase + pt, plane, pt);
@@ -2059,6 +2102,7 @@ struct qgk_bucket {
   std::vector<std::vector<double*>> own_tab;  // [which][member]
   cd* tw_x = nullptr;
   cd* tw_y = nullptr;
+  double* tri_tab = nullptr;  // [M][3][nz] bands of S for the tridiagonal solve
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
@@ -2725,7 +2769,11 @@ int qgk_set_const(qgk_bucket* b, int which, int member, const double* host) {
     dst = own;
     param_slot(b->hp[member], which) = dst;
   }
-  if (which == QGK_A) b->gvalid = false;
+  if (which == QGK_A) {
+    b->gvalid = false;
+    for (int m = (member < 0 ? 0 : member); m < (member < 0 ? b->M : member + 1); ++m)
+      b->hp[m].tri = nullptr;  // a new table is used as given
+  }
   b->gen++;
   b->params_dirty = true;
   const size_t rows = const_rows(b, which);
@@ -2757,7 +2805,10 @@ int qgk_set_const_all(qgk_bucket* b, int which, const double* host) {
   }
   HIPC(hipMemcpyAsync(tab, src, n * b->M * sizeof(double), hipMemcpyHostToDevice, b->stream));
   HIPC(hipStreamSynchronize(b->stream));
-  for (int m = 0; m < b->M; ++m) param_slot(b->hp[m], which) = tab + (size_t)m * n;
+  for (int m = 0; m < b->M; ++m) {
+    param_slot(b->hp[m], which) = tab + (size_t)m * n;
+    if (which == QGK_A) b->hp[m].tri = nullptr;
+  }
   b->params_dirty = true;
   b->primed = false;
   b->gvalid = false;
@@ -2778,6 +2829,45 @@ int qgk_set_inversion_2layer(qgk_bucket* b, int member, double F1, double F2) {
   b->primed = false;
   b->gvalid = false;
   b->gen++;
+  return 0;
+}
+
+int qgk_set_inversion_tridiagonal(qgk_bucket* b, int member, const double* bands) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (b->nz < 3) return fail(QGK_E_ARG, "the tridiagonal solve is for nz >= 3 (nz=%d)", b->nz);
+  if (member >= b->M) return fail(QGK_E_ARG, "member %d out of range", member);
+  const int m0 = member < 0 ? 0 : member, m1 = member < 0 ? b->M : member + 1;
+  if (bands) {
+    for (int i = 0; i < b->nz; ++i) {
+      const double lo = bands[i], dg = bands[b->nz + i], hi = bands[2 * b->nz + i];
+      if ((i == 0 && lo != 0.0) || (i == b->nz - 1 && hi != 0.0))
+        return fail(QGK_E_ARG, "bands[0][0] and bands[2][nz-1] must be 0");
+      if (!(dg <= 0.0) || lo < 0.0 || hi < 0.0 || -dg < lo + hi)
+        return fail(QGK_E_ARG, "layer %d: the solve needs a diagonally dominant S with a "
+                               "non-positive diagonal and non-negative off-diagonals", i);
+    }
+    const size_t per = (size_t)3 * b->nz;
+    if (!b->tri_tab) {
+      int rc = dev_alloc(b, (void**)&b->tri_tab, per * b->M * sizeof(double));
+      if (rc) return rc;
+    }
+    for (int m = m0; m < m1; ++m)
+      HIPC(hipMemcpyAsync(b->tri_tab + m * per, bands, per * sizeof(double),
+                          hipMemcpyHostToDevice, b->stream));
+    HIPC(hipStreamSynchronize(b->stream));
+  }
+  for (int m = m0; m < m1; ++m) b->hp[m].tri = bands ? b->tri_tab + m * (size_t)3 * b->nz : nullptr;
+  b->params_dirty = true;
+  b->primed = false;
+  b->gvalid = false;
+  b->gen++;
+  return 0;
+}
+
+int qgk_get_inversion_tridiagonal(qgk_bucket* b, int member, int* on) {
+  if (!b || !on) return fail(QGK_E_ARG, "NULL argument");
+  if (member < 0 || member >= b->M) return fail(QGK_E_ARG, "member %d out of range", member);
+  *on = b->hp[member].tri != nullptr;
   return 0;
 }
 

@@ -136,6 +136,8 @@ class Ensemble:
         b.set_const("k2l2", gr["k2l2"])
         b.set_const("filtr", C.spectral_filter(gr["k"], gr["l"], gr["dx"], gr["dy"], filterfac))
         b.set_const("a", a)
+        if nz >= 3:
+            b.set_inversion_tridiagonal(S)  # fused steps solve instead of reading a
         b.set_const("Ubg", Ubg)
         b.set_const("Qy", Qy)
         b.set_rek(rek)

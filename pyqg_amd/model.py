@@ -6,8 +6,8 @@ order of pyqg's Model / QGModel / LayeredModel / BTModel / SQGModel
 sqg_model.py).  They only build the constants the kernel consumes (wavenumbers,
 filter, background, inversion matrix) and orchestrate the kernel's phases;
 all per-step arithmetic happens in libqgk, including the time-averaged
-diagnostics (pyqg_amd.diagnostics).  xarray output and linear stability
-analysis are not part of this package.
+diagnostics (pyqg_amd.diagnostics).  Linear stability analysis is not part of
+this package; xarray output (Model.to_dataset) is in pyqg_amd.dataset.
 """
 import logging
 import warnings
@@ -480,8 +480,12 @@ class LayeredModel(Model):
 
     def _initialize_inversion_matrix(self):
         """a = (S - kappa^2 I)^-1 per wavenumber, zero at kappa = 0
-        (layered_model.py:211-229)."""
+        (layered_model.py:211-229).  With three or more layers the fused steps
+        solve (S - kappa^2 I) ph = qh with the tridiagonal S instead of reading
+        the nz^2 table; the per-method phases keep using a."""
         self.a = C.layered_inversion(self.S, self.wv2)
+        if self.nz >= 3:
+            self._bucket.set_inversion_tridiagonal(self.S)
 
     def modal_projection(self, p, forward=True):
         """Modal amplitudes of p in the basis pmodes, or the inverse

@@ -146,6 +146,36 @@ def test_layered_nz10_ensemble_shared_inversion(pq):
         assert rel_l2(ens.get("q", i), o2.q) < TOL_TRAJ, i
 
 
+def test_layered_tridiagonal_solve_matches_table(pq):
+    """Fused nz=10 steps solving (S - kappa^2 I) ph = qh agree with the same
+    steps reading the inversion table a, and with the oracle; writing a
+    switches the solve off."""
+    H, rho, U = pq.constants.exponential_stratification()
+    kw = dict(nz=10, nx=64, H=H, rho=rho, U=U, V=np.zeros(10), L=2e6, beta=1.5e-11,
+              rek=5.787e-7, dt=3600.0, f=0.0001236812857687059, g=9.81)
+    m1 = pq.LayeredModel(log_level=0, **kw)
+    m2 = pq.LayeredModel(log_level=0, **kw)
+    assert m1._bucket.inversion_tridiagonal() and m2._bucket.inversion_tridiagonal()
+    m2._bucket.set_inversion_tridiagonal(None)
+    assert not m2._bucket.inversion_tridiagonal()
+    o = OracleLayered(**kw)
+    o.a = np.ascontiguousarray(m1.a.real)
+    q0 = seeded_q(10, 64, 77)
+    for m in (m1, m2, o):
+        m.q = q0
+    m1._fused_steps(60)
+    m2._fused_steps(60)
+    for _ in range(60):
+        o._step_forward()
+    assert rel_l2(m1.q, m2.q) < 1e-12
+    assert rel_l2(m1.ph, m2.ph) < 1e-12
+    assert rel_l2(m1.q, o.q) < TOL_TRAJ
+    m1.a = m1.a.real.copy()          # a user-written table is used as given
+    assert not m1._bucket.inversion_tridiagonal()
+    with pytest.raises(Exception):   # two layers: no tridiagonal solve
+        pq.QGModel(nx=32, log_level=0)._bucket.set_inversion_tridiagonal(np.eye(2))
+
+
 def test_full_size_invariants_headline_ensemble(pq):
     """256 members of nx=256 (the measured workload): members with identical
     inputs stay bit-identical, and the state stays finite and bounded."""
