@@ -1242,6 +1242,144 @@ __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
   }
 }
 
+// The layer-split kernel is used where its two transform buffers fit the
+// 160 KB of LDS and it compiles without spills: its blocks are twice the
+// register kernel's, so the register budget per thread halves (radix 4 fits
+// at up to 1024 threads, radix 6 and 8 at up to 512, radix 12 and 16 at 256).
+template <int NY, int E>
+constexpr bool split_fits() {
+  using G = FGeo<NY, E>;
+  constexpr int threads = 2 * G::T * G::TK;
+  constexpr bool regs = G::E <= 4 || (G::E <= 8 ? threads <= 512 : threads <= 256);
+  return regs && 2 * NY * G::TK * (int)sizeof(cd) <= 160 * 1024;
+}
+
+// ---- nz = 2, one member: the layers split over the block.  A one-member
+// launch is a handful of blocks whose time is the latency of their serial
+// chain of transforms; here each layer has its own group of T x TK threads
+// (the same column geometry and radix as k_col_step_reg), so the two layers'
+// transforms run side by side instead of one after the other.  The inversion
+// couples the layers at every point: the groups swap their qh through LDS
+// (each group's transform buffer) before the tendency and again before the
+// prologue.
+template <int NY, int E>
+__global__ __launch_bounds__((2 * FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step_split(
+    const cd* __restrict__ qh_old, cd* __restrict__ qh_new, const cd* __restrict__ p,
+    cd* __restrict__ pp_to_newp, const cd* __restrict__ UQm, const cd* __restrict__ VQm,
+    cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
+    const MemberParams* __restrict__ params, StepCoeffs cf, int nk, int ldk,
+    const cd* __restrict__ tw) {
+#pragma clang fp contract(off)
+  using G = FGeo<NY, E>;
+  constexpr int T = G::T, TK = G::TK, EE = G::E;
+  __shared__ cd lds[2][NY * TK];
+  const int lz = threadIdx.x / (T * TK);
+  const int r = threadIdx.x % (T * TK);
+  const int c = r % TK, t = r / TK;
+  const int k = blockIdx.x * TK + c;
+  const int m = blockIdx.y;
+  const bool ok = k < nk;
+  const int kc = ok ? k : 0;
+  const MemberParams& P = params[m];
+  const size_t plane = (size_t)NY * ldk;
+  const size_t mbase = (size_t)m * 2 * plane;
+  const size_t zoff = mbase + (size_t)lz * plane;
+  const double kk = P.kk[kc];
+  cd* L = lds[lz];
+  const cd* Lo = lds[1 - lz];
+
+  // own layer's old qh and the tendency's transforms of UQm, VQm
+  cd Q[EE], xa[EE], xb[EE];
+#pragma unroll
+  for (int s = 0; s < EE; ++s) {
+    const size_t pt = (size_t)(t + s * T) * ldk + kc;
+    Q[s] = ok ? ld_stream(qh_old + zoff + pt) : cd{0.0, 0.0};
+    xa[s] = ok ? ld_stream(UQm + zoff + pt) : cd{0.0, 0.0};
+    xb[s] = ok ? ld_stream(VQm + zoff + pt) : cd{0.0, 0.0};
+  }
+  qgk::fft_block<NY, EE, -1, TK>(xa, L, t, c, tw);
+  qgk::fft_block<NY, EE, -1, TK>(xb, L, t, c, tw);
+  __syncthreads();  // the transforms' last LDS reads are done
+#pragma unroll
+  for (int s = 0; s < EE; ++s) L[(t + s * T) * TK + c] = Q[s];
+  __syncthreads();
+  // -(ik uqh + il vqh + ikQy ph) [+ drag], AB3 + filter
+  if (ok) {
+    const double kq = kk * P.Qy[lz];
+    const bool drag = lz == 1 && P.rek != 0.0;
+#pragma unroll
+    for (int s = 0; s < EE; ++s) {
+      const int l = t + s * T;
+      const size_t pt = (size_t)l * ldk + k;
+      const size_t o = zoff + pt;
+      const cd qo = Lo[l * TK + c];
+      const cd qs[2] = {lz == 0 ? Q[s] : qo, lz == 0 ? qo : Q[s]};
+      const cd ph = invert_regs<2>(P, lz, qs, plane, pt);
+      const double ll = P.ll[l];
+      const cd S = {-kk * xa[s].y + -ll * xb[s].y, kk * xa[s].x + ll * xb[s].x};
+      cd d = {S.x + -kq * ph.y, S.y + kq * ph.x};
+      d = {-d.x, -d.y};
+      if (drag) {
+        const double cr = P.rek * P.k2l2[pt];
+        d = {d.x + cr * ph.x, d.y + cr * ph.y};
+      }
+      const double f = P.filtr[pt];
+      const cd q = Q[s], pv = ld_stream(p + o), ppv = ld_stream(pp_to_newp + o);
+      double rx = q.x + cf.dt1 * d.x;
+      double ry = q.y + cf.dt1 * d.y;
+      rx = rx + cf.dt2 * pv.x;
+      ry = ry + cf.dt2 * pv.y;
+      rx = rx + cf.dt3 * ppv.x;
+      ry = ry + cf.dt3 * ppv.y;
+      const cd qn = {f * rx, f * ry};
+      st_stream(qh_new + o, qn);
+      st_stream(pp_to_newp + o, d);
+      Q[s] = qn;
+    }
+  } else {
+    // lanes past nk write zeros so every tile row is a full 128-byte line
+#pragma unroll
+    for (int s = 0; s < EE; ++s) {
+      const size_t o = zoff + (size_t)(t + s * T) * ldk + k;
+      st_stream(qh_new + o, cd{0.0, 0.0});
+      st_stream(pp_to_newp + o, cd{0.0, 0.0});
+    }
+  }
+  // next step's prologue: ph from both layers' new qh, then y IFFT of q, u, v
+  __syncthreads();  // every read of the old qh swap is done
+#pragma unroll
+  for (int s = 0; s < EE; ++s) L[(t + s * T) * TK + c] = Q[s];
+  __syncthreads();
+  cd ph[EE];
+#pragma unroll
+  for (int s = 0; s < EE; ++s) {
+    const cd qo = Lo[(t + s * T) * TK + c];
+    const cd qs[2] = {lz == 0 ? Q[s] : qo, lz == 0 ? qo : Q[s]};
+    ph[s] = ok ? invert_regs<2>(P, lz, qs, plane, (size_t)(t + s * T) * ldk + kc)
+               : cd{0.0, 0.0};
+  }
+  __syncthreads();  // every read of the new qh swap is done before L is reused
+  cd x[EE];
+#pragma unroll
+  for (int s = 0; s < EE; ++s) x[s] = Q[s];
+  qgk::fft_block<NY, EE, +1, TK>(x, L, t, c, tw);
+#pragma unroll
+  for (int s = 0; s < EE; ++s) st_stream(Qm + zoff + (size_t)(t + s * T) * ldk + k, x[s]);
+#pragma unroll
+  for (int s = 0; s < EE; ++s) {
+    const double ll = P.ll[t + s * T];
+    x[s] = {ll * ph[s].y, -ll * ph[s].x};
+  }
+  qgk::fft_block<NY, EE, +1, TK>(x, L, t, c, tw);
+#pragma unroll
+  for (int s = 0; s < EE; ++s) st_stream(Um + zoff + (size_t)(t + s * T) * ldk + k, x[s]);
+#pragma unroll
+  for (int s = 0; s < EE; ++s) x[s] = {-kk * ph[s].y, kk * ph[s].x};
+  qgk::fft_block<NY, EE, +1, TK>(x, L, t, c, tw);
+#pragma unroll
+  for (int s = 0; s < EE; ++s) st_stream(Vm + zoff + (size_t)(t + s * T) * ldk + k, x[s]);
+}
+
 template <int NY, int NZ, int E>
 __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_prologue_reg(
     const cd* __restrict__ qh, cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
@@ -2119,6 +2257,7 @@ struct qgk_bucket {
   bool gvalid = false;
   // register radix of the fused row and column kernels (autotuned)
   int erow = 8, ecol = 8;
+  bool csplit = false;  // nz = 2: column steps by k_col_step_split (one member)
   double* small = nullptr;  // 256 doubles: learned-closure scales (input [0,128), output [128,256))
   std::vector<double> small_in_h, small_out_h;  // values last uploaded there
   // diagnostics: per-member parameters, accumulators and work space
@@ -2495,7 +2634,12 @@ struct ColStepL {
       hipLaunchKernelGGL((k_col_step_reg<NY, 1, E>), grid, block, 0, b->lst ? b->lst : b->stream, qo, qn, Pp(b),
                          PP(b), b->UQm, b->VQm, b->Qm, b->Um, b->Vm, b->dp, cf, b->nk, b->ldk,
                          b->tw_y);
-    else if (b->nz == 2)
+    else if (b->nz == 2 && b->csplit && split_fits<NY, E>()) {
+      if constexpr (split_fits<NY, E>())
+        hipLaunchKernelGGL((k_col_step_split<NY, E>), grid, dim3(2 * G::T * G::TK), 0,
+                           b->lst ? b->lst : b->stream, qo, qn, Pp(b), PP(b), b->UQm, b->VQm,
+                           b->Qm, b->Um, b->Vm, b->dp, cf, b->nk, b->ldk, b->tw_y);
+    } else if (b->nz == 2)
       hipLaunchKernelGGL((k_col_step_reg<NY, 2, E>), grid, block, 0, b->lst ? b->lst : b->stream, qo, qn, Pp(b),
                          PP(b), b->UQm, b->VQm, b->Qm, b->Um, b->Vm, b->dp, cf, b->nk, b->ldk,
                          b->tw_y);
@@ -2654,9 +2798,11 @@ int qgk_create(int nmembers, int nz, int ny, int nx, int flags, qgk_bucket** out
     // One member: each fused launch is a handful of workgroups and bound by
     // the latency of its serial transform passes; radix 4 (twice the lanes
     // per transform) measured 12-30 % less time per fused step than radix 8
-    // for nx = 32 ... 512, one and two layers.
+    // for nx = 32 ... 512, one and two layers, and two layers step with the
+    // layer-split column kernel.
     b->erow = 4;
     b->ecol = 4;
+    b->csplit = nz == 2;
   }
   b->nplane_c = (size_t)b->nl * b->ldk;
   b->nplane_r = (size_t)ny * nx;
