@@ -206,6 +206,16 @@ def test_fused_radix_variants(pq, erow, ecol):
         for _ in range(nsteps):
             o._step_forward()
         assert rel_l2(ens.get("q", i), o.q) < TOL_TRAJ
+    # one two-layer member: the layer-split column kernel at this radix
+    qm = pq.QGModel(nx=64, dt=3600., log_level=0, **QG_PHYS)
+    qm._bucket.set_variant(erow, ecol)
+    oq = OracleQG(nx=64, dt=3600., **QG_PHYS)
+    qm.q = q0[1]
+    oq.q = q0[1]
+    qm._fused_steps(nsteps)
+    for _ in range(nsteps):
+        oq._step_forward()
+    assert rel_l2(qm.q, oq.q) < TOL_TRAJ
     from oracle.qg_oracle import OracleBT
     bt = pq.BTModel(L=2 * np.pi, nx=64, beta=0.5, rd=0.3, dt=0.005, log_level=0)
     bt._bucket.set_variant(erow, ecol)
