@@ -137,7 +137,10 @@ class Ensemble:
         b.set_const("filtr", C.spectral_filter(gr["k"], gr["l"], gr["dx"], gr["dy"], filterfac))
         b.set_const("a", a)
         if nz >= 3:
-            b.set_inversion_tridiagonal(S)  # fused steps solve instead of reading a
+            try:
+                b.set_inversion_tridiagonal(S)  # fused steps solve instead of reading a
+            except (_lib.QGKError, ValueError):
+                pass  # an S the solve does not take: the fused steps read a
         b.set_const("Ubg", Ubg)
         b.set_const("Qy", Qy)
         b.set_rek(rek)

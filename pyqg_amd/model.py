@@ -485,7 +485,10 @@ class LayeredModel(Model):
         the nz^2 table; the per-method phases keep using a."""
         self.a = C.layered_inversion(self.S, self.wv2)
         if self.nz >= 3:
-            self._bucket.set_inversion_tridiagonal(self.S)
+            try:
+                self._bucket.set_inversion_tridiagonal(self.S)
+            except (_lib.QGKError, ValueError):
+                pass  # an S the solve does not take: the fused steps read a
 
     def modal_projection(self, p, forward=True):
         """Modal amplitudes of p in the basis pmodes, or the inverse
