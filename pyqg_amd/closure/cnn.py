@@ -250,13 +250,21 @@ class CNN:
                                  self.b32[l].numel(), lr, b1, b2, eps, self.step_count, 0.0,
                                  _stream()))
 
-    def train_step(self, xp, target, lr=1e-3):
+    def train_step(self, xp, target, lr=1e-3, clip=None):
         """One Adam step on the MSE between output channels and target
-        (B,H,W,channels[-1]) FP32; returns the loss tensor."""
+        (B,H,W,channels[-1]) FP32; returns the loss tensor.  The global
+        gradient norm is kept in `last_grad_norm` (a device scalar); with
+        `clip` the gradients are scaled down to at most that norm first."""
         out, acts = self.forward(xp, keep=True)
         grad = torch.empty(out.shape, dtype=torch.bfloat16, device=out.device)
         loss = mse(out, target, grad)
         dws, dbs = self.backward(acts, grad)
+        gn = torch.sqrt(sum((d * d).sum() for d in dws) + sum((d * d).sum() for d in dbs))
+        self.last_grad_norm = gn
+        if clip is not None:
+            s = torch.clamp(clip / (gn + 1e-30), max=1.0)
+            for d in list(dws) + list(dbs):
+                d.mul_(s)
         self.adam_step(dws, dbs, lr=lr)
         return loss
 

@@ -17,14 +17,20 @@ def _normalized(x, y, scales, idx, net):
 
 
 def train(net: CNN, x, y, scales: Scales, epochs=4, batch=64, lr=1e-3, lr_min=1e-5, seed=0,
-          log=None):
+          log=None, clip_factor=4.0):
     """Adam on the MSE of normalised forcing, cosine learning-rate decay.
-    x (N, H, W, 6) and y (N, H, W, 2) FP32 tensors on the GPU."""
+    x (N, H, W, 6) and y (N, H, W, 2) FP32 tensors on the GPU.
+
+    After 50 steps the global gradient norm is clipped at clip_factor times
+    its running mean: without it one outsized gradient at the peak learning
+    rate could switch off the ReLUs and the loss return to 1 for good (one
+    seed in three at the default settings)."""
     n = x.shape[0]
     g = torch.Generator(device="cpu").manual_seed(seed)
     steps = epochs * max(n // batch, 1)
     it = 0
     history = []
+    gn_mean = 0.0
     t0 = time.perf_counter()
     for ep in range(epochs):
         perm = torch.randperm(n, generator=g)
@@ -32,12 +38,16 @@ def train(net: CNN, x, y, scales: Scales, epochs=4, batch=64, lr=1e-3, lr_min=1e
             idx = perm[s * batch:(s + 1) * batch].to(x.device)
             xb, yb = _normalized(x, y, scales, idx, net)
             cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * it / steps))
-            loss = net.train_step(xb, yb, lr=cur)
+            clip = clip_factor * gn_mean if (clip_factor and it >= 50) else None
+            loss = net.train_step(xb, yb, lr=cur, clip=clip)
+            gn = float(net.last_grad_norm)
+            used = min(gn, clip) if clip is not None else gn
+            gn_mean = used if it == 0 else 0.98 * gn_mean + 0.02 * used
             it += 1
             if it % 50 == 0 or it == steps:
                 history.append((it, float(loss)))
                 if log:
-                    log(f"step {it}/{steps} loss {float(loss):.4f}")
+                    log(f"step {it}/{steps} loss {float(loss):.4f} grad norm {gn:.3g}")
     torch.cuda.synchronize()
     return {"steps": it, "seconds": time.perf_counter() - t0, "history": history}
 
