@@ -180,6 +180,12 @@ int qgk_subgrid_forcing(qgk_bucket* hi, qgk_bucket* lo, const double* filter_lo)
 int qgk_pack_inputs(qgk_bucket* b, const double* inv_scale, int cinp, void* dst);
 int qgk_add_forcing_device(qgk_bucket* b, const float* net_out, int coutp, const double* scale);
 
+/* Two-layer column steps by the layer-split kernel (each layer its own
+ * thread group) instead of the register kernel (no counterpart in the
+ * reference).  Default: on when members x max(nx, ny) <= 2048, off otherwise; results
+ * agree to rounding.  Error for on with nz != 2. */
+int qgk_set_layer_split(qgk_bucket* b, int on);
+int qgk_get_layer_split(qgk_bucket* b, int* on);
 /* Register radix (4, 8 or 16 complex values per thread) of the fused row and
  * column kernels; it fixes the Stockham pass structure and the tile shapes. */
 int qgk_set_variant(qgk_bucket* b, int erow, int ecol);

@@ -31,7 +31,8 @@ EXPORTS = ["qgk_create", "qgk_destroy", "qgk_last_error", "qgk_set_const", "qgk_
            "qgk_timestep", "qgk_q_to_qh", "qgk_qh_to_q", "qgk_fft", "qgk_ifft", "qgk_step",
            "qgk_timing_enable", "qgk_timing", "qgk_sync", "qgk_step_buckets",
            "qgk_copy_probe", "qgk_pattern_probe", "qgk_capture_begin", "qgk_capture_end",
-           "qgk_graph_replay", "qgk_stream", "qgk_set_variant", "qgk_get_variant", "qgk_autotune",
+           "qgk_graph_replay", "qgk_stream", "qgk_set_variant", "qgk_get_variant", "qgk_set_layer_split",
+           "qgk_get_layer_split", "qgk_autotune",
            "qgk_device_name", "qgk_put_device", "qgk_get_device", "qgk_coarsen",
            "qgk_subgrid_forcing", "qgk_pack_inputs", "qgk_add_forcing_device",
            "qgk_diag_set_params", "qgk_diag_accumulate", "qgk_diag_read", "qgk_diag_reset",
@@ -116,6 +117,8 @@ def _load():
         "qgk_stream": (P, [P]),
         "qgk_set_variant": (I, [P, I, I]),
         "qgk_get_variant": (I, [P, ctypes.POINTER(I), ctypes.POINTER(I)]),
+        "qgk_set_layer_split": (I, [P, I]),
+        "qgk_get_layer_split": (I, [P, ctypes.POINTER(I)]),
         "qgk_autotune": (I, [P, I, ctypes.POINTER(I), ctypes.POINTER(I), DP, DP]),
         "qgk_device_name": (I, [ctypes.c_char_p, I]),
         "qgk_put_device": (I, [P, I, P]),
@@ -262,6 +265,14 @@ class Bucket:
 
     def set_variant(self, erow, ecol):
         check(lib.qgk_set_variant(self.h, int(erow), int(ecol)))
+
+    def set_layer_split(self, on):
+        check(lib.qgk_set_layer_split(self.h, int(bool(on))))
+
+    def layer_split(self):
+        on = ctypes.c_int()
+        check(lib.qgk_get_layer_split(self.h, ctypes.byref(on)))
+        return bool(on.value)
 
     def variant(self):
         r, c = ctypes.c_int(), ctypes.c_int()

@@ -2794,12 +2794,14 @@ int qgk_create(int nmembers, int nz, int ny, int nx, int flags, qgk_bucket** out
   b->nk = nx / 2 + 1;
   b->ldk = ldk_of(nx);
   b->flags = flags;
-  if (nmembers == 1) {
-    // One member: each fused launch is a handful of workgroups and bound by
-    // the latency of its serial transform passes; radix 4 (twice the lanes
-    // per transform) measured 12-30 % less time per fused step than radix 8
-    // for nx = 32 ... 512, one and two layers, and two layers step with the
-    // layer-split column kernel.
+  if ((long)nmembers * (nx > ny ? nx : ny) <= 2048) {
+    // Small buckets: each fused launch is a handful of workgroups and bound
+    // by the latency of its serial transform passes.  Radix 4 (twice the
+    // lanes per transform) and, with two layers, the layer-split column
+    // kernel measured fastest up to members x n = 2048: per step 20-26 us
+    // against 31-37 us for radix 8 at n = 64 with 1 ... 32 members, 41-49
+    // against 52-57 us at n = 256 with 1 ... 4.  Larger buckets keep radix 8
+    // and the register kernel (the autotuner may change the radix).
     b->erow = 4;
     b->ecol = 4;
     b->csplit = nz == 2;
@@ -3548,6 +3550,21 @@ int qgk_set_variant(qgk_bucket* b, int erow, int ecol) {
   b->ecol = ecol;
   b->gvalid = false;
   b->gen++;
+  return 0;
+}
+
+int qgk_set_layer_split(qgk_bucket* b, int on) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (on && b->nz != 2) return fail(QGK_E_ARG, "the layer-split column kernel needs nz == 2");
+  b->csplit = on != 0;
+  b->gvalid = false;
+  b->gen++;
+  return 0;
+}
+
+int qgk_get_layer_split(qgk_bucket* b, int* on) {
+  if (!b || !on) return fail(QGK_E_ARG, "NULL argument");
+  *on = b->csplit ? 1 : 0;
   return 0;
 }
 
