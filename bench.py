@@ -15,6 +15,9 @@ Other workloads (--workload):
   layered10  64 LayeredModel members, nz = 10, nx = 256, L = 2e6 m,
              exponential stratification, first deformation radius 20 km
   cnn        the learned closure's CNN: forward and training-step throughput
+  single512  one two-layer QGModel at nx=512, dt=1800 s (the single-GPU parity
+             configuration): run() with fused stretches, a _step_forward loop and
+             the oracle on one host core; no graphs, so it can run under a tracer
   dropin     the reference's example runs (five model-years of single QGModels,
              nx = 64 and 256, with and without closures) through the drop-in API,
              beside the wall times the reference's notebook records
@@ -348,7 +351,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--members", type=int, default=MEMBERS)
-    ap.add_argument("--workload", choices=["qg256", "uneven", "layered10", "cnn", "dropin"],
+    ap.add_argument("--workload", choices=["qg256", "uneven", "layered10", "cnn", "dropin",
+                                           "single512"],
                     default="qg256")
     ap.add_argument("--cnn-inference-only", action="store_true",
                     help="cnn workload: skip the training-step timing (profiling runs)")
@@ -364,6 +368,15 @@ def main():
         return bench_cnn(args)
     if args.workload == "dropin":
         return bench_dropin(args)
+    if args.workload == "single512":
+        out = bench_config2(args)
+        print(json.dumps({
+            "metric": "model-days/s of one two-layer QGModel at nx=512 (run(), fused stretches)",
+            "value": out["run_model_days_per_s"], "unit": "model-days/s", "n_gpus": 1,
+            "higher_is_better": True, "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded 1e-7 N(0,1) initial PV)",
+            "config": {"workload": "one QGModel, nx=512, dt=1800 s, diagnostics off", **out}}))
+        return None
     cpu = None if args.no_cpu else cpu_baseline(args.workload)
 
     import numpy as np
