@@ -262,6 +262,32 @@ def test_single_member_default_radix(pq, n):
     assert rel_l2(m.q, o.q) < 1e-12
 
 
+def test_small_bucket_defaults_and_layer_split(pq):
+    """Buckets with members x n <= 2048 default to radix 4 and the layer-split
+    column kernel, larger ones to radix 8 and the register kernel; both column
+    kernels give the same trajectory to rounding, and the oracle's."""
+    small = pq.Ensemble.qg(np.full(4, 64), dt=3600.)
+    large = pq.Ensemble.qg(np.full(40, 64), dt=3600.)
+    assert small.buckets[0].variant() == (4, 4) and small.buckets[0].layer_split()
+    assert large.buckets[0].variant() == (8, 8) and not large.buckets[0].layer_split()
+    other = pq.Ensemble.qg(np.full(4, 64), dt=3600.)
+    other.buckets[0].set_layer_split(False)
+    q0 = [seeded_q(2, 64, 90 + i) for i in range(4)]
+    small.set_q(q0)
+    other.set_q(q0)
+    small.step(40)
+    other.step(40)
+    o = OracleQG(nx=64, dt=3600., **QG_PHYS)
+    o.q = q0[2]
+    for _ in range(40):
+        o._step_forward()
+    for i in range(4):
+        assert rel_l2(small.get("q", i), other.get("q", i)) < 1e-12
+    assert rel_l2(small.get("q", 2), o.q) < TOL_TRAJ
+    with pytest.raises(Exception):
+        pq.BTModel(nx=32, log_level=0)._bucket.set_layer_split(True)
+
+
 def test_autotune_keeps_state_and_parity(pq, tmp_path):
     ens = pq.Ensemble.qg(np.array([64, 64, 128]), dt=3600.)
     q0 = [seeded_q(2, n, 60 + i) for i, n in enumerate([64, 64, 128])]
