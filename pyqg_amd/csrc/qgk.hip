@@ -147,18 +147,31 @@ __device__ __forceinline__ cd invert_point(const MemberParams& P, int nz, int k2
   return acc;
 }
 
+// Up to kPlaneJobs fields per transform launch: field f of launch block row
+// blockIdx.y = f * nplanes + p is plane p of jobs.in[f] / jobs.out[f].
+constexpr int kPlaneMax = 64;
+constexpr int kPlaneJobs = 4;
+
+struct PlaneJobs {
+  const void* in[kPlaneJobs];
+  void* out[kPlaneJobs];
+};
+
 // ==========================================================================
 // Column (y) transforms of spectral planes [nl][nk]
 // ==========================================================================
 template <int NY, int SIGN>
 __global__ __launch_bounds__(Geo<NY>::T * Geo<NY>::TK) void k_col_fft(
-    const cd* in, cd* out, int nk, int ldk, const cd* __restrict__ tw) {
+    PlaneJobs jobs, int nplanes, int nk, int ldk, const cd* __restrict__ tw) {
   using G = Geo<NY>;
   __shared__ cd lds[NY * G::TK];
+  const int f = blockIdx.y / nplanes, pl = blockIdx.y - f * nplanes;
+  const cd* in = static_cast<const cd*>(jobs.in[f]);
+  cd* out = static_cast<cd*>(jobs.out[f]);
   const int c = threadIdx.x % G::TK;
   const int t = threadIdx.x / G::TK;
   const int k = blockIdx.x * G::TK + c;
-  const size_t plane = (size_t)blockIdx.y * NY * ldk;
+  const size_t plane = (size_t)pl * NY * ldk;
   const bool ok = k < nk;
   cd x[G::E];
 #pragma unroll
@@ -194,17 +207,19 @@ __device__ __forceinline__ cd hext(const cd* __restrict__ h, int p) {
 // c2r of planes [ny][nk] -> [ny][nx], scaled by `scale`
 template <int NX>
 __global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_c2r(
-    const cd* __restrict__ in, double* __restrict__ out, int ny, double scale,
-    const cd* __restrict__ tw) {
+    PlaneJobs jobs, int nplanes, int ny, double scale, const cd* __restrict__ tw) {
   using G = Geo<NX>;
   constexpr int NK = ldk_of(NX);  // row stride
   __shared__ cd lds[NX * G::RP];
+  const int f = blockIdx.y / nplanes, pl = blockIdx.y - f * nplanes;
+  const cd* __restrict__ in = static_cast<const cd*>(jobs.in[f]);
+  double* __restrict__ out = static_cast<double*>(jobs.out[f]);
   const int t = threadIdx.x % G::T;
   const int rp = threadIdx.x / G::T;
   const int jp = blockIdx.x * G::RP + rp;
   const bool ok = 2 * jp < ny;
   const int j0 = ok ? 2 * jp : 0;
-  const cd* r0 = in + ((size_t)blockIdx.y * ny + j0) * NK;
+  const cd* r0 = in + ((size_t)pl * ny + j0) * NK;
   const cd* r1 = r0 + NK;
   cd x[G::E];
 #pragma unroll
@@ -215,7 +230,7 @@ __global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_c2r(
   }
   qgk::fft_block_rows<NX, G::E, +1>(x, lds, t, rp, tw);
   if (ok) {
-    double* o0 = out + ((size_t)blockIdx.y * ny + j0) * NX;
+    double* o0 = out + ((size_t)pl * ny + j0) * NX;
     double* o1 = o0 + NX;
 #pragma unroll
     for (int s = 0; s < G::E; ++s) {
@@ -245,16 +260,19 @@ __device__ __forceinline__ void split_store(const cd* lds, int t, int rp, bool o
 // r2c of planes [ny][nx] -> [ny][nk]
 template <int NX>
 __global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_r2c(
-    const double* __restrict__ in, cd* __restrict__ out, int ny, const cd* __restrict__ tw) {
+    PlaneJobs jobs, int nplanes, int ny, const cd* __restrict__ tw) {
   using G = Geo<NX>;
   constexpr int NK = ldk_of(NX);  // row stride
   __shared__ cd lds[NX * G::RP];
+  const int f = blockIdx.y / nplanes, pl = blockIdx.y - f * nplanes;
+  const double* __restrict__ in = static_cast<const double*>(jobs.in[f]);
+  cd* __restrict__ out = static_cast<cd*>(jobs.out[f]);
   const int t = threadIdx.x % G::T;
   const int rp = threadIdx.x / G::T;
   const int jp = blockIdx.x * G::RP + rp;
   const bool ok = 2 * jp < ny;
   const int j0 = ok ? 2 * jp : 0;
-  const double* i0 = in + ((size_t)blockIdx.y * ny + j0) * NX;
+  const double* i0 = in + ((size_t)pl * ny + j0) * NX;
   const double* i1 = i0 + NX;
   cd x[G::E];
 #pragma unroll
@@ -267,7 +285,7 @@ __global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_r2c(
 #pragma unroll
   for (int s = 0; s < G::E; ++s) lds[qgk::RowSwizzled<NX>::at(t, s * G::T, rp)] = x[s];
   __syncthreads();
-  cd* o0 = out + ((size_t)blockIdx.y * ny + j0) * NK;
+  cd* o0 = out + ((size_t)pl * ny + j0) * NK;
   split_store<NX, G::RP>(lds, t, rp, ok, o0, o0 + NK);
 }
 
@@ -283,13 +301,6 @@ __global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_r2c(
 // follows a barrier, after the reads of the previous use.  A launch carries up
 // to kPlaneJobs fields of nplanes planes each.
 // ==========================================================================
-constexpr int kPlaneMax = 64;
-constexpr int kPlaneJobs = 4;
-
-struct PlaneJobs {
-  const void* in[kPlaneJobs];
-  void* out[kPlaneJobs];
-};
 
 // T threads per transform; N/2 + 1 groups of them: the columns of the half
 // spectrum in the column pass, the N/2 row pairs (and one idle group) in the
@@ -2392,15 +2403,16 @@ int grid_1d(size_t total) {
   return (int)(g ? g : 1);
 }
 
+// Launches over nf fields of nplanes planes each (block rows f * nplanes + p).
 template <int SIGN>
-int launch_col_fft(qgk_bucket* b, const cd* in, cd* out, int nplanes) {
+int launch_col_fft_n(qgk_bucket* b, const PlaneJobs& j, int nf, int nplanes) {
   const int ny = b->ny, nk = b->nk;
 #define QGK_COL(NY)                                                                     \
   case NY: {                                                                            \
     using G = Geo<NY>;                                                                  \
-    dim3 grid((nk + G::TK - 1) / G::TK, nplanes);                                       \
-    hipLaunchKernelGGL((k_col_fft<NY, SIGN>), grid, dim3(G::T * G::TK), 0, b->stream, in, \
-                       out, nk, b->ldk, b->tw_y);                                       \
+    dim3 grid((nk + G::TK - 1) / G::TK, nf * nplanes);                                  \
+    hipLaunchKernelGGL((k_col_fft<NY, SIGN>), grid, dim3(G::T * G::TK), 0, b->stream, j,  \
+                       nplanes, nk, b->ldk, b->tw_y);                                   \
     break;                                                                              \
   }
   switch (ny) {
@@ -2413,15 +2425,15 @@ int launch_col_fft(qgk_bucket* b, const cd* in, cd* out, int nplanes) {
   return 0;
 }
 
-int launch_row_c2r(qgk_bucket* b, const cd* in, double* out, int nplanes) {
+int launch_row_c2r_n(qgk_bucket* b, const PlaneJobs& j, int nf, int nplanes) {
   const int nx = b->nx, ny = b->ny;
   const double scale = 1.0 / ((double)nx * (double)ny);
 #define QGK_ROW(NX)                                                                       \
   case NX: {                                                                              \
     using G = Geo<NX>;                                                                    \
-    dim3 grid((ny / 2 + G::RP - 1) / G::RP, nplanes);                                     \
-    hipLaunchKernelGGL((k_row_c2r<NX>), grid, dim3(G::T * G::RP), 0, b->stream, in, out, ny, \
-                       scale, b->tw_x);                                                   \
+    dim3 grid((ny / 2 + G::RP - 1) / G::RP, nf * nplanes);                                \
+    hipLaunchKernelGGL((k_row_c2r<NX>), grid, dim3(G::T * G::RP), 0, b->stream, j, nplanes, \
+                       ny, scale, b->tw_x);                                               \
     break;                                                                                \
   }
   switch (nx) {
@@ -2434,14 +2446,14 @@ int launch_row_c2r(qgk_bucket* b, const cd* in, double* out, int nplanes) {
   return 0;
 }
 
-int launch_row_r2c(qgk_bucket* b, const double* in, cd* out, int nplanes) {
+int launch_row_r2c_n(qgk_bucket* b, const PlaneJobs& j, int nf, int nplanes) {
   const int nx = b->nx, ny = b->ny;
 #define QGK_ROW(NX)                                                                        \
   case NX: {                                                                               \
     using G = Geo<NX>;                                                                     \
-    dim3 grid((ny / 2 + G::RP - 1) / G::RP, nplanes);                                      \
-    hipLaunchKernelGGL((k_row_r2c<NX>), grid, dim3(G::T * G::RP), 0, b->stream, in, out, ny, \
-                       b->tw_x);                                                           \
+    dim3 grid((ny / 2 + G::RP - 1) / G::RP, nf * nplanes);                                 \
+    hipLaunchKernelGGL((k_row_r2c<NX>), grid, dim3(G::T * G::RP), 0, b->stream, j, nplanes,  \
+                       ny, b->tw_x);                                                       \
     break;                                                                                 \
   }
   switch (nx) {
@@ -2452,6 +2464,26 @@ int launch_row_r2c(qgk_bucket* b, const double* in, cd* out, int nplanes) {
 #undef QGK_ROW
   HIPC(hipGetLastError());
   return 0;
+}
+
+PlaneJobs one_job(const void* in, void* out) {
+  PlaneJobs j{};
+  j.in[0] = in;
+  j.out[0] = out;
+  return j;
+}
+
+template <int SIGN>
+int launch_col_fft(qgk_bucket* b, const cd* in, cd* out, int nplanes) {
+  return launch_col_fft_n<SIGN>(b, one_job(in, out), 1, nplanes);
+}
+
+int launch_row_c2r(qgk_bucket* b, const cd* in, double* out, int nplanes) {
+  return launch_row_c2r_n(b, one_job(in, out), 1, nplanes);
+}
+
+int launch_row_r2c(qgk_bucket* b, const double* in, cd* out, int nplanes) {
+  return launch_row_r2c_n(b, one_job(in, out), 1, nplanes);
 }
 
 // one-launch plane transforms: square grids of n <= kPlaneMax
@@ -2529,6 +2561,21 @@ int r2c2d_n(qgk_bucket* b, int nf, const double* const* in, cd* const* out, int 
     }
     return launch_plane<PlaneR2CL>(b, j, nf, nplanes);
   }
+  if ((long)nf * nplanes <= 65535 &&
+      !jobs_overlap(nf, reinterpret_cast<const void* const*>(in), ib,
+                    reinterpret_cast<void* const*>(out), ob)) {
+    // one row launch and one column launch (in place on the outputs) for all fields
+    PlaneJobs j{}, jc{};
+    for (int i = 0; i < nf; ++i) {
+      j.in[i] = in[i];
+      j.out[i] = out[i];
+      jc.in[i] = out[i];
+      jc.out[i] = out[i];
+    }
+    int rc = launch_row_r2c_n(b, j, nf, nplanes);
+    if (rc) return rc;
+    return launch_col_fft_n<-1>(b, jc, nf, nplanes);
+  }
   for (int i = 0; i < nf; ++i) {
     int rc = launch_row_r2c(b, in[i], out[i], nplanes);
     if (rc) return rc;
@@ -2551,8 +2598,24 @@ int c2r2d_n(qgk_bucket* b, int nf, const cd* const* in, double* const* out, int 
     }
     return launch_plane<PlaneC2RL>(b, j, nf, nplanes);
   }
-  // y pass into the spectral scratch, x pass into out
+  // y pass into spectral scratch, x pass into out
   if (nplanes > b->M * b->nz) return fail(QGK_E_ARG, "too many planes for scratch");
+  if (nf <= 3 && (long)nf * nplanes <= 65535) {
+    // one column launch and one row launch for all fields; the scratch of
+    // fields 1 and 2 is the fused step's x-mixed flux pair, which the fused
+    // row kernel rewrites before anything reads it
+    cd* scratch[3] = {b->ctmp, b->UQm, b->VQm};
+    PlaneJobs jc{}, jr{};
+    for (int i = 0; i < nf; ++i) {
+      jc.in[i] = in[i];
+      jc.out[i] = scratch[i];
+      jr.in[i] = scratch[i];
+      jr.out[i] = out[i];
+    }
+    int rc = launch_col_fft_n<+1>(b, jc, nf, nplanes);
+    if (rc) return rc;
+    return launch_row_c2r_n(b, jr, nf, nplanes);
+  }
   for (int i = 0; i < nf; ++i) {
     int rc = launch_col_fft<+1>(b, in[i], b->ctmp, nplanes);
     if (rc) return rc;
