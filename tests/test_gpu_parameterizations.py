@@ -139,3 +139,51 @@ def test_ensemble_closure_matches_single_models(pq, which):
         for _ in range(steps):
             m._step_forward()
         assert rel_l2(ens.get("q", i), m.q) < 1e-13, i
+
+
+def test_closures_on_a_larger_grid_match_oracle(pq):
+    """nx=128, past the one-launch plane transforms: the closures' derivative
+    fields are formed by their own pointwise kernels and transformed by row
+    and column launches that carry several fields.  Closure outputs on one
+    state, and 8 steps with each closure as the model's parameterization,
+    against the oracle."""
+    from oracle.qg_diagnostics import attach_constants
+    from oracle.qg_oracle import OracleQG
+    import oracle.qg_parameterizations as P
+    print("closures nx=128", flush=True)
+    kw = dict(nx=128, dt=3600.0, **QG_PHYS)
+    q0 = 1e-6 * np.random.default_rng(128).standard_normal((2, 128, 128))
+    m = pq.QGModel(log_level=0, **kw)
+    m.q = q0
+    m._invert()
+    o = OracleQG(**kw)
+    o.q = q0
+    o._invert()
+    attach_constants(o)
+    du, dv = pq.Smagorinsky()(m)
+    odu, odv = P.smagorinsky(o)
+    assert rel_l2(du, odu) < 1e-11 and rel_l2(dv, odv) < 1e-11
+    assert rel_l2(pq.Smagorinsky(0.2)(m, just_viscosity=True),
+                  P.smagorinsky(o, 0.2, just_viscosity=True)) < 1e-11
+    assert rel_l2(pq.BackscatterBiharmonic()(m), P.backscatter_biharmonic(o)) < 1e-11
+    du, dv = pq.ZannaBolton2020()(m)
+    odu, odv = P.zanna_bolton_2020(o)
+    assert rel_l2(du, odu) < 1e-11 and rel_l2(dv, odv) < 1e-11
+    runs = {
+        "smag": ("uv_parameterization", lambda: pq.Smagorinsky(0.15),
+                 lambda mm: P.smagorinsky(mm, 0.15)),
+        "back": ("q_parameterization", lambda: pq.BackscatterBiharmonic(),
+                 lambda mm: P.backscatter_biharmonic(mm)),
+        "zb": ("uv_parameterization", lambda: pq.ZannaBolton2020(),
+               lambda mm: P.zanna_bolton_2020(mm)),
+    }
+    for name, (kind, dev, ref) in runs.items():
+        a = pq.QGModel(log_level=0, **{kind: dev()}, **kw)
+        b = OracleQG(**{kind: ref}, **kw)
+        attach_constants(b)
+        a.q = q0
+        b.q = q0
+        for _ in range(8):
+            a._step_forward()
+            b._step_forward()
+        assert rel_l2(a.q, b.q) < 1e-10, name
