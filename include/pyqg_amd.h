@@ -182,8 +182,8 @@ int qgk_add_forcing_device(qgk_bucket* b, const float* net_out, int coutp, const
 
 /* Two-layer column steps by the layer-split kernel (each layer its own
  * thread group) instead of the register kernel (no counterpart in the
- * reference).  Default: on when members x max(nx, ny) <= 2048, off otherwise; results
- * agree to rounding.  Error for on with nz != 2. */
+ * reference).  Default: on for one member or members x max(nx, ny)^2 <= 2^19, off
+ * otherwise; results agree to rounding.  Error for on with nz != 2. */
 int qgk_set_layer_split(qgk_bucket* b, int on);
 int qgk_get_layer_split(qgk_bucket* b, int* on);
 /* Register radix (4, 8 or 16 complex values per thread) of the fused row and

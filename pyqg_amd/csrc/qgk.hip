@@ -2857,14 +2857,17 @@ int qgk_create(int nmembers, int nz, int ny, int nx, int flags, qgk_bucket** out
   b->nk = nx / 2 + 1;
   b->ldk = ldk_of(nx);
   b->flags = flags;
-  if ((long)nmembers * (nx > ny ? nx : ny) <= 2048) {
-    // Small buckets: each fused launch is a handful of workgroups and bound
-    // by the latency of its serial transform passes.  Radix 4 (twice the
-    // lanes per transform) and, with two layers, the layer-split column
-    // kernel measured fastest up to members x n = 2048: per step 20-26 us
-    // against 31-37 us for radix 8 at n = 64 with 1 ... 32 members, 41-49
-    // against 52-57 us at n = 256 with 1 ... 4.  Larger buckets keep radix 8
-    // and the register kernel (the autotuner may change the radix).
+  const long nmax = nx > ny ? nx : ny;
+  if (nmembers == 1 || (long)nmembers * nmax * nmax <= (1L << 19)) {
+    // Small buckets: the fused launches are few workgroups and bound by the
+    // latency of their serial transform passes.  Radix 4 (twice the lanes
+    // per transform) and, with two layers, the layer-split column kernel
+    // measured fastest up to members x n^2 = 2^19: per step 20-50 us against
+    // 31-62 us for radix 8 and the register kernel at n = 64 with 1 ... 128
+    // members, 41-55 against 52-64 us at n = 256 with 1 ... 8; beyond that
+    // (n = 128 with 64 members, n = 256 with 16) the split kernel loses.
+    // Larger buckets keep radix 8 and the register kernel (the autotuner
+    // may change the radix).
     b->erow = 4;
     b->ecol = 4;
     b->csplit = nz == 2;

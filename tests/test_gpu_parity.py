@@ -263,11 +263,12 @@ def test_single_member_default_radix(pq, n):
 
 
 def test_small_bucket_defaults_and_layer_split(pq):
-    """Buckets with members x n <= 2048 default to radix 4 and the layer-split
-    column kernel, larger ones to radix 8 and the register kernel; both column
-    kernels give the same trajectory to rounding, and the oracle's."""
+    """Buckets with members x n^2 <= 2^19 (and single members) default to
+    radix 4 and the layer-split column kernel, larger ones to radix 8 and the
+    register kernel; both column kernels give the same trajectory to rounding,
+    and the oracle's."""
     small = pq.Ensemble.qg(np.full(4, 64), dt=3600.)
-    large = pq.Ensemble.qg(np.full(40, 64), dt=3600.)
+    large = pq.Ensemble.qg(np.full(40, 128), dt=3600.)
     assert small.buckets[0].variant() == (4, 4) and small.buckets[0].layer_split()
     assert large.buckets[0].variant() == (8, 8) and not large.buckets[0].layer_split()
     other = pq.Ensemble.qg(np.full(4, 64), dt=3600.)
