@@ -42,6 +42,10 @@ int qgcnn_conv_wgrad_ws(const void* dy, const void* x, int B, int H, int W, int 
 int qgcnn_flip_transpose(const void* w, void* wt, int cinp, int coutp, void* stream);
 /* db[c] += sum_px dy[px][c] */
 int qgcnn_bias_grad(const void* dy, size_t npix, int coutp, float* db, void* stream);
+/* As qgcnn_bias_grad with a fixed summation order (repeated runs agree bit for
+ * bit): ws holds at least 256 * coutp floats of per-block partial sums. */
+int qgcnn_bias_grad_ws(const void* dy, size_t npix, int coutp, float* db, float* ws,
+                       size_t ws_floats, void* stream);
 /* loss += mean over px and the first nch channels of (pred - target)^2;
  * grad = d loss / d pred * gscale as BF16 over all coutp channels (may be NULL).
  * pred is [npix][coutp] FP32, target [npix][nch] FP32. */

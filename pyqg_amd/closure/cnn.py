@@ -32,6 +32,7 @@ def _load():
         "qgcnn_conv_wgrad": (I, [P, P, I, I, I, I, I, P, P]),
         "qgcnn_flip_transpose": (I, [P, P, I, I, P]),
         "qgcnn_bias_grad": (I, [P, Z, I, P, P]),
+        "qgcnn_bias_grad_ws": (I, [P, Z, I, P, P, Z, P]),
         "qgcnn_mse": (I, [P, P, Z, I, I, F, P, P, P]),
         "qgcnn_f32_to_bf16": (I, [P, P, Z, P]),
         "qgcnn_bf16_to_f32": (I, [P, P, Z, P]),
@@ -49,7 +50,8 @@ def _load():
 lib = _load()
 EXPORTS = ["qgcnn_last_error", "qgcnn_conv_fwd", "qgcnn_conv_wgrad", "qgcnn_flip_transpose",
            "qgcnn_bias_grad", "qgcnn_mse", "qgcnn_f32_to_bf16", "qgcnn_bf16_to_f32",
-           "qgcnn_adam", "qgcnn_wgrad_workspace_bytes", "qgcnn_conv_wgrad_ws"]
+           "qgcnn_adam", "qgcnn_wgrad_workspace_bytes", "qgcnn_conv_wgrad_ws",
+           "qgcnn_bias_grad_ws"]
 
 
 def check(rc):
@@ -123,8 +125,11 @@ def flip_transpose(w, wt):
 
 
 def bias_grad(dy, db):
+    """db += the bias gradient, summed in a fixed order (bit-reproducible)."""
     coutp = dy.shape[-1]
-    check(lib.qgcnn_bias_grad(_p(dy), dy.numel() // coutp, coutp, _p(db), _stream()))
+    ws = _workspace(256 * coutp * 4, dy.device)
+    check(lib.qgcnn_bias_grad_ws(_p(dy), dy.numel() // coutp, coutp, _p(db), _p(ws),
+                                 ws.numel() // 4, _stream()))
 
 
 def mse(pred, target, grad=None, gscale=1.0):

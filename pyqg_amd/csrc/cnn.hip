@@ -440,6 +440,37 @@ __global__ __launch_bounds__(256) void k_bias_grad(const bf16_t* __restrict__ dy
   }
 }
 
+// The same sum in a fixed order: each block writes its per-channel partial
+// sums to part[block][c] (no atomics), then k_bias_grad_sum adds the blocks'
+// partials channel by channel in block order, so repeated runs agree bit for bit.
+__global__ __launch_bounds__(256) void k_bias_grad_part(const bf16_t* __restrict__ dy,
+                                                        size_t npix, int coutp,
+                                                        float* __restrict__ part) {
+  __shared__ float red[256];
+  const int c = threadIdx.x % coutp;
+  const int lanes_per_c = blockDim.x / coutp;
+  const int sub = threadIdx.x / coutp;
+  float s = 0.f;
+  for (size_t p = blockIdx.x * (size_t)lanes_per_c + sub; p < npix;
+       p += (size_t)gridDim.x * lanes_per_c)
+    s += bf2f(dy[p * coutp + c]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (sub == 0) {
+    for (int k = 1; k < lanes_per_c; ++k) s += red[k * coutp + c];
+    part[(size_t)blockIdx.x * coutp + c] = s;
+  }
+}
+
+__global__ void k_bias_grad_sum(const float* __restrict__ part, int nblocks, int coutp,
+                                float* __restrict__ db) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= coutp) return;
+  float s = 0.f;
+  for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * coutp + c];
+  db[c] += s;
+}
+
 // MSE over the first `nch` of `coutp` channels: loss += sum (pred - target)^2 * inv_n;
 // grad (bf16, all coutp channels, zero in the padding) = 2 (pred - target) * inv_n * gscale
 __global__ void k_mse(const float* __restrict__ pred, const float* __restrict__ target,
@@ -642,6 +673,20 @@ int qgcnn_bias_grad(const void* dy, size_t npix, int coutp, float* db, void* str
   if (!dy || !db || !mult16(coutp)) return fail(QGCNN_E_ARG, "bad arguments");
   hipLaunchKernelGGL(k_bias_grad, dim3(256), dim3(256 / coutp * coutp), 0, (hipStream_t)stream,
                      (const bf16_t*)dy, npix, coutp, db);
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+int qgcnn_bias_grad_ws(const void* dy, size_t npix, int coutp, float* db, float* ws,
+                       size_t ws_floats, void* stream) {
+  constexpr int kBlocks = 256;
+  if (!dy || !db || !ws || !mult16(coutp)) return fail(QGCNN_E_ARG, "bad arguments");
+  if (ws_floats < (size_t)kBlocks * coutp)
+    return fail(QGCNN_E_ARG, "workspace of %zu floats, %d needed", ws_floats, kBlocks * coutp);
+  hipLaunchKernelGGL(k_bias_grad_part, dim3(kBlocks), dim3(256 / coutp * coutp), 0,
+                     (hipStream_t)stream, (const bf16_t*)dy, npix, coutp, ws);
+  hipLaunchKernelGGL(k_bias_grad_sum, dim3((coutp + 127) / 128), dim3(128), 0,
+                     (hipStream_t)stream, ws, kBlocks, coutp, db);
   HIPC(hipGetLastError());
   return 0;
 }

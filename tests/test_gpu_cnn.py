@@ -194,3 +194,21 @@ def test_training_reduces_loss(cnn):
     for _ in range(150):
         last = float(net.train_step(xp, t, lr=3e-3))
     assert last < 0.2 * first
+
+
+def test_training_is_bit_reproducible(cnn):
+    """Two networks from the same seed, trained on the same batches, end with
+    identical weights: every gradient is summed in a fixed order."""
+    import torch
+    g = torch.Generator().manual_seed(3)
+    x = torch.randn((8, 32, 64, 6), generator=g).cuda()
+    t = torch.randn((8, 32, 64, 2), generator=g).cuda()
+    nets = [cnn.CNN(channels=(6, 32, 32, 2), seed=9) for _ in range(2)]
+    for net in nets:
+        xp = net.pad_input(x)
+        for _ in range(5):
+            net.train_step(xp, t, lr=1e-3, clip=10.0)
+    torch.cuda.synchronize()
+    for l in range(nets[0].nlayers):
+        assert torch.equal(nets[0].w32[l], nets[1].w32[l]), l
+        assert torch.equal(nets[0].b32[l], nets[1].b32[l]), l
