@@ -462,13 +462,20 @@ __global__ __launch_bounds__(256) void k_bias_grad_part(const bf16_t* __restrict
   }
 }
 
-__global__ void k_bias_grad_sum(const float* __restrict__ part, int nblocks, int coutp,
-                                float* __restrict__ db) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= coutp) return;
-  float s = 0.f;
-  for (int b = 0; b < nblocks; ++b) s += part[(size_t)b * coutp + c];
-  db[c] += s;
+// one block per channel: the nblocks (<= 256) partials loaded side by side and
+// added by a tree in LDS, whose order is fixed
+__global__ __launch_bounds__(256) void k_bias_grad_sum(const float* __restrict__ part,
+                                                       int nblocks, int coutp,
+                                                       float* __restrict__ db) {
+  __shared__ float red[256];
+  const int c = blockIdx.x;
+  red[threadIdx.x] = threadIdx.x < nblocks ? part[(size_t)threadIdx.x * coutp + c] : 0.f;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[c] += red[0];
 }
 
 // MSE over the first `nch` of `coutp` channels: loss += sum (pred - target)^2 * inv_n;
@@ -685,8 +692,8 @@ int qgcnn_bias_grad_ws(const void* dy, size_t npix, int coutp, float* db, float*
     return fail(QGCNN_E_ARG, "workspace of %zu floats, %d needed", ws_floats, kBlocks * coutp);
   hipLaunchKernelGGL(k_bias_grad_part, dim3(kBlocks), dim3(256 / coutp * coutp), 0,
                      (hipStream_t)stream, (const bf16_t*)dy, npix, coutp, ws);
-  hipLaunchKernelGGL(k_bias_grad_sum, dim3((coutp + 127) / 128), dim3(128), 0,
-                     (hipStream_t)stream, ws, kBlocks, coutp, db);
+  hipLaunchKernelGGL(k_bias_grad_sum, dim3(coutp), dim3(256), 0, (hipStream_t)stream, ws,
+                     kBlocks, coutp, db);
   HIPC(hipGetLastError());
   return 0;
 }
