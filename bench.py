@@ -104,7 +104,10 @@ def cpu_baseline(workload="qg256", nsteps=None):
     os.environ["OPENBLAS_NUM_THREADS"] = "1"
     if nsteps is None:
         nsteps = {"qg256": 500, "uneven": 500, "layered10": 60}[workload]
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    # one member per core of the CPU share a GPU box allots to one GPU (16
+    # cores; the host may show more, see affinity and cpu_count below)
+    affinity = len(os.sched_getaffinity(0))
+    cores = min(affinity, 16)
     with cf.ProcessPoolExecutor(max_workers=cores, mp_context=mp.get_context("spawn")) as ex:
         t0 = time.perf_counter()
         times = list(ex.map(_cpu_worker, [(nsteps, s, workload) for s in range(cores)]))
@@ -123,7 +126,8 @@ def cpu_baseline(workload="qg256", nsteps=None):
             "uneven": "two-layer QG oracle members of nx drawn from {64,128,256}",
             "layered10": "the nz=10 nx=256 LayeredModel oracle"}[workload]
     return {"value": per_member * cores, "unit": "model-days/s", "cores": cores,
-            "kind": "port",
+            "kind": "port", "affinity_cores": affinity, "cpu_count": os.cpu_count(),
+            "cpu_model": model,
             "sample": (f"{cores} members x {nsteps} steps of {what}, one single-threaded "
                        f"process per core ({model}); aggregate = mean per-member rate x "
                        f"cores; {wall:.1f} s wall")}
@@ -356,6 +360,8 @@ def main():
                     default="qg256")
     ap.add_argument("--cnn-inference-only", action="store_true",
                     help="cnn workload: skip the training-step timing (profiling runs)")
+    ap.add_argument("--graph-steps", type=int, default=100,
+                    help="steps timed again on the graph-replay path (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=1 << 30,
                     help="bytes moved by the copy kernel that calibrates HBM counters (0: skip)")
     args = ap.parse_args()
@@ -445,17 +451,33 @@ def main():
                    "row_kernel_over_bound": per_bucket[0]["row_kernel_ms"] / ms_r,
                    "col_kernel_over_bound": per_bucket[0]["col_kernel_ms"] / ms_c}
 
-    for m in (0, M - 1):
-        if not np.isfinite(ens.get("qh", m)).all():
-            raise SystemExit("non-finite state after the timed steps")
+    # the same steps replayed from a hipGraph of two fused steps per bucket
+    graph_ms = None
+    if args.graph_steps:
+        ens.step(2, use_graph=True)       # capture, untimed
+        for b in ens.buckets:
+            b.sync()
+        t0 = time.perf_counter()
+        ens.step(args.graph_steps, use_graph=True)
+        graph_ms = (time.perf_counter() - t0) * 1e3 / args.graph_steps
+
+    # every member's health (the reference's CFL < 1 assertion, per member)
+    cfl = ens.member_cfl()
+    if not (np.isfinite(cfl).all() and cfl.max() < 1.0):
+        raise SystemExit(f"members failed the health check after the timed steps: "
+                         f"{np.nonzero(~(np.isfinite(cfl) & (cfl < 1.0)))[0].tolist()}")
 
     model_days = M * args.steps * DT / 86400.
     value = model_days / wall
     bytes_step = float(sum(algorithmic_bytes(nz, int(n), int(n)) for n in nxs))
     achieved = bytes_step / (step_ms * 1e-3) / 1e9
-    traffic = None
+    traffic, traffic_src = None, None
     if args.workload == "qg256" and M == MEMBERS:
         traffic = measured_traffic(*ens.buckets[0].variant())
+        if traffic is not None:
+            traffic_src = ("profiles/fused_step_qg2_nx256_256members.md: rocprofv3 --pmc "
+                           "FETCH_SIZE and WRITE_SIZE of the same two kernels at this "
+                           "workload, copy-calibrated (not measured in this run)")
     out = {
         "metric": METRIC,
         "value": value,
@@ -474,11 +496,18 @@ def main():
                    "dt": DT},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "fused step = k_row_flux + k_col_step (per bucket, summed)",
                      "algorithmic_bytes_per_launch": bytes_step,
                      "kernel_ms_per_step": step_ms, "buckets": per_bucket,
                      "copy_kernel": copy, "access_pattern_bound": pattern, "autotune": tune},
         "cpu_baseline": cpu,
+        "graph_path": None if graph_ms is None else {
+            "steps": args.graph_steps, "ms_per_step": graph_ms,
+            "value": M * DT / 86400. / (graph_ms * 1e-3),
+            "note": "the same fused steps replayed from one hipGraph of two steps of "
+                    "every bucket (wall clock, after the event-timed eager steps)"},
+        "health": {"cfl_max": float(cfl.max()), "members_checked": int(cfl.size)},
     }
     print(json.dumps(out))
 

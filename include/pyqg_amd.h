@@ -146,6 +146,17 @@ int qgk_ifft(qgk_bucket* b, int nplanes, const double* in_complex, double* out_r
  * hipGraph once the weights stop changing. */
 int qgk_step(qgk_bucket* b, int nsteps, const double* coeffs, int use_graph);
 
+/* Per-member health check, the counterpart of the reference's CFL assertion
+ * every twrite steps (pyqg/model.py:564-574, pyqg/qg_diagnostics.py:76-79)
+ * for every member of a bucket at once: speed_host[m] = max over the grid of
+ * |u + Ubg| and |v| of member m's current state (CFL = speed * dt / dx); NaN
+ * (or inf) when any of the member's q, u or v is not finite.
+ * qgk_mask_member zeroes every state field of one member (its PV, tendency
+ * history and intermediates), so a member that blew up rests at q = 0 instead
+ * of carrying NaNs into bucket-wide sums. */
+int qgk_member_speed(qgk_bucket* b, double* speed_host);
+int qgk_mask_member(qgk_bucket* b, int member);
+
 /* Kernel timing for the measurement harness: when enabled, the fused step
  * records HIP events around each of its kernels; qgk_timing returns the
  * summed milliseconds and launch counts of the row-phase and column-phase

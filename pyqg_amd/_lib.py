@@ -37,7 +37,8 @@ EXPORTS = ["qgk_create", "qgk_destroy", "qgk_last_error", "qgk_set_const", "qgk_
            "qgk_subgrid_forcing", "qgk_pack_inputs", "qgk_add_forcing_device",
            "qgk_diag_set_params", "qgk_diag_accumulate", "qgk_diag_read", "qgk_diag_reset",
            "qgk_param_smagorinsky", "qgk_smagorinsky_viscosity", "qgk_param_zb2020",
-           "qgk_param_backscatter", "qgk_apply_q_param", "qgk_apply_uv_param"]
+           "qgk_param_backscatter", "qgk_apply_q_param", "qgk_apply_uv_param",
+           "qgk_member_speed", "qgk_mask_member"]
 
 # device diagnostics, in the order of the qgk_diag enum
 DIAGS = ["Ensspec", "KEspec", "EKEdiss", "KEfrictionspec", "EKE", "Dissspec", "ENSDissspec",
@@ -137,6 +138,8 @@ def _load():
         "qgk_param_backscatter": (I, [P, P, P, D, D, D, D, I, P]),
         "qgk_apply_q_param": (I, [P]),
         "qgk_apply_uv_param": (I, [P]),
+        "qgk_member_speed": (I, [P, P]),
+        "qgk_mask_member": (I, [P, I]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -262,6 +265,16 @@ class Bucket:
 
     def sync(self):
         check(lib.qgk_sync(self.h))
+
+    def member_speed(self):
+        """Largest |u + Ubg|, |v| over each member's grid (NaN or inf when the
+        member's state is not finite)."""
+        out = np.empty(self.M)
+        check(lib.qgk_member_speed(self.h, ptr(out)))
+        return out
+
+    def mask_member(self, member):
+        check(lib.qgk_mask_member(self.h, int(member)))
 
     def set_variant(self, erow, ecol):
         check(lib.qgk_set_variant(self.h, int(erow), int(ecol)))

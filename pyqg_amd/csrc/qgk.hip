@@ -788,6 +788,79 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP), row_flux_waves<
   split_store_t<NX, G::T, G::RP>(lds, t, rp, ok, VQm + row0, VQm + row0 + NK);
 }
 
+// Health check of every member: the reference asserts CFL < 1 every twrite
+// steps (model.py:564-574 with qg_diagnostics.py:76-79), CFL being
+// max|u + Ubg|, |v| over the grid times dt/dx.  Here one pass over a bucket
+// gives each member's largest speed from the primed x-mixed q, u, v, by the
+// row kernel's inverse x transforms.  The maximum is kept as the bit pattern of
+// a non-negative double, which orders like its value, so an unsigned atomic
+// max builds it; |NaN| has a larger pattern than +inf, so a NaN in u or v, or
+// a non-finite q anywhere in the member, leaves a NaN (or inf) in its word.
+__device__ __forceinline__ unsigned long long abs_bits(double x) {
+  return (unsigned long long)__double_as_longlong(x) & 0x7fffffffffffffffULL;
+}
+
+template <int NX, int E>
+__global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP)) void k_row_speed(
+    const cd* __restrict__ Qm, const cd* __restrict__ Um, const cd* __restrict__ Vm,
+    const MemberParams* __restrict__ params, int nz, int ny, double scale,
+    const cd* __restrict__ tw, unsigned long long* __restrict__ speed) {
+  using G = FGeo<NX, E>;
+  constexpr int NK = ldk_of(NX);
+  constexpr unsigned long long kInf = 0x7ff0000000000000ULL;
+  __shared__ cd lds[NX * G::RP];
+  __shared__ cd twl[NX];
+  tw = stage_twiddles<NX>(twl, tw);
+  const int t = threadIdx.x % G::T;
+  const int rp = threadIdx.x / G::T;
+  const int jp = blockIdx.x * G::RP + rp;
+  const int j0 = 2 * jp < ny ? 2 * jp : 0;  // spare row pairs repeat rows 0 and 1
+  const int plane_id = blockIdx.y;
+  const int m = plane_id / nz, z = plane_id - m * nz;
+  const double ubg = params[m].Ubg[z];
+  const size_t row0 = ((size_t)plane_id * ny + j0) * NK;
+  constexpr int H = G::E / 2;
+  cd hq[2][H], hu[2][H], hv[2][H];
+#pragma unroll
+  for (int s = 0; s < H; ++s) {
+    const int p = t + s * G::T;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      hq[r][s] = ld_stream(Qm + row0 + r * NK + p);
+      hu[r][s] = ld_stream(Um + row0 + r * NK + p);
+      hv[r][s] = ld_stream(Vm + row0 + r * NK + p);
+    }
+  }
+  cd zq[G::E], zu[G::E], zv[G::E];
+  row_pair_extend<NX, G::E, G::T, G::RP>(hq, Qm + row0, NK, zq, lds, t, rp);
+  qgk::fft_block_rows<NX, G::E, +1>(zq, lds, t, rp, tw);
+  row_pair_extend<NX, G::E, G::T, G::RP>(hu, Um + row0, NK, zu, lds, t, rp);
+  qgk::fft_block_rows<NX, G::E, +1>(zu, lds, t, rp, tw);
+  row_pair_extend<NX, G::E, G::T, G::RP>(hv, Vm + row0, NK, zv, lds, t, rp);
+  qgk::fft_block_rows<NX, G::E, +1>(zv, lds, t, rp, tw);
+  unsigned long long w = 0;
+#pragma unroll
+  for (int s = 0; s < G::E; ++s) {
+    const unsigned long long b[6] = {
+        abs_bits(zu[s].x * scale + ubg), abs_bits(zu[s].y * scale + ubg),
+        abs_bits(zv[s].x * scale), abs_bits(zv[s].y * scale),
+        abs_bits(zq[s].x * scale), abs_bits(zq[s].y * scale)};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w = b[i] > w ? b[i] : w;
+#pragma unroll
+    for (int i = 4; i < 6; ++i) w = (b[i] >= kInf && b[i] > w) ? b[i] : w;
+  }
+  // wave maximum, then one atomic per wave
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned lo = __shfl_xor((unsigned)(w & 0xffffffffULL), off);
+    const unsigned hi = __shfl_xor((unsigned)(w >> 32), off);
+    const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+    w = o > w ? o : w;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(speed + m, w);
+}
+
 // ---- any nz: the inversion re-reads qh from global memory (cache hits)
 
 // From qh (all layers) build ph, uh, vh of layer z at this thread's points and
@@ -2269,6 +2342,7 @@ struct qgk_bucket {
   // register radix of the fused row and column kernels (autotuned)
   int erow = 8, ecol = 8;
   bool csplit = false;  // nz = 2: column steps by k_col_step_split (one member)
+  unsigned long long* speed = nullptr;  // [M] member health words (qgk_member_speed)
   double* small = nullptr;  // 256 doubles: learned-closure scales (input [0,128), output [128,256))
   std::vector<double> small_in_h, small_out_h;  // values last uploaded there
   // diagnostics: per-member parameters, accumulators and work space
@@ -2350,8 +2424,20 @@ const double*& param_slot(MemberParams& p, int which) {
   }
 }
 
+// Entry points that read or write host memory, synchronize, or change a
+// constant refuse to run while the bucket's stream is being captured: a copy
+// from pageable host memory recorded into a graph would read that host buffer
+// again at every replay, after it may be gone, and a synchronization is
+// illegal inside a capture.
+int refuse_in_capture(const qgk_bucket* b, const char* what) {
+  if (b->capturing) return fail(QGK_E_STATE, "%s is not allowed during graph capture", what);
+  return 0;
+}
+
 int upload_params(qgk_bucket* b) {
   if (!b->params_dirty) return 0;
+  if (b->capturing)
+    return fail(QGK_E_STATE, "member constants changed during graph capture");
   HIPC(hipMemcpyAsync(b->dp, b->hp.data(), sizeof(MemberParams) * b->M, hipMemcpyHostToDevice,
                       b->stream));
   b->params_dirty = false;
@@ -2604,12 +2690,12 @@ int c2r2d_n(qgk_bucket* b, int nf, const cd* const* in, double* const* out, int 
     // one column launch and one row launch for all fields; the scratch of
     // fields 1 and 2 is the fused step's x-mixed flux pair, which the fused
     // row kernel rewrites before anything reads it
-    cd* scratch[3] = {b->ctmp, b->UQm, b->VQm};
+    cd* stage[3] = {b->ctmp, b->UQm, b->VQm};
     PlaneJobs jc{}, jr{};
     for (int i = 0; i < nf; ++i) {
       jc.in[i] = in[i];
-      jc.out[i] = scratch[i];
-      jr.in[i] = scratch[i];
+      jc.out[i] = stage[i];
+      jr.in[i] = stage[i];
       jr.out[i] = out[i];
     }
     int rc = launch_col_fft_n<+1>(b, jc, nf, nplanes);
@@ -2662,6 +2748,17 @@ struct RowFluxL {
     hipLaunchKernelGGL((k_row_flux<NX, E>), grid, dim3(G::T * G::RP), 0,
                        b->lst ? b->lst : b->stream, b->Qm,
                        b->Um, b->Vm, b->UQm, b->VQm, b->dp, b->nz, b->ny, scale, b->tw_x);
+  }
+};
+
+struct RowSpeedL {
+  template <int NX, int E>
+  static void go(qgk_bucket* b, unsigned long long* speed) {
+    using G = FGeo<NX, E>;
+    const double scale = 1.0 / ((double)b->nx * (double)b->ny);
+    dim3 grid((b->ny / 2 + G::RP - 1) / G::RP, b->M * b->nz);
+    hipLaunchKernelGGL((k_row_speed<NX, E>), grid, dim3(G::T * G::RP), 0, b->stream, b->Qm,
+                       b->Um, b->Vm, b->dp, b->nz, b->ny, scale, b->tw_x, speed);
   }
 };
 
@@ -2967,6 +3064,7 @@ int qgk_destroy(qgk_bucket* b) {
 
 int qgk_set_const(qgk_bucket* b, int which, int member, const double* host) {
   if (!b || !host) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "setting a constant")) return rc0;
   if (which < 0 || which >= QGK_NCONSTS) return fail(QGK_E_ARG, "bad constant id %d", which);
   if (member >= b->M) return fail(QGK_E_ARG, "member %d out of range", member);
   const size_t bytes = const_len(b, which) * sizeof(double);
@@ -3005,6 +3103,7 @@ int qgk_set_const(qgk_bucket* b, int which, int member, const double* host) {
 
 int qgk_set_const_all(qgk_bucket* b, int which, const double* host) {
   if (!b || !host) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "setting a constant")) return rc0;
   if (which < 0 || which >= QGK_NCONSTS) return fail(QGK_E_ARG, "bad constant id %d", which);
   const size_t n = const_len(b, which);
   double* tab = nullptr;
@@ -3032,6 +3131,7 @@ int qgk_set_const_all(qgk_bucket* b, int which, const double* host) {
 
 int qgk_set_inversion_2layer(qgk_bucket* b, int member, double F1, double F2) {
   if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "setting the inversion")) return rc0;
   if (b->nz != 2) return fail(QGK_E_ARG, "closed-form inversion needs nz == 2");
   if (member >= b->M) return fail(QGK_E_ARG, "member %d out of range", member);
   for (int m = (member < 0 ? 0 : member); m < (member < 0 ? b->M : member + 1); ++m) {
@@ -3048,6 +3148,7 @@ int qgk_set_inversion_2layer(qgk_bucket* b, int member, double F1, double F2) {
 
 int qgk_set_inversion_tridiagonal(qgk_bucket* b, int member, const double* bands) {
   if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "setting the inversion")) return rc0;
   if (b->nz < 3) return fail(QGK_E_ARG, "the tridiagonal solve is for nz >= 3 (nz=%d)", b->nz);
   if (member >= b->M) return fail(QGK_E_ARG, "member %d out of range", member);
   const int m0 = member < 0 ? 0 : member, m1 = member < 0 ? b->M : member + 1;
@@ -3087,6 +3188,7 @@ int qgk_get_inversion_tridiagonal(qgk_bucket* b, int member, int* on) {
 
 int qgk_set_rek(qgk_bucket* b, int member, double rek) {
   if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "setting rek")) return rc0;
   if (member >= b->M) return fail(QGK_E_ARG, "member %d out of range", member);
   for (int m = (member < 0 ? 0 : member); m < (member < 0 ? b->M : member + 1); ++m)
     b->hp[m].rek = rek;
@@ -3098,6 +3200,7 @@ int qgk_set_rek(qgk_bucket* b, int member, double rek) {
 
 int qgk_put(qgk_bucket* b, int field, const void* host) {
   if (!b || !host) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "qgk_put")) return rc0;
   int rc = check_field(b, field);
   if (rc) return rc;
   const size_t n = (size_t)b->M * b->nz;
@@ -3115,6 +3218,7 @@ int qgk_put(qgk_bucket* b, int field, const void* host) {
 
 int qgk_get(qgk_bucket* b, int field, void* host) {
   if (!b || !host) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "qgk_get")) return rc0;
   int rc = check_field(b, field);
   if (rc) return rc;
   if (field == QGK_Q && (rc = refresh_q(b))) return rc;
@@ -3206,6 +3310,7 @@ int qgk_apply_q_param(qgk_bucket* b) {
 
 int qgk_add_q_param(qgk_bucket* b, const double* dq_host) {
   if (!b || !dq_host) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "a host dq")) return rc0;
   if (!b->field_r[QGK_DQ]) return fail(QGK_E_STATE, "dq is not allocated (no q parameterization)");
   const int np = b->M * b->nz;
   HIPC(hipMemcpyAsync(b->field_r[QGK_DQ], dq_host, (size_t)np * b->nplane_r * sizeof(double),
@@ -3235,6 +3340,7 @@ int qgk_apply_uv_param(qgk_bucket* b) {
 
 int qgk_add_uv_param(qgk_bucket* b, const double* du_host, const double* dv_host) {
   if (!b || !du_host || !dv_host) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "a host du, dv")) return rc0;
   if (!b->field_r[QGK_DU])
     return fail(QGK_E_STATE, "du/dv are not allocated (no uv parameterization)");
   const int np = b->M * b->nz;
@@ -3325,6 +3431,7 @@ int qgk_graph_replay(qgk_bucket* b, int n) {
 
 int qgk_q_to_qh(qgk_bucket* b) {
   if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "qgk_q_to_qh")) return rc0;
   int rc = r2c2d(b, b->field_r[QGK_Q], b->field_c[QGK_QH], b->M * b->nz);
   if (rc) return rc;
   b->primed = false;
@@ -3335,6 +3442,7 @@ int qgk_q_to_qh(qgk_bucket* b) {
 
 int qgk_qh_to_q(qgk_bucket* b) {
   if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "qgk_qh_to_q")) return rc0;
   int rc = c2r2d(b, b->field_c[QGK_QH], b->field_r[QGK_Q], b->M * b->nz);
   if (rc) return rc;
   b->q_stale = false;
@@ -3344,6 +3452,7 @@ int qgk_qh_to_q(qgk_bucket* b) {
 
 int qgk_fft(qgk_bucket* b, int nplanes, const double* in_real, double* out_complex) {
   if (!b || !in_real || !out_complex) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "qgk_fft")) return rc0;
   if (nplanes < 1 || nplanes > b->M * b->nz)
     return fail(QGK_E_ARG, "nplanes must be in [1, %d]", b->M * b->nz);
   HIPC(hipMemcpyAsync(b->rtmp, in_real, (size_t)nplanes * b->nplane_r * sizeof(double),
@@ -3361,6 +3470,7 @@ int qgk_fft(qgk_bucket* b, int nplanes, const double* in_real, double* out_compl
 
 int qgk_ifft(qgk_bucket* b, int nplanes, const double* in_complex, double* out_real) {
   if (!b || !in_complex || !out_real) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "qgk_ifft")) return rc0;
   if (nplanes < 1 || nplanes > b->M * b->nz)
     return fail(QGK_E_ARG, "nplanes must be in [1, %d]", b->M * b->nz);
   HIPC(copy_spectral(b, b->ctmp, in_complex, (size_t)nplanes * b->nl, hipMemcpyHostToDevice));
@@ -3375,6 +3485,7 @@ int qgk_ifft(qgk_bucket* b, int nplanes, const double* in_complex, double* out_r
 
 int qgk_timing_enable(qgk_bucket* b, int enable) {
   if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "qgk_timing_enable")) return rc0;
   int rc = t_collect(b);
   if (rc) return rc;
   b->timing = enable != 0;
@@ -3385,6 +3496,7 @@ int qgk_timing_enable(qgk_bucket* b, int enable) {
 
 int qgk_timing(qgk_bucket* b, double* ms_row, long* n_row, double* ms_col, long* n_col) {
   if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "qgk_timing")) return rc0;
   int rc = t_collect(b);
   if (rc) return rc;
   if (ms_row) *ms_row = b->ms_row;
@@ -3396,6 +3508,7 @@ int qgk_timing(qgk_bucket* b, double* ms_row, long* n_row, double* ms_col, long*
 
 int qgk_sync(qgk_bucket* b) {
   if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "qgk_sync")) return rc0;
   HIPC(hipStreamSynchronize(b->stream));
   return 0;
 }
@@ -3469,6 +3582,7 @@ int qgk_step_buckets(qgk_bucket** bs, int nb, int nsteps, const double* coeffs, 
   for (int i = 0; i < nb; ++i) {
     qgk_bucket* b = bs[i];
     if (!b) return fail(QGK_E_ARG, "NULL bucket");
+    if (int rc0 = refuse_in_capture(b, "a fused step")) return rc0;
     int rc = upload_params(b);
     if (rc) return rc;
     if (!b->primed) {
@@ -3610,6 +3724,7 @@ int qgk_pattern_probe(int ny, int ldk, int members, int nz, int iters, double* m
 
 int qgk_set_variant(qgk_bucket* b, int erow, int ecol) {
   if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "qgk_set_variant")) return rc0;
   auto okr = [](int e) { return e == 4 || e == 8 || e == 16; };
   if (!okr(erow) || !okr(ecol)) return fail(QGK_E_ARG, "radix must be 4, 8 or 16");
   b->erow = erow;
@@ -3621,6 +3736,7 @@ int qgk_set_variant(qgk_bucket* b, int erow, int ecol) {
 
 int qgk_set_layer_split(qgk_bucket* b, int on) {
   if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "qgk_set_layer_split")) return rc0;
   if (on && b->nz != 2) return fail(QGK_E_ARG, "the layer-split column kernel needs nz == 2");
   b->csplit = on != 0;
   b->gvalid = false;
@@ -3644,6 +3760,7 @@ int qgk_get_variant(qgk_bucket* b, int* erow, int* ecol) {
 int qgk_autotune(qgk_bucket* b, int reps, int* erow, int* ecol, double* ms_row3,
                  double* ms_col3) {
   if (!b || reps < 1 || !erow || !ecol) return fail(QGK_E_ARG, "bad arguments");
+  if (int rc0 = refuse_in_capture(b, "qgk_autotune")) return rc0;
   int rc = upload_params(b);
   if (rc) return rc;
   const size_t bytes = (size_t)b->M * b->nz * b->nplane_c * sizeof(cd);
@@ -3736,6 +3853,7 @@ int qgk_device_name(char* buf, int len) {
 
 int qgk_put_device(qgk_bucket* b, int field, const void* dptr) {
   if (!b || !dptr) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "qgk_put_device")) return rc0;
   int rc = check_field(b, field);
   if (rc) return rc;
   const size_t n = (size_t)b->M * b->nz;
@@ -3754,6 +3872,7 @@ int qgk_put_device(qgk_bucket* b, int field, const void* dptr) {
 
 int qgk_get_device(qgk_bucket* b, int field, void* dptr) {
   if (!b || !dptr) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "qgk_get_device")) return rc0;
   int rc = check_field(b, field);
   if (rc) return rc;
   if (field == QGK_Q && (rc = refresh_q(b))) return rc;
@@ -3772,6 +3891,8 @@ int qgk_get_device(qgk_bucket* b, int field, void* dptr) {
 int qgk_coarsen(qgk_bucket* hi, qgk_bucket* lo, int field_hi, int field_lo,
                 const double* filter_lo) {
   if (!hi || !lo) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(hi, "qgk_coarsen")) return rc0;
+  if (int rc0 = refuse_in_capture(lo, "qgk_coarsen")) return rc0;
   if (hi->M != lo->M || hi->nz != lo->nz || hi->nx != hi->ny || lo->nx != lo->ny ||
       lo->nx > hi->nx)
     return fail(QGK_E_ARG, "coarsening needs equal members/layers and square grids, lo <= hi");
@@ -3805,6 +3926,7 @@ int qgk_coarsen(qgk_bucket* hi, qgk_bucket* lo, int field_hi, int field_lo,
 
 int qgk_subgrid_forcing(qgk_bucket* hi, qgk_bucket* lo, const double* filter_lo) {
   if (!hi || !lo) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(hi, "qgk_subgrid_forcing")) return rc0;
   if (!(hi->flags & QGK_FULL_STATE) || !(lo->flags & QGK_FULL_STATE) || !lo->field_r[QGK_DQ])
     return fail(QGK_E_STATE,
                 "subgrid forcing needs full-state handles and dq on the coarse one");
@@ -3884,6 +4006,48 @@ int qgk_add_forcing_device(qgk_bucket* b, const float* net_out, int coutp, const
 
 int qgk_step(qgk_bucket* b, int nsteps, const double* coeffs, int use_graph) {
   return qgk_step_buckets(&b, 1, nsteps, coeffs, use_graph);
+}
+
+int qgk_member_speed(qgk_bucket* b, double* speed_host) {
+  if (!b || !speed_host) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "qgk_member_speed")) return rc0;
+  int rc = upload_params(b);
+  if (rc) return rc;
+  if (!b->speed && (rc = dev_alloc(b, (void**)&b->speed, b->M * sizeof(unsigned long long))))
+    return rc;
+  if (!b->primed) {
+    if ((rc = launch_col_prologue(b))) return rc;
+    b->primed = true;
+  }
+  HIPC(hipMemsetAsync(b->speed, 0, b->M * sizeof(unsigned long long), b->stream));
+  if ((rc = with_ne<RowSpeedL>(b->nx, b->erow, b, b->speed))) return rc;
+  HIPC(hipGetLastError());
+  std::vector<unsigned long long> w(b->M);
+  HIPC(hipMemcpyAsync(w.data(), b->speed, b->M * sizeof(unsigned long long),
+                      hipMemcpyDeviceToHost, b->stream));
+  HIPC(hipStreamSynchronize(b->stream));
+  for (int m = 0; m < b->M; ++m) std::memcpy(speed_host + m, &w[m], sizeof(double));
+  return 0;
+}
+
+int qgk_mask_member(qgk_bucket* b, int member) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "qgk_mask_member")) return rc0;
+  if (member < 0 || member >= b->M) return fail(QGK_E_ARG, "member %d out of range", member);
+  const size_t cm = (size_t)b->nz * b->nplane_c, rm = (size_t)b->nz * b->nplane_r;
+  cd* cs[] = {b->field_c[QGK_QH], b->qh_alt, b->hist[0], b->hist[1], b->hist[2], b->Qm, b->Um,
+              b->Vm, b->UQm, b->VQm, b->phm, b->field_c[QGK_PH], b->field_c[QGK_UH],
+              b->field_c[QGK_VH], b->field_c[QGK_UQH], b->field_c[QGK_VQH],
+              b->field_c[QGK_DQH], b->field_c[QGK_DUH], b->field_c[QGK_DVH]};
+  double* rs[] = {b->field_r[QGK_Q], b->field_r[QGK_U], b->field_r[QGK_V], b->field_r[QGK_UQ],
+                  b->field_r[QGK_VQ], b->field_r[QGK_DQ], b->field_r[QGK_DU],
+                  b->field_r[QGK_DV]};
+  for (cd* f : cs)
+    if (f) HIPC(hipMemsetAsync(f + member * cm, 0, cm * sizeof(cd), b->stream));
+  for (double* f : rs)
+    if (f) HIPC(hipMemsetAsync(f + member * rm, 0, rm * sizeof(double), b->stream));
+  HIPC(hipStreamSynchronize(b->stream));
+  return 0;
 }
 
 }  // extern "C"
@@ -3993,6 +4157,7 @@ extern "C" {
 
 int qgk_diag_set_params(qgk_bucket* b, int member, const double* params) {
   if (!b || !params) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "diagnostics")) return rc0;
   if (member < 0 || member >= b->M) return fail(QGK_E_ARG, "member %d out of range", member);
   if (b->nz > kDiagNzMax) return fail(QGK_E_ARG, "diagnostics support nz <= %d", kDiagNzMax);
   const int plen = diag_plen(b);
@@ -4011,6 +4176,7 @@ int qgk_diag_set_params(qgk_bucket* b, int member, const double* params) {
 
 int qgk_diag_reset(qgk_bucket* b) {
   if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "diagnostics")) return rc0;
   const size_t plane = (size_t)b->nl * b->ldk;
   if (b->dacc)
     HIPC(hipMemsetAsync(b->dacc, 0, (size_t)b->nslot * b->M * plane * sizeof(double), b->stream));
@@ -4022,6 +4188,7 @@ int qgk_diag_reset(qgk_bucket* b) {
 int qgk_diag_accumulate(qgk_bucket* b, unsigned int mask, int instant, double dt, double dt1,
                         double dt2, double dt3) {
   if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "diagnostics")) return rc0;
   int rc = need_full(b, "diagnostics");
   if (rc) return rc;
   if (b->nz > kDiagNzMax) return fail(QGK_E_ARG, "diagnostics support nz <= %d", kDiagNzMax);
@@ -4147,6 +4314,7 @@ int qgk_diag_accumulate(qgk_bucket* b, unsigned int mask, int instant, double dt
 
 int qgk_diag_read(qgk_bucket* b, int diag, int member, int instant, double* host) {
   if (!b || !host) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "diagnostics")) return rc0;
   if (diag < 0 || diag >= QGK_NDIAG) return fail(QGK_E_ARG, "bad diagnostic id %d", diag);
   diag_layout(b);
   if (b->dslot[diag] < 0)
@@ -4391,6 +4559,7 @@ int qgk_param_smagorinsky(qgk_bucket* b, const double* dx, double constant, doub
 int qgk_smagorinsky_viscosity(qgk_bucket* b, const double* dx, double constant,
                               double* nu_host) {
   if (!b || !dx || !nu_host) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "qgk_smagorinsky_viscosity")) return rc0;
   int rc;
   if ((rc = param_setup(b, dx, nullptr, true)) || (rc = smag_viscosity(b, constant))) return rc;
   const size_t n = (size_t)b->M * b->nz * b->nplane_r;

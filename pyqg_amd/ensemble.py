@@ -42,6 +42,8 @@ class Ensemble:
         self.diag_count = {}
         self.closure_dx = []   # per bucket: grid spacing [M] and layer thicknesses [M][nz]
         self.closure_Hi = []
+        self.failed = set()    # members masked after a failed health check
+        self.cfl = None        # per member: CFL number at the last health check
 
     @property
     def dt(self):
@@ -188,9 +190,22 @@ class Ensemble:
         return [out[bi][j] for (bi, j) in self.members]
 
     # ------------------------------------------------------------ stepping
-    def step(self, nsteps=1, use_graph=False):
-        """nsteps fused steps of every member."""
+    def step(self, nsteps=1, use_graph=False, twrite=None, on_failure="mask"):
+        """nsteps fused steps of every member.  With twrite, every member's
+        health is checked whenever tc reaches a multiple of twrite, as the
+        reference asserts CFL < 1 at its status steps (model.py:564-574);
+        check_members says what happens to a member that fails."""
         nsteps = int(nsteps)
+        while nsteps > 0:
+            n = nsteps
+            if twrite:
+                n = min(n, int(twrite) - self.tc % int(twrite))
+            self._step(n, use_graph)
+            nsteps -= n
+            if twrite and self.tc % int(twrite) == 0:
+                self.check_members(on_failure)
+
+    def _step(self, nsteps, use_graph):
         if nsteps <= 0:
             return
         coeffs = np.empty((nsteps, 3))
@@ -201,6 +216,38 @@ class Ensemble:
         for _ in range(nsteps):
             self.t += self._dt
         self.tc += nsteps
+
+    def member_cfl(self):
+        """CFL number max(|u + Ubg|, |v|) dt / dx of every member's current
+        state (qg_diagnostics.py:76-79), computed on the device for all members
+        at once; NaN or inf for a member whose state is not finite."""
+        cfl = np.empty(self.nmembers)
+        for bi, (b, dx) in enumerate(zip(self.buckets, self.closure_dx)):
+            sp = b.member_speed()
+            for j, m in enumerate(self._bucket_members(bi)):
+                cfl[m] = sp[j] * self._dt / dx[j]
+        return cfl
+
+    def check_members(self, on_failure="mask"):
+        """Health check of every member: a member fails when its CFL number is
+        not below 1 or its state is not finite.  on_failure="mask" zeroes a
+        failed member's state (it rests at q = 0 and no longer feeds NaNs into
+        member sums) and records it in self.failed; "raise" raises the
+        reference's AssertionError instead.  Returns the members that failed
+        in this check."""
+        if on_failure not in ("mask", "raise"):
+            raise ValueError("on_failure must be 'mask' or 'raise'")
+        self.cfl = cfl = self.member_cfl()
+        bad = [m for m in range(self.nmembers)
+               if m not in self.failed and not (np.isfinite(cfl[m]) and cfl[m] < 1.0)]
+        if bad and on_failure == "raise":
+            raise AssertionError(f"CFL condition violated in members {bad} "
+                                 f"(CFL {[float(cfl[m]) for m in bad]})")
+        for m in bad:
+            bi, j = self.members[m]
+            self.buckets[bi].mask_member(j)
+            self.failed.add(m)
+        return bad
 
     def autotune(self, reps=5, cache_path=None):
         """Pick the fastest register radix of the fused row and column kernels
@@ -256,7 +303,9 @@ class Ensemble:
             from .parameterizations import _capture_steps
             uv = p.parameterization_type == 'uv_parameterization'
             w, self.ablevel = C.ab_weights(self.ablevel, self._dt)
-            key = (id(p), repr(p), tuple(w))
+            key = (p._device_key(), tuple(w),
+                   tuple(np.asarray(x, dtype=float).tobytes() for x in self.closure_dx),
+                   tuple(np.asarray(x, dtype=float).tobytes() for x in self.closure_Hi))
             L = _lib.lib
 
             def replayable():

@@ -180,7 +180,9 @@ class Model(DiagnosticsMixin, PseudoSpectralKernel):
             b, dx, Hi = _device_target(self)
             qp, uvp = self.q_parameterization, self.uv_parameterization
             w, self._ablevel = C.ab_weights(self._ablevel, self._dt)
-            key = (id(qp), repr(qp), id(uvp), repr(uvp), tuple(w), dx.tobytes(), Hi.tobytes())
+            key = (None if qp is None else qp._device_key(),
+                   None if uvp is None else uvp._device_key(), tuple(w), dx.tobytes(),
+                   Hi.tobytes())
             self._flush()
             ok = getattr(self, "_graph_key", None) == key
             if not ok or L.qgk_graph_replay(b.h, 0) != 0:

@@ -87,6 +87,12 @@ class Parameterization(ABC):
     def _device_bucket(self, b, dx, Hi, weight, accumulate):
         raise NotImplementedError
 
+    def _device_key(self):
+        """Every value a device evaluation bakes into a captured step graph, at
+        full precision (floats compare exactly); a kept graph is replayed only
+        while this key is unchanged."""
+        raise NotImplementedError
+
     def _device(self, m, weight=1.0, accumulate=False):
         b, dx, Hi = _require_device(m, type(self).__name__)
         self._device_bucket(b, dx, Hi, weight, accumulate)
@@ -114,6 +120,9 @@ class CompositeParameterization(Parameterization):
         for i, p in enumerate(self.params):
             p._device_bucket(b, dx, Hi, weight, accumulate or i > 0)
 
+    def _device_key(self):
+        return ("Composite",) + tuple(p._device_key() for p in self.params)
+
     def __repr__(self):
         return f"CompositeParameterization{self.params}"
 
@@ -137,6 +146,9 @@ class WeightedParameterization(Parameterization):
 
     def _device_bucket(self, b, dx, Hi, weight, accumulate):
         self.param._device_bucket(b, dx, Hi, weight * self.weight, accumulate)
+
+    def _device_key(self):
+        return ("Weighted", float(self.weight), self.param._device_key())
 
     def __repr__(self):
         return f"{self.weight} * {self.param}"
@@ -186,6 +198,9 @@ class Smagorinsky(UVParameterization):
                                                   float(weight), int(bool(accumulate)), None,
                                                   None))
 
+    def _device_key(self):
+        return ("Smagorinsky", float(self.constant))
+
     def __repr__(self):
         return f"Smagorinsky(Cs={self.constant})"
 
@@ -218,6 +233,10 @@ class BackscatterBiharmonic(QParameterization):
     def _device_bucket(self, b, dx, Hi, weight, accumulate):
         self._run(b, dx, Hi, weight, accumulate, None)
 
+    def _device_key(self):
+        return ("BackscatterBiharmonic", float(self.smagorinsky.constant),
+                float(self.back_constant), float(self.eps))
+
     def __repr__(self):
         return (f"BackscatterBiharmonic(Cs={self.smagorinsky.constant}, "
                 f"Cb={self.back_constant})")
@@ -245,6 +264,9 @@ class ZannaBolton2020(UVParameterization):
     def _device_bucket(self, b, dx, Hi, weight, accumulate):
         _lib.check(_lib.lib.qgk_param_zb2020(b.h, float(self.constant), float(weight),
                                              int(bool(accumulate)), None, None))
+
+    def _device_key(self):
+        return ("ZannaBolton2020", float(self.constant))
 
     def __repr__(self):
         return f"ZannaBolton2020(κ={self.constant:.2e})"

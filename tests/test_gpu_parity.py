@@ -189,33 +189,38 @@ def test_full_size_invariants_headline_ensemble(pq):
     assert all(np.isfinite(x).all() for x in qh)
 
 
-@pytest.mark.parametrize("erow,ecol", [(4, 4), (4, 8), (8, 16), (16, 4), (16, 16)])
+@pytest.mark.parametrize("erow,ecol", [(4, 4), (4, 8), (8, 16), (16, 4), (16, 16), (8, 8)])
 def test_fused_radix_variants(pq, erow, ecol):
-    """Every register radix of the fused kernels gives the oracle's trajectory
-    (two-layer register path, one-layer register path, generic nz path)."""
+    """Every register radix of the fused kernels gives the oracle's trajectory:
+    the two-layer register column kernel (layer split off) and the layer-split
+    one, the one-layer register path and the generic nz path."""
     nsteps = 12
-    ens = pq.Ensemble.qg(np.array([32, 64, 64]), dt=3600.)
-    for b in ens.buckets:
-        b.set_variant(erow, ecol)
     q0 = [seeded_q(2, n, 40 + i) for i, n in enumerate([32, 64, 64])]
-    ens.set_q(q0)
-    ens.step(nsteps)
+    refs = []
     for i, n in enumerate([32, 64, 64]):
         o = OracleQG(nx=n, dt=3600., **QG_PHYS)
         o.q = q0[i]
         for _ in range(nsteps):
             o._step_forward()
-        assert rel_l2(ens.get("q", i), o.q) < TOL_TRAJ
-    # one two-layer member: the layer-split column kernel at this radix
-    qm = pq.QGModel(nx=64, dt=3600., log_level=0, **QG_PHYS)
-    qm._bucket.set_variant(erow, ecol)
-    oq = OracleQG(nx=64, dt=3600., **QG_PHYS)
-    qm.q = q0[1]
-    oq.q = q0[1]
-    qm._fused_steps(nsteps)
-    for _ in range(nsteps):
-        oq._step_forward()
-    assert rel_l2(qm.q, oq.q) < TOL_TRAJ
+        refs.append(o.q.copy())
+    for split in (True, False):
+        ens = pq.Ensemble.qg(np.array([32, 64, 64]), dt=3600.)
+        for b in ens.buckets:
+            b.set_variant(erow, ecol)
+            b.set_layer_split(split)
+            assert b.layer_split() == split
+        ens.set_q(q0)
+        ens.step(nsteps)
+        for i in range(3):
+            assert rel_l2(ens.get("q", i), refs[i]) < TOL_TRAJ, (split, i)
+    # one two-layer member: both column kernels at this radix
+    for split in (True, False):
+        qm = pq.QGModel(nx=64, dt=3600., log_level=0, **QG_PHYS)
+        qm._bucket.set_variant(erow, ecol)
+        qm._bucket.set_layer_split(split)
+        qm.q = q0[1]
+        qm._fused_steps(nsteps)
+        assert rel_l2(qm.q, refs[1]) < TOL_TRAJ, split
     from oracle.qg_oracle import OracleBT
     bt = pq.BTModel(L=2 * np.pi, nx=64, beta=0.5, rd=0.3, dt=0.005, log_level=0)
     bt._bucket.set_variant(erow, ecol)
