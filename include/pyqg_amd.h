@@ -156,6 +156,17 @@ int qgk_step(qgk_bucket* b, int nsteps, const double* coeffs, int use_graph);
  * of carrying NaNs into bucket-wide sums. */
 int qgk_member_speed(qgk_bucket* b, double* speed_host);
 int qgk_mask_member(qgk_bucket* b, int member);
+/* qgk_member_speed without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  speed_dev is a
+ * caller-provided DEVICE buffer of M doubles; the result equals
+ * qgk_member_speed's.  The per-member words live in a static device array of
+ * 4096 entries (32 KB per GPU, held until the process ends), so buckets of more
+ * than 4096 members are rejected with QGK_E_ARG.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (different streams or threads)
+ * race.  Returns after the work is done, except under graph capture of the
+ * bucket's stream, where it only enqueues. */
+int qgk_member_speed_noalloc(qgk_bucket* b, double* speed_dev);
 
 /* Kernel timing for the measurement harness: when enabled, the fused step
  * records HIP events around each of its kernels; qgk_timing returns the

@@ -38,7 +38,7 @@ EXPORTS = ["qgk_create", "qgk_destroy", "qgk_last_error", "qgk_set_const", "qgk_
            "qgk_diag_set_params", "qgk_diag_accumulate", "qgk_diag_read", "qgk_diag_reset",
            "qgk_param_smagorinsky", "qgk_smagorinsky_viscosity", "qgk_param_zb2020",
            "qgk_param_backscatter", "qgk_apply_q_param", "qgk_apply_uv_param",
-           "qgk_member_speed", "qgk_mask_member"]
+           "qgk_member_speed", "qgk_mask_member", "qgk_member_speed_noalloc"]
 
 # device diagnostics, in the order of the qgk_diag enum
 DIAGS = ["Ensspec", "KEspec", "EKEdiss", "KEfrictionspec", "EKE", "Dissspec", "ENSDissspec",
@@ -140,6 +140,7 @@ def _load():
         "qgk_apply_uv_param": (I, [P]),
         "qgk_member_speed": (I, [P, P]),
         "qgk_mask_member": (I, [P, I]),
+        "qgk_member_speed_noalloc": (I, [P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

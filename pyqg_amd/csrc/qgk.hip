@@ -23,6 +23,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <utility>
 #include <string>
 #include <vector>
@@ -859,6 +860,24 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP)) void k_row_spee
     w = o > w ? o : w;
   }
   if ((threadIdx.x & 63) == 0) atomicMax(speed + m, w);
+}
+
+// Scratch of the allocation-free health check (qgk_member_speed_noalloc): one
+// health word per member in a file-scope device array, so the call needs no
+// device allocation.  Host code uses its per-device address from
+// hipGetSymbolAddress, never the array's name.
+constexpr int kNoAllocMaxMembers = 4096;
+static __device__ __align__(256) unsigned long long scratch[kNoAllocMaxMembers];
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, int n, unsigned long long v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_u64_to_double(const unsigned long long* __restrict__ w,
+                                double* __restrict__ out, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = __longlong_as_double((long long)w[i]);
 }
 
 // ---- any nz: the inversion re-reads qh from global memory (cache hits)
@@ -2444,6 +2463,27 @@ int upload_params(qgk_bucket* b) {
   return 0;
 }
 
+// Device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array).
+constexpr int kMaxDevices = 64;
+unsigned long long* g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_address(unsigned long long** out) {
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(QGK_E_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(QGK_E_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_dev[dev] = static_cast<unsigned long long*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
 cd* D(qgk_bucket* b) { return b->hist[b->iD]; }
 cd* Pp(qgk_bucket* b) { return b->hist[b->iP]; }
 cd* PP(qgk_bucket* b) { return b->hist[b->iPP]; }
@@ -2978,6 +3018,10 @@ int qgk_create(int nmembers, int nz, int ny, int nx, int flags, qgk_bucket** out
   };
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(QGK_E_HIP, "hipStreamCreate failed"));
+  {  // set-up of the allocation-free entry point's scratch address on this device
+    unsigned long long* w = nullptr;
+    if ((rc = scratch_address(&w))) return cleanup(rc);
+  }
   const size_t nc = (size_t)nmembers * nz * b->nplane_c * sizeof(cd);
   const size_t nr = (size_t)nmembers * nz * b->nplane_r * sizeof(double);
   auto ac = [&](cd** p) { return dev_alloc(b, (void**)p, nc); };
@@ -4027,6 +4071,27 @@ int qgk_member_speed(qgk_bucket* b, double* speed_host) {
                       hipMemcpyDeviceToHost, b->stream));
   HIPC(hipStreamSynchronize(b->stream));
   for (int m = 0; m < b->M; ++m) std::memcpy(speed_host + m, &w[m], sizeof(double));
+  return 0;
+}
+
+int qgk_member_speed_noalloc(qgk_bucket* b, double* speed_dev) {
+  if (!b || !speed_dev) return fail(QGK_E_ARG, "NULL argument");
+  if (b->M > kNoAllocMaxMembers)
+    return fail(QGK_E_ARG, "qgk_member_speed_noalloc takes at most %d members (bucket has %d)",
+                kNoAllocMaxMembers, b->M);
+  unsigned long long* w = nullptr;
+  int rc = scratch_address(&w);
+  if (rc || (rc = upload_params(b))) return rc;
+  if (!b->primed) {
+    if ((rc = launch_col_prologue(b))) return rc;
+    b->primed = true;
+  }
+  const int blocks = (b->M + 255) / 256;
+  hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, b->stream, w, b->M, 0ULL);
+  if ((rc = with_ne<RowSpeedL>(b->nx, b->erow, b, w))) return rc;
+  hipLaunchKernelGGL(k_u64_to_double, dim3(blocks), dim3(256), 0, b->stream, w, speed_dev, b->M);
+  HIPC(hipGetLastError());
+  HIPC(stream_done(b));
   return 0;
 }
 
