@@ -178,7 +178,7 @@ struct RowSwizzled {
 // One Stockham pass of radix R on sub-transforms of length L (L*R <= N).
 // Butterfly b (0 <= b < N/R) reads positions b + s*N/R, writes k*L*R + j + m*L
 // with j = b % L, k = b / L, after twiddling input s by w^(j*s*N/(L*R)).
-template <int N, int E, int R, int L, int SIGN, class LY, bool FIRST, bool LAST>
+template <int N, int E, int R, int L, int SIGN, class LY, bool FIRST, bool LAST, int TAG>
 __device__ __forceinline__ void stockham_pass(cd (&x)[E], cd* lds, int t, int lane,
                                               const cd* __restrict__ tw) {
   constexpr int T = N / E;
@@ -236,7 +236,7 @@ __device__ __forceinline__ void stockham_pass(cd (&x)[E], cd* lds, int t, int la
 
 constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
 
-template <int N, int E, int SIGN, class LY, int L>
+template <int N, int E, int SIGN, class LY, int L, int TAG>
 struct StockhamFrom {
   static constexpr int REM = N / L;
   // the largest radix that divides both what is left of N and E (E itself
@@ -245,27 +245,32 @@ struct StockhamFrom {
   static_assert(R > 1, "E must hold every prime factor of N");
   __device__ __forceinline__ static void run(cd (&x)[E], cd* lds, int t, int lane,
                                              const cd* __restrict__ tw) {
-    stockham_pass<N, E, R, L, SIGN, LY, (L == 1), (L * R == N)>(x, lds, t, lane, tw);
-    if constexpr (L * R < N) StockhamFrom<N, E, SIGN, LY, L * R>::run(x, lds, t, lane, tw);
+    stockham_pass<N, E, R, L, SIGN, LY, (L == 1), (L * R == N), TAG>(x, lds, t, lane, tw);
+    if constexpr (L * R < N) StockhamFrom<N, E, SIGN, LY, L * R, TAG>::run(x, lds, t, lane, tw);
   }
 };
 
 // Full transform.  All threads of the block must call it (it contains barriers
 // whenever N > E).  lds must hold N*LANES complex values.
-template <int N, int E, int SIGN, int LANES>
+//
+// TAG only makes distinct instantiations: a hot kernel passes its own TAG so
+// that no other kernel shares its transform code.  When two kernels share one
+// instantiation the compiler has generated different, worse code for one of
+// them (generic instead of global/LDS addressing, and register spills).
+template <int N, int E, int SIGN, int LANES, int TAG = 0>
 __device__ __forceinline__ void fft_block(cd (&x)[E], cd* lds, int t, int lane,
                                           const cd* __restrict__ tw) {
   static_assert(N % E == 0, "E must divide N");
-  StockhamFrom<N, E, SIGN, Interleaved<LANES>, 1>::run(x, lds, t, lane, tw);
+  StockhamFrom<N, E, SIGN, Interleaved<LANES>, 1, TAG>::run(x, lds, t, lane, tw);
 }
 
 // As fft_block, for transforms along rows: transform `row` of the block keeps
 // its own swizzled LDS region (RowSwizzled<N>); lds must hold N*rows values.
-template <int N, int E, int SIGN>
+template <int N, int E, int SIGN, int TAG = 0>
 __device__ __forceinline__ void fft_block_rows(cd (&x)[E], cd* lds, int t, int row,
                                                const cd* __restrict__ tw) {
   static_assert(N % E == 0 && N % 8 == 0, "E must divide N; rows are multiples of 8");
-  StockhamFrom<N, E, SIGN, RowSwizzled<N>, 1>::run(x, lds, t, row, tw);
+  StockhamFrom<N, E, SIGN, RowSwizzled<N>, 1, TAG>::run(x, lds, t, row, tw);
 }
 
 }  // namespace qgk

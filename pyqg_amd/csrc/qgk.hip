@@ -625,6 +625,10 @@ struct FGeo {
   static constexpr int RP = RPraw < 1 ? 1 : RPraw;
 };
 
+// Transform instantiation tags of the fused kernels (see fft_block): each hot
+// kernel compiles its own copy of the transforms.
+constexpr int kTagRow = 1, kTagColReg = 2, kTagColSplit = 3, kTagColPh = 4;
+
 // Streaming accesses.  The fused step touches every spectral field once per
 // step, so its loads (and the column kernels' stores) carry the non-temporal
 // hint: the L2 is left to the small tables every block re-reads (filter,
@@ -761,11 +765,11 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP), row_flux_waves<
   }
   cd zq[G::E], zu[G::E], zv[G::E];
   row_pair_extend<NX, G::E, G::T, G::RP>(hq, Qm + row0, NK, zq, lds, t, rp);
-  qgk::fft_block_rows<NX, G::E, +1>(zq, lds, t, rp, tw);
+  qgk::fft_block_rows<NX, G::E, +1, kTagRow>(zq, lds, t, rp, tw);
   row_pair_extend<NX, G::E, G::T, G::RP>(hu, Um + row0, NK, zu, lds, t, rp);
-  qgk::fft_block_rows<NX, G::E, +1>(zu, lds, t, rp, tw);
+  qgk::fft_block_rows<NX, G::E, +1, kTagRow>(zu, lds, t, rp, tw);
   row_pair_extend<NX, G::E, G::T, G::RP>(hv, Vm + row0, NK, zv, lds, t, rp);
-  qgk::fft_block_rows<NX, G::E, +1>(zv, lds, t, rp, tw);
+  qgk::fft_block_rows<NX, G::E, +1, kTagRow>(zv, lds, t, rp, tw);
   // fluxes of both rows: uq = (u + Ubg) q, vq = v q (kernel.pyx:372-378)
 #pragma unroll
   for (int s = 0; s < G::E; ++s) {
@@ -775,13 +779,13 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP), row_flux_waves<
     zu[s] = {(u0 + ubg) * q0, (u1 + ubg) * q1};
     zv[s] = {v0 * q0, v1 * q1};
   }
-  qgk::fft_block_rows<NX, G::E, -1>(zu, lds, t, rp, tw);
+  qgk::fft_block_rows<NX, G::E, -1, kTagRow>(zu, lds, t, rp, tw);
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < G::E; ++s) lds[qgk::RowSwizzled<NX>::at(t, s * G::T, rp)] = zu[s];
   __syncthreads();
   split_store_t<NX, G::T, G::RP>(lds, t, rp, ok, UQm + row0, UQm + row0 + NK);
-  qgk::fft_block_rows<NX, G::E, -1>(zv, lds, t, rp, tw);
+  qgk::fft_block_rows<NX, G::E, -1, kTagRow>(zv, lds, t, rp, tw);
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < G::E; ++s) lds[qgk::RowSwizzled<NX>::at(t, s * G::T, rp)] = zv[s];
@@ -791,67 +795,34 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP), row_flux_waves<
 
 // Health check of every member: the reference asserts CFL < 1 every twrite
 // steps (model.py:564-574 with qg_diagnostics.py:76-79), CFL being
-// max|u + Ubg|, |v| over the grid times dt/dx.  Here one pass over a bucket
-// gives each member's largest speed from the primed x-mixed q, u, v, by the
-// row kernel's inverse x transforms.  The maximum is kept as the bit pattern of
-// a non-negative double, which orders like its value, so an unsigned atomic
-// max builds it; |NaN| has a larger pattern than +inf, so a NaN in u or v, or
-// a non-finite q anywhere in the member, leaves a NaN (or inf) in its word.
+// max|u + Ubg|, |v| over the grid times dt/dx.  Here each member's largest
+// speed comes from the primed x-mixed q, u, v, transformed back to real space
+// one field at a time.  The maximum is kept as the bit pattern of a
+// non-negative double, which orders like its value, so an unsigned atomic max
+// builds it; |NaN| has a larger pattern than +inf, so a NaN in u or v, or a
+// non-finite q anywhere in the member, leaves a NaN (or inf) in its word.
 __device__ __forceinline__ unsigned long long abs_bits(double x) {
   return (unsigned long long)__double_as_longlong(x) & 0x7fffffffffffffffULL;
 }
 
-template <int NX, int E>
-__global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP)) void k_row_speed(
-    const cd* __restrict__ Qm, const cd* __restrict__ Um, const cd* __restrict__ Vm,
-    const MemberParams* __restrict__ params, int nz, int ny, double scale,
-    const cd* __restrict__ tw, unsigned long long* __restrict__ speed) {
-  using G = FGeo<NX, E>;
-  constexpr int NK = ldk_of(NX);
+// One field of every member in real space (the x inverse transform of its
+// x-mixed copy, [M][nz][ny][nx]) folded into the members' health words: mode 0
+// adds the layer's Ubg (u), mode 1 takes the value (v), mode 2 only flags
+// non-finite values (q).  Block per (part of a plane); a wave maximum, then
+// one atomic per wave.
+__global__ __launch_bounds__(256) void k_speed_fold(const double* __restrict__ r, int nplane,
+                                                    int nz, const MemberParams* __restrict__ params,
+                                                    int mode, unsigned long long* __restrict__ speed) {
   constexpr unsigned long long kInf = 0x7ff0000000000000ULL;
-  __shared__ cd lds[NX * G::RP];
-  __shared__ cd twl[NX];
-  tw = stage_twiddles<NX>(twl, tw);
-  const int t = threadIdx.x % G::T;
-  const int rp = threadIdx.x / G::T;
-  const int jp = blockIdx.x * G::RP + rp;
-  const int j0 = 2 * jp < ny ? 2 * jp : 0;  // spare row pairs repeat rows 0 and 1
   const int plane_id = blockIdx.y;
   const int m = plane_id / nz, z = plane_id - m * nz;
-  const double ubg = params[m].Ubg[z];
-  const size_t row0 = ((size_t)plane_id * ny + j0) * NK;
-  constexpr int H = G::E / 2;
-  cd hq[2][H], hu[2][H], hv[2][H];
-#pragma unroll
-  for (int s = 0; s < H; ++s) {
-    const int p = t + s * G::T;
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      hq[r][s] = ld_stream(Qm + row0 + r * NK + p);
-      hu[r][s] = ld_stream(Um + row0 + r * NK + p);
-      hv[r][s] = ld_stream(Vm + row0 + r * NK + p);
-    }
-  }
-  cd zq[G::E], zu[G::E], zv[G::E];
-  row_pair_extend<NX, G::E, G::T, G::RP>(hq, Qm + row0, NK, zq, lds, t, rp);
-  qgk::fft_block_rows<NX, G::E, +1>(zq, lds, t, rp, tw);
-  row_pair_extend<NX, G::E, G::T, G::RP>(hu, Um + row0, NK, zu, lds, t, rp);
-  qgk::fft_block_rows<NX, G::E, +1>(zu, lds, t, rp, tw);
-  row_pair_extend<NX, G::E, G::T, G::RP>(hv, Vm + row0, NK, zv, lds, t, rp);
-  qgk::fft_block_rows<NX, G::E, +1>(zv, lds, t, rp, tw);
+  const double add = mode == 0 ? params[m].Ubg[z] : 0.0;
+  const double* p = r + (size_t)plane_id * nplane;
   unsigned long long w = 0;
-#pragma unroll
-  for (int s = 0; s < G::E; ++s) {
-    const unsigned long long b[6] = {
-        abs_bits(zu[s].x * scale + ubg), abs_bits(zu[s].y * scale + ubg),
-        abs_bits(zv[s].x * scale), abs_bits(zv[s].y * scale),
-        abs_bits(zq[s].x * scale), abs_bits(zq[s].y * scale)};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) w = b[i] > w ? b[i] : w;
-#pragma unroll
-    for (int i = 4; i < 6; ++i) w = (b[i] >= kInf && b[i] > w) ? b[i] : w;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nplane; i += gridDim.x * blockDim.x) {
+    const unsigned long long b = abs_bits(p[i] + add);
+    if (mode < 2 || b >= kInf) w = b > w ? b : w;
   }
-  // wave maximum, then one atomic per wave
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     const unsigned lo = __shfl_xor((unsigned)(w & 0xffffffffULL), off);
@@ -859,7 +830,7 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP)) void k_row_spee
     const unsigned long long o = ((unsigned long long)hi << 32) | lo;
     w = o > w ? o : w;
   }
-  if ((threadIdx.x & 63) == 0) atomicMax(speed + m, w);
+  if ((threadIdx.x & 63) == 0 && w) atomicMax(speed + m, w);
 }
 
 // Scratch of the allocation-free health check (qgk_member_speed_noalloc): one
@@ -1117,7 +1088,7 @@ __device__ __forceinline__ void col_invert_points(const cd* __restrict__ qh, cd*
 }
 
 // y IFFT of q, u = -il ph, v = ik ph of layer z from qh and the ph buffer
-template <int NY, int E>
+template <int NY, int E, int TAG = 0>
 __device__ __forceinline__ void col_prologue_layer_ph(const cd* __restrict__ qh,
                                                       const cd* __restrict__ phm,
                                                       cd* __restrict__ Qm, cd* __restrict__ Um,
@@ -1136,7 +1107,7 @@ __device__ __forceinline__ void col_prologue_layer_ph(const cd* __restrict__ qh,
     x[s] = ok ? qh[zoff + pt] : cd{0.0, 0.0};
     ph[s] = ok ? phm[zoff + pt] : cd{0.0, 0.0};
   }
-  qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+  qgk::fft_block<NY, G::E, +1, G::TK, TAG>(x, lds, t, c, tw);
   {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
     for (int s = 0; s < G::E; ++s) st_stream(Qm + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
@@ -1146,7 +1117,7 @@ __device__ __forceinline__ void col_prologue_layer_ph(const cd* __restrict__ qh,
     const double ll = P.ll[t + s * G::T];
     x[s] = {ll * ph[s].y, -ll * ph[s].x};
   }
-  qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+  qgk::fft_block<NY, G::E, +1, G::TK, TAG>(x, lds, t, c, tw);
   {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
     for (int s = 0; s < G::E; ++s) st_stream(Um + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
@@ -1154,7 +1125,7 @@ __device__ __forceinline__ void col_prologue_layer_ph(const cd* __restrict__ qh,
   const double kk = P.kk[kc];
 #pragma unroll
   for (int s = 0; s < G::E; ++s) x[s] = {-kk * ph[s].y, kk * ph[s].x};
-  qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+  qgk::fft_block<NY, G::E, +1, G::TK, TAG>(x, lds, t, c, tw);
   {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
     for (int s = 0; s < G::E; ++s) st_stream(Vm + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
@@ -1212,8 +1183,8 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
       xa[s] = ok ? ld_stream(UQm + zoff + pt) : cd{0.0, 0.0};
       xb[s] = ok ? ld_stream(VQm + zoff + pt) : cd{0.0, 0.0};
     }
-    qgk::fft_block<NY, G::E, -1, G::TK>(xa, lds, t, c, tw);
-    qgk::fft_block<NY, G::E, -1, G::TK>(xb, lds, t, c, tw);
+    qgk::fft_block<NY, G::E, -1, G::TK, kTagColPh>(xa, lds, t, c, tw);
+    qgk::fft_block<NY, G::E, -1, G::TK, kTagColPh>(xb, lds, t, c, tw);
     if (ok) {
       const double kq = kk * P.Qy[z];
       const bool drag = (z == nz - 1) && P.rek != 0.0;
@@ -1254,12 +1225,20 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
   }
   col_invert_points<NY, E>(qh_new, phm, P, nz, t, k, ok, mbase, ldk);
   for (int z = 0; z < nz; ++z)
-    col_prologue_layer_ph<NY, E>(qh_new, phm, Qm, Um, Vm, P, z, ldk, t, c, k, ok, mbase, lds,
+    col_prologue_layer_ph<NY, E, kTagColPh>(qh_new, phm, Qm, Um, Vm, P, z, ldk, t, c, k, ok, mbase, lds,
                                  tw);
 }
 
 // ---- nz <= 2: every field is read exactly once; the layers' qh and ph at the
 // thread's points stay in registers.
+
+// Loads of the member constants through global-address-space pointers: a
+// pointer read from MemberParams is generic, and generic (flat) loads count
+// against both the memory and the LDS wait counters, so every LDS wait of the
+// transforms would also wait for them.
+__device__ __forceinline__ double gld(const double* p, size_t i) {
+  return ((const __attribute__((address_space(1))) double*)p)[i];
+}
 
 // ph[z] = sum_k1 a[z,k1] q[k1] with the reference's summation order
 template <int NZ>
@@ -1270,14 +1249,14 @@ __device__ __forceinline__ cd invert_regs(const MemberParams& P, int z, const cd
     cd acc = {0.0, 0.0};
 #pragma unroll
     for (int k1 = 0; k1 < NZ; ++k1) {
-      const double a = P.a[((size_t)z * NZ + k1) * plane + pt];
+      const double a = gld(P.a, ((size_t)z * NZ + k1) * plane + pt);
       acc.x = acc.x + a * q[k1].x;
       acc.y = acc.y + a * q[k1].y;
     }
     return acc;
   }
   if constexpr (NZ == 2) {
-    const double w = P.k2l2[pt];
+    const double w = gld(P.k2l2, pt);
     const double det = w * (w + P.F1 + P.F2);
     if (det == 0.0) return {0.0, 0.0};
     const double di = 1.0 / det;
@@ -1294,8 +1273,39 @@ __device__ __forceinline__ cd invert_regs(const MemberParams& P, int z, const cd
   }
 }
 
+// Both layers' ph at one point (two layers), with invert_regs' arithmetic:
+// the closed form's determinant and its reciprocal are formed once.
+__device__ __forceinline__ void invert2_both(const MemberParams& P, cd q0, cd q1, size_t plane,
+                                             size_t pt, cd& ph0, cd& ph1) {
+#pragma clang fp contract(off)
+  if (P.a) {
+    const cd q[2] = {q0, q1};
+    ph0 = invert_regs<2>(P, 0, q, plane, pt);
+    ph1 = invert_regs<2>(P, 1, q, plane, pt);
+    return;
+  }
+  const double w = gld(P.k2l2, pt);
+  const double det = w * (w + P.F1 + P.F2);
+  if (det == 0.0) {
+    ph0 = {0.0, 0.0};
+    ph1 = {0.0, 0.0};
+    return;
+  }
+  const double di = 1.0 / det;
+  const double a00 = -(w + P.F2) * di, a01 = -P.F1 * di;
+  const double a10 = -P.F2 * di, a11 = -(w + P.F1) * di;
+  ph0.x = 0.0 + a00 * q0.x;
+  ph0.y = 0.0 + a00 * q0.y;
+  ph0.x = ph0.x + a01 * q1.x;
+  ph0.y = ph0.y + a01 * q1.y;
+  ph1.x = 0.0 + a10 * q0.x;
+  ph1.y = 0.0 + a10 * q0.y;
+  ph1.x = ph1.x + a11 * q1.x;
+  ph1.y = ph1.y + a11 * q1.y;
+}
+
 // inversion + y IFFT of q, u, v for all layers from register-held qh
-template <int NY, int NZ, int E>
+template <int NY, int NZ, int E, int TAG = 0>
 __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
                                                   cd* __restrict__ Qm, cd* __restrict__ Um,
                                                   cd* __restrict__ Vm, const MemberParams& P,
@@ -1306,7 +1316,7 @@ __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
   using G = FGeo<NY, E>;
   const size_t plane = (size_t)NY * ldk;
   const int kc = ok ? k : 0;
-  const double kk = P.kk[kc];
+  const double kk = gld(P.kk, kc);
 #pragma unroll
   for (int z = 0; z < NZ; ++z) {
     const size_t zoff = mbase + (size_t)z * plane;
@@ -1320,24 +1330,24 @@ __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
                  : cd{0.0, 0.0};
       x[s] = Q[z][s];
     }
-    qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+    qgk::fft_block<NY, G::E, +1, G::TK, TAG>(x, lds, t, c, tw);
     {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
       for (int s = 0; s < G::E; ++s) st_stream(Qm + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
     }
 #pragma unroll
     for (int s = 0; s < G::E; ++s) {
-      const double ll = P.ll[t + s * G::T];
+      const double ll = gld(P.ll, t + s * G::T);
       x[s] = {ll * ph[s].y, -ll * ph[s].x};
     }
-    qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+    qgk::fft_block<NY, G::E, +1, G::TK, TAG>(x, lds, t, c, tw);
     {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
       for (int s = 0; s < G::E; ++s) st_stream(Um + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
     }
 #pragma unroll
     for (int s = 0; s < G::E; ++s) x[s] = {-kk * ph[s].y, kk * ph[s].x};
-    qgk::fft_block<NY, G::E, +1, G::TK>(x, lds, t, c, tw);
+    qgk::fft_block<NY, G::E, +1, G::TK, TAG>(x, lds, t, c, tw);
     {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
       for (int s = 0; s < G::E; ++s) st_stream(Vm + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
@@ -1400,8 +1410,8 @@ __global__ __launch_bounds__((2 * FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_
     xa[s] = ok ? ld_stream(UQm + zoff + pt) : cd{0.0, 0.0};
     xb[s] = ok ? ld_stream(VQm + zoff + pt) : cd{0.0, 0.0};
   }
-  qgk::fft_block<NY, EE, -1, TK>(xa, L, t, c, tw);
-  qgk::fft_block<NY, EE, -1, TK>(xb, L, t, c, tw);
+  qgk::fft_block<NY, EE, -1, TK, kTagColSplit>(xa, L, t, c, tw);
+  qgk::fft_block<NY, EE, -1, TK, kTagColSplit>(xb, L, t, c, tw);
   __syncthreads();  // the transforms' last LDS reads are done
 #pragma unroll
   for (int s = 0; s < EE; ++s) L[(t + s * T) * TK + c] = Q[s];
@@ -1465,7 +1475,7 @@ __global__ __launch_bounds__((2 * FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_
   cd x[EE];
 #pragma unroll
   for (int s = 0; s < EE; ++s) x[s] = Q[s];
-  qgk::fft_block<NY, EE, +1, TK>(x, L, t, c, tw);
+  qgk::fft_block<NY, EE, +1, TK, kTagColSplit>(x, L, t, c, tw);
 #pragma unroll
   for (int s = 0; s < EE; ++s) st_stream(Qm + zoff + (size_t)(t + s * T) * ldk + k, x[s]);
 #pragma unroll
@@ -1473,12 +1483,12 @@ __global__ __launch_bounds__((2 * FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_
     const double ll = P.ll[t + s * T];
     x[s] = {ll * ph[s].y, -ll * ph[s].x};
   }
-  qgk::fft_block<NY, EE, +1, TK>(x, L, t, c, tw);
+  qgk::fft_block<NY, EE, +1, TK, kTagColSplit>(x, L, t, c, tw);
 #pragma unroll
   for (int s = 0; s < EE; ++s) st_stream(Um + zoff + (size_t)(t + s * T) * ldk + k, x[s]);
 #pragma unroll
   for (int s = 0; s < EE; ++s) x[s] = {-kk * ph[s].y, kk * ph[s].x};
-  qgk::fft_block<NY, EE, +1, TK>(x, L, t, c, tw);
+  qgk::fft_block<NY, EE, +1, TK, kTagColSplit>(x, L, t, c, tw);
 #pragma unroll
   for (int s = 0; s < EE; ++s) st_stream(Vm + zoff + (size_t)(t + s * T) * ldk + k, x[s]);
 }
@@ -1509,8 +1519,14 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_prol
   col_prologue_regs<NY, NZ, G::E>(Q, Qm, Um, Vm, P, nk, ldk, t, c, k, ok, mbase, lds, tw);
 }
 
+// Waves per SIMD the register column kernel is compiled for: with two layers
+// at radix 8 the compiler would otherwise take all 256 VGPRs (one wave per
+// SIMD); bounded to two waves it fits without spills and runs 10 % faster.
+template <int NZ, int E>
+constexpr int col_reg_waves() { return NZ == 2 && E == 8 ? 2 : 1; }
+
 template <int NY, int NZ, int E>
-__global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step_reg(
+__global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK), (col_reg_waves<NZ, E>())) void k_col_step_reg(
     const cd* __restrict__ qh_old, cd* __restrict__ qh_new, const cd* __restrict__ p,
     cd* __restrict__ pp_to_newp, const cd* __restrict__ UQm, const cd* __restrict__ VQm,
     cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
@@ -1530,10 +1546,12 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
   const MemberParams& P = params[m];
   const size_t plane = (size_t)NY * ldk;
   const size_t mbase = (size_t)m * NZ * plane;
-  const double kk = P.kk[kc];
+  const double kk = gld(P.kk, kc);
 
   cd Q[NZ][G::E];
-  cd Q0old[NZ == 2 ? G::E : 1];
+  // two layers: layer 1's ph from the old qh of both layers, formed together
+  // with layer 0's (one determinant per point) before layer 0 is replaced
+  cd ph1[NZ == 2 ? G::E : 1];
 #pragma unroll
   for (int z = 0; z < NZ; ++z)
 #pragma unroll
@@ -1551,34 +1569,35 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
       xa[s] = ok ? ld_stream(UQm + zoff + pt) : cd{0.0, 0.0};
       xb[s] = ok ? ld_stream(VQm + zoff + pt) : cd{0.0, 0.0};
     }
-    qgk::fft_block<NY, G::E, -1, G::TK>(xa, lds, t, c, tw);
-    qgk::fft_block<NY, G::E, -1, G::TK>(xb, lds, t, c, tw);
+    qgk::fft_block<NY, G::E, -1, G::TK, kTagColReg>(xa, lds, t, c, tw);
+    qgk::fft_block<NY, G::E, -1, G::TK, kTagColReg>(xb, lds, t, c, tw);
     if (ok) {
-      const double kq = kk * P.Qy[z];
+      const double kq = kk * gld(P.Qy, z);
       const bool drag = (z == NZ - 1) && P.rek != 0.0;
 #pragma unroll
       for (int s = 0; s < G::E; ++s) {
         const int l = t + s * G::T;
         const size_t pt = (size_t)l * ldk + k;
         const size_t o = zoff + pt;
-        const double ll = P.ll[l];
+        const double ll = gld(P.ll, l);
         const cd S = {-kk * xa[s].y + -ll * xb[s].y, kk * xa[s].x + ll * xb[s].x};
-        // ph from the old qh of every layer (layer 0 was already replaced in Q
-        // when z == 1; its old value is kept in Q0old)
-        cd qs[NZ];
-#pragma unroll
-        for (int k1 = 0; k1 < NZ; ++k1) qs[k1] = Q[k1][s];
+        // ph from the old qh of every layer
+        cd ph;
         if constexpr (NZ == 2) {
-          if (z == 1) qs[0] = Q0old[s];
+          if (z == 0)
+            invert2_both(P, Q[0][s], Q[1][s], plane, pt, ph, ph1[s]);
+          else
+            ph = ph1[s];
+        } else {
+          ph = invert_regs<NZ>(P, z, Q[0] + s, plane, pt);
         }
-        const cd ph = invert_regs<NZ>(P, z, qs, plane, pt);
         cd d = {S.x + -kq * ph.y, S.y + kq * ph.x};
         d = {-d.x, -d.y};
         if (drag) {
-          const double cr = P.rek * P.k2l2[pt];
+          const double cr = P.rek * gld(P.k2l2, pt);
           d = {d.x + cr * ph.x, d.y + cr * ph.y};
         }
-        const double f = P.filtr[pt];
+        const double f = gld(P.filtr, pt);
         const cd q = Q[z][s], pv = ld_stream(p + o), ppv = ld_stream(pp_to_newp + o);
         double rx = q.x + cf.dt1 * d.x;
         double ry = q.y + cf.dt1 * d.y;
@@ -1589,9 +1608,6 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
         const cd qn = {f * rx, f * ry};
         st_stream(qh_new + o, qn);
         st_stream(pp_to_newp + o, d);
-        if constexpr (NZ == 2) {
-          if (z == 0) Q0old[s] = q;
-        }
         Q[z][s] = qn;
       }
     } else {
@@ -1604,7 +1620,7 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
       }
     }
   }
-  col_prologue_regs<NY, NZ, G::E>(Q, Qm, Um, Vm, P, nk, ldk, t, c, k, ok, mbase, lds, tw);
+  col_prologue_regs<NY, NZ, G::E, kTagColReg>(Q, Qm, Um, Vm, P, nk, ldk, t, c, k, ok, mbase, lds, tw);
 }
 
 // ==========================================================================
@@ -2791,17 +2807,6 @@ struct RowFluxL {
   }
 };
 
-struct RowSpeedL {
-  template <int NX, int E>
-  static void go(qgk_bucket* b, unsigned long long* speed) {
-    using G = FGeo<NX, E>;
-    const double scale = 1.0 / ((double)b->nx * (double)b->ny);
-    dim3 grid((b->ny / 2 + G::RP - 1) / G::RP, b->M * b->nz);
-    hipLaunchKernelGGL((k_row_speed<NX, E>), grid, dim3(G::T * G::RP), 0, b->stream, b->Qm,
-                       b->Um, b->Vm, b->dp, b->nz, b->ny, scale, b->tw_x, speed);
-  }
-};
-
 struct ColPrologueL {
   template <int NY, int E>
   static void go(qgk_bucket* b) {
@@ -2871,6 +2876,30 @@ int launch_col_prologue(qgk_bucket* b) {
 int launch_col_step(qgk_bucket* b, StepCoeffs cf) {
   int rc = with_ne<ColStepL>(b->ny, b->ecol, b, cf);
   if (rc) return rc;
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+// Members' health words into w[M] (cleared first, by a fill kernel, so the
+// work can be captured): x inverse transforms of the primed Qm, Um, Vm into
+// the real scratch, each folded into the words.
+int launch_member_speed(qgk_bucket* b, unsigned long long* w) {
+  int rc = upload_params(b);
+  if (rc) return rc;
+  if (!b->primed) {
+    if ((rc = launch_col_prologue(b))) return rc;
+    b->primed = true;
+  }
+  hipLaunchKernelGGL(k_fill_u64, dim3((b->M + 255) / 256), dim3(256), 0, b->stream, w, b->M,
+                     0ULL);
+  const int np = b->M * b->nz;
+  const int parts = (int)std::min<size_t>(8, (b->nplane_r + 2047) / 2048);
+  const cd* src[3] = {b->Um, b->Vm, b->Qm};
+  for (int f = 0; f < 3; ++f) {
+    if ((rc = launch_row_c2r(b, src[f], b->rtmp, np))) return rc;
+    hipLaunchKernelGGL(k_speed_fold, dim3(parts, np), dim3(256), 0, b->stream, b->rtmp,
+                       (int)b->nplane_r, b->nz, b->dp, f, w);
+  }
   HIPC(hipGetLastError());
   return 0;
 }
@@ -4055,17 +4084,10 @@ int qgk_step(qgk_bucket* b, int nsteps, const double* coeffs, int use_graph) {
 int qgk_member_speed(qgk_bucket* b, double* speed_host) {
   if (!b || !speed_host) return fail(QGK_E_ARG, "NULL argument");
   if (int rc0 = refuse_in_capture(b, "qgk_member_speed")) return rc0;
-  int rc = upload_params(b);
-  if (rc) return rc;
+  int rc = 0;
   if (!b->speed && (rc = dev_alloc(b, (void**)&b->speed, b->M * sizeof(unsigned long long))))
     return rc;
-  if (!b->primed) {
-    if ((rc = launch_col_prologue(b))) return rc;
-    b->primed = true;
-  }
-  HIPC(hipMemsetAsync(b->speed, 0, b->M * sizeof(unsigned long long), b->stream));
-  if ((rc = with_ne<RowSpeedL>(b->nx, b->erow, b, b->speed))) return rc;
-  HIPC(hipGetLastError());
+  if ((rc = launch_member_speed(b, b->speed))) return rc;
   std::vector<unsigned long long> w(b->M);
   HIPC(hipMemcpyAsync(w.data(), b->speed, b->M * sizeof(unsigned long long),
                       hipMemcpyDeviceToHost, b->stream));
@@ -4081,15 +4103,9 @@ int qgk_member_speed_noalloc(qgk_bucket* b, double* speed_dev) {
                 kNoAllocMaxMembers, b->M);
   unsigned long long* w = nullptr;
   int rc = scratch_address(&w);
-  if (rc || (rc = upload_params(b))) return rc;
-  if (!b->primed) {
-    if ((rc = launch_col_prologue(b))) return rc;
-    b->primed = true;
-  }
-  const int blocks = (b->M + 255) / 256;
-  hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, b->stream, w, b->M, 0ULL);
-  if ((rc = with_ne<RowSpeedL>(b->nx, b->erow, b, w))) return rc;
-  hipLaunchKernelGGL(k_u64_to_double, dim3(blocks), dim3(256), 0, b->stream, w, speed_dev, b->M);
+  if (rc || (rc = launch_member_speed(b, w))) return rc;
+  hipLaunchKernelGGL(k_u64_to_double, dim3((b->M + 255) / 256), dim3(256), 0, b->stream, w,
+                     speed_dev, b->M);
   HIPC(hipGetLastError());
   HIPC(stream_done(b));
   return 0;

@@ -67,10 +67,53 @@ def test_coarsen_is_a_projection_and_commutes_with_inversion(mods):
     _lib.check(_lib.lib.qgk_coarsen(hb.h, lo2.buckets[0].h, _lib.FIELD_ID["ph"],
                                     _lib.FIELD_ID["qh"], None))
     assert rel_l2(lo2.buckets[0].get("qh"), lb.get("ph")) < 1e-13
-    # Parseval on the retained modes: the coarse field's variance is the
-    # retained part of the fine field's
+    # Parseval on the retained modes, exactly: the coarse field's mean square
+    # on the coarse grid equals the mean square, on the fine grid, of the fine
+    # field restricted to the modes the coarse grid keeps (numpy's transforms;
+    # a wrong (n_lo/n_hi)^2 scale fails it)
     q_lo = lb.get("q")
-    assert np.isfinite(q_lo).all() and q_lo.var() < hi.buckets[0].get("q").var()
+    qh_hi = hb.get("qh")
+    keep = np.zeros((128, 65), bool)
+    for l in range(64):
+        if l != 32:
+            keep[(l if l < 32 else l - 64) % 128, :32] = True
+    r_hi = np.fft.irfftn(np.where(keep, qh_hi, 0.0), s=(128, 128), axes=(-2, -1))
+    ms_lo = (q_lo ** 2).mean(axis=(-2, -1))
+    ms_hi = (r_hi ** 2).mean(axis=(-2, -1))
+    assert np.all(np.abs(ms_lo - ms_hi) <= 1e-12 * ms_hi), (ms_lo, ms_hi)
+
+
+def test_coarsen_plane_waves(mods):
+    """A plane wave the coarse grid resolves comes through with its amplitude
+    and phase: the coarse field is the same cosine sampled at every second
+    fine grid point (coarse index j is fine index 2j); one beyond the coarse
+    grid's band, or on its Nyquist row, disappears."""
+    pq = mods[0]
+    from pyqg_amd import _lib
+    n_hi, n_lo, L = 128, 64, 1e6
+    x_hi = (np.arange(n_hi) + 0.5) * L / n_hi
+    two_pi = 2 * np.pi / L
+
+    def wave(x, i, j, amp, phase):
+        X, Y = np.meshgrid(x, x)
+        return amp * np.cos(two_pi * (i * X + j * Y) + phase)
+
+    cases = [((3, 5), True), ((31, -7), True), ((-12, 30), True), ((40, 2), False),
+             ((5, 32), False), ((0, 50), False)]
+    hi = pq.Ensemble.qg(np.full(len(cases), n_hi), full_state=True)
+    lo = pq.Ensemble.qg(np.full(len(cases), n_lo), full_state=True)
+    q_hi = [np.stack([wave(x_hi, i, j, 2.5, 0.3), wave(x_hi, i, j, -1.0, 1.1)])
+            for (i, j), _ in cases]
+    hi.set_q(q_hi)
+    _lib.check(_lib.lib.qgk_coarsen(hi.buckets[0].h, lo.buckets[0].h, _lib.FIELD_ID["qh"],
+                                    _lib.FIELD_ID["qh"], None))
+    q_lo = lo.buckets[0].get("q")
+    for m, ((i, j), kept) in enumerate(cases):
+        if kept:
+            want = q_hi[m][:, ::2, ::2]
+            assert np.abs(q_lo[m] - want).max() < 1e-12 * 2.5, (i, j)
+        else:
+            assert np.abs(q_lo[m]).max() < 1e-12, (i, j)
 
 
 def test_subgrid_forcing_matches_oracle(mods):
