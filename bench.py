@@ -196,7 +196,7 @@ def bench_cnn(args):
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
         ev[0].record(st)
         for _ in range(args.steps):
-            y, _ = C.conv_fwd(h, net.wb[l], net.b32[l], net.chp[l + 1], relu=not last)
+            y, _ = net._layer(l, h, relu=not last, out_bf16=True, out_f32=False)
         ev[1].record(st)
         torch.cuda.synchronize()
         layer_ms.append(ev[0].elapsed_time(ev[1]) / args.steps)
@@ -241,6 +241,10 @@ def bench_cnn(args):
                    "batch": B},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": CNN_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / CNN_PEAK_TFLOPS, "traffic": None,
+                     "layer_kernels": ["k_conv_first" if (l == 0 and net.first_packed) else
+                                       "k_conv_last" if (l == net.nlayers - 1 and
+                                                         net.last_packed) else "k_conv_fwd_mt"
+                                       for l in range(net.nlayers)],
                      "kernel": (f"k_conv_fwd_mt<64, 4, 32>: layer {dom + 1} "
                                 f"({net.channels[dom]}->{net.channels[dom + 1]} channels), "
                                 f"batch {B}"),

@@ -28,6 +28,22 @@ const char* qgcnn_last_error(void);
 int qgcnn_conv_fwd(const void* x, int B, int H, int W, int cinp, const void* w,
                    const float* bias, int coutp, int relu, const void* mask, void* y, float* y32,
                    void* stream);
+/* First layer of a network with at most 8 real input channels: the same
+ * y = act(conv5x5(x, w) + bias) with the 25 taps x 8 channels packed into the
+ * matrix K dimension.  x keeps its cinp-channel layout (channels 8.. unused);
+ * w8 is w repacked by qgcnn_pack_first: [coutp][28][8] BF16 (device,
+ * coutp * 448 bytes), refreshed whenever w changes. */
+int qgcnn_pack_first(const void* w, int cinp, int coutp, void* w8, void* stream);
+int qgcnn_conv_first(const void* x, int B, int H, int W, int cinp, const void* w8,
+                     const float* bias, int coutp, int relu, void* y, float* y32, void* stream);
+/* Last layer with cout real output channels, 5 * cout <= 16, cinp a multiple
+ * of 32, no activation: y = conv5x5(x, w) + bias over all coutp channels
+ * (channels >= cout get their bias), with the 5 horizontal taps of each real
+ * output channel as rows of the matrix tile.  wl is w repacked by
+ * qgcnn_pack_last: [16][5][cinp] BF16 (device, 160 * cinp bytes). */
+int qgcnn_pack_last(const void* w, int cinp, int coutp, int cout, void* wl, void* stream);
+int qgcnn_conv_last(const void* x, int B, int H, int W, int cinp, const void* wl,
+                    const float* bias, int coutp, int cout, void* y, float* y32, void* stream);
 /* dw[co][tap][ci] += sum_px dy[px][co] x[px + tap][ci] (FP32 accumulation) */
 int qgcnn_conv_wgrad(const void* dy, const void* x, int B, int H, int W, int cinp, int coutp,
                      float* dw, void* stream);

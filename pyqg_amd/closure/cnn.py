@@ -39,6 +39,10 @@ def _load():
         "qgcnn_adam": (I, [P, P, P, P, P, Z, F, F, F, F, I, F, P]),
         "qgcnn_wgrad_workspace_bytes": (Z, [I, I, I, I, I]),
         "qgcnn_conv_wgrad_ws": (I, [P, P, I, I, I, I, I, P, P, Z, P]),
+        "qgcnn_pack_first": (I, [P, I, I, P, P]),
+        "qgcnn_conv_first": (I, [P, I, I, I, I, P, P, I, I, P, P, P]),
+        "qgcnn_pack_last": (I, [P, I, I, I, P, P]),
+        "qgcnn_conv_last": (I, [P, I, I, I, I, P, P, I, I, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -51,7 +55,8 @@ lib = _load()
 EXPORTS = ["qgcnn_last_error", "qgcnn_conv_fwd", "qgcnn_conv_wgrad", "qgcnn_flip_transpose",
            "qgcnn_bias_grad", "qgcnn_mse", "qgcnn_f32_to_bf16", "qgcnn_bf16_to_f32",
            "qgcnn_adam", "qgcnn_wgrad_workspace_bytes", "qgcnn_conv_wgrad_ws",
-           "qgcnn_bias_grad_ws"]
+           "qgcnn_bias_grad_ws", "qgcnn_pack_first", "qgcnn_conv_first", "qgcnn_pack_last",
+           "qgcnn_conv_last"]
 
 
 def check(rc):
@@ -88,6 +93,54 @@ def conv_fwd(x, w, bias, coutp, relu=True, mask=None, out_bf16=True, out_f32=Fal
     check(lib.qgcnn_conv_fwd(_p(x), B, H, W, cinp, _p(w), _p(bias), coutp, int(relu), _p(mask),
                              _p(y), _p(y32), stream if stream is not None else _stream()))
     return y, y32
+
+
+def conv_first(x, w8, bias, coutp, relu=True, y=None, y32=None, out_bf16=True, out_f32=False,
+               stream=None):
+    """First layer with at most 8 real input channels: as conv_fwd with the taps
+    packed into the matrix K dimension; w8 from pack_first."""
+    B, H, W, cinp = x.shape
+    if out_bf16 and y is None:
+        y = torch.empty((B, H, W, coutp), dtype=torch.bfloat16, device=x.device)
+    if out_f32 and y32 is None:
+        y32 = torch.empty((B, H, W, coutp), dtype=torch.float32, device=x.device)
+    y = y if out_bf16 else None
+    y32 = y32 if out_f32 else None
+    check(lib.qgcnn_conv_first(_p(x), B, H, W, cinp, _p(w8), _p(bias), coutp, int(relu), _p(y),
+                               _p(y32), stream if stream is not None else _stream()))
+    return y, y32
+
+
+def pack_first(w, w8=None):
+    coutp, _, cinp = w.shape
+    if w8 is None:
+        w8 = torch.empty((coutp, 28, 8), dtype=torch.bfloat16, device=w.device)
+    check(lib.qgcnn_pack_first(_p(w), cinp, coutp, _p(w8), _stream()))
+    return w8
+
+
+def conv_last(x, wl, bias, coutp, cout, y=None, y32=None, out_bf16=False, out_f32=True,
+              stream=None):
+    """Last layer with 5 * cout <= 16 real outputs: as conv_fwd without
+    activation, the horizontal taps as matrix rows; wl from pack_last."""
+    B, H, W, cinp = x.shape
+    if out_bf16 and y is None:
+        y = torch.empty((B, H, W, coutp), dtype=torch.bfloat16, device=x.device)
+    if out_f32 and y32 is None:
+        y32 = torch.empty((B, H, W, coutp), dtype=torch.float32, device=x.device)
+    y = y if out_bf16 else None
+    y32 = y32 if out_f32 else None
+    check(lib.qgcnn_conv_last(_p(x), B, H, W, cinp, _p(wl), _p(bias), coutp, cout, _p(y),
+                              _p(y32), stream if stream is not None else _stream()))
+    return y, y32
+
+
+def pack_last(w, cout, wl=None):
+    coutp, _, cinp = w.shape
+    if wl is None:
+        wl = torch.empty((16, 5, cinp), dtype=torch.bfloat16, device=w.device)
+    check(lib.qgcnn_pack_last(_p(w), cinp, coutp, cout, _p(wl), _stream()))
+    return wl
 
 
 _WS = {}
@@ -166,6 +219,11 @@ class CNN:
             self.wt.append(torch.empty((cinp, 25, coutp), dtype=torch.bfloat16, device=device))
         self._adam = None
         self.step_count = 0
+        # the first and last layers run special kernels when their shapes allow
+        self.first_packed = self.channels[0] <= 8
+        self.last_packed = 5 * self.channels[-1] <= 16 and self.chp[-2] % 32 == 0
+        self.w_first = self.w_last = None
+        self._repack()
 
     @property
     def nlayers(self):
@@ -178,6 +236,27 @@ class CNN:
     def refresh_bf16(self):
         for l in range(self.nlayers):
             self.wb[l] = self.w32[l].to(torch.bfloat16)
+        self._repack()
+
+    def _repack(self):
+        """The first and last layers' weights in their special kernels' layouts
+        (after every change of the BF16 weights)."""
+        if self.first_packed:
+            self.w_first = pack_first(self.wb[0], self.w_first)
+        if self.last_packed:
+            self.w_last = pack_last(self.wb[-1], self.channels[-1], self.w_last)
+
+    def _layer(self, l, h, relu, out_bf16, out_f32, y=None, y32=None, stream=None):
+        """Forward of layer l on the kernel that fits its shape."""
+        coutp = self.chp[l + 1]
+        if l == 0 and self.first_packed:
+            return conv_first(h, self.w_first, self.b32[0], coutp, relu=relu, y=y, y32=y32,
+                              out_bf16=out_bf16, out_f32=out_f32, stream=stream)
+        if l == self.nlayers - 1 and self.last_packed and not relu:
+            return conv_last(h, self.w_last, self.b32[l], coutp, self.channels[-1], y=y, y32=y32,
+                             out_bf16=out_bf16, out_f32=out_f32, stream=stream)
+        return conv_fwd(h, self.wb[l], self.b32[l], coutp, relu=relu, out_bf16=out_bf16,
+                        out_f32=out_f32, y=y, y32=y32, stream=stream)
 
     def pad_input(self, x):
         """(B, H, W, channels[0]) float -> padded bf16 NHWC."""
@@ -193,8 +272,7 @@ class CNN:
         h = xp
         for l in range(self.nlayers):
             last = l == self.nlayers - 1
-            y, y32 = conv_fwd(h, self.wb[l], self.b32[l], self.chp[l + 1], relu=not last,
-                              out_bf16=not last or keep, out_f32=last)
+            y, y32 = self._layer(l, h, relu=not last, out_bf16=not last or keep, out_f32=last)
             if not last:
                 h = y
                 if keep:
@@ -218,9 +296,9 @@ class CNN:
         h = xp
         for l in range(self.nlayers):
             last = l == self.nlayers - 1
-            conv_fwd(h, self.wb[l], self.b32[l], self.chp[l + 1], relu=not last,
-                     out_bf16=not last, out_f32=last, y=None if last else bufs[l],
-                     y32=bufs[l] if last else None, stream=stream)
+            self._layer(l, h, relu=not last, out_bf16=not last, out_f32=last,
+                        y=None if last else bufs[l], y32=bufs[l] if last else None,
+                        stream=stream)
             h = bufs[l]
         return bufs[-1]
 
@@ -254,6 +332,7 @@ class CNN:
             check(lib.qgcnn_adam(_p(self.b32[l]), _p(dbs[l]), _p(mb), _p(vb), None,
                                  self.b32[l].numel(), lr, b1, b2, eps, self.step_count, 0.0,
                                  _stream()))
+        self._repack()
 
     def train_step(self, xp, target, lr=1e-3, clip=None):
         """One Adam step on the MSE between output channels and target

@@ -33,12 +33,24 @@ from . import train as train_mod
 from .. import diagnostic_tools as tools
 
 
+# The two-layer model's time-mean diagnostics compared between the runs (the
+# registry entries pyqg's diagnostic_similarities compares, with the
+# parameterization terms KEflux and APEflux are summed with).
+COMPARED = ["KEspec", "Ensspec", "EKE", "EKEdiss", "KEfrictionspec", "Dissspec", "ENSDissspec",
+            "entspec", "ENSflux", "ENSgenspec", "ENSfrictionspec", "APEgenspec", "APEflux",
+            "KEflux", "APEgen", "paramspec_APEflux", "paramspec_KEflux"]
+# energy-budget terms among them
+BUDGET = ["KEflux", "APEflux", "APEgenspec", "KEfrictionspec", "Dissspec", "EKEdiss", "APEgen"]
+
+
 def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=64, lr=1e-3,
-        online_members=128, online_days=720, avg_days=360, dt_lo=14400., seed=0, log=print):
+        online_members=128, online_days=720, avg_days=360, dt_lo=14400., seed=0, ema=0.998,
+        ema_from=0.5, closures=("backscatter", "zb2020", "smagorinsky"), log=print):
     out = {"config": dict(members=members, samples=samples, spinup_days=spinup_days,
                           every=every, epochs=epochs, batch=batch, lr=lr,
                           online_members=online_members, online_days=online_days,
-                          avg_days=avg_days, dt_hi=3600., dt_lo=dt_lo)}
+                          avg_days=avg_days, dt_hi=3600., dt_lo=dt_lo, seed=seed, ema=ema,
+                          ema_from=ema_from)}
     L = data_mod.QG_EDDY["L"]
     g_hi, g_lo = tools.grid(256, L), tools.grid(64, L)
     # ---------------------------------------------------------- 1. data
@@ -58,7 +70,7 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
         gen.hi.step(every)
         hb.sync()
         t_hi_steps += time.perf_counter() - ts
-        gen.hi.accumulate_diagnostics(["KEspec"])
+        gen.hi.accumulate_diagnostics(COMPARED)
         x, y = gen.sample()
         xs.append(x)
         ys.append(y)
@@ -67,6 +79,7 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
         if nsnap % 50 == 0:
             log(f"sampled {n} pairs")
     e_hi = gen.hi.get_diagnostic("KEspec", mean=True)[0]
+    view_hi = gen.hi.mean_model(0)
     x = torch.cat(xs)
     y = torch.cat(ys)
     del xs, ys
@@ -82,7 +95,7 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
     net = cnn_mod.CNN(seed=seed)
     out["n_params"] = net.n_params()
     info = train_mod.train(net, x_tr, y_tr, scales, epochs=epochs, batch=batch, lr=lr,
-                           log=log)
+                           log=log, seed=seed, ema=ema, ema_from=ema_from)
     out["train"] = {"steps": info["steps"], "seconds": info["seconds"],
                     "final_loss": info["history"][-1][1] if info["history"] else None}
     out["offline"] = train_mod.evaluate(net, x_te, y_te, scales)
@@ -101,8 +114,10 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
     runs = {}
     spectra_lo = {}
     from .. import parameterizations as builtin
-    for name, prm in (("plain", None), ("cnn", p), ("backscatter", builtin.BackscatterBiharmonic()),
-                      ("zb2020", builtin.ZannaBolton2020()), ("smagorinsky", builtin.Smagorinsky())):
+    choices = {"backscatter": builtin.BackscatterBiharmonic(), "zb2020": builtin.ZannaBolton2020(),
+               "smagorinsky": builtin.Smagorinsky()}
+    views = {}
+    for name, prm in [("plain", None), ("cnn", p)] + [(c, choices[c]) for c in closures]:
         ce = param_mod.ClosedEnsemble(online_members, 64, dt_lo, gen.phys, param=prm, seed=seed)
         ce.ens.set_q([qlo[i % members] for i in range(online_members)])
         per_day = int(round(86400. / dt_lo))
@@ -116,13 +131,14 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
             if s0 > done:
                 ce.step(s0 - done)
                 done = s0
-            ce.ens.accumulate_diagnostics(["KEspec"])
+            ce.ens.accumulate_diagnostics(COMPARED)
         if nsteps > done:
             ce.step(nsteps - done)
         cb.sync()
         wall = time.perf_counter() - t1
         e = ce.ens.get_diagnostic("KEspec", mean=True)[0]
         spectra_lo[name] = e
+        views[name] = ce.ens.mean_model(0)
         finite = bool(np.isfinite(ce.ens.get("q", 0)).all())
         runs[name] = {"ke_rmse": spectra.ke_errors(e, g_lo, e_hi, g_hi),
                       "ke_rel_err": spectra.relative_errors(e, g_lo, e_hi, g_hi),
@@ -134,6 +150,11 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
         if name != "plain":
             runs[name]["similarity_vs_plain"] = spectra.similarity(runs[name]["ke_rmse"],
                                                                   runs["plain"]["ke_rmse"])
+            # pyqg's comparison over every compared diagnostic, per layer
+            sims = tools.diagnostic_similarities(views[name], view_hi, views["plain"])
+            runs[name]["diagnostic_similarities"] = {k: float(v) for k, v in sims.items()}
+            budget = [v for k, v in sims.items() if k.rstrip("12") in BUDGET]
+            runs[name]["budget_similarity_mean"] = float(np.mean(budget))
     out["online"] = runs
     out["hi_res_total_ke"] = float(tools.spec_sum(e_hi).sum())
     # cost per member and simulated day
@@ -159,12 +180,24 @@ def main():
     ap.add_argument("--avg-days", type=int, default=360)
     ap.add_argument("--dt-lo", type=float, default=14400.)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--ema", type=float, default=0.998,
+                    help="weight averaging decay over the second half of training (0: off)")
+    ap.add_argument("--ema-from", type=float, default=0.5,
+                    help="fraction of training after which the weights are averaged")
+    ap.add_argument("--closures", default="backscatter,zb2020,smagorinsky",
+                    help="built-in closures run online beside the network (comma list)")
+    ap.add_argument("--out", default=None, help="also append the JSON line to this file")
     a = ap.parse_args()
     res = run(members=a.members, samples=a.samples, spinup_days=a.spinup_days, every=a.every,
               epochs=a.epochs, batch=a.batch, lr=a.lr, online_members=a.online_members,
               online_days=a.online_days, avg_days=a.avg_days, dt_lo=a.dt_lo, seed=a.seed,
+              ema=a.ema, ema_from=a.ema_from, closures=[c for c in a.closures.split(",") if c],
               log=lambda s: print(s, flush=True))
-    print(json.dumps(res))
+    line = json.dumps(res)
+    print(line)
+    if a.out:
+        with open(a.out, "a") as f:
+            f.write(line + "\n")
 
 
 if __name__ == "__main__":

@@ -17,20 +17,26 @@ def _normalized(x, y, scales, idx, net):
 
 
 def train(net: CNN, x, y, scales: Scales, epochs=4, batch=64, lr=1e-3, lr_min=1e-5, seed=0,
-          log=None, clip_factor=4.0):
+          log=None, clip_factor=4.0, ema=0.998, ema_from=0.5):
     """Adam on the MSE of normalised forcing, cosine learning-rate decay.
     x (N, H, W, 6) and y (N, H, W, 2) FP32 tensors on the GPU.
 
     After 50 steps the global gradient norm is clipped at clip_factor times
     its running mean: without it one outsized gradient at the peak learning
     rate could switch off the ReLUs and the loss return to 1 for good (one
-    seed in three at the default settings)."""
+    seed in three at the default settings).
+
+    With ema > 0 the network ends with an exponential moving average of its
+    weights over the steps after the fraction ema_from of training (Polyak
+    averaging): the closed model's long-run statistics are sensitive to the
+    last steps' weight noise, which the average smooths."""
     n = x.shape[0]
     g = torch.Generator(device="cpu").manual_seed(seed)
     steps = epochs * max(n // batch, 1)
     it = 0
     history = []
     gn_mean = 0.0
+    avg = None
     t0 = time.perf_counter()
     for ep in range(epochs):
         perm = torch.randperm(n, generator=g)
@@ -40,6 +46,13 @@ def train(net: CNN, x, y, scales: Scales, epochs=4, batch=64, lr=1e-3, lr_min=1e
             cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * it / steps))
             clip = clip_factor * gn_mean if (clip_factor and it >= 50) else None
             loss = net.train_step(xb, yb, lr=cur, clip=clip)
+            if ema and it >= int(ema_from * steps):
+                params = list(net.w32) + list(net.b32)
+                if avg is None:
+                    avg = [p.clone() for p in params]
+                else:
+                    torch._foreach_mul_(avg, ema)
+                    torch._foreach_add_(avg, params, alpha=1.0 - ema)
             gn = float(net.last_grad_norm)
             used = min(gn, clip) if clip is not None else gn
             gn_mean = used if it == 0 else 0.98 * gn_mean + 0.02 * used
@@ -48,8 +61,15 @@ def train(net: CNN, x, y, scales: Scales, epochs=4, batch=64, lr=1e-3, lr_min=1e
                 history.append((it, float(loss)))
                 if log:
                     log(f"step {it}/{steps} loss {float(loss):.4f} grad norm {gn:.3g}")
+    if avg is not None:
+        nl = net.nlayers
+        for l in range(nl):
+            net.w32[l].copy_(avg[l])
+            net.b32[l].copy_(avg[nl + l])
+        net.refresh_bf16()
     torch.cuda.synchronize()
-    return {"steps": it, "seconds": time.perf_counter() - t0, "history": history}
+    return {"steps": it, "seconds": time.perf_counter() - t0, "history": history,
+            "ema": ema if avg is not None else 0.0}
 
 
 @torch.no_grad()

@@ -50,12 +50,11 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float bf2f(bf16_t h) {
   return __uint_as_float(((unsigned)h) << 16);
 }
-// round to nearest even
+// round to nearest even: the compiler's float -> bfloat conversion, one
+// v_cvt_pk_bf16_f32 on gfx950 (NaN stays NaN, quieted)
 __device__ __forceinline__ bf16_t f2bf(float f) {
-  unsigned u = __float_as_uint(f);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (bf16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (bf16_t)(u >> 16);
+  const __bf16 h = (__bf16)f;
+  return __builtin_bit_cast(bf16_t, h);
 }
 
 constexpr int KS = 5;       // kernel size
@@ -193,6 +192,231 @@ __global__ __launch_bounds__(256, RW) void k_conv_fwd_mt(const bf16_t* __restric
       }
     }
   }
+}
+
+// --------------------------------------------------------------------------
+// First layer (at most 8 real input channels): the taps go into the K
+// dimension.  K = tap * 8 + channel over the first 8 channels of the stored
+// input and 25 taps padded to 28, so one K=32 MFMA covers 4 taps and the layer
+// takes 7 matrix steps per tile instead of 25 K=16 ones over 16 padded
+// channels.  In the operand layout lane group g holds K slots [8g, 8g + 8):
+// tap 4c + g of step c, i.e. one 16-byte read of the staged pixel shifted by
+// that tap.  Weights are repacked once per update: w8[co][28][8].
+// --------------------------------------------------------------------------
+constexpr int kFirstCh = 8;
+constexpr int kFirstTaps = 28;
+
+template <int W, int MT, int RW>
+__global__ __launch_bounds__(256, RW) void k_conv_first(const bf16_t* __restrict__ x,
+                                                       const bf16_t* __restrict__ w8,
+                                                       const float* __restrict__ bias, int H,
+                                                       int cinp, int coutp, int relu,
+                                                       bf16_t* __restrict__ y,
+                                                       float* __restrict__ y32) {
+  typedef v8bf op;
+  constexpr int NT = W / 16;
+  constexpr int ROWS = 4 * RW;
+  constexpr int SR = ROWS + KS - 1;
+  constexpr int CH = kFirstCh;
+  __shared__ __attribute__((aligned(16))) bf16_t lds[SR * W * CH];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int row0 = blockIdx.x * ROWS, b = blockIdx.y;
+  const int co0 = blockIdx.z * (16 * MT);
+  const int wrow0 = row0 + wave * RW;
+  const bool wave_ok = wrow0 < H;
+  const int li = lane % 16, lg = lane / 16;
+  v4f acc[MT][RW][NT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int rr = 0; rr < RW; ++rr)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[m][rr][t] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int q = threadIdx.x; q < SR * W; q += 256) {
+    const int pix = q % W, r = q / W;
+    const int yy = ((row0 + r - 2) % H + H) % H;
+    const uint4 v = *reinterpret_cast<const uint4*>(x + (((size_t)b * H + yy) * W + pix) * cinp);
+    *reinterpret_cast<uint4*>(lds + (r * W + pix) * CH) = v;
+  }
+  __syncthreads();
+  if (wave_ok) {
+    const bf16_t* wrow = w8 + (size_t)(co0 + li) * kFirstTaps * CH + lg * CH;
+#pragma unroll
+    for (int c = 0; c < kFirstTaps / 4; ++c) {
+      // taps 25-27 have zero weights; their operand reads stay on tap 24
+      const int tap = 4 * c + lg < NTAP ? 4 * c + lg : NTAP - 1;
+      const int dy = tap / KS, dx = tap % KS;
+      op a[MT];
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        a[m] = *reinterpret_cast<const op*>(wrow + ((size_t)m * 16 * kFirstTaps + 4 * c) * CH);
+#pragma unroll
+      for (int rr = 0; rr < RW; ++rr) {
+        const bf16_t* lrow = lds + (wave * RW + rr + dy) * W * CH;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const int pix = (t * 16 + li + dx - 2 + W) % W;
+          const op bv = *reinterpret_cast<const op*>(lrow + pix * CH);
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+            acc[m][rr][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], bv, acc[m][rr][t], 0, 0, 0);
+        }
+      }
+    }
+  }
+  if (!wave_ok) return;
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) {
+    const int row = wrow0 + rr;
+    if (row >= H) break;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int cb = co0 + m * 16 + lg * 4;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const size_t o = (((size_t)b * H + row) * W + t * 16 + li) * coutp + cb;
+        float v[4] = {acc[m][rr][t][0], acc[m][rr][t][1], acc[m][rr][t][2], acc[m][rr][t][3]};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (bias) v[j] += bias[cb + j];
+          if (relu) v[j] = v[j] > 0.f ? v[j] : 0.f;
+        }
+        if (y) {
+          v4s out;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) out[j] = (short)f2bf(v[j]);
+          *reinterpret_cast<v4s*>(y + o) = out;
+        }
+        if (y32) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) y32[o + j] = v[j];
+        }
+      }
+    }
+  }
+}
+
+// w8[co][tap][c] = w[co][tap][c] for tap < 25 and c < 8, else 0
+__global__ void k_pack_first(const bf16_t* __restrict__ w, bf16_t* __restrict__ w8, int cinp,
+                             int coutp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= coutp * kFirstTaps * kFirstCh) return;
+  const int c = i % kFirstCh, tap = (i / kFirstCh) % kFirstTaps, co = i / (kFirstCh * kFirstTaps);
+  w8[i] = (tap < NTAP && c < cinp) ? w[((size_t)co * NTAP + tap) * cinp + c] : (bf16_t)0;
+}
+
+// --------------------------------------------------------------------------
+// Last layer (cout output channels with 5 * cout <= 16): the horizontal taps go
+// into the M dimension.  Row c*5 + j of the MFMA tile is output channel c's
+// column-offset-j part D[c, j](p) = sum over dy, ci of W[c][dy*5 + j][ci]
+// x[row + dy - 2][p][ci] (K = 5 vertical taps x the input channels, no
+// horizontal shift), and y[p][c] = bias[c] + sum_j D[c, j](p + j - 2), the
+// horizontal sum taken through LDS.  Per pixel tile 5 * cinp/32 matrix steps
+// instead of 25 * cinp/32 for a 16-channel tile of which cout are real.
+// Weights repacked once per update: wl[16][5][cinp] (rows >= 5 cout zero).
+// --------------------------------------------------------------------------
+template <int W, int RW>
+__global__ __launch_bounds__(256) void k_conv_last(const bf16_t* __restrict__ x,
+                                                   const bf16_t* __restrict__ wl,
+                                                   const float* __restrict__ bias, int H,
+                                                   int cinp, int coutp, int cout,
+                                                   bf16_t* __restrict__ y,
+                                                   float* __restrict__ y32) {
+  typedef v8bf op;
+  constexpr int KC = 32;
+  constexpr int NT = W / 16;
+  constexpr int ROWS = 4 * RW;
+  constexpr int SR = ROWS + KS - 1;
+  constexpr int PS = KC + 8;
+  constexpr int PIECES = KC / 8;
+  __shared__ __attribute__((aligned(16))) bf16_t lds[SR * W * PS];
+  __shared__ float part[4][RW][16][W + 4];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int row0 = blockIdx.x * ROWS, b = blockIdx.y;
+  const int wrow0 = row0 + wave * RW;
+  const bool wave_ok = wrow0 < H;
+  const int li = lane % 16, lg = lane / 16;
+  v4f acc[RW][NT];
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[rr][t] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int ci0 = 0; ci0 < cinp; ci0 += KC) {
+    __syncthreads();
+    for (int q = threadIdx.x; q < SR * W * PIECES; q += 256) {
+      const int pc = q % PIECES, pix = (q / PIECES) % W, r = q / (PIECES * W);
+      const int yy = ((row0 + r - 2) % H + H) % H;
+      const uint4 v = *reinterpret_cast<const uint4*>(
+          x + (((size_t)b * H + yy) * W + pix) * cinp + ci0 + pc * 8);
+      *reinterpret_cast<uint4*>(lds + (r * W + pix) * PS + pc * 8) = v;
+    }
+    __syncthreads();
+    if (wave_ok) {
+#pragma unroll
+      for (int dy = 0; dy < KS; ++dy) {
+        const op a = *reinterpret_cast<const op*>(wl + ((size_t)li * KS + dy) * cinp + ci0 +
+                                                  lg * 8);
+#pragma unroll
+        for (int rr = 0; rr < RW; ++rr) {
+          const bf16_t* lrow = lds + (wave * RW + rr + dy) * W * PS + lg * 8;
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+            const op bv = *reinterpret_cast<const op*>(lrow + (t * 16 + li) * PS);
+            acc[rr][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, acc[rr][t], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+  if (!wave_ok) return;
+  // D[4 lg + j][p] of the wave's rows into LDS (its own region), then the
+  // horizontal sums; every lane handles one pixel
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) part[wave][rr][4 * lg + j][t * 16 + li] = acc[rr][t][j];
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the wave's LDS writes are done
+  __builtin_amdgcn_wave_barrier();
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr) {
+    const int row = wrow0 + rr;
+    if (row >= H) break;
+    const int p = lane;
+    const size_t o = (((size_t)b * H + row) * W + p) * coutp;
+    for (int c0 = 0; c0 < coutp; c0 += 4) {
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int c = c0 + i;
+        v[i] = bias ? bias[c] : 0.f;
+        if (c < cout) {
+          float sum = 0.f;
+#pragma unroll
+          for (int j = 0; j < KS; ++j) sum += part[wave][rr][c * KS + j][(p + j - 2 + W) % W];
+          v[i] += sum;
+        }
+      }
+      if (y) {
+        v4s out;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out[i] = (short)f2bf(v[i]);
+        *reinterpret_cast<v4s*>(y + o + c0) = out;
+      }
+      if (y32) *reinterpret_cast<v4f*>(y32 + o + c0) = v4f{v[0], v[1], v[2], v[3]};
+    }
+  }
+}
+
+// wl[r][dy][ci] = w[r / 5][dy * 5 + r % 5][ci] for r < 5 cout, else 0
+__global__ void k_pack_last(const bf16_t* __restrict__ w, bf16_t* __restrict__ wl, int cinp,
+                            int cout) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 16 * KS * cinp) return;
+  const int ci = i % cinp, dy = (i / cinp) % KS, r = i / (cinp * KS);
+  const int c = r / KS, j = r % KS;
+  wl[i] = c < cout ? w[((size_t)c * NTAP + dy * KS + j) * cinp + ci] : (bf16_t)0;
 }
 
 // --------------------------------------------------------------------------
@@ -579,6 +803,69 @@ int qgcnn_conv_fwd(const void* x, int B, int H, int W, int cinp, const void* w,
   }
 #undef QGCNN_FWD_RW
 #undef QGCNN_FWD
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+int qgcnn_pack_first(const void* w, int cinp, int coutp, void* w8, void* stream) {
+  if (!w || !w8) return fail(QGCNN_E_ARG, "NULL argument");
+  if (!mult16(cinp) || !mult16(coutp))
+    return fail(QGCNN_E_ARG, "channel counts must be multiples of 16 in [16, 128]");
+  const int n = coutp * kFirstTaps * kFirstCh;
+  hipLaunchKernelGGL(k_pack_first, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     (const bf16_t*)w, (bf16_t*)w8, cinp, coutp);
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+int qgcnn_conv_first(const void* x, int B, int H, int W, int cinp, const void* w8,
+                     const float* bias, int coutp, int relu, void* y, float* y32, void* stream) {
+  if (!x || !w8 || (!y && !y32)) return fail(QGCNN_E_ARG, "NULL argument");
+  if (!mult16(cinp) || !mult16(coutp))
+    return fail(QGCNN_E_ARG, "channel counts must be multiples of 16 in [16, 128]");
+  if (B < 1 || H < KS || W != 64)
+    return fail(QGCNN_E_ARG, "images must be W=64 wide and at least 5 rows (got %dx%d)", H, W);
+  hipStream_t st = (hipStream_t)stream;
+  const int mt = coutp % 64 == 0 ? 4 : (coutp % 32 == 0 ? 2 : 1);
+  const int rw = H % 8 == 0 ? 2 : 1;
+  dim3 grid((H + 4 * rw - 1) / (4 * rw), B, coutp / (16 * mt));
+#define QGCNN_FIRST(MT, RW)                                                                    \
+  hipLaunchKernelGGL((k_conv_first<64, MT, RW>), grid, dim3(256), 0, st, (const bf16_t*)x,    \
+                     (const bf16_t*)w8, bias, H, cinp, coutp, relu, (bf16_t*)y, y32)
+  if (rw == 2) {
+    if (mt == 4) QGCNN_FIRST(4, 2); else if (mt == 2) QGCNN_FIRST(2, 2); else QGCNN_FIRST(1, 2);
+  } else {
+    if (mt == 4) QGCNN_FIRST(4, 1); else if (mt == 2) QGCNN_FIRST(2, 1); else QGCNN_FIRST(1, 1);
+  }
+#undef QGCNN_FIRST
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+int qgcnn_pack_last(const void* w, int cinp, int coutp, int cout, void* wl, void* stream) {
+  if (!w || !wl) return fail(QGCNN_E_ARG, "NULL argument");
+  if (!mult16(cinp) || cinp % 32 != 0 || !mult16(coutp) || cout < 1 || KS * cout > 16 ||
+      cout > coutp)
+    return fail(QGCNN_E_ARG, "last-layer packing needs cinp a multiple of 32 and 5*cout <= 16");
+  const int n = 16 * KS * cinp;
+  hipLaunchKernelGGL(k_pack_last, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     (const bf16_t*)w, (bf16_t*)wl, cinp, cout);
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+int qgcnn_conv_last(const void* x, int B, int H, int W, int cinp, const void* wl,
+                    const float* bias, int coutp, int cout, void* y, float* y32, void* stream) {
+  if (!x || !wl || (!y && !y32)) return fail(QGCNN_E_ARG, "NULL argument");
+  if (!mult16(cinp) || cinp % 32 != 0 || !mult16(coutp) || cout < 1 || KS * cout > 16 ||
+      cout > coutp)
+    return fail(QGCNN_E_ARG, "the last-layer kernel needs cinp a multiple of 32 and 5*cout <= 16");
+  if (B < 1 || H < KS || W != 64)
+    return fail(QGCNN_E_ARG, "images must be W=64 wide and at least 5 rows (got %dx%d)", H, W);
+  dim3 grid((H + 3) / 4, B);
+  hipLaunchKernelGGL((k_conv_last<64, 1>), grid, dim3(256), 0, (hipStream_t)stream,
+                     (const bf16_t*)x, (const bf16_t*)wl, bias, H, cinp, coutp, cout, (bf16_t*)y,
+                     y32);
   HIPC(hipGetLastError());
   return 0;
 }

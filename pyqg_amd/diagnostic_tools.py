@@ -68,8 +68,12 @@ def calc_ispec(model, _var_dens, averaging=True, truncate=True, nd_wavenumber=Fa
         phr[i] *= 2
     kr = kr + dkr / 2
     if nd_wavenumber:
-        kr = kr / kmin
-        phr = phr * kmin
+        # The reference divides by kmin, which is 0 there (diagnostic_tools.py:
+        # 118-121), giving inf; here the wavenumbers are made nondimensional
+        # with the fundamental wavenumber of the grid instead.
+        kfund = min(model.dk, model.dl)
+        kr = kr / kfund
+        phr = phr * kfund
     return kr, phr
 
 

@@ -387,6 +387,15 @@ class Ensemble:
         per = [b.diag_read(name, member=-1) / cnt for b in self.buckets]
         return [per[bi][j] for (bi, j) in self.members]
 
+    def mean_model(self, bucket=0):
+        """The time-mean diagnostics of one bucket, averaged over its members,
+        as an object pyqg's comparison tools accept in place of a model
+        (diagnostic_tools.diagnostic_differences / diagnostic_similarities,
+        pyqg/diagnostic_tools.py:128-262): a registry of the diagnostics
+        accumulated so far with their dims, sums_with and skip_comparison
+        entries, get_diagnostic, nz and the grid attributes calc_ispec reads."""
+        return _MeanModel(self, bucket)
+
     def reset_diagnostics(self):
         for b in self.buckets:
             if b.flags & _lib.FULL_STATE and self.diag_count:
@@ -401,3 +410,31 @@ class Ensemble:
         for b in self.buckets:
             b.close()
         self.buckets = []
+
+
+class _MeanModel:
+    """Member-mean time averages of one ensemble bucket, shaped like a model
+    for the diagnostic comparison tools (see Ensemble.mean_model)."""
+
+    def __init__(self, ens, bucket):
+        from . import diagnostic_tools as tools
+        from .diagnostics import CORE, LAYERED_ONLY, QG_ONLY, QGDIAG
+        b = ens.buckets[bucket]
+        L = float(ens.closure_dx[bucket][0]) * b.nx
+        g = tools.grid(b.nx, L)
+        for a in ("kk", "ll", "dk", "dl", "wv", "wv2", "M", "nx", "ny"):
+            setattr(self, a, getattr(g, a))
+        self.nz = b.nz
+        self._ens, self._bucket = ens, bucket
+        table = dict(CORE, **QGDIAG, **(QG_ONLY if b.nz == 2 else LAYERED_ONLY))
+        self.diagnostics = {}
+        for name, (desc, units, dims, extra) in table.items():
+            if ens.diag_count.get(name, 0) == 0:
+                continue
+            entry = dict(description=desc, units=units, dims=dims, **extra)
+            entry["function"] = lambda m, n=name: m.get_diagnostic(n)
+            self.diagnostics[name] = entry
+
+    def get_diagnostic(self, name):
+        return self._ens.get_diagnostic(name, mean=True)[self._bucket]
+

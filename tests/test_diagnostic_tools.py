@@ -42,3 +42,18 @@ def test_spec_var_and_sum():
     np.testing.assert_allclose(spec_var(m, ph), p.var(axis=(-2, -1)), rtol=1e-12)
     np.testing.assert_allclose(spec_sum(np.abs(ph) ** 2 / m.M ** 2),
                                (p ** 2).mean(axis=(-2, -1)), rtol=1e-12)
+
+
+def test_calc_ispec_nondimensional_wavenumber():
+    """nd_wavenumber gives finite wavenumbers in units of the fundamental
+    wavenumber and keeps the spectrum's integral (the reference divides by a
+    zero kmin instead)."""
+    m = grid(64, 1e6)
+    p = np.random.default_rng(3).random((64, 64))
+    Eh = np.abs(rfft2(p)) ** 2
+    kr, Ehr = calc_ispec(m, Eh, truncate=False, averaging=False)
+    krn, Ehn = calc_ispec(m, Eh, truncate=False, averaging=False, nd_wavenumber=True)
+    assert np.isfinite(krn).all() and np.isfinite(Ehn).all()
+    np.testing.assert_allclose(krn, kr / m.dk, rtol=1e-14)
+    np.testing.assert_allclose(Ehn.sum() * (krn[1] - krn[0]), Ehr.sum() * (kr[1] - kr[0]),
+                               rtol=1e-12)

@@ -50,6 +50,56 @@ def test_conv_forward(cnn, cinp, coutp, relu):
         ref = torch.relu(ref)
     assert rel(y32.cpu(), ref) < 1e-5
     assert rel(y_bf.float().cpu(), ref) < 1e-2  # BF16 output rounding
+    # the BF16 output is the FP32 result rounded to nearest even
+    assert torch.equal(y_bf.cpu(), y32.cpu().to(torch.bfloat16))
+
+
+@pytest.mark.parametrize("cin,coutp,H,relu", [(6, 128, 8, True), (8, 64, 16, True),
+                                              (3, 32, 12, False), (6, 16, 5, True)])
+def test_first_layer_kernel(cnn, cin, coutp, H, relu):
+    """The first-layer kernel (25 taps x 8 channels packed into K) against the
+    float reference, the input stored with 16 channels of which cin are real
+    and the rest zero (the network input's layout)."""
+    import torch
+    g = torch.Generator().manual_seed(cin * 100 + coutp + H)
+    B, W = 2, 64
+    x = torch.zeros((B, H, W, 16))
+    x[..., :cin] = rnd_bf16((B, H, W, cin), 1.0, g)
+    w = torch.zeros((coutp, 25, 16))
+    w[..., :cin] = rnd_bf16((coutp, 25, cin), 0.1, g)
+    b = torch.randn(coutp, generator=g) * 0.1
+    w8 = cnn.pack_first(w.to(torch.bfloat16).cuda())
+    y_bf, y32 = cnn.conv_first(x.to(torch.bfloat16).cuda(), w8, b.cuda(), coutp, relu=relu,
+                               out_f32=True)
+    ref = ref_conv(x, w, b)
+    if relu:
+        ref = torch.relu(ref)
+    assert rel(y32.cpu(), ref) < 1e-5
+    assert rel(y_bf.float().cpu(), ref) < 1e-2
+    assert torch.equal(y_bf.cpu(), y32.cpu().to(torch.bfloat16))
+
+
+@pytest.mark.parametrize("cinp,cout,H", [(32, 2, 8), (64, 3, 12), (32, 1, 5), (128, 2, 64)])
+def test_last_layer_kernel(cnn, cinp, cout, H):
+    """The last-layer kernel (horizontal taps as matrix rows, the horizontal sum
+    through LDS) against the float reference; the padded output channels carry
+    their bias."""
+    import torch
+    g = torch.Generator().manual_seed(cinp + 10 * cout + H)
+    B, W, coutp = 3, 64, 16
+    x = rnd_bf16((B, H, W, cinp), 1.0, g)
+    w = torch.zeros((coutp, 25, cinp))
+    w[:cout] = rnd_bf16((cout, 25, cinp), 0.1, g)
+    b = torch.zeros(coutp)
+    b[:cout] = torch.randn(cout, generator=g) * 0.1
+    wl = cnn.pack_last(w.to(torch.bfloat16).cuda(), cout)
+    y_bf, y32 = cnn.conv_last(x.to(torch.bfloat16).cuda(), wl, b.cuda(), coutp, cout,
+                              out_bf16=True, out_f32=True)
+    ref = ref_conv(x, w, b)
+    assert rel(y32.cpu(), ref) < 1e-5
+    assert rel(y_bf.float().cpu(), ref) < 1e-2
+    assert torch.equal(y_bf.cpu(), y32.cpu().to(torch.bfloat16))
+    assert not y32[..., cout:].any()
 
 
 def test_backward_data_with_relu_mask(cnn):
