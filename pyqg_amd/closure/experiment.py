@@ -45,12 +45,12 @@ BUDGET = ["KEflux", "APEflux", "APEgenspec", "KEfrictionspec", "Dissspec", "EKEd
 
 def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=64, lr=1e-3,
         online_members=128, online_days=720, avg_days=360, dt_lo=14400., seed=0, ema=0.998,
-        ema_from=0.5, soup=1, closures=("backscatter", "zb2020", "smagorinsky"), log=print):
+        ema_from=0.5, closures=("backscatter", "zb2020", "smagorinsky"), log=print):
     out = {"config": dict(members=members, samples=samples, spinup_days=spinup_days,
                           every=every, epochs=epochs, batch=batch, lr=lr,
                           online_members=online_members, online_days=online_days,
                           avg_days=avg_days, dt_hi=3600., dt_lo=dt_lo, seed=seed, ema=ema,
-                          ema_from=ema_from, soup=soup)}
+                          ema_from=ema_from)}
     L = data_mod.QG_EDDY["L"]
     g_hi, g_lo = tools.grid(256, L), tools.grid(64, L)
     # ---------------------------------------------------------- 1. data
@@ -96,24 +96,7 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
     out["n_params"] = net.n_params()
     info = train_mod.train(net, x_tr, y_tr, scales, epochs=epochs, batch=batch, lr=lr,
                            log=log, seed=seed, ema=ema, ema_from=ema_from)
-    if soup > 1:
-        # further networks from the same initial weights, trained on other
-        # batch orders; the closure uses their weight average
-        out["offline_members"] = [train_mod.evaluate(net, x_te, y_te, scales)["r2"]]
-        nets = [net]
-        for i in range(1, soup):
-            other = cnn_mod.CNN(seed=seed)
-            info_i = train_mod.train(other, x_tr, y_tr, scales, epochs=epochs, batch=batch,
-                                     lr=lr, log=None, seed=seed + 1000 * i, ema=ema,
-                                     ema_from=ema_from)
-            info["seconds"] += info_i["seconds"]
-            out["offline_members"].append(train_mod.evaluate(other, x_te, y_te, scales)["r2"])
-            nets.append(other)
-        for l in range(net.nlayers):
-            net.w32[l].copy_(sum(n.w32[l] for n in nets) / soup)
-            net.b32[l].copy_(sum(n.b32[l] for n in nets) / soup)
-        net.refresh_bf16()
-    out["train"] = {"steps": info["steps"], "seconds": info["seconds"], "networks": soup,
+    out["train"] = {"steps": info["steps"], "seconds": info["seconds"],
                     "final_loss": info["history"][-1][1] if info["history"] else None}
     out["offline"] = train_mod.evaluate(net, x_te, y_te, scales)
     flop_per_sample = 2 * 64 * 64 * 25 * sum(
@@ -201,9 +184,6 @@ def main():
                     help="weight averaging decay over the second half of training (0: off)")
     ap.add_argument("--ema-from", type=float, default=0.5,
                     help="fraction of training after which the weights are averaged")
-    ap.add_argument("--soup", type=int, default=1,
-                    help="networks trained from the same initial weights on different batch "
-                         "orders and averaged (weight average)")
     ap.add_argument("--closures", default="backscatter,zb2020,smagorinsky",
                     help="built-in closures run online beside the network (comma list)")
     ap.add_argument("--out", default=None, help="also append the JSON line to this file")
@@ -211,7 +191,7 @@ def main():
     res = run(members=a.members, samples=a.samples, spinup_days=a.spinup_days, every=a.every,
               epochs=a.epochs, batch=a.batch, lr=a.lr, online_members=a.online_members,
               online_days=a.online_days, avg_days=a.avg_days, dt_lo=a.dt_lo, seed=a.seed,
-              ema=a.ema, ema_from=a.ema_from, soup=a.soup, closures=[c for c in a.closures.split(",") if c],
+              ema=a.ema, ema_from=a.ema_from, closures=[c for c in a.closures.split(",") if c],
               log=lambda s: print(s, flush=True))
     line = json.dumps(res)
     print(line)

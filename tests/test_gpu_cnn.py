@@ -54,6 +54,34 @@ def test_conv_forward(cnn, cinp, coutp, relu):
     assert torch.equal(y_bf.cpu(), y32.cpu().to(torch.bfloat16))
 
 
+@pytest.mark.parametrize("cinp,coutp,H,relu,masked", [(32, 32, 8, True, False),
+                                                     (64, 32, 12, True, False),
+                                                     (128, 64, 64, False, False),
+                                                     (32, 64, 6, True, True),
+                                                     (96, 32, 16, False, True)])
+def test_conv_forward_narrow_layers(cnn, cinp, coutp, H, relu, masked):
+    """Layers with 32 or 64 output channels against the float reference, at row
+    counts that fill partial blocks, with and without the ReLU mask of the
+    backward pass."""
+    import torch
+    g = torch.Generator().manual_seed(cinp * 7 + coutp + H)
+    B, W = 2, 64
+    x = rnd_bf16((B, H, W, cinp), 1.0, g)
+    w = rnd_bf16((coutp, 25, cinp), 0.1, g)
+    b = torch.randn(coutp, generator=g) * 0.1
+    mk = torch.relu(rnd_bf16((B, H, W, coutp), 1.0, g)) if masked else None
+    y_bf, y32 = cnn.conv_fwd(x.to(torch.bfloat16).cuda(), w.to(torch.bfloat16).cuda(), b.cuda(),
+                             coutp, relu=relu, out_f32=True,
+                             mask=None if mk is None else mk.to(torch.bfloat16).cuda())
+    ref = ref_conv(x, w, b)
+    if relu:
+        ref = torch.relu(ref)
+    if masked:
+        ref = ref * (mk > 0)
+    assert rel(y32.cpu(), ref) < 1e-5
+    assert torch.equal(y_bf.cpu(), y32.cpu().to(torch.bfloat16))
+
+
 @pytest.mark.parametrize("cin,coutp,H,relu", [(6, 128, 8, True), (8, 64, 16, True),
                                               (3, 32, 12, False), (6, 16, 5, True)])
 def test_first_layer_kernel(cnn, cin, coutp, H, relu):
