@@ -474,7 +474,11 @@ def main():
     model_days = M * args.steps * DT / 86400.
     value = model_days / wall
     bytes_step = float(sum(algorithmic_bytes(nz, int(n), int(n)) for n in nxs))
-    achieved = bytes_step / (step_ms * 1e-3) / 1e9
+    # With several buckets each runs on its own stream and their kernels
+    # overlap, so the sum of the per-kernel event times overstates the step;
+    # the step's span is then the timed region's wall clock per step.
+    span_ms = step_ms if len(ens.buckets) == 1 else wall * 1e3 / args.steps
+    achieved = bytes_step / (span_ms * 1e-3) / 1e9
     traffic, traffic_src = None, None
     if args.workload == "qg256" and M == MEMBERS:
         traffic = measured_traffic(*ens.buckets[0].variant())
@@ -501,7 +505,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "kernel": "fused step = k_row_flux + k_col_step (per bucket, summed)",
+                     "kernel": ("fused step = k_row_flux + k_col_step" if len(ens.buckets) == 1
+                                else "fused steps of every bucket, concurrent on their streams: "
+                                     "achieved over the timed region's wall clock per step"),
                      "algorithmic_bytes_per_launch": bytes_step,
                      "kernel_ms_per_step": step_ms, "buckets": per_bucket,
                      "copy_kernel": copy, "access_pattern_bound": pattern, "autotune": tune},
