@@ -166,6 +166,34 @@ def cnn_cpu_baseline(channels, batch=4, reps=3):
             "sample": f"{reps} forward passes of {batch} samples, torch CPU FP32 conv2d"}
 
 
+def cnn_library_reference(channels, batch, reps):
+    """The same layers through torch.nn.functional.conv2d (MIOpen), BF16,
+    channels_last, on inputs padded by two pixels beforehand (no bias, no
+    ReLU): the vendor library's time for each convolution on this GPU."""
+    import torch
+    import torch.nn.functional as F
+    torch.backends.cudnn.benchmark = True
+    ms = []
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for l in range(len(channels) - 1):
+        x = torch.randn(batch, channels[l], 68, 68, device="cuda", dtype=torch.bfloat16)
+        w = torch.randn(channels[l + 1], channels[l], 5, 5, device="cuda", dtype=torch.bfloat16)
+        x = x.contiguous(memory_format=torch.channels_last)
+        w = w.contiguous(memory_format=torch.channels_last)
+        for _ in range(5):
+            F.conv2d(x, w)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            F.conv2d(x, w)
+        e1.record()
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1) / reps)
+    return {"kind": "MIOpen through torch.nn.functional.conv2d, BF16 channels_last, "
+                    "pre-padded input, no bias or ReLU",
+            "layer_ms": ms, "sum_ms": sum(ms)}
+
+
 def bench_cnn(args):
     """Inference and training throughput of the closure network on libqgcnn."""
     import numpy as np
@@ -223,6 +251,7 @@ def bench_cnn(args):
         torch.cuda.synchronize()
         ms_train = e0.elapsed_time(e1) / nt
         train_tflops = 3 * Bt * fl / (ms_train * 1e-3) / 1e12
+    library = cnn_library_reference(net.channels, B, args.steps) if args.cnn_library else None
     out = {
         "metric": "CNN subgrid-closure inference throughput (64x64 inputs, 8 conv layers)",
         "value": B * args.steps / wall,
@@ -255,6 +284,8 @@ def bench_cnn(args):
                      "train_step_ms_batch64": ms_train, "train_tflops": train_tflops},
         "cpu_baseline": cpu,
     }
+    if library is not None:
+        out["library_reference"] = library
     print(json.dumps(out))
 
 
@@ -362,6 +393,8 @@ def main():
     ap.add_argument("--workload", choices=["qg256", "uneven", "layered10", "cnn", "dropin",
                                            "single512"],
                     default="qg256")
+    ap.add_argument("--cnn-library", action="store_true",
+                    help="cnn workload: also time each layer with MIOpen (torch conv2d)")
     ap.add_argument("--cnn-inference-only", action="store_true",
                     help="cnn workload: skip the training-step timing (profiling runs)")
     ap.add_argument("--graph-steps", type=int, default=100,

@@ -72,6 +72,8 @@ constexpr int NTAP = KS * KS;
 // MFMA lane layout (A 16 x K, B K x 16): lane (i = lane%16, g = lane/16) holds
 // A[i][g*K/4 .. +K/4] and B[g*K/4 .. +K/4][i]; the accumulator lane holds
 // D[4g + j][i].  A and B use the same K slots, so the sum covers every channel.
+// Row r of tile m is output channel tile_channel<MT>(co0, m, r) (below), so a
+// lane's accumulators over the MT tiles are consecutive channels of its pixel.
 // --------------------------------------------------------------------------
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
@@ -91,6 +93,65 @@ struct MfmaK<16> {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
   }
 };
+
+// Output channels of a block's MT matrix tiles.  The rows of the tiles are
+// assigned so that accumulator lane group g (rows 4g..4g+3 of every tile)
+// holds the 4*MT consecutive channels co0 + 4*MT*g + [0, 4*MT): tile m, row r
+// computes channel co0 + 4*MT*(r/4) + 4*m + r%4.  A lane then stores each of
+// its pixels' channels as one contiguous 8*MT-byte run, and the four lane
+// groups together write the block's 32*MT bytes of the pixel in one piece
+// (one 128-byte line at MT = 4) instead of MT scattered 32-byte fragments.
+template <int MT>
+__device__ __forceinline__ int tile_channel(int co0, int m, int r) {
+  return co0 + 4 * MT * (r / 4) + 4 * m + r % 4;
+}
+
+typedef short v8s __attribute__((ext_vector_type(8)));
+
+// Epilogue of one pixel's 4*MT consecutive channels from cb (flat NHWC offset
+// o): bias, ReLU, mask (channels whose mask value is <= 0 are zeroed), then
+// BF16 and/or FP32 stores, each as whole 16-byte pieces where MT >= 2.
+template <int MT>
+__device__ __forceinline__ void store_channels(float (&v)[4 * MT], const float* __restrict__ bias,
+                                               int cb, int relu, const bf16_t* __restrict__ mask,
+                                               bf16_t* __restrict__ y, float* __restrict__ y32,
+                                               size_t o) {
+#pragma unroll
+  for (int i = 0; i < 4 * MT; ++i) {
+    if (bias) v[i] += bias[cb + i];
+    if (relu) v[i] = v[i] > 0.f ? v[i] : 0.f;
+  }
+  if (mask) {
+#pragma unroll
+    for (int p = 0; p < MT; ++p) {
+      const v4s mk = *reinterpret_cast<const v4s*>(mask + o + 4 * p);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (bf2f((bf16_t)mk[j]) <= 0.f) v[4 * p + j] = 0.f;
+    }
+  }
+  if (y) {
+    if constexpr (MT >= 2) {
+#pragma unroll
+      for (int p = 0; p < MT / 2; ++p) {
+        v8s out;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) out[j] = (short)f2bf(v[8 * p + j]);
+        *reinterpret_cast<v8s*>(y + o + 8 * p) = out;
+      }
+    } else {
+      v4s out;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) out[j] = (short)f2bf(v[j]);
+      *reinterpret_cast<v4s*>(y + o) = out;
+    }
+  }
+  if (y32) {
+#pragma unroll
+    for (int p = 0; p < MT; ++p)
+      *reinterpret_cast<v4f*>(y32 + o + 4 * p) = v4f{v[4 * p], v[4 * p + 1], v[4 * p + 2], v[4 * p + 3]};
+  }
+}
 
 template <int W, int MT, int KC, int RW>
 __global__ __launch_bounds__(256, RW) void k_conv_fwd_mt(const bf16_t* __restrict__ x,
@@ -133,14 +194,15 @@ __global__ __launch_bounds__(256, RW) void k_conv_fwd_mt(const bf16_t* __restric
     }
     __syncthreads();
     if (wave_ok) {
-      const bf16_t* wrow = w + (size_t)(co0 + li) * NTAP * cinp + ci0 + lg * KG;
+      const bf16_t* wrow =
+          w + (size_t)tile_channel<MT>(co0, 0, li) * NTAP * cinp + ci0 + lg * KG;
 #pragma unroll 1
       for (int tap = 0; tap < NTAP; ++tap) {
         const int dy = tap / KS, dx = tap % KS;
         op a[MT];
 #pragma unroll
         for (int m = 0; m < MT; ++m)
-          a[m] = *reinterpret_cast<const op*>(wrow + ((size_t)m * 16 * NTAP + tap) * cinp);
+          a[m] = *reinterpret_cast<const op*>(wrow + ((size_t)m * 4 * NTAP + tap) * cinp);
 #pragma unroll
         for (int rr = 0; rr < RW; ++rr) {
           const bf16_t* lrow = lds + (wave * RW + rr + dy) * W * PS + lg * KG;
@@ -156,40 +218,20 @@ __global__ __launch_bounds__(256, RW) void k_conv_fwd_mt(const bf16_t* __restric
     }
   }
   if (!wave_ok) return;
+  const int cb = tile_channel<MT>(co0, 0, 4 * lg);
 #pragma unroll
   for (int rr = 0; rr < RW; ++rr) {
     const int row = wrow0 + rr;
     if (row >= H) break;
 #pragma unroll
-    for (int m = 0; m < MT; ++m) {
-      const int cb = co0 + m * 16 + lg * 4;
+    for (int t = 0; t < NT; ++t) {
+      const size_t o = (((size_t)b * H + row) * W + t * 16 + li) * coutp + cb;
+      float v[4 * MT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const int pix = t * 16 + li;
-        const size_t o = (((size_t)b * H + row) * W + pix) * coutp + cb;
-        float v[4] = {acc[m][rr][t][0], acc[m][rr][t][1], acc[m][rr][t][2], acc[m][rr][t][3]};
+      for (int m = 0; m < MT; ++m)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if (bias) v[j] += bias[cb + j];
-          if (relu) v[j] = v[j] > 0.f ? v[j] : 0.f;
-        }
-        if (mask) {
-          const v4s mk = *reinterpret_cast<const v4s*>(mask + o);
-#pragma unroll
-          for (int j = 0; j < 4; ++j)
-            if (bf2f((bf16_t)mk[j]) <= 0.f) v[j] = 0.f;
-        }
-        if (y) {
-          v4s out;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) out[j] = (short)f2bf(v[j]);
-          *reinterpret_cast<v4s*>(y + o) = out;
-        }
-        if (y32) {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) y32[o + j] = v[j];
-        }
-      }
+        for (int j = 0; j < 4; ++j) v[4 * m + j] = acc[m][rr][t][j];
+      store_channels<MT>(v, bias, cb, relu, mask, y, y32, o);
     }
   }
 }
@@ -240,7 +282,7 @@ __global__ __launch_bounds__(256, RW) void k_conv_first(const bf16_t* __restrict
   }
   __syncthreads();
   if (wave_ok) {
-    const bf16_t* wrow = w8 + (size_t)(co0 + li) * kFirstTaps * CH + lg * CH;
+    const bf16_t* wrow = w8 + (size_t)tile_channel<MT>(co0, 0, li) * kFirstTaps * CH + lg * CH;
 #pragma unroll
     for (int c = 0; c < kFirstTaps / 4; ++c) {
       // taps 25-27 have zero weights; their operand reads stay on tap 24
@@ -249,7 +291,7 @@ __global__ __launch_bounds__(256, RW) void k_conv_first(const bf16_t* __restrict
       op a[MT];
 #pragma unroll
       for (int m = 0; m < MT; ++m)
-        a[m] = *reinterpret_cast<const op*>(wrow + ((size_t)m * 16 * kFirstTaps + 4 * c) * CH);
+        a[m] = *reinterpret_cast<const op*>(wrow + ((size_t)m * 4 * kFirstTaps + 4 * c) * CH);
 #pragma unroll
       for (int rr = 0; rr < RW; ++rr) {
         const bf16_t* lrow = lds + (wave * RW + rr + dy) * W * CH;
@@ -265,33 +307,20 @@ __global__ __launch_bounds__(256, RW) void k_conv_first(const bf16_t* __restrict
     }
   }
   if (!wave_ok) return;
+  const int cb = tile_channel<MT>(co0, 0, 4 * lg);
 #pragma unroll
   for (int rr = 0; rr < RW; ++rr) {
     const int row = wrow0 + rr;
     if (row >= H) break;
 #pragma unroll
-    for (int m = 0; m < MT; ++m) {
-      const int cb = co0 + m * 16 + lg * 4;
+    for (int t = 0; t < NT; ++t) {
+      const size_t o = (((size_t)b * H + row) * W + t * 16 + li) * coutp + cb;
+      float v[4 * MT];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const size_t o = (((size_t)b * H + row) * W + t * 16 + li) * coutp + cb;
-        float v[4] = {acc[m][rr][t][0], acc[m][rr][t][1], acc[m][rr][t][2], acc[m][rr][t][3]};
+      for (int m = 0; m < MT; ++m)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          if (bias) v[j] += bias[cb + j];
-          if (relu) v[j] = v[j] > 0.f ? v[j] : 0.f;
-        }
-        if (y) {
-          v4s out;
-#pragma unroll
-          for (int j = 0; j < 4; ++j) out[j] = (short)f2bf(v[j]);
-          *reinterpret_cast<v4s*>(y + o) = out;
-        }
-        if (y32) {
-#pragma unroll
-          for (int j = 0; j < 4; ++j) y32[o + j] = v[j];
-        }
-      }
+        for (int j = 0; j < 4; ++j) v[4 * m + j] = acc[m][rr][t][j];
+      store_channels<MT>(v, bias, cb, relu, nullptr, y, y32, o);
     }
   }
 }
