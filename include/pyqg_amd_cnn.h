@@ -24,7 +24,8 @@ const char* qgcnn_last_error(void);
 
 /* y = act(conv5x5(x, w) + bias), act = ReLU if relu; if mask != NULL the result
  * is zeroed where mask <= 0 (backward through a ReLU).  Writes BF16 y and/or
- * FP32 y32 (either may be NULL). */
+ * FP32 y32 (either may be NULL).  Every pointer given must be 16-byte aligned
+ * (QGCNN_E_ARG otherwise); the same holds for qgcnn_conv_first. */
 int qgcnn_conv_fwd(const void* x, int B, int H, int W, int cinp, const void* w,
                    const float* bias, int coutp, int relu, const void* mask, void* y, float* y32,
                    void* stream);

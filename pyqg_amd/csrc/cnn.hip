@@ -17,7 +17,9 @@
 #include <hip/hip_bf16.h>
 
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
+#include <initializer_list>
 #include <string>
 
 #include "../../include/pyqg_amd_cnn.h"
@@ -108,17 +110,30 @@ __device__ __forceinline__ int tile_channel(int co0, int m, int r) {
 
 typedef short v8s __attribute__((ext_vector_type(8)));
 
-// Epilogue of one pixel's 4*MT consecutive channels from cb (flat NHWC offset
-// o): bias, ReLU, mask (channels whose mask value is <= 0 are zeroed), then
-// BF16 and/or FP32 stores, each as whole 16-byte pieces where MT >= 2.
+// Epilogue of one pixel's 4*MT consecutive channels (flat NHWC offset o):
+// bias (bv, read once per lane by load_bias), ReLU, mask (channels whose mask
+// value is <= 0 are zeroed), then BF16 and/or FP32 stores, each as whole
+// 16-byte pieces where MT >= 2.
 template <int MT>
-__device__ __forceinline__ void store_channels(float (&v)[4 * MT], const float* __restrict__ bias,
-                                               int cb, int relu, const bf16_t* __restrict__ mask,
+__device__ __forceinline__ void load_bias(const float* __restrict__ bias, int cb,
+                                          float (&bv)[4 * MT]) {
+#pragma unroll
+  for (int p = 0; p < MT; ++p) {
+    const v4f q = bias ? *reinterpret_cast<const v4f*>(bias + cb + 4 * p) : v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) bv[4 * p + j] = q[j];
+  }
+}
+
+template <int MT>
+__device__ __forceinline__ void store_channels(float (&v)[4 * MT], const float (&bv)[4 * MT],
+                                               bool has_bias, int relu,
+                                               const bf16_t* __restrict__ mask,
                                                bf16_t* __restrict__ y, float* __restrict__ y32,
                                                size_t o) {
 #pragma unroll
   for (int i = 0; i < 4 * MT; ++i) {
-    if (bias) v[i] += bias[cb + i];
+    if (has_bias) v[i] += bv[i];
     if (relu) v[i] = v[i] > 0.f ? v[i] : 0.f;
   }
   if (mask) {
@@ -150,6 +165,37 @@ __device__ __forceinline__ void store_channels(float (&v)[4 * MT], const float* 
 #pragma unroll
     for (int p = 0; p < MT; ++p)
       *reinterpret_cast<v4f*>(y32 + o + 4 * p) = v4f{v[4 * p], v[4 * p + 1], v[4 * p + 2], v[4 * p + 3]};
+  }
+}
+
+// Staging of input rows row0 - 2 .. row0 - 2 + SR - 1 (periodic in y) of sample
+// b, channels ci0 .. ci0 + 8*PIECES - 1, into lds[(r*W + pix)*PS + c] by the
+// block's 256 threads.  Every thread issues all of its global loads before its
+// first LDS write, so the block waits for one memory latency per staging
+// instead of one per 16-byte piece.
+template <int SR, int W, int PIECES, int PS>
+__device__ __forceinline__ void stage_rows(const bf16_t* __restrict__ x, bf16_t* lds, int b,
+                                           int H, int row0, int cinp, int ci0) {
+  constexpr int TOTAL = SR * W * PIECES;
+  constexpr int NQ = (TOTAL + 255) / 256;
+  uint4 v[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    const int q = threadIdx.x + 256 * i;
+    if (TOTAL % 256 == 0 || q < TOTAL) {
+      const int pc = q % PIECES, pix = (q / PIECES) % W, r = q / (PIECES * W);
+      const int yy = ((row0 + r - 2) % H + H) % H;
+      v[i] = *reinterpret_cast<const uint4*>(x + (((size_t)b * H + yy) * W + pix) * cinp + ci0 +
+                                             pc * 8);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) {
+    const int q = threadIdx.x + 256 * i;
+    if (TOTAL % 256 == 0 || q < TOTAL) {
+      const int pc = q % PIECES, pix = (q / PIECES) % W, r = q / (PIECES * W);
+      *reinterpret_cast<uint4*>(lds + (r * W + pix) * PS + pc * 8) = v[i];
+    }
   }
 }
 
@@ -185,13 +231,7 @@ __global__ __launch_bounds__(256, RW) void k_conv_fwd_mt(const bf16_t* __restric
       for (int t = 0; t < NT; ++t) acc[m][rr][t] = v4f{0.f, 0.f, 0.f, 0.f};
   for (int ci0 = 0; ci0 < cinp; ci0 += KC) {
     __syncthreads();
-    for (int q = threadIdx.x; q < SR * W * PIECES; q += 256) {
-      const int pc = q % PIECES, pix = (q / PIECES) % W, r = q / (PIECES * W);
-      const int yy = ((row0 + r - 2) % H + H) % H;
-      const uint4 v = *reinterpret_cast<const uint4*>(
-          x + (((size_t)b * H + yy) * W + pix) * cinp + ci0 + pc * 8);
-      *reinterpret_cast<uint4*>(lds + (r * W + pix) * PS + pc * 8) = v;
-    }
+    stage_rows<SR, W, PIECES, PS>(x, lds, b, H, row0, cinp, ci0);
     __syncthreads();
     if (wave_ok) {
       const bf16_t* wrow =
@@ -219,6 +259,8 @@ __global__ __launch_bounds__(256, RW) void k_conv_fwd_mt(const bf16_t* __restric
   }
   if (!wave_ok) return;
   const int cb = tile_channel<MT>(co0, 0, 4 * lg);
+  float bv[4 * MT];
+  load_bias<MT>(bias, cb, bv);
 #pragma unroll
   for (int rr = 0; rr < RW; ++rr) {
     const int row = wrow0 + rr;
@@ -231,7 +273,7 @@ __global__ __launch_bounds__(256, RW) void k_conv_fwd_mt(const bf16_t* __restric
       for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int j = 0; j < 4; ++j) v[4 * m + j] = acc[m][rr][t][j];
-      store_channels<MT>(v, bias, cb, relu, mask, y, y32, o);
+      store_channels<MT>(v, bv, bias != nullptr, relu, mask, y, y32, o);
     }
   }
 }
@@ -274,12 +316,7 @@ __global__ __launch_bounds__(256, RW) void k_conv_first(const bf16_t* __restrict
     for (int rr = 0; rr < RW; ++rr)
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[m][rr][t] = v4f{0.f, 0.f, 0.f, 0.f};
-  for (int q = threadIdx.x; q < SR * W; q += 256) {
-    const int pix = q % W, r = q / W;
-    const int yy = ((row0 + r - 2) % H + H) % H;
-    const uint4 v = *reinterpret_cast<const uint4*>(x + (((size_t)b * H + yy) * W + pix) * cinp);
-    *reinterpret_cast<uint4*>(lds + (r * W + pix) * CH) = v;
-  }
+  stage_rows<SR, W, 1, CH>(x, lds, b, H, row0, cinp, 0);
   __syncthreads();
   if (wave_ok) {
     const bf16_t* wrow = w8 + (size_t)tile_channel<MT>(co0, 0, li) * kFirstTaps * CH + lg * CH;
@@ -308,6 +345,8 @@ __global__ __launch_bounds__(256, RW) void k_conv_first(const bf16_t* __restrict
   }
   if (!wave_ok) return;
   const int cb = tile_channel<MT>(co0, 0, 4 * lg);
+  float bv[4 * MT];
+  load_bias<MT>(bias, cb, bv);
 #pragma unroll
   for (int rr = 0; rr < RW; ++rr) {
     const int row = wrow0 + rr;
@@ -320,7 +359,7 @@ __global__ __launch_bounds__(256, RW) void k_conv_first(const bf16_t* __restrict
       for (int m = 0; m < MT; ++m)
 #pragma unroll
         for (int j = 0; j < 4; ++j) v[4 * m + j] = acc[m][rr][t][j];
-      store_channels<MT>(v, bias, cb, relu, nullptr, y, y32, o);
+      store_channels<MT>(v, bv, bias != nullptr, relu, nullptr, y, y32, o);
     }
   }
 }
@@ -372,13 +411,7 @@ __global__ __launch_bounds__(256) void k_conv_last(const bf16_t* __restrict__ x,
     for (int t = 0; t < NT; ++t) acc[rr][t] = v4f{0.f, 0.f, 0.f, 0.f};
   for (int ci0 = 0; ci0 < cinp; ci0 += KC) {
     __syncthreads();
-    for (int q = threadIdx.x; q < SR * W * PIECES; q += 256) {
-      const int pc = q % PIECES, pix = (q / PIECES) % W, r = q / (PIECES * W);
-      const int yy = ((row0 + r - 2) % H + H) % H;
-      const uint4 v = *reinterpret_cast<const uint4*>(
-          x + (((size_t)b * H + yy) * W + pix) * cinp + ci0 + pc * 8);
-      *reinterpret_cast<uint4*>(lds + (r * W + pix) * PS + pc * 8) = v;
-    }
+    stage_rows<SR, W, PIECES, PS>(x, lds, b, H, row0, cinp, ci0);
     __syncthreads();
     if (wave_ok) {
 #pragma unroll
@@ -798,6 +831,15 @@ int grid1(size_t n) {
 
 bool mult16(int c) { return c > 0 && c % 16 == 0 && c <= 128; }
 
+// The convolution kernels read and write 16-byte vectors (inputs, weights,
+// bias, mask, outputs) at offsets that are multiples of 16 bytes from each base.
+bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+bool all_aligned16(std::initializer_list<const void*> ps) {
+  for (const void* p : ps)
+    if (!aligned16(p)) return false;
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -812,6 +854,8 @@ int qgcnn_conv_fwd(const void* x, int B, int H, int W, int cinp, const void* w,
     return fail(QGCNN_E_ARG, "channel counts must be multiples of 16 in [16, 128]");
   if (B < 1 || H < KS || W != 64)
     return fail(QGCNN_E_ARG, "images must be W=64 wide and at least 5 rows (got %dx%d)", H, W);
+  if (!all_aligned16({x, w, bias, mask, y, y32}))
+    return fail(QGCNN_E_ARG, "tensors must start on 16-byte boundaries");
   hipStream_t st = (hipStream_t)stream;
   const int mt = coutp % 64 == 0 ? 4 : (coutp % 32 == 0 ? 2 : 1);
   // two output rows per wave (every weight operand then feeds twice the MFMAs)
@@ -854,6 +898,8 @@ int qgcnn_conv_first(const void* x, int B, int H, int W, int cinp, const void* w
     return fail(QGCNN_E_ARG, "channel counts must be multiples of 16 in [16, 128]");
   if (B < 1 || H < KS || W != 64)
     return fail(QGCNN_E_ARG, "images must be W=64 wide and at least 5 rows (got %dx%d)", H, W);
+  if (!all_aligned16({x, w8, bias, y, y32}))
+    return fail(QGCNN_E_ARG, "tensors must start on 16-byte boundaries");
   hipStream_t st = (hipStream_t)stream;
   const int mt = coutp % 64 == 0 ? 4 : (coutp % 32 == 0 ? 2 : 1);
   const int rw = H % 8 == 0 ? 2 : 1;
