@@ -858,10 +858,10 @@ int qgcnn_conv_fwd(const void* x, int B, int H, int W, int cinp, const void* w,
     return fail(QGCNN_E_ARG, "tensors must start on 16-byte boundaries");
   hipStream_t st = (hipStream_t)stream;
   const int mt = coutp % 64 == 0 ? 4 : (coutp % 32 == 0 ? 2 : 1);
-  // two output rows per wave (every weight operand then feeds twice the MFMAs)
-  // where the weights dominate: 64 output channels per block or at least 64
-  // input channels; smaller layers keep more, shorter blocks
-  const int rw = H % 8 == 0 && (mt == 4 || cinp >= 64) ? 2 : 1;
+  // two output rows per wave wherever the height allows: every weight operand
+  // then feeds twice the MFMAs (measured 12 % faster at 32 -> 32 channels too,
+  // although the larger LDS tile halves the blocks per CU)
+  const int rw = H % 8 == 0 ? 2 : 1;
   dim3 grid((H + 4 * rw - 1) / (4 * rw), B, coutp / (16 * mt));
 #define QGCNN_FWD(MT, KC, RW)                                                                  \
   hipLaunchKernelGGL((k_conv_fwd_mt<64, MT, KC, RW>), grid, dim3(256), 0, st, (const bf16_t*)x, \
