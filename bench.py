@@ -399,6 +399,8 @@ def main():
                     help="cnn workload: skip the training-step timing (profiling runs)")
     ap.add_argument("--graph-steps", type=int, default=100,
                     help="steps timed again on the graph-replay path (0: skip)")
+    ap.add_argument("--unpacked", action="store_true",
+                    help="exchange nx/2 + 1 x-mixed columns per row (packed exchange off)")
     ap.add_argument("--copy-probe", type=int, default=1 << 30,
                     help="bytes moved by the copy kernel that calibrates HBM counters (0: skip)")
     args = ap.parse_args()
@@ -452,6 +454,9 @@ def main():
         nxs, nz = np.full(M, 256), 10
         workload = ("LayeredModel ensemble: 64 members, nz=10, nx=ny=256, L=2e6 m, exponential "
                     "stratification (H ~ exp(-z/800 m), 20 km first radius), dt=3600 s, FP64")
+    if args.unpacked:
+        for b in ens.buckets:
+            b.set_packed_exchange(False)
     ens.set_q([1e-7 * np.random.default_rng(s).standard_normal((nz, int(n), int(n)))
                for s, n in enumerate(nxs)])
     ens.step(3)                           # AB start-up, untimed
@@ -483,8 +488,10 @@ def main():
         # fused kernels would take if their transforms were free
         b0 = ens.buckets[0]
         ldk = ((b0.nx // 2 + 1) + 7) // 8 * 8
-        ms_r, ms_c = pyqg_amd._lib.pattern_probe(b0.ny, ldk, b0.M, nz)
-        pattern = {"row_ms": ms_r, "col_ms": ms_c,
+        ncols = b0.exchange_columns()
+        ncols = ncols if ncols % 8 == 0 else ldk  # unpacked rows are written whole
+        ms_r, ms_c = pyqg_amd._lib.pattern_probe(b0.ny, ldk, b0.M, nz, ncols=ncols)
+        pattern = {"row_ms": ms_r, "col_ms": ms_c, "columns_per_row": ncols,
                    "row_kernel_over_bound": per_bucket[0]["row_kernel_ms"] / ms_r,
                    "col_kernel_over_bound": per_bucket[0]["col_kernel_ms"] / ms_c}
 

@@ -212,6 +212,15 @@ int qgk_get_layer_split(qgk_bucket* b, int* on);
  * column kernels; it fixes the Stockham pass structure and the tile shapes. */
 int qgk_set_variant(qgk_bucket* b, int erow, int ecol);
 int qgk_get_variant(qgk_bucket* b, int* erow, int* ecol);
+/* Columns per row of the x-mixed fields the fused row and column kernels
+ * exchange (no counterpart in the reference): nx/2 when the register column
+ * kernels step the bucket (nz <= 2, layer split off; the real x bins 0 and
+ * nx/2 share one complex slot), nx/2 + 1 otherwise. */
+int qgk_exchange_columns(qgk_bucket* b, int* ncols);
+/* Packed exchange on (default) or off for this bucket (no counterpart in the
+ * reference): off, the register column kernels exchange nx/2 + 1 columns per
+ * row as the other column kernels do.  Results agree to rounding. */
+int qgk_set_packed_exchange(qgk_bucket* b, int on);
 /* Autotuner: times every radix for the row and the column kernel on this
  * bucket (reps launches each; the model state is saved and restored), selects
  * the fastest, and reports the per-candidate milliseconds (radix 4, 8, 16; -1
@@ -226,13 +235,14 @@ int qgk_device_name(char* buf, int len);
 int qgk_copy_probe(size_t nbytes, int iters, double* ms_per_copy);
 /* Measurement aid: memory-only stand-ins for the fused step's two kernels at
  * the shape of a bucket (members x nz planes of ny rows with row stride ldk
- * complex128): ms_rows times three fields read and two written as sequential
- * streams (the row kernel's traffic), ms_cols five fields read and five
- * written by (8-column tile, member) blocks over every row and layer (the
- * column kernel's traffic and access pattern).  Average ms over iters
- * launches; allocates and frees 9 fields. */
-int qgk_pattern_probe(int ny, int ldk, int members, int nz, int iters, double* ms_rows,
-                      double* ms_cols);
+ * complex128, of which the first ncols are touched, a multiple of 8):
+ * ms_rows times three fields read and two written as sequential streams (the
+ * row kernel's traffic), ms_cols five fields read and five written by
+ * (8-column tile, member) blocks over every row and layer (the column
+ * kernel's traffic and access pattern).  Average ms over iters launches;
+ * allocates and frees 9 fields. */
+int qgk_pattern_probe(int ny, int ldk, int ncols, int members, int nz, int iters,
+                      double* ms_rows, double* ms_cols);
 
 /* Multi-bucket stepping for uneven ensembles: nsteps fused steps of every
  * bucket, all launched on the first bucket's stream (one graph when

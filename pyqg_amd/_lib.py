@@ -30,7 +30,7 @@ EXPORTS = ["qgk_create", "qgk_destroy", "qgk_last_error", "qgk_set_const", "qgk_
            "qgk_advect", "qgk_friction", "qgk_add_q_param", "qgk_add_uv_param",
            "qgk_timestep", "qgk_q_to_qh", "qgk_qh_to_q", "qgk_fft", "qgk_ifft", "qgk_step",
            "qgk_timing_enable", "qgk_timing", "qgk_sync", "qgk_step_buckets",
-           "qgk_copy_probe", "qgk_pattern_probe", "qgk_capture_begin", "qgk_capture_end",
+           "qgk_copy_probe", "qgk_pattern_probe", "qgk_exchange_columns", "qgk_set_packed_exchange", "qgk_capture_begin", "qgk_capture_end",
            "qgk_graph_replay", "qgk_stream", "qgk_set_variant", "qgk_get_variant", "qgk_set_layer_split",
            "qgk_get_layer_split", "qgk_autotune",
            "qgk_device_name", "qgk_put_device", "qgk_get_device", "qgk_coarsen",
@@ -111,7 +111,9 @@ def _load():
         "qgk_sync": (I, [P]),
         "qgk_step_buckets": (I, [ctypes.POINTER(P), I, I, P, I]),
         "qgk_copy_probe": (I, [ctypes.c_size_t, I, DP]),
-        "qgk_pattern_probe": (I, [I, I, I, I, I, DP, DP]),
+        "qgk_pattern_probe": (I, [I, I, I, I, I, I, DP, DP]),
+        "qgk_exchange_columns": (I, [P, ctypes.POINTER(I)]),
+        "qgk_set_packed_exchange": (I, [P, I]),
         "qgk_capture_begin": (I, [P]),
         "qgk_capture_end": (I, [P, I]),
         "qgk_graph_replay": (I, [P, I]),
@@ -293,6 +295,16 @@ class Bucket:
         check(lib.qgk_get_variant(self.h, ctypes.byref(r), ctypes.byref(c)))
         return r.value, c.value
 
+    def set_packed_exchange(self, on):
+        check(lib.qgk_set_packed_exchange(self.h, int(bool(on))))
+
+    def exchange_columns(self):
+        """Columns per row of the x-mixed fields the fused kernels exchange:
+        nx/2 on the packed register path, nx/2 + 1 otherwise."""
+        n = ctypes.c_int()
+        check(lib.qgk_exchange_columns(self.h, ctypes.byref(n)))
+        return n.value
+
     # ---------------------------------------------------------- diagnostics
     def diag_slots(self, name):
         """Leading shape of one member's value of a device diagnostic."""
@@ -357,13 +369,13 @@ def copy_probe(nbytes, iters=20):
     return ms.value
 
 
-def pattern_probe(ny, ldk, members, nz, iters=10):
+def pattern_probe(ny, ldk, members, nz, iters=10, ncols=None):
     """(ms_rows, ms_cols): memory-only stand-ins for the fused step's row and
     column kernels at a bucket's shape (same fields, bytes and access
-    patterns, no transforms)."""
+    patterns, no transforms); ncols = columns touched per row (default ldk)."""
     r, c = ctypes.c_double(), ctypes.c_double()
-    check(lib.qgk_pattern_probe(int(ny), int(ldk), int(members), int(nz), int(iters),
-                                ctypes.byref(r), ctypes.byref(c)))
+    check(lib.qgk_pattern_probe(int(ny), int(ldk), int(ncols or ldk), int(members), int(nz),
+                                int(iters), ctypes.byref(r), ctypes.byref(c)))
     return r.value, c.value
 
 

@@ -192,10 +192,14 @@ __global__ __launch_bounds__(Geo<NY>::T * Geo<NY>::TK) void k_col_fft(
 
 // Hermitian extension of a half spectrum row h[0..N/2] at position p, with the
 // imaginary parts of bins 0 and N/2 discarded (numpy / pocketfft c2r semantics).
-template <int NX>
+// PK: the row is in the packed x-mixed form of the fused register path (slot 0
+// holds bin 0 in its real part and bin N/2 in its imaginary part; N/2 slots).
+template <int NX, bool PK = false>
 __device__ __forceinline__ cd hext(const cd* __restrict__ h, int p) {
   cd v;
-  if (p <= NX / 2) {
+  if (PK && p == NX / 2) {
+    v = {h[0].y, 0.0};
+  } else if (p <= NX / 2) {
     v = h[p];
     if (p == 0 || p == NX / 2) v.y = 0.0;
   } else {
@@ -205,8 +209,8 @@ __device__ __forceinline__ cd hext(const cd* __restrict__ h, int p) {
   return v;
 }
 
-// c2r of planes [ny][nk] -> [ny][nx], scaled by `scale`
-template <int NX>
+// c2r of planes [ny][nk] -> [ny][nx], scaled by `scale` (PK: packed x-mixed rows)
+template <int NX, bool PK = false>
 __global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_c2r(
     PlaneJobs jobs, int nplanes, int ny, double scale, const cd* __restrict__ tw) {
   using G = Geo<NX>;
@@ -226,7 +230,7 @@ __global__ __launch_bounds__(Geo<NX>::T * Geo<NX>::RP) void k_row_c2r(
 #pragma unroll
   for (int s = 0; s < G::E; ++s) {
     const int p = t + s * G::T;
-    cd a = hext<NX>(r0, p), b = hext<NX>(r1, p);
+    cd a = hext<NX, PK>(r0, p), b = hext<NX, PK>(r1, p);
     x[s] = {a.x - b.y, a.y + b.x};
   }
   qgk::fft_block_rows<NX, G::E, +1>(x, lds, t, rp, tw);
@@ -654,10 +658,30 @@ __device__ __forceinline__ const cd* stage_twiddles(cd* twl, const cd* __restric
   return twl;
 }
 
-template <int NX, int T, int RP>
+// Splits the transform Z of a row pair (in LDS) into the rows' half spectra
+// and stores them.  Unpacked: slots 0..NX/2 and zeros in the padding up to the
+// row stride, so whole 128-byte lines are written.  Packed (PK): slots
+// 0..NX/2-1, slot 0 holding the real bins 0 and NX/2 as one complex value.
+template <int NX, int T, int RP, bool PK = false>
 __device__ __forceinline__ void split_store_t(const cd* lds, int t, int rp, bool ok, cd* o0,
                                               cd* o1) {
   if (!ok) return;
+  if constexpr (PK) {
+    for (int k = t; k < NX / 2; k += T) {
+      cd A = lds[qgk::RowSwizzled<NX>::at(k, rp)];
+      if (k == 0) {
+        // Z(0) and Z(NX/2) are real for each row: re -> row j, im -> row j+1
+        const cd H = lds[qgk::RowSwizzled<NX>::at(NX / 2, rp)];
+        o0[0] = {A.x, H.x};
+        o1[0] = {A.y, H.y};
+      } else {
+        cd B = lds[qgk::RowSwizzled<NX>::at(NX - k, rp)];
+        o0[k] = {0.5 * (A.x + B.x), 0.5 * (A.y - B.y)};
+        o1[k] = {0.5 * (A.y + B.y), -0.5 * (A.x - B.x)};
+      }
+    }
+    return;
+  }
   constexpr int NK = ldk_of(NX);
   for (int k = t; k < NK; k += T) {
     if (k <= NX / 2) {
@@ -677,8 +701,9 @@ __device__ __forceinline__ void split_store_t(const cd* lds, int t, int rp, bool
 // conj(h_r[NX - p]) above, with the imaginary parts of bins 0 and NX/2
 // dropped (numpy's c2r).  h holds the thread's own slots p < NX/2 of both rows;
 // they are exchanged through LDS (rows at [0, NX/2) and [NX/2, NX) of each
-// lane) for the mirrored positions; the Nyquist bin is read by thread 0.
-template <int NX, int E, int T, int RP>
+// lane) for the mirrored positions; the Nyquist bin is read by thread 0 (PK:
+// it is the imaginary part of the packed slot 0 thread 0 already holds).
+template <int NX, int E, int T, int RP, bool PK = false>
 __device__ __forceinline__ void row_pair_extend(const cd (&h)[2][E / 2],
                                                 const cd* __restrict__ row, int NK,
                                                 cd (&z)[E], cd* lds, int t, int rp) {
@@ -702,10 +727,15 @@ __device__ __forceinline__ void row_pair_extend(const cd (&h)[2][E / 2],
         b.y = 0.0;
       }
     } else if (p == HALF) {
-      a = ld_stream(row + HALF);
-      b = ld_stream(row + NK + HALF);
-      a.y = 0.0;
-      b.y = 0.0;
+      if constexpr (PK) {
+        a = {h[0][0].y, 0.0};
+        b = {h[1][0].y, 0.0};
+      } else {
+        a = ld_stream(row + HALF);
+        b = ld_stream(row + NK + HALF);
+        a.y = 0.0;
+        b.y = 0.0;
+      }
     } else {
       // element NX - p = -t + (NX - s*T)
       a = lds[qgk::RowSwizzled<NX>::at(-t, NX - s * T, rp)];
@@ -720,6 +750,10 @@ __device__ __forceinline__ void row_pair_extend(const cd (&h)[2][E / 2],
 // Row phase: per (member, layer, row pair): x-c2r of q, u, v, flux, x-r2c.
 // In:  Qm, Um, Vm  [M][nz][ny][ldk] (y already inverse transformed)
 // Out: UQm, VQm    [M][nz][ny][ldk] (x already forward transformed)
+// PK selects the packed x-mixed rows of the register column kernels: NX/2
+// slots per row, bins 0 and NX/2 (real after an x transform of real data, and
+// only their real parts enter a c2r) sharing slot 0.  The row stride stays
+// ldk, so a row is NX/2 * 16 bytes of whole lines instead of ldk * 16.
 // Each complex transform carries the two rows of the pair (row j in the real
 // part, row j+1 in the imaginary part), so the six input rows are loaded at
 // once and the five transforms run back to back.
@@ -728,7 +762,7 @@ __device__ __forceinline__ void row_pair_extend(const cd (&h)[2][E / 2],
 template <int E>
 constexpr int row_flux_waves() { return E == 8 ? 3 : 1; }
 
-template <int NX, int E>
+template <int NX, int E, bool PK>
 __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP), row_flux_waves<E>()) void k_row_flux(
     const cd* __restrict__ Qm, const cd* __restrict__ Um, const cd* __restrict__ Vm,
     cd* __restrict__ UQm, cd* __restrict__ VQm, const MemberParams* __restrict__ params,
@@ -764,11 +798,11 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP), row_flux_waves<
     }
   }
   cd zq[G::E], zu[G::E], zv[G::E];
-  row_pair_extend<NX, G::E, G::T, G::RP>(hq, Qm + row0, NK, zq, lds, t, rp);
+  row_pair_extend<NX, G::E, G::T, G::RP, PK>(hq, Qm + row0, NK, zq, lds, t, rp);
   qgk::fft_block_rows<NX, G::E, +1, kTagRow>(zq, lds, t, rp, tw);
-  row_pair_extend<NX, G::E, G::T, G::RP>(hu, Um + row0, NK, zu, lds, t, rp);
+  row_pair_extend<NX, G::E, G::T, G::RP, PK>(hu, Um + row0, NK, zu, lds, t, rp);
   qgk::fft_block_rows<NX, G::E, +1, kTagRow>(zu, lds, t, rp, tw);
-  row_pair_extend<NX, G::E, G::T, G::RP>(hv, Vm + row0, NK, zv, lds, t, rp);
+  row_pair_extend<NX, G::E, G::T, G::RP, PK>(hv, Vm + row0, NK, zv, lds, t, rp);
   qgk::fft_block_rows<NX, G::E, +1, kTagRow>(zv, lds, t, rp, tw);
   // fluxes of both rows: uq = (u + Ubg) q, vq = v q (kernel.pyx:372-378)
 #pragma unroll
@@ -784,13 +818,13 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP), row_flux_waves<
 #pragma unroll
   for (int s = 0; s < G::E; ++s) lds[qgk::RowSwizzled<NX>::at(t, s * G::T, rp)] = zu[s];
   __syncthreads();
-  split_store_t<NX, G::T, G::RP>(lds, t, rp, ok, UQm + row0, UQm + row0 + NK);
+  split_store_t<NX, G::T, G::RP, PK>(lds, t, rp, ok, UQm + row0, UQm + row0 + NK);
   qgk::fft_block_rows<NX, G::E, -1, kTagRow>(zv, lds, t, rp, tw);
   __syncthreads();
 #pragma unroll
   for (int s = 0; s < G::E; ++s) lds[qgk::RowSwizzled<NX>::at(t, s * G::T, rp)] = zv[s];
   __syncthreads();
-  split_store_t<NX, G::T, G::RP>(lds, t, rp, ok, VQm + row0, VQm + row0 + NK);
+  split_store_t<NX, G::T, G::RP, PK>(lds, t, rp, ok, VQm + row0, VQm + row0 + NK);
 }
 
 // Health check of every member: the reference asserts CFL < 1 every twrite
@@ -1304,19 +1338,35 @@ __device__ __forceinline__ void invert2_both(const MemberParams& P, cd q0, cd q1
   ph1.y = ph1.y + a11 * q1.y;
 }
 
-// inversion + y IFFT of q, u, v for all layers from register-held qh
-template <int NY, int NZ, int E, int TAG = 0>
+// inversion + y IFFT of q, u, v for all layers from register-held qh.
+// PK (packed x-mixed rows): the lane of column 0 also carries the Nyquist
+// column nk-1, whose spectra of q, u and v (all layers) it finds in
+// nyq[(f * NZ + z) * NY + l] for f = 0, 1, 2 (every row, written before a
+// barrier): it transforms column 0 + i * column nk-1 as one complex sequence.
+// The c2r uses only the real parts of the y inverse transforms of these two
+// columns, i.e. the transforms of their Hermitian parts H(l) = (X(l) +
+// conj X(-l)) / 2, so those are what is packed.  Column 0 is Hermitian to
+// rounding except in the self-conjugate rows 0 and NY/2, where the tendency's
+// i l VQ term leaves an imaginary part (kept in the state, as the reference
+// keeps it, and dropped by its c2r): there its real part is packed.  The
+// Nyquist column's ik factor makes i k UQ and v anti-Hermitian in every row,
+// so its Hermitian part is formed from the LDS copy.  The transform of the
+// packed column is then real + i * real: the two x-mixed bins the row kernel
+// takes.
+template <int NY, int NZ, int E, int TAG = 0, bool PK = false>
 __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
                                                   cd* __restrict__ Qm, cd* __restrict__ Um,
                                                   cd* __restrict__ Vm, const MemberParams& P,
                                                   int nk, int ldk, int t, int c, int k, bool ok,
                                                   size_t mbase, cd* lds,
-                                                  const cd* __restrict__ tw) {
+                                                  const cd* __restrict__ tw,
+                                                  const cd* nyq = nullptr) {
 #pragma clang fp contract(off)
   using G = FGeo<NY, E>;
   const size_t plane = (size_t)NY * ldk;
   const int kc = ok ? k : 0;
   const double kk = gld(P.kk, kc);
+  const bool dual = PK && ok && k == 0;
 #pragma unroll
   for (int z = 0; z < NZ; ++z) {
     const size_t zoff = mbase + (size_t)z * plane;
@@ -1330,6 +1380,19 @@ __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
                  : cd{0.0, 0.0};
       x[s] = Q[z][s];
     }
+    if constexpr (PK) {
+      if (dual) {
+#pragma unroll
+        for (int s = 0; s < G::E; ++s) {
+          const int l = t + s * G::T;
+          const cd X = nyq[z * NY + l], Xm = nyq[z * NY + (l == 0 ? 0 : NY - l)];
+          const cd b = {0.5 * (X.x + Xm.x), 0.5 * (X.y - Xm.y)};  // Hermitian part
+          cd a = x[s];
+          if (l == 0 || 2 * l == NY) a.y = 0.0;  // self-conjugate rows: real parts
+          x[s] = {a.x - b.y, a.y + b.x};
+        }
+      }
+    }
     qgk::fft_block<NY, G::E, +1, G::TK, TAG>(x, lds, t, c, tw);
     {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
@@ -1340,6 +1403,19 @@ __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
       const double ll = gld(P.ll, t + s * G::T);
       x[s] = {ll * ph[s].y, -ll * ph[s].x};
     }
+    if constexpr (PK) {
+      if (dual) {
+#pragma unroll
+        for (int s = 0; s < G::E; ++s) {
+          const int l = t + s * G::T;
+          const cd X = nyq[(NZ + z) * NY + l], Xm = nyq[(NZ + z) * NY + (l == 0 ? 0 : NY - l)];
+          const cd b = {0.5 * (X.x + Xm.x), 0.5 * (X.y - Xm.y)};  // Hermitian part
+          cd a = x[s];
+          if (l == 0 || 2 * l == NY) a.y = 0.0;  // self-conjugate rows: real parts
+          x[s] = {a.x - b.y, a.y + b.x};
+        }
+      }
+    }
     qgk::fft_block<NY, G::E, +1, G::TK, TAG>(x, lds, t, c, tw);
     {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
@@ -1347,6 +1423,19 @@ __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
     }
 #pragma unroll
     for (int s = 0; s < G::E; ++s) x[s] = {-kk * ph[s].y, kk * ph[s].x};
+    if constexpr (PK) {
+      if (dual) {
+#pragma unroll
+        for (int s = 0; s < G::E; ++s) {
+          const int l = t + s * G::T;
+          const cd X = nyq[(2 * NZ + z) * NY + l], Xm = nyq[(2 * NZ + z) * NY + (l == 0 ? 0 : NY - l)];
+          const cd b = {0.5 * (X.x + Xm.x), 0.5 * (X.y - Xm.y)};  // Hermitian part
+          cd a = x[s];
+          if (l == 0 || 2 * l == NY) a.y = 0.0;  // self-conjugate rows: real parts
+          x[s] = {a.x - b.y, a.y + b.x};
+        }
+      }
+    }
     qgk::fft_block<NY, G::E, +1, G::TK, TAG>(x, lds, t, c, tw);
     {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
@@ -1493,18 +1582,79 @@ __global__ __launch_bounds__((2 * FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_
   for (int s = 0; s < EE; ++s) st_stream(Vm + zoff + (size_t)(t + s * T) * ldk + k, x[s]);
 }
 
-template <int NY, int NZ, int E>
+// Tendency and AB3 update at one spectral point (kernel.pyx:385-394, 444-461,
+// 496-517): S = ik UQ + il VQ from the y-transformed fluxes, dqhdt = -(S + ik Qy
+// ph) (+ rek kappa^2 ph in the bottom layer), qh_new = filtr (qh + dt1 dqhdt +
+// dt2 dqhdt_p + dt3 dqhdt_pp), in the reference's operation order.
+__device__ __forceinline__ cd tend_ab3_point(const MemberParams& P, cd UQ, cd VQ, cd ph, cd q,
+                                             double kk, double kq, double ll, bool drag,
+                                             size_t pt, const cd* __restrict__ p_pt,
+                                             const cd* __restrict__ pp_pt, StepCoeffs cf,
+                                             cd& d_out) {
+#pragma clang fp contract(off)
+  const cd S = {-kk * UQ.y + -ll * VQ.y, kk * UQ.x + ll * VQ.x};
+  cd d = {S.x + -kq * ph.y, S.y + kq * ph.x};
+  d = {-d.x, -d.y};
+  if (drag) {
+    const double cr = P.rek * gld(P.k2l2, pt);
+    d = {d.x + cr * ph.x, d.y + cr * ph.y};
+  }
+  const double f = gld(P.filtr, pt);
+  const cd pv = ld_stream(p_pt), ppv = ld_stream(pp_pt);
+  double rx = q.x + cf.dt1 * d.x;
+  double ry = q.y + cf.dt1 * d.y;
+  rx = rx + cf.dt2 * pv.x;
+  ry = ry + cf.dt2 * pv.y;
+  rx = rx + cf.dt3 * ppv.x;
+  ry = ry + cf.dt3 * ppv.y;
+  d_out = d;
+  return {f * rx, f * ry};
+}
+
+// Columns handled by a launch of the register column kernels: nk, or nk - 1
+// with packed x-mixed rows (the Nyquist column rides on column 0).
+__host__ __device__ constexpr int col_count(int nk, bool pk) { return pk ? nk - 1 : nk; }
+
+// The Nyquist column (nk-1) after the update of every layer: its new qh
+// (nyq[0][z][l], written by the step or loaded by the prologue kernel) gives
+// ph and so the y-spectra of u and v (nyq[1][z][l], nyq[2][z][l]), which the
+// packed prologue adds to column 0's as imaginary parts.  All threads of the
+// block take rows l = threadIdx.x, threadIdx.x + blockDim.x, ... (the same
+// rows as the step's Nyquist update, so a thread reads only what it wrote).
+template <int NY, int NZ>
+__device__ __forceinline__ void nyquist_uv(const MemberParams& P, cd* nyq, int nk, int ldk) {
+#pragma clang fp contract(off)
+  const size_t plane = (size_t)NY * ldk;
+  const double kkN = gld(P.kk, nk - 1);
+  for (int l = threadIdx.x; l < NY; l += blockDim.x) {
+    const size_t ptN = (size_t)l * ldk + (nk - 1);
+    const double ll = gld(P.ll, l);
+    cd qs[NZ];
+#pragma unroll
+    for (int k1 = 0; k1 < NZ; ++k1) qs[k1] = nyq[k1 * NY + l];
+#pragma unroll
+    for (int z = 0; z < NZ; ++z) {
+      const cd ph = invert_regs<NZ>(P, z, qs, plane, ptN);
+      nyq[(NZ + z) * NY + l] = {ll * ph.y, -ll * ph.x};
+      nyq[(2 * NZ + z) * NY + l] = {-kkN * ph.y, kkN * ph.x};
+    }
+  }
+}
+
+template <int NY, int NZ, int E, bool PK>
 __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_prologue_reg(
     const cd* __restrict__ qh, cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
     const MemberParams* __restrict__ params, int nk, int ldk, const cd* __restrict__ tw) {
 #pragma clang fp contract(off)
   using G = FGeo<NY, E>;
   __shared__ cd lds[NY * G::TK];
+  __shared__ cd nyq[PK ? 3 * NZ * NY : 1];
   const int c = threadIdx.x % G::TK;
   const int t = threadIdx.x / G::TK;
-  const int k = blockIdx.x * G::TK + c;
   const int m = blockIdx.y;
-  const bool ok = k < nk;
+  const int tile = PK ? (blockIdx.x + m) % gridDim.x : blockIdx.x;  // as k_col_step_reg
+  const int k = tile * G::TK + c;
+  const bool ok = k < col_count(nk, PK);
   const int kc = ok ? k : 0;
   const MemberParams& P = params[m];
   const size_t plane = (size_t)NY * ldk;
@@ -1516,7 +1666,18 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_prol
     for (int s = 0; s < G::E; ++s)
       Q[z][s] = ok ? ld_stream(qh + mbase + z * plane + (size_t)(t + s * G::T) * ldk + kc)
                    : cd{0.0, 0.0};
-  col_prologue_regs<NY, NZ, G::E>(Q, Qm, Um, Vm, P, nk, ldk, t, c, k, ok, mbase, lds, tw);
+  if constexpr (PK) {
+    if (tile == 0) {  // the Nyquist column, for the lane that packs it
+      for (int l = threadIdx.x; l < NY; l += blockDim.x)
+#pragma unroll
+        for (int z = 0; z < NZ; ++z)
+          nyq[z * NY + l] = ld_stream(qh + mbase + z * plane + (size_t)l * ldk + (nk - 1));
+      nyquist_uv<NY, NZ>(P, nyq, nk, ldk);
+      __syncthreads();  // every row of nyq is written
+    }
+  }
+  col_prologue_regs<NY, NZ, G::E, 0, PK>(Q, Qm, Um, Vm, P, nk, ldk, t, c, k, ok, mbase, lds, tw,
+                                          nyq);
 }
 
 // Waves per SIMD the register column kernel is compiled for: with two layers
@@ -1525,7 +1686,17 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_prol
 template <int NZ, int E>
 constexpr int col_reg_waves() { return NZ == 2 && E == 8 ? 2 : 1; }
 
-template <int NY, int NZ, int E>
+// PK: packed x-mixed rows (see k_row_flux).  The launch covers columns
+// 0..nk-2; the lane of column 0 transforms the packed slot, column 0 +
+// i * column nk-1, and separates the two spectra by their symmetry:
+// F = A + iB with A, B Hermitian in l gives A(l) = (F(l) + conj F(-l)) / 2,
+// B(l) = (F(l) - conj F(-l)) / 2i.  F is exchanged through LDS columns 0-1 of
+// the transform buffer, A and B go to columns 2-5.  After the block's own
+// columns, every thread of block 0 steps rows of the Nyquist column (B from
+// LDS; its qh, dqhdt_p and dqhdt_pp from memory) and keeps the new qh and the
+// derived u, v spectra in LDS for the packed prologue.  The next transform's
+// first LDS write follows a barrier, so columns 2-5 live until then.
+template <int NY, int NZ, int E, bool PK>
 __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK), (col_reg_waves<NZ, E>())) void k_col_step_reg(
     const cd* __restrict__ qh_old, cd* __restrict__ qh_new, const cd* __restrict__ p,
     cd* __restrict__ pp_to_newp, const cd* __restrict__ UQm, const cd* __restrict__ VQm,
@@ -1534,19 +1705,27 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK), (col_reg_waves<
     const cd* __restrict__ tw) {
 #pragma clang fp contract(off)
   using G = FGeo<NY, E>;
+  static_assert(!PK || G::TK >= 6, "the packed exchange uses six LDS columns");
   __shared__ cd lds[NY * G::TK];
   __shared__ cd twl[NY];
+  __shared__ cd nyq[PK ? 3 * NZ * NY : 1];
   tw = stage_twiddles<NY>(twl, tw);
   const int c = threadIdx.x % G::TK;
   const int t = threadIdx.x / G::TK;
-  const int k = blockIdx.x * G::TK + c;
   const int m = blockIdx.y;
-  const bool ok = k < nk;
+  // packed: tiles rotated by member, so the tiles carrying the Nyquist column
+  // (tile 0) spread over the XCDs (blocks go to XCDs round robin, and with 16
+  // tiles per member an unrotated tile 0 would always land on the same one)
+  const int tile = PK ? (blockIdx.x + m) % gridDim.x : blockIdx.x;
+  const int k = tile * G::TK + c;
+  const bool ok = k < col_count(nk, PK);
   const int kc = ok ? k : 0;
   const MemberParams& P = params[m];
   const size_t plane = (size_t)NY * ldk;
   const size_t mbase = (size_t)m * NZ * plane;
   const double kk = gld(P.kk, kc);
+  const bool dual = PK && ok && k == 0;
+  const int kN = nk - 1;
 
   cd Q[NZ][G::E];
   // two layers: layer 1's ph from the old qh of both layers, formed together
@@ -1571,6 +1750,37 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK), (col_reg_waves<
     }
     qgk::fft_block<NY, G::E, -1, G::TK, kTagColReg>(xa, lds, t, c, tw);
     qgk::fft_block<NY, G::E, -1, G::TK, kTagColReg>(xb, lds, t, c, tw);
+    if constexpr (PK) {
+      if (tile == 0) {  // block-uniform: the barriers below are safe
+        __syncthreads();      // the transforms' last LDS reads are done
+        if (dual) {
+#pragma unroll
+          for (int s = 0; s < G::E; ++s) {
+            lds[t + s * G::T] = xa[s];
+            lds[NY + t + s * G::T] = xb[s];
+          }
+        }
+        __syncthreads();
+        if (dual) {
+#pragma nounroll
+          for (int s = 0; s < G::E; ++s) {
+            const int l = t + s * G::T;
+            const int lm = l == 0 ? 0 : NY - l;  // -l mod NY
+#pragma unroll
+            for (int f = 0; f < 2; ++f) {
+              const cd F = lds[f * NY + l], Fm = lds[f * NY + lm];
+              lds[(2 + f) * NY + l] = {0.5 * (F.x + Fm.x), 0.5 * (F.y - Fm.y)};   // A
+              lds[(4 + f) * NY + l] = {0.5 * (F.y + Fm.y), -0.5 * (F.x - Fm.x)};  // B
+            }
+          }
+#pragma unroll
+          for (int s = 0; s < G::E; ++s) {
+            xa[s] = lds[2 * NY + t + s * G::T];
+            xb[s] = lds[3 * NY + t + s * G::T];
+          }
+        }
+      }
+    }
     if (ok) {
       const double kq = kk * gld(P.Qy, z);
       const bool drag = (z == NZ - 1) && P.rek != 0.0;
@@ -1580,7 +1790,6 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK), (col_reg_waves<
         const size_t pt = (size_t)l * ldk + k;
         const size_t o = zoff + pt;
         const double ll = gld(P.ll, l);
-        const cd S = {-kk * xa[s].y + -ll * xb[s].y, kk * xa[s].x + ll * xb[s].x};
         // ph from the old qh of every layer
         cd ph;
         if constexpr (NZ == 2) {
@@ -1591,27 +1800,16 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK), (col_reg_waves<
         } else {
           ph = invert_regs<NZ>(P, z, Q[0] + s, plane, pt);
         }
-        cd d = {S.x + -kq * ph.y, S.y + kq * ph.x};
-        d = {-d.x, -d.y};
-        if (drag) {
-          const double cr = P.rek * gld(P.k2l2, pt);
-          d = {d.x + cr * ph.x, d.y + cr * ph.y};
-        }
-        const double f = gld(P.filtr, pt);
-        const cd q = Q[z][s], pv = ld_stream(p + o), ppv = ld_stream(pp_to_newp + o);
-        double rx = q.x + cf.dt1 * d.x;
-        double ry = q.y + cf.dt1 * d.y;
-        rx = rx + cf.dt2 * pv.x;
-        ry = ry + cf.dt2 * pv.y;
-        rx = rx + cf.dt3 * ppv.x;
-        ry = ry + cf.dt3 * ppv.y;
-        const cd qn = {f * rx, f * ry};
+        cd d;
+        const cd qn = tend_ab3_point(P, xa[s], xb[s], ph, Q[z][s], kk, kq, ll, drag, pt, p + o,
+                                     pp_to_newp + o, cf, d);
         st_stream(qh_new + o, qn);
         st_stream(pp_to_newp + o, d);
         Q[z][s] = qn;
       }
-    } else {
+    } else if (!PK || k != kN) {
       // lanes past nk write zeros so every tile row is a full 128-byte line
+      // (not over the Nyquist column, which the packed lane of column 0 steps)
 #pragma unroll
       for (int s = 0; s < G::E; ++s) {
         const size_t o = zoff + (size_t)(t + s * G::T) * ldk + k;
@@ -1619,8 +1817,37 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK), (col_reg_waves<
         st_stream(pp_to_newp + o, cd{0.0, 0.0});
       }
     }
+    if constexpr (PK) {
+      if (tile == 0) {  // the Nyquist column's tendency and AB3 update
+        __syncthreads();      // B of every row is in LDS
+        const double kkN = gld(P.kk, kN);
+        const double kqN = kkN * gld(P.Qy, z);
+        const bool drag = (z == NZ - 1) && P.rek != 0.0;
+        for (int l = threadIdx.x; l < NY; l += blockDim.x) {
+          const size_t ptN = (size_t)l * ldk + kN;
+          cd qs[NZ];
+#pragma unroll
+          for (int k1 = 0; k1 < NZ; ++k1) qs[k1] = ld_stream(qh_old + mbase + k1 * plane + ptN);
+          const cd phN = invert_regs<NZ>(P, z, qs, plane, ptN);
+          cd dN;
+          const cd qnN = tend_ab3_point(P, lds[4 * NY + l], lds[5 * NY + l], phN, qs[z], kkN, kqN,
+                                        gld(P.ll, l), drag, ptN, p + zoff + ptN,
+                                        pp_to_newp + zoff + ptN, cf, dN);
+          st_stream(qh_new + zoff + ptN, qnN);
+          st_stream(pp_to_newp + zoff + ptN, dN);
+          nyq[z * NY + l] = qnN;
+        }
+      }
+    }
   }
-  col_prologue_regs<NY, NZ, G::E, kTagColReg>(Q, Qm, Um, Vm, P, nk, ldk, t, c, k, ok, mbase, lds, tw);
+  if constexpr (PK) {
+    if (tile == 0) {
+      nyquist_uv<NY, NZ>(P, nyq, nk, ldk);
+      __syncthreads();  // every row of nyq is written
+    }
+  }
+  col_prologue_regs<NY, NZ, G::E, kTagColReg, PK>(Q, Qm, Um, Vm, P, nk, ldk, t, c, k, ok, mbase,
+                                                   lds, tw, nyq);
 }
 
 // ==========================================================================
@@ -2257,12 +2484,15 @@ __global__ void k_stream_copy(const double2* __restrict__ in, double2* __restric
 // access patterns, cache hints and bytes, no transforms.  Their times bound what
 // the fused kernels can reach on this memory system.
 //
-// Row phase: three fields read and two written, each a sequential stream.
+// Row phase: three fields read and two written, each a sequential stream over
+// the first ncols values of every row of stride ldk.
 __global__ void k_probe_rows(const cd* __restrict__ a, const cd* __restrict__ b,
                              const cd* __restrict__ c, cd* __restrict__ d, cd* __restrict__ e,
-                             size_t n) {
-  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i < n) {
+                             size_t nrows, int ncols, int ldk) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t row = g / (size_t)ncols;
+  if (row < nrows) {
+    const size_t i = row * ldk + (g - row * ncols);
     const cd x = ld_stream(a + i), y = ld_stream(b + i), z = ld_stream(c + i);
     st_stream(d + i, {x.x + y.x, x.y + y.y});
     st_stream(e + i, {z.x + x.x, z.y + x.y});
@@ -2377,6 +2607,7 @@ struct qgk_bucket {
   // register radix of the fused row and column kernels (autotuned)
   int erow = 8, ecol = 8;
   bool csplit = false;  // nz = 2: column steps by k_col_step_split (one member)
+  bool pk_off = false;  // packed x-mixed exchange switched off (qgk_set_packed_exchange)
   unsigned long long* speed = nullptr;  // [M] member health words (qgk_member_speed)
   double* small = nullptr;  // 256 doubles: learned-closure scales (input [0,128), output [128,256))
   std::vector<double> small_in_h, small_out_h;  // values last uploaded there
@@ -2567,15 +2798,20 @@ int launch_col_fft_n(qgk_bucket* b, const PlaneJobs& j, int nf, int nplanes) {
   return 0;
 }
 
-int launch_row_c2r_n(qgk_bucket* b, const PlaneJobs& j, int nf, int nplanes) {
+// pk: the input rows are the fused register path's packed x-mixed rows
+int launch_row_c2r_n(qgk_bucket* b, const PlaneJobs& j, int nf, int nplanes, bool pk = false) {
   const int nx = b->nx, ny = b->ny;
   const double scale = 1.0 / ((double)nx * (double)ny);
 #define QGK_ROW(NX)                                                                       \
   case NX: {                                                                              \
     using G = Geo<NX>;                                                                    \
     dim3 grid((ny / 2 + G::RP - 1) / G::RP, nf * nplanes);                                \
-    hipLaunchKernelGGL((k_row_c2r<NX>), grid, dim3(G::T * G::RP), 0, b->stream, j, nplanes, \
-                       ny, scale, b->tw_x);                                               \
+    if (pk)                                                                               \
+      hipLaunchKernelGGL((k_row_c2r<NX, true>), grid, dim3(G::T * G::RP), 0, b->stream, j, \
+                         nplanes, ny, scale, b->tw_x);                                    \
+    else                                                                                  \
+      hipLaunchKernelGGL((k_row_c2r<NX>), grid, dim3(G::T * G::RP), 0, b->stream, j,      \
+                         nplanes, ny, scale, b->tw_x);                                    \
     break;                                                                                \
   }
   switch (nx) {
@@ -2620,8 +2856,8 @@ int launch_col_fft(qgk_bucket* b, const cd* in, cd* out, int nplanes) {
   return launch_col_fft_n<SIGN>(b, one_job(in, out), 1, nplanes);
 }
 
-int launch_row_c2r(qgk_bucket* b, const cd* in, double* out, int nplanes) {
-  return launch_row_c2r_n(b, one_job(in, out), 1, nplanes);
+int launch_row_c2r(qgk_bucket* b, const cd* in, double* out, int nplanes, bool pk = false) {
+  return launch_row_c2r_n(b, one_job(in, out), 1, nplanes, pk);
 }
 
 int launch_row_r2c(qgk_bucket* b, const double* in, cd* out, int nplanes) {
@@ -2795,17 +3031,90 @@ int with_ne(int n, int e, A... a) {
 #undef QGK_NE
 }
 
+// The packed register column kernel: six LDS columns of the transform buffer
+// for its exchange, and its LDS (transform buffer, twiddles and the Nyquist
+// column's q, u, v spectra of every layer) within the CU's 160 KB
+template <int NY, int NZ, int E>
+constexpr bool pk_fits() {
+  using G = FGeo<NY, E>;
+  return G::TK >= 6 && (NY * G::TK + NY + 3 * NZ * NY) * (int)sizeof(cd) <= 160 * 1024;
+}
+
+struct PackedL {
+  template <int NY, int E>
+  static void go(const qgk_bucket* b, bool* pk) {
+    const bool split = b->nz == 2 && b->csplit && split_fits<NY, E>();
+    *pk = (b->nz == 1 && pk_fits<NY, 1, E>()) || (b->nz == 2 && !split && pk_fits<NY, 2, E>());
+  }
+};
+
+// The register column kernels (nz <= 2, unless the layer-split kernel steps
+// the bucket) and the row kernel exchange packed x-mixed rows: NX/2 slots of
+// whole lines per row, bins 0 and nx/2 sharing slot 0 (k_row_flux).
+bool packed_exchange(const qgk_bucket* b) {
+  if (b->nz > 2 || b->pk_off) return false;
+  bool pk = false;
+  if (with_ne<PackedL>(b->ny, b->ecol, b, &pk)) return false;
+  return pk;
+}
+
 struct RowFluxL {
   template <int NX, int E>
   static void go(qgk_bucket* b) {
     using G = FGeo<NX, E>;
     const double scale = 1.0 / ((double)b->nx * (double)b->ny);
     dim3 grid((b->ny / 2 + G::RP - 1) / G::RP, b->M * b->nz);
-    hipLaunchKernelGGL((k_row_flux<NX, E>), grid, dim3(G::T * G::RP), 0,
-                       b->lst ? b->lst : b->stream, b->Qm,
-                       b->Um, b->Vm, b->UQm, b->VQm, b->dp, b->nz, b->ny, scale, b->tw_x);
+    if (packed_exchange(b))
+      hipLaunchKernelGGL((k_row_flux<NX, E, true>), grid, dim3(G::T * G::RP), 0,
+                         b->lst ? b->lst : b->stream, b->Qm, b->Um, b->Vm, b->UQm, b->VQm, b->dp,
+                         b->nz, b->ny, scale, b->tw_x);
+    else
+      hipLaunchKernelGGL((k_row_flux<NX, E, false>), grid, dim3(G::T * G::RP), 0,
+                         b->lst ? b->lst : b->stream, b->Qm, b->Um, b->Vm, b->UQm, b->VQm, b->dp,
+                         b->nz, b->ny, scale, b->tw_x);
   }
 };
+
+// k_col_prologue_reg / k_col_step_reg of NZ layers, packed when pk (and the
+// packed kernel fits)
+template <int NY, int NZ, int E>
+void launch_prologue_reg(qgk_bucket* b, bool pk) {
+  using G = FGeo<NY, E>;
+  const dim3 block(G::T * G::TK);
+  const cd* qh = b->field_c[QGK_QH];
+  if constexpr (pk_fits<NY, NZ, E>()) {
+    if (pk) {
+      dim3 grid((col_count(b->nk, true) + G::TK - 1) / G::TK, b->M);
+      hipLaunchKernelGGL((k_col_prologue_reg<NY, NZ, E, true>), grid, block, 0, b->stream, qh,
+                         b->Qm, b->Um, b->Vm, b->dp, b->nk, b->ldk, b->tw_y);
+      return;
+    }
+  }
+  dim3 grid((b->nk + G::TK - 1) / G::TK, b->M);
+  hipLaunchKernelGGL((k_col_prologue_reg<NY, NZ, E, false>), grid, block, 0, b->stream, qh,
+                     b->Qm, b->Um, b->Vm, b->dp, b->nk, b->ldk, b->tw_y);
+}
+
+template <int NY, int NZ, int E>
+void launch_step_reg(qgk_bucket* b, bool pk, StepCoeffs cf) {
+  using G = FGeo<NY, E>;
+  const dim3 block(G::T * G::TK);
+  hipStream_t st = b->lst ? b->lst : b->stream;
+  const cd* qo = b->field_c[QGK_QH];
+  cd* qn = b->qh_alt;
+  if constexpr (pk_fits<NY, NZ, E>()) {
+    if (pk) {
+      dim3 grid((col_count(b->nk, true) + G::TK - 1) / G::TK, b->M);
+      hipLaunchKernelGGL((k_col_step_reg<NY, NZ, E, true>), grid, block, 0, st, qo, qn, Pp(b),
+                         PP(b), b->UQm, b->VQm, b->Qm, b->Um, b->Vm, b->dp, cf, b->nk, b->ldk,
+                         b->tw_y);
+      return;
+    }
+  }
+  dim3 grid((b->nk + G::TK - 1) / G::TK, b->M);
+  hipLaunchKernelGGL((k_col_step_reg<NY, NZ, E, false>), grid, block, 0, st, qo, qn, Pp(b), PP(b),
+                     b->UQm, b->VQm, b->Qm, b->Um, b->Vm, b->dp, cf, b->nk, b->ldk, b->tw_y);
+}
 
 struct ColPrologueL {
   template <int NY, int E>
@@ -2813,12 +3122,11 @@ struct ColPrologueL {
     using G = FGeo<NY, E>;
     dim3 grid((b->nk + G::TK - 1) / G::TK, b->M), block(G::T * G::TK);
     const cd* qh = b->field_c[QGK_QH];
+    const bool pk = packed_exchange(b);
     if (b->nz == 1)
-      hipLaunchKernelGGL((k_col_prologue_reg<NY, 1, E>), grid, block, 0, b->stream, qh, b->Qm,
-                         b->Um, b->Vm, b->dp, b->nk, b->ldk, b->tw_y);
+      launch_prologue_reg<NY, 1, E>(b, pk);
     else if (b->nz == 2)
-      hipLaunchKernelGGL((k_col_prologue_reg<NY, 2, E>), grid, block, 0, b->stream, qh, b->Qm,
-                         b->Um, b->Vm, b->dp, b->nk, b->ldk, b->tw_y);
+      launch_prologue_reg<NY, 2, E>(b, pk);
     else if (b->phm)
       hipLaunchKernelGGL((k_col_prologue_ph<NY, E>), grid, block, 0, b->stream, qh, b->phm,
                          b->Qm, b->Um, b->Vm, b->dp, b->nz, b->nk, b->ldk, b->tw_y);
@@ -2835,19 +3143,16 @@ struct ColStepL {
     dim3 grid((b->nk + G::TK - 1) / G::TK, b->M), block(G::T * G::TK);
     const cd* qo = b->field_c[QGK_QH];
     cd* qn = b->qh_alt;
+    const bool pk = packed_exchange(b);
     if (b->nz == 1)
-      hipLaunchKernelGGL((k_col_step_reg<NY, 1, E>), grid, block, 0, b->lst ? b->lst : b->stream, qo, qn, Pp(b),
-                         PP(b), b->UQm, b->VQm, b->Qm, b->Um, b->Vm, b->dp, cf, b->nk, b->ldk,
-                         b->tw_y);
+      launch_step_reg<NY, 1, E>(b, pk, cf);
     else if (b->nz == 2 && b->csplit && split_fits<NY, E>()) {
       if constexpr (split_fits<NY, E>())
         hipLaunchKernelGGL((k_col_step_split<NY, E>), grid, dim3(2 * G::T * G::TK), 0,
                            b->lst ? b->lst : b->stream, qo, qn, Pp(b), PP(b), b->UQm, b->VQm,
                            b->Qm, b->Um, b->Vm, b->dp, cf, b->nk, b->ldk, b->tw_y);
     } else if (b->nz == 2)
-      hipLaunchKernelGGL((k_col_step_reg<NY, 2, E>), grid, block, 0, b->lst ? b->lst : b->stream, qo, qn, Pp(b),
-                         PP(b), b->UQm, b->VQm, b->Qm, b->Um, b->Vm, b->dp, cf, b->nk, b->ldk,
-                         b->tw_y);
+      launch_step_reg<NY, 2, E>(b, pk, cf);
     else if (b->phm)
       hipLaunchKernelGGL((k_col_step_ph<NY, E>), grid, block, 0, b->lst ? b->lst : b->stream, qo,
                          qn, Pp(b), PP(b), b->UQm, b->VQm, b->phm, b->Qm, b->Um, b->Vm, b->dp, cf,
@@ -2895,8 +3200,9 @@ int launch_member_speed(qgk_bucket* b, unsigned long long* w) {
   const int np = b->M * b->nz;
   const int parts = (int)std::min<size_t>(8, (b->nplane_r + 2047) / 2048);
   const cd* src[3] = {b->Um, b->Vm, b->Qm};
+  const bool pk = packed_exchange(b);
   for (int f = 0; f < 3; ++f) {
-    if ((rc = launch_row_c2r(b, src[f], b->rtmp, np))) return rc;
+    if ((rc = launch_row_c2r(b, src[f], b->rtmp, np, pk))) return rc;
     hipLaunchKernelGGL(k_speed_fold, dim3(parts, np), dim3(256), 0, b->stream, b->rtmp,
                        (int)b->nplane_r, b->nz, b->dp, f, w);
   }
@@ -3748,12 +4054,14 @@ int qgk_copy_probe(size_t nbytes, int iters, double* ms_per_copy) {
   return rc;
 }
 
-int qgk_pattern_probe(int ny, int ldk, int members, int nz, int iters, double* ms_rows,
-                      double* ms_cols) {
+int qgk_pattern_probe(int ny, int ldk, int ncols, int members, int nz, int iters,
+                      double* ms_rows, double* ms_cols) {
   if (ny < 32 || ny % 32 || ldk < 8 || ldk % 8 || members < 1 || nz < 1 || iters < 1 ||
       !ms_rows || !ms_cols)
     return fail(QGK_E_ARG, "pattern probe: bad arguments");
+  if (ncols < 8 || ncols > ldk || ncols % 8) return fail(QGK_E_ARG, "pattern probe: bad ncols");
   const size_t n = (size_t)members * nz * ny * ldk;
+  const size_t nrows = (size_t)members * nz * ny;
   if ((n + 255) / 256 > 0x7fffffffu || members > 65535)
     return fail(QGK_E_ARG, "pattern probe: size too large");
   cd* f[9] = {};
@@ -3780,11 +4088,11 @@ int qgk_pattern_probe(int ny, int ldk, int members, int nz, int iters, double* m
       *out = ms / iters;
   };
   timed([&] {
-    hipLaunchKernelGGL(k_probe_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, f[0],
-                       f[1], f[2], f[3], f[4], n);
+    hipLaunchKernelGGL(k_probe_rows, dim3((unsigned)((nrows * ncols + 255) / 256)), dim3(256),
+                       0, st, f[0], f[1], f[2], f[3], f[4], nrows, ncols, ldk);
   }, ms_rows);
   timed([&] {
-    hipLaunchKernelGGL(k_probe_cols, dim3(ldk / 8, members), dim3(256), 0, st, f[0], f[1], f[2],
+    hipLaunchKernelGGL(k_probe_cols, dim3(ncols / 8, members), dim3(256), 0, st, f[0], f[1], f[2],
                        f[3], f[4], f[5], f[6], f[7], f[8], ny, ldk, nz);
   }, ms_cols);
   if (e0) hipEventDestroy(e0);
@@ -3802,6 +4110,7 @@ int qgk_set_variant(qgk_bucket* b, int erow, int ecol) {
   if (!okr(erow) || !okr(ecol)) return fail(QGK_E_ARG, "radix must be 4, 8 or 16");
   b->erow = erow;
   b->ecol = ecol;
+  b->primed = false;  // the x-mixed exchange format may change with the column kernel
   b->gvalid = false;
   b->gen++;
   return 0;
@@ -3812,6 +4121,7 @@ int qgk_set_layer_split(qgk_bucket* b, int on) {
   if (int rc0 = refuse_in_capture(b, "qgk_set_layer_split")) return rc0;
   if (on && b->nz != 2) return fail(QGK_E_ARG, "the layer-split column kernel needs nz == 2");
   b->csplit = on != 0;
+  b->primed = false;  // the x-mixed exchange format may change with the column kernel
   b->gvalid = false;
   b->gen++;
   return 0;
@@ -3820,6 +4130,22 @@ int qgk_set_layer_split(qgk_bucket* b, int on) {
 int qgk_get_layer_split(qgk_bucket* b, int* on) {
   if (!b || !on) return fail(QGK_E_ARG, "NULL argument");
   *on = b->csplit ? 1 : 0;
+  return 0;
+}
+
+int qgk_set_packed_exchange(qgk_bucket* b, int on) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "qgk_set_packed_exchange")) return rc0;
+  b->pk_off = on == 0;
+  b->primed = false;  // the x-mixed exchange format changes
+  b->gvalid = false;
+  b->gen++;
+  return 0;
+}
+
+int qgk_exchange_columns(qgk_bucket* b, int* ncols) {
+  if (!b || !ncols) return fail(QGK_E_ARG, "NULL argument");
+  *ncols = packed_exchange(b) ? b->nk - 1 : b->nk;
   return 0;
 }
 

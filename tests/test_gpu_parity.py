@@ -248,6 +248,66 @@ def test_fused_radix_variants(pq, erow, ecol):
     assert rel_l2(lm.q, ol.q) < TOL_TRAJ
 
 
+@pytest.mark.parametrize("n,erow,ecol,split", [(64, 8, 8, False), (64, 4, 4, True),
+                                                (256, 8, 8, False), (48, 8, 8, False)])
+def test_fused_nyquist_column_unfiltered(pq, n, erow, ecol, split):
+    """With the exponential filter off (filterfac = 0) the Nyquist column
+    kx = nx/2 keeps its full amplitude (the default filter scales it by ~1e-15
+    every step, so q alone cannot show it).  After fused steps qh and both
+    tendency histories match the oracle over the whole field, and in column 0
+    and the Nyquist column on their own: the register column kernels carry
+    these two columns in one packed slot of the x-mixed exchange."""
+    phys = dict(QG_PHYS, filterfac=0.0)
+    nsteps = 8
+    q0 = seeded_q(2, n, 71)
+    m = pq.QGModel(nx=n, dt=3600., log_level=0, **phys)
+    m._bucket.set_variant(erow, ecol)
+    m._bucket.set_layer_split(split)
+    o = OracleQG(nx=n, dt=3600., **phys)
+    m.q = q0
+    o.q = q0
+    m._fused_steps(nsteps)
+    for _ in range(nsteps):
+        o._step_forward()
+    for name in ("qh", "dqhdt_p", "dqhdt_pp"):
+        a, b = np.asarray(getattr(m, name)), np.asarray(getattr(o, name))
+        assert np.abs(b[..., -1]).max() > 1e-6 * np.abs(b).max(), name  # Nyquist not negligible
+        assert rel_l2(a, b) < 1e-11, name
+        assert rel_l2(a[..., 0], b[..., 0]) < 1e-11, name
+        assert rel_l2(a[..., -1], b[..., -1]) < 1e-11, name
+    assert rel_l2(m.q, o.q) < 1e-11
+
+
+def test_packed_exchange_switch(pq):
+    """The register column kernels exchange nx/2 x-mixed columns per row
+    (bins 0 and nx/2 packed into one slot); switched off they exchange
+    nx/2 + 1.  Both give the same trajectory to rounding, and the oracle's,
+    with the filter off so that the Nyquist column carries weight."""
+    phys = dict(QG_PHYS, filterfac=0.0)
+    q0 = [seeded_q(2, 128, 80 + i) for i in range(40)]
+    runs = []
+    for on in (True, False):
+        ens = pq.Ensemble.qg(np.full(40, 128), dt=3600., **phys)
+        b = ens.buckets[0]
+        assert not b.layer_split()
+        b.set_packed_exchange(on)
+        assert b.exchange_columns() == (64 if on else 65)
+        ens.set_q(q0)
+        ens.step(10)
+        runs.append([np.asarray(ens.get("qh", i)) for i in (0, 39)])
+    o = OracleQG(nx=128, dt=3600., **phys)
+    o.q = q0[39]
+    for _ in range(10):
+        o._step_forward()
+    for a, b in zip(*runs):
+        assert rel_l2(a, b) < 1e-12
+        assert rel_l2(a[..., -1], b[..., -1]) < 1e-12
+    assert rel_l2(runs[0][1], np.asarray(o.qh)) < 1e-11
+    assert rel_l2(runs[0][1][..., -1], np.asarray(o.qh)[..., -1]) < 1e-11
+    split = pq.Ensemble.qg(np.full(2, 64), dt=3600.).buckets[0]
+    assert split.layer_split() and split.exchange_columns() == 33
+
+
 @pytest.mark.parametrize("n", [128, 256, 1024, 384, 768])
 def test_single_member_default_radix(pq, n):
     """A one-member model steps its fused stretches with radix 4 (radix 6 for
