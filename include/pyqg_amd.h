@@ -215,7 +215,9 @@ int qgk_get_variant(qgk_bucket* b, int* erow, int* ecol);
 /* Columns per row of the x-mixed fields the fused row and column kernels
  * exchange (no counterpart in the reference): nx/2 when the register column
  * kernels step the bucket (nz <= 2, layer split off; the real x bins 0 and
- * nx/2 share one complex slot), nx/2 + 1 otherwise. */
+ * nx/2 share one complex slot; it also needs every member's ll and k2l2 to
+ * be the standard tables, which the kernels then form themselves), nx/2 + 1
+ * otherwise. */
 int qgk_exchange_columns(qgk_bucket* b, int* ncols);
 /* Packed exchange on (default) or off for this bucket (no counterpart in the
  * reference): off, the register column kernels exchange nx/2 + 1 columns per

@@ -77,6 +77,10 @@ struct MemberParams {
   // the diagonal) or nullptr: the fused nz >= 3 steps then solve
   // (S - kappa^2 I) ph = qh instead of reading the table a
   const double* tri;
+  // ll[1]: with a standard grid (ll[l] = dl * l' for l' = l or l - nl, and
+  // k2l2 = kk^2 + ll^2 exactly, checked on the device) the packed register
+  // kernels form ll and kappa^2 from kk and dl instead of loading them
+  double dl;
 };
 
 struct StepCoeffs {
@@ -827,6 +831,34 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP), row_flux_waves<
   split_store_t<NX, G::T, G::RP, PK>(lds, t, rp, ok, VQm + row0, VQm + row0 + NK);
 }
 
+// Standard-grid check of every member (block per member): ll[l] = dl * l'
+// (l' = l below ny/2, l - ny above) and k2l2[l][k] = kk[k]^2 + ll[l]^2, both
+// exactly, as the model's wavenumber tables are built (model.py:461-463,
+// kernel.pyx:543-546).  The packed register kernels then form ll and kappa^2
+// themselves instead of loading them; tables set to anything else keep the
+// table-reading kernels.
+__global__ __launch_bounds__(256) void k_grid_check(const MemberParams* __restrict__ params,
+                                                    int ny, int nk, int ldk,
+                                                    int* __restrict__ ok) {
+#pragma clang fp contract(off)
+  const MemberParams& P = params[blockIdx.x];
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  int b = 0;
+  for (int l = threadIdx.x; l < ny; l += blockDim.x) {
+    const double ll = P.dl * (double)(l < ny / 2 ? l : l - ny);
+    if (!(P.ll[l] == ll)) b = 1;
+    for (int k = 0; k < nk; ++k) {
+      const double kk = P.kk[k];
+      if (!(P.k2l2[(size_t)l * ldk + k] == kk * kk + ll * ll)) b = 1;
+    }
+  }
+  if (b) atomicOr(&bad, 1);
+  __syncthreads();
+  if (threadIdx.x == 0) ok[blockIdx.x] = bad ? 0 : 1;
+}
+
 // Health check of every member: the reference asserts CFL < 1 every twrite
 // steps (model.py:564-574 with qg_diagnostics.py:76-79), CFL being
 // max|u + Ubg|, |v| over the grid times dt/dx.  Here each member's largest
@@ -1275,9 +1307,10 @@ __device__ __forceinline__ double gld(const double* p, size_t i) {
 }
 
 // ph[z] = sum_k1 a[z,k1] q[k1] with the reference's summation order
-template <int NZ>
+// KW: kappa^2 at the point is given (wk) instead of read from the k2l2 table
+template <int NZ, bool KW = false>
 __device__ __forceinline__ cd invert_regs(const MemberParams& P, int z, const cd* q,
-                                          size_t plane, size_t pt) {
+                                          size_t plane, size_t pt, double wk = 0.0) {
 #pragma clang fp contract(off)
   if (P.a) {
     cd acc = {0.0, 0.0};
@@ -1290,7 +1323,7 @@ __device__ __forceinline__ cd invert_regs(const MemberParams& P, int z, const cd
     return acc;
   }
   if constexpr (NZ == 2) {
-    const double w = gld(P.k2l2, pt);
+    const double w = KW ? wk : gld(P.k2l2, pt);
     const double det = w * (w + P.F1 + P.F2);
     if (det == 0.0) return {0.0, 0.0};
     const double di = 1.0 / det;
@@ -1309,8 +1342,9 @@ __device__ __forceinline__ cd invert_regs(const MemberParams& P, int z, const cd
 
 // Both layers' ph at one point (two layers), with invert_regs' arithmetic:
 // the closed form's determinant and its reciprocal are formed once.
+template <bool KW = false>
 __device__ __forceinline__ void invert2_both(const MemberParams& P, cd q0, cd q1, size_t plane,
-                                             size_t pt, cd& ph0, cd& ph1) {
+                                             size_t pt, cd& ph0, cd& ph1, double wk = 0.0) {
 #pragma clang fp contract(off)
   if (P.a) {
     const cd q[2] = {q0, q1};
@@ -1318,7 +1352,7 @@ __device__ __forceinline__ void invert2_both(const MemberParams& P, cd q0, cd q1
     ph1 = invert_regs<2>(P, 1, q, plane, pt);
     return;
   }
-  const double w = gld(P.k2l2, pt);
+  const double w = KW ? wk : gld(P.k2l2, pt);
   const double det = w * (w + P.F1 + P.F2);
   if (det == 0.0) {
     ph0 = {0.0, 0.0};
@@ -1376,7 +1410,9 @@ __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
       cd qs[NZ];
 #pragma unroll
       for (int k1 = 0; k1 < NZ; ++k1) qs[k1] = Q[k1][s];
-      ph[s] = ok ? invert_regs<NZ>(P, z, qs, plane, (size_t)(t + s * G::T) * ldk + kc)
+      const double ll = PK ? std_ll<NY>(P, t + s * G::T) : 0.0;
+      ph[s] = ok ? invert_regs<NZ, PK>(P, z, qs, plane, (size_t)(t + s * G::T) * ldk + kc,
+                                       kk * kk + ll * ll)
                  : cd{0.0, 0.0};
       x[s] = Q[z][s];
     }
@@ -1400,7 +1436,7 @@ __device__ __forceinline__ void col_prologue_regs(cd (&Q)[NZ][E],
     }
 #pragma unroll
     for (int s = 0; s < G::E; ++s) {
-      const double ll = gld(P.ll, t + s * G::T);
+      const double ll = PK ? std_ll<NY>(P, t + s * G::T) : gld(P.ll, t + s * G::T);
       x[s] = {ll * ph[s].y, -ll * ph[s].x};
     }
     if constexpr (PK) {
@@ -1582,21 +1618,30 @@ __global__ __launch_bounds__((2 * FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_
   for (int s = 0; s < EE; ++s) st_stream(Vm + zoff + (size_t)(t + s * T) * ldk + k, x[s]);
 }
 
+// ll[l] on a standard grid (the packed register kernels): dl * l for
+// l < NY/2, dl * (l - NY) above, as the wavenumber table is built
+// (model.py:461-463), so the values are the table's to the bit.
+template <int NY>
+__device__ __forceinline__ double std_ll(const MemberParams& P, int l) {
+  return P.dl * (double)(l < NY / 2 ? l : l - NY);
+}
+
 // Tendency and AB3 update at one spectral point (kernel.pyx:385-394, 444-461,
 // 496-517): S = ik UQ + il VQ from the y-transformed fluxes, dqhdt = -(S + ik Qy
 // ph) (+ rek kappa^2 ph in the bottom layer), qh_new = filtr (qh + dt1 dqhdt +
 // dt2 dqhdt_p + dt3 dqhdt_pp), in the reference's operation order.
+template <bool KW = false>
 __device__ __forceinline__ cd tend_ab3_point(const MemberParams& P, cd UQ, cd VQ, cd ph, cd q,
                                              double kk, double kq, double ll, bool drag,
                                              size_t pt, const cd* __restrict__ p_pt,
                                              const cd* __restrict__ pp_pt, StepCoeffs cf,
-                                             cd& d_out) {
+                                             cd& d_out, double wk = 0.0) {
 #pragma clang fp contract(off)
   const cd S = {-kk * UQ.y + -ll * VQ.y, kk * UQ.x + ll * VQ.x};
   cd d = {S.x + -kq * ph.y, S.y + kq * ph.x};
   d = {-d.x, -d.y};
   if (drag) {
-    const double cr = P.rek * gld(P.k2l2, pt);
+    const double cr = P.rek * (KW ? wk : gld(P.k2l2, pt));
     d = {d.x + cr * ph.x, d.y + cr * ph.y};
   }
   const double f = gld(P.filtr, pt);
@@ -1628,13 +1673,14 @@ __device__ __forceinline__ void nyquist_uv(const MemberParams& P, cd* nyq, int n
   const double kkN = gld(P.kk, nk - 1);
   for (int l = threadIdx.x; l < NY; l += blockDim.x) {
     const size_t ptN = (size_t)l * ldk + (nk - 1);
-    const double ll = gld(P.ll, l);
+    const double ll = std_ll<NY>(P, l);
+    const double w = kkN * kkN + ll * ll;
     cd qs[NZ];
 #pragma unroll
     for (int k1 = 0; k1 < NZ; ++k1) qs[k1] = nyq[k1 * NY + l];
 #pragma unroll
     for (int z = 0; z < NZ; ++z) {
-      const cd ph = invert_regs<NZ>(P, z, qs, plane, ptN);
+      const cd ph = invert_regs<NZ, true>(P, z, qs, plane, ptN, w);
       nyq[(NZ + z) * NY + l] = {ll * ph.y, -ll * ph.x};
       nyq[(2 * NZ + z) * NY + l] = {-kkN * ph.y, kkN * ph.x};
     }
@@ -1789,20 +1835,22 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK), (col_reg_waves<
         const int l = t + s * G::T;
         const size_t pt = (size_t)l * ldk + k;
         const size_t o = zoff + pt;
-        const double ll = gld(P.ll, l);
+        // packed: a standard grid, ll and kappa^2 formed here (std_ll)
+        const double ll = PK ? std_ll<NY>(P, l) : gld(P.ll, l);
+        const double w = PK ? kk * kk + ll * ll : 0.0;
         // ph from the old qh of every layer
         cd ph;
         if constexpr (NZ == 2) {
           if (z == 0)
-            invert2_both(P, Q[0][s], Q[1][s], plane, pt, ph, ph1[s]);
+            invert2_both<PK>(P, Q[0][s], Q[1][s], plane, pt, ph, ph1[s], w);
           else
             ph = ph1[s];
         } else {
-          ph = invert_regs<NZ>(P, z, Q[0] + s, plane, pt);
+          ph = invert_regs<NZ, PK>(P, z, Q[0] + s, plane, pt, w);
         }
         cd d;
-        const cd qn = tend_ab3_point(P, xa[s], xb[s], ph, Q[z][s], kk, kq, ll, drag, pt, p + o,
-                                     pp_to_newp + o, cf, d);
+        const cd qn = tend_ab3_point<PK>(P, xa[s], xb[s], ph, Q[z][s], kk, kq, ll, drag, pt, p + o,
+                                         pp_to_newp + o, cf, d, w);
         st_stream(qh_new + o, qn);
         st_stream(pp_to_newp + o, d);
         Q[z][s] = qn;
@@ -1825,14 +1873,16 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK), (col_reg_waves<
         const bool drag = (z == NZ - 1) && P.rek != 0.0;
         for (int l = threadIdx.x; l < NY; l += blockDim.x) {
           const size_t ptN = (size_t)l * ldk + kN;
+          const double ll = std_ll<NY>(P, l);
+          const double w = kkN * kkN + ll * ll;
           cd qs[NZ];
 #pragma unroll
           for (int k1 = 0; k1 < NZ; ++k1) qs[k1] = ld_stream(qh_old + mbase + k1 * plane + ptN);
-          const cd phN = invert_regs<NZ>(P, z, qs, plane, ptN);
+          const cd phN = invert_regs<NZ, true>(P, z, qs, plane, ptN, w);
           cd dN;
-          const cd qnN = tend_ab3_point(P, lds[4 * NY + l], lds[5 * NY + l], phN, qs[z], kkN, kqN,
-                                        gld(P.ll, l), drag, ptN, p + zoff + ptN,
-                                        pp_to_newp + zoff + ptN, cf, dN);
+          const cd qnN = tend_ab3_point<true>(P, lds[4 * NY + l], lds[5 * NY + l], phN, qs[z], kkN,
+                                              kqN, ll, drag, ptN, p + zoff + ptN,
+                                              pp_to_newp + zoff + ptN, cf, dN, w);
           st_stream(qh_new + zoff + ptN, qnN);
           st_stream(pp_to_newp + zoff + ptN, dN);
           nyq[z * NY + l] = qnN;
@@ -2608,6 +2658,8 @@ struct qgk_bucket {
   int erow = 8, ecol = 8;
   bool csplit = false;  // nz = 2: column steps by k_col_step_split (one member)
   bool pk_off = false;  // packed x-mixed exchange switched off (qgk_set_packed_exchange)
+  bool grid_std = false;  // every member's ll and k2l2 tables are standard (k_grid_check)
+  int* gflag = nullptr;   // [M] per-member result of k_grid_check
   unsigned long long* speed = nullptr;  // [M] member health words (qgk_member_speed)
   double* small = nullptr;  // 256 doubles: learned-closure scales (input [0,128), output [128,256))
   std::vector<double> small_in_h, small_out_h;  // values last uploaded there
@@ -2707,6 +2759,27 @@ int upload_params(qgk_bucket* b) {
   HIPC(hipMemcpyAsync(b->dp, b->hp.data(), sizeof(MemberParams) * b->M, hipMemcpyHostToDevice,
                       b->stream));
   b->params_dirty = false;
+  if (b->nz <= 2) {  // the packed register kernels form ll and kappa^2 on a standard grid
+    if (!b->gflag) {
+      int rc = dev_alloc(b, (void**)&b->gflag, b->M * sizeof(int));
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_grid_check, dim3(b->M), dim3(256), 0, b->stream, b->dp, b->ny, b->nk,
+                       b->ldk, b->gflag);
+    HIPC(hipGetLastError());
+    std::vector<int> f(b->M);
+    HIPC(hipMemcpyAsync(f.data(), b->gflag, b->M * sizeof(int), hipMemcpyDeviceToHost,
+                        b->stream));
+    HIPC(hipStreamSynchronize(b->stream));
+    bool all = true;
+    for (int v : f) all = all && v == 1;
+    if (all != b->grid_std) {  // the exchange format may change with it
+      b->grid_std = all;
+      b->primed = false;
+      b->gvalid = false;
+      b->gen++;
+    }
+  }
   return 0;
 }
 
@@ -3052,7 +3125,7 @@ struct PackedL {
 // the bucket) and the row kernel exchange packed x-mixed rows: NX/2 slots of
 // whole lines per row, bins 0 and nx/2 sharing slot 0 (k_row_flux).
 bool packed_exchange(const qgk_bucket* b) {
-  if (b->nz > 2 || b->pk_off) return false;
+  if (b->nz > 2 || b->pk_off || !b->grid_std) return false;
   bool pk = false;
   if (with_ne<PackedL>(b->ny, b->ecol, b, &pk)) return false;
   return pk;
@@ -3465,6 +3538,9 @@ int qgk_set_const(qgk_bucket* b, int which, int member, const double* host) {
     for (int m = (member < 0 ? 0 : member); m < (member < 0 ? b->M : member + 1); ++m)
       b->hp[m].tri = nullptr;  // a new table is used as given
   }
+  if (which == QGK_LL)
+    for (int m = (member < 0 ? 0 : member); m < (member < 0 ? b->M : member + 1); ++m)
+      b->hp[m].dl = b->nl > 1 ? host[1] : 0.0;
   b->gen++;
   b->params_dirty = true;
   const size_t rows = const_rows(b, which);
@@ -3500,6 +3576,7 @@ int qgk_set_const_all(qgk_bucket* b, int which, const double* host) {
   for (int m = 0; m < b->M; ++m) {
     param_slot(b->hp[m], which) = tab + (size_t)m * n;
     if (which == QGK_A) b->hp[m].tri = nullptr;
+    if (which == QGK_LL) b->hp[m].dl = b->nl > 1 ? host[(size_t)m * n + 1] : 0.0;
   }
   b->params_dirty = true;
   b->primed = false;
@@ -4145,6 +4222,8 @@ int qgk_set_packed_exchange(qgk_bucket* b, int on) {
 
 int qgk_exchange_columns(qgk_bucket* b, int* ncols) {
   if (!b || !ncols) return fail(QGK_E_ARG, "NULL argument");
+  if (int rc0 = refuse_in_capture(b, "qgk_exchange_columns")) return rc0;
+  if (int rc = upload_params(b)) return rc;  // settles the standard-grid check
   *ncols = packed_exchange(b) ? b->nk - 1 : b->nk;
   return 0;
 }

@@ -308,6 +308,31 @@ def test_packed_exchange_switch(pq):
     assert split.layer_split() and split.exchange_columns() == 33
 
 
+def test_nonstandard_wavenumber_tables_use_the_table_kernels(pq):
+    """The packed kernels form ll and kappa^2 themselves when every member's
+    tables are the standard ones.  A kappa^2 table that differs from
+    kk^2 + ll^2 anywhere (here by one ulp at one point, through the drop-in
+    _k2l2 setter) keeps the table-reading kernels, whose trajectory then
+    agrees with the standard run's to rounding."""
+    n, nsteps = 64, 12
+    q0 = seeded_q(2, n, 95)
+    runs = []
+    for perturb in (False, True):
+        m = pq.QGModel(nx=n, dt=3600., log_level=0, **QG_PHYS)
+        m._bucket.set_variant(8, 8)
+        m._bucket.set_layer_split(False)
+        assert m._bucket.exchange_columns() == n // 2
+        if perturb:
+            k2 = np.array(m._k2l2)
+            k2[5, 7] = np.nextafter(k2[5, 7], np.inf)
+            m._k2l2 = k2
+            assert m._bucket.exchange_columns() == n // 2 + 1
+        m.q = q0
+        m._fused_steps(nsteps)
+        runs.append(np.array(m.q))
+    assert rel_l2(runs[1], runs[0]) < 1e-12
+
+
 @pytest.mark.parametrize("n", [128, 256, 1024, 384, 768])
 def test_single_member_default_radix(pq, n):
     """A one-member model steps its fused stretches with radix 4 (radix 6 for
