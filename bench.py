@@ -51,7 +51,8 @@ def measured_traffic(radix_row, radix_col, path=None):
         return None
     text = text.split("## HBM traffic", 1)[1]
     total = 0.0
-    for name in (f"k_row_flux<256, {radix_row}>", f"k_col_step_reg<256, 2, {radix_col}>"):
+    for name in (f"k_row_flux<256, {radix_row}, true>",
+                 f"k_col_step_reg<256, 2, {radix_col}, true>"):
         for line in text.splitlines():
             cells = [c.strip() for c in line.split("|")]
             if len(cells) > 3 and cells[1] == name:
