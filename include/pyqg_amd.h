@@ -192,7 +192,8 @@ int qgk_get_device(qgk_bucket* b, int field, void* dptr);
  *   subgrid PV forcing S = C(T_hi(q)) - T_lo(C(q)) into lo's dqh (spectral) and
  *   dq (real), T = the model tendency of _invert + _do_advection.
  * qgk_pack_inputs: lo's q, u, v (real) as an NHWC BF16 network input with
- *   channels f*nz + z (f = q, u, v) times inv_scale[3*nz], zero-padded to cinp.
+ *   channels f*nz + z (f = q, u, v) times inv_scale[3*nz], zero-padded to cinp
+ *   (a multiple of 8; dst 16-byte aligned).
  * qgk_add_forcing_device: dq = net_out[..., z] * scale[z] (network output FP32
  *   NHWC with coutp channels), then dqhdt += rfft2(dq), as
  *   _do_q_subgrid_parameterization does with a host array (kernel.pyx:424-442). */

@@ -14,6 +14,7 @@
 // Weight layout W[cout][25][cin] (tap = dy*5 + dx, dy, dx in 0..4 for offsets
 // -2..2).  All shapes are checked on the host before any launch.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <hip/hip_bf16.h>
 
 #include <cstdarg>
@@ -236,23 +237,57 @@ __global__ __launch_bounds__(256, RW) void k_conv_fwd_mt(const bf16_t* __restric
     if (wave_ok) {
       const bf16_t* wrow =
           w + (size_t)tile_channel<MT>(co0, 0, li) * NTAP * cinp + ci0 + lg * KG;
-#pragma unroll 1
-      for (int tap = 0; tap < NTAP; ++tap) {
-        const int dy = tap / KS, dx = tap % KS;
-        op a[MT];
+      auto load_a = [&](op (&a)[MT], int tap) {
 #pragma unroll
         for (int m = 0; m < MT; ++m)
           a[m] = *reinterpret_cast<const op*>(wrow + ((size_t)m * 4 * NTAP + tap) * cinp);
+      };
+      // One tap: every B operand read from LDS before the first matrix op (the
+      // LDS latency is paid once per tap, not per operand pair), then the
+      // MT x RW x NT matrix ops.  Up to two weight tiles, the weight operands
+      // of the next tap are loaded first, so their L2 latency overlaps this
+      // tap's work (two register sets, taps taken in pairs).  The scheduling
+      // barriers keep
+      // that order: the compiler's own schedule interleaves the reads with
+      // the matrix ops two at a time and waits for each pair.
+      auto tap_step = [&](const op (&a)[MT], op (&a_next)[MT], int tap, auto next_c) {
+        constexpr bool next = decltype(next_c)::value;
+        if constexpr (next) load_a(a_next, tap + 1);
+        const int dy = tap / KS, dx = tap % KS;
+        op bv[RW][NT];
 #pragma unroll
         for (int rr = 0; rr < RW; ++rr) {
           const bf16_t* lrow = lds + (wave * RW + rr + dy) * W * PS + lg * KG;
 #pragma unroll
           for (int t = 0; t < NT; ++t) {
             const int pix = (t * 16 + li + dx - 2 + W) % W;
-            const op bv = *reinterpret_cast<const op*>(lrow + pix * PS);
-#pragma unroll
-            for (int m = 0; m < MT; ++m) acc[m][rr][t] = M::mma(a[m], bv, acc[m][rr][t]);
+            bv[rr][t] = *reinterpret_cast<const op*>(lrow + pix * PS);
           }
+        }
+#pragma unroll
+        for (int rr = 0; rr < RW; ++rr)
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int m = 0; m < MT; ++m) acc[m][rr][t] = M::mma(a[m], bv[rr][t], acc[m][rr][t]);
+        __builtin_amdgcn_sched_group_barrier(0x020, next ? MT : 0, 0);
+        __builtin_amdgcn_sched_group_barrier(0x100, RW * NT, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, MT * RW * NT, 0);
+      };
+      op a0[MT], a1[MT];
+      if constexpr (MT <= 2) {
+        load_a(a0, 0);
+#pragma unroll 1
+        for (int tap = 0; tap < NTAP - 1; tap += 2) {  // NTAP is odd
+          tap_step(a0, a1, tap, std::true_type{});
+          tap_step(a1, a0, tap + 1, std::true_type{});
+        }
+        tap_step(a0, a1, NTAP - 1, std::false_type{});
+      } else {  // four weight tiles: no room for a second register set
+#pragma unroll 1
+        for (int tap = 0; tap < NTAP; ++tap) {
+          load_a(a0, tap);
+          tap_step(a0, a1, tap, std::false_type{});
         }
       }
     }

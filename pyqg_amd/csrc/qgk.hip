@@ -1940,25 +1940,40 @@ __global__ void k_csub(const cd* __restrict__ a, const cd* __restrict__ b, cd* _
 
 // NHWC BF16 network input from real fields: channel c = f * nz + z of field f
 // (f over q, u, v) scaled by inv_scale[c]; channels cin..cinp-1 are zero.
+// One thread per pixel and group of 8 channels: the group is rounded to BF16
+// (nearest even) and written as one 16-byte store; consecutive threads take
+// consecutive pixels, so the field reads are coalesced.
 __global__ void k_pack_nhwc(const double* __restrict__ q, const double* __restrict__ u,
                             const double* __restrict__ v, const double* __restrict__ inv_scale,
                             unsigned short* __restrict__ out, int M, int nz, int npix, int cinp) {
-  const size_t total = (size_t)M * npix * cinp;
+  const int groups = cinp / 8;
+  const size_t total = (size_t)M * npix * groups;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
        i += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i % cinp);
-    const size_t mp = i / cinp;
+    const size_t mp = i % ((size_t)M * npix);  // member * npix + pixel
+    const int g = (int)(i / ((size_t)M * npix));
     const int p = (int)(mp % npix);
     const int m = (int)(mp / npix);
-    float val = 0.f;
-    if (c < 3 * nz) {
-      const int f = c / nz, z = c % nz;
-      const double* src = f == 0 ? q : (f == 1 ? u : v);
-      val = (float)(src[((size_t)m * nz + z) * npix + p] * inv_scale[c]);
+    unsigned short h[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int c = 8 * g + j;
+      float val = 0.f;
+      if (c < 3 * nz) {
+        const int f = c / nz, z = c % nz;
+        const double* src = f == 0 ? q : (f == 1 ? u : v);
+        val = (float)(src[((size_t)m * nz + z) * npix + p] * inv_scale[c]);
+      }
+      unsigned int bits = __float_as_uint(val);
+      bits += 0x7fffu + ((bits >> 16) & 1u);
+      h[j] = (unsigned short)(bits >> 16);
     }
-    unsigned int bits = __float_as_uint(val);
-    bits += 0x7fffu + ((bits >> 16) & 1u);
-    out[i] = (unsigned short)(bits >> 16);
+    uint4 w;
+    w.x = h[0] | ((unsigned int)h[1] << 16);
+    w.y = h[2] | ((unsigned int)h[3] << 16);
+    w.z = h[4] | ((unsigned int)h[5] << 16);
+    w.w = h[6] | ((unsigned int)h[7] << 16);
+    *reinterpret_cast<uint4*>(out + mp * cinp + 8 * g) = w;
   }
 }
 
@@ -4449,8 +4464,10 @@ int qgk_pack_inputs(qgk_bucket* b, const double* inv_scale, int cinp, void* dst)
   if (3 * b->nz > 128) return fail(QGK_E_ARG, "too many layers for the packing constants");
   double* sc = b->small;
   if ((rc = upload_cached(b, inv_scale, 3 * b->nz, b->small_in_h, sc))) return rc;
+  if (cinp % 8) return fail(QGK_E_ARG, "cinp must be a multiple of 8");
+  if (reinterpret_cast<uintptr_t>(dst) % 16) return fail(QGK_E_ARG, "dst must be 16-byte aligned");
   const int npix = (int)b->nplane_r;
-  const size_t n = (size_t)b->M * npix * cinp;
+  const size_t n = (size_t)b->M * npix * (cinp / 8);
   hipLaunchKernelGGL(k_pack_nhwc, dim3(grid_1d(n)), dim3(256), 0, b->stream, b->field_r[QGK_Q],
                      b->field_r[QGK_U], b->field_r[QGK_V], sc, (unsigned short*)dst, b->M, b->nz,
                      npix, cinp);
