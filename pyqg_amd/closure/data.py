@@ -24,9 +24,15 @@ QG_EDDY = dict(L=1e6, rd=15000.0, delta=0.25, U1=0.025, U2=0.0, beta=1.5e-11, re
                H1=500.0, filterfac=23.6)
 
 
-def coarse_filter(n_lo, L):
-    """Gaussian filter of width Delta = L / n_lo on the coarse rfft2 grid."""
+def coarse_filter(n_lo, L, kind="gaussian"):
+    """Filter on the coarse rfft2 grid applied after the spectral truncation:
+    "gaussian", exp(-(kappa Delta)^2 / 24) with Delta = L / n_lo, or "sharp",
+    the truncation alone (every resolved mode kept whole)."""
     g = C.grid(n_lo, n_lo, L, L)
+    if kind == "sharp":
+        return np.ones_like(g["wv2"])
+    if kind != "gaussian":
+        raise ValueError(f"unknown coarse filter {kind!r}")
     delta = L / n_lo
     return np.exp(-g["wv2"] * delta ** 2 / 24.0)
 
@@ -47,7 +53,8 @@ def ctypes_ptr(t):
 class PairGenerator:
     """High-resolution ensemble + its coarse-graining onto a coarse grid."""
 
-    def __init__(self, nmembers=64, n_hi=256, n_lo=64, dt=3600., seed=0, phys=None):
+    def __init__(self, nmembers=64, n_hi=256, n_lo=64, dt=3600., seed=0, phys=None,
+                 filter_kind="gaussian"):
         self.phys = dict(QG_EDDY, **(phys or {}))
         self.n_hi, self.n_lo, self.dt = n_hi, n_lo, dt
         self.hi = Ensemble.qg(np.full(nmembers, n_hi), dt=dt, full_state=True, **self.phys)
@@ -55,7 +62,7 @@ class PairGenerator:
                               **self.phys)
         rng = np.random.default_rng(seed)
         self.hi.set_q([1e-7 * rng.standard_normal((2, n_hi, n_hi)) for _ in range(nmembers)])
-        self.G = np.ascontiguousarray(coarse_filter(n_lo, self.phys["L"]))
+        self.G = np.ascontiguousarray(coarse_filter(n_lo, self.phys["L"], filter_kind))
 
     def spin_up(self, nsteps, chunk=2000):
         while nsteps > 0:

@@ -45,16 +45,18 @@ BUDGET = ["KEflux", "APEflux", "APEgenspec", "KEfrictionspec", "Dissspec", "EKEd
 
 def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=64, lr=1e-3,
         online_members=128, online_days=720, avg_days=360, dt_lo=14400., seed=0, ema=0.998,
-        ema_from=0.5, closures=("backscatter", "zb2020", "smagorinsky"), log=print):
+        ema_from=0.5, closures=("backscatter", "zb2020", "smagorinsky"), calibrate="none",
+        filter_kind="gaussian", log=print):
     out = {"config": dict(members=members, samples=samples, spinup_days=spinup_days,
                           every=every, epochs=epochs, batch=batch, lr=lr,
                           online_members=online_members, online_days=online_days,
                           avg_days=avg_days, dt_hi=3600., dt_lo=dt_lo, seed=seed, ema=ema,
-                          ema_from=ema_from)}
+                          ema_from=ema_from, calibrate=calibrate, filter=filter_kind)}
     L = data_mod.QG_EDDY["L"]
     g_hi, g_lo = tools.grid(256, L), tools.grid(64, L)
     # ---------------------------------------------------------- 1. data
-    gen = data_mod.PairGenerator(nmembers=members, n_hi=256, n_lo=64, dt=3600., seed=seed)
+    gen = data_mod.PairGenerator(nmembers=members, n_hi=256, n_lo=64, dt=3600., seed=seed,
+                                 filter_kind=filter_kind)
     t0 = time.perf_counter()
     gen.spin_up(spinup_days * 24)
     hb = gen.hi.buckets[0]
@@ -99,6 +101,16 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
     out["train"] = {"steps": info["steps"], "seconds": info["seconds"],
                     "final_loss": info["history"][-1][1] if info["history"] else None}
     out["offline"] = train_mod.evaluate(net, x_te, y_te, scales)
+    amp = train_mod.amplitude(net, x_te, y_te, scales)
+    out["offline"]["amplitude"] = amp
+    if calibrate == "variance":
+        # restore the forcing's held-out mean square, layer by layer: the MSE fit
+        # leaves out the part of the forcing its inputs do not determine, and
+        # with it energy the coarse model then lacks
+        gain = np.sqrt(np.asarray(amp["target_ms"]) / np.asarray(amp["pred_ms"]))
+        scales = param_mod.Scales(scales.in_scale, scales.out_scale * gain)
+        out["offline"]["gain"] = gain.tolist()
+        log(f"variance calibration gain {gain}")
     flop_per_sample = 2 * 64 * 64 * 25 * sum(
         net.chp[l] * net.chp[l + 1] for l in range(net.nlayers))
     out["train"]["tflops"] = 3 * flop_per_sample * info["steps"] * batch / info["seconds"] / 1e12
@@ -186,12 +198,18 @@ def main():
                     help="fraction of training after which the weights are averaged")
     ap.add_argument("--closures", default="backscatter,zb2020,smagorinsky",
                     help="built-in closures run online beside the network (comma list)")
+    ap.add_argument("--calibrate", choices=["none", "variance"], default="none",
+                    help="variance: scale each output layer so the network's held-out "
+                         "mean-square forcing equals the target's")
+    ap.add_argument("--filter", choices=["gaussian", "sharp"], default="gaussian",
+                    help="coarse-graining after the spectral truncation")
     ap.add_argument("--out", default=None, help="also append the JSON line to this file")
     a = ap.parse_args()
     res = run(members=a.members, samples=a.samples, spinup_days=a.spinup_days, every=a.every,
               epochs=a.epochs, batch=a.batch, lr=a.lr, online_members=a.online_members,
               online_days=a.online_days, avg_days=a.avg_days, dt_lo=a.dt_lo, seed=a.seed,
               ema=a.ema, ema_from=a.ema_from, closures=[c for c in a.closures.split(",") if c],
+              calibrate=a.calibrate, filter_kind=a.filter,
               log=lambda s: print(s, flush=True))
     line = json.dumps(res)
     print(line)

@@ -84,3 +84,27 @@ def evaluate(net: CNN, x, y, scales: Scales, batch=128):
         se += float(((out - yb) ** 2).sum())
         ss += float((yb ** 2).sum())
     return {"mse_net": se / ss, "mse_zero": 1.0, "r2": 1.0 - se / ss}
+
+
+@torch.no_grad()
+def amplitude(net: CNN, x, y, scales: Scales, batch=128):
+    """Per output channel (layer) on held-out pairs: mean square of the target,
+    of the prediction, and their mean product (normalised units).  An MSE
+    fit shrinks the prediction towards the conditional mean: its mean square
+    falls short of the target's by the unexplained variance."""
+    nout = y.shape[-1]
+    yy = torch.zeros(nout, dtype=torch.float64, device=x.device)
+    pp = torch.zeros_like(yy)
+    yp = torch.zeros_like(yy)
+    n = 0
+    for s in range(0, x.shape[0], batch):
+        idx = torch.arange(s, min(s + batch, x.shape[0]), device=x.device)
+        xb, yb = _normalized(x, y, scales, idx, net)
+        out = net.forward(xb)[..., :nout].double()
+        yb = yb.double()
+        yy += (yb ** 2).sum(dim=(0, 1, 2))
+        pp += (out ** 2).sum(dim=(0, 1, 2))
+        yp += (out * yb).sum(dim=(0, 1, 2))
+        n += yb.shape[0] * yb.shape[1] * yb.shape[2]
+    return {"target_ms": (yy / n).tolist(), "pred_ms": (pp / n).tolist(),
+            "cross": (yp / n).tolist()}
