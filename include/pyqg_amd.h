@@ -203,6 +203,23 @@ int qgk_subgrid_forcing(qgk_bucket* hi, qgk_bucket* lo, const double* filter_lo)
 int qgk_pack_inputs(qgk_bucket* b, const double* inv_scale, int cinp, void* dst);
 int qgk_add_forcing_device(qgk_bucket* b, const float* net_out, int coutp, const double* scale);
 
+/* Fused closed step (no counterpart in the reference: Model._step_forward with
+ * a q parameterization, model.py:406-436, as two fused kernels around the
+ * network).  One step is qgk_fused_row_net, the network on the bucket's
+ * stream (reading net_in, writing net_out), then qgk_fused_col_forcing:
+ *   qgk_fused_row_net: the fused row kernel, which also writes the network
+ *     input as qgk_pack_inputs does (channels f*nz + z of q, u, v times
+ *     inv_scale, NHWC BF16; channels from 3*nz up to cinp are not written and
+ *     must hold zeros);
+ *   qgk_fused_col_forcing: dq = net_out * scale as qgk_add_forcing_device,
+ *     dqh = rfft2(dq), and the fused column kernel with dqh added to the
+ *     tendency after the drag, AB weights dt1..dt3; ends the step.
+ * Two-layer square grids up to 128 on the packed register kernels (layer split
+ * off, standard wavenumber grid); capturable (qgk_capture_begin). */
+int qgk_fused_row_net(qgk_bucket* b, const double* inv_scale, int cinp, void* net_in);
+int qgk_fused_col_forcing(qgk_bucket* b, const float* net_out, int coutp, const double* scale,
+                          double dt1, double dt2, double dt3);
+
 /* Two-layer column steps by the layer-split kernel (each layer its own
  * thread group) instead of the register kernel (no counterpart in the
  * reference).  Default: on for one member or members x max(nx, ny)^2 <= 2^19, off

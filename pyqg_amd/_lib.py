@@ -35,6 +35,7 @@ EXPORTS = ["qgk_create", "qgk_destroy", "qgk_last_error", "qgk_set_const", "qgk_
            "qgk_get_layer_split", "qgk_autotune",
            "qgk_device_name", "qgk_put_device", "qgk_get_device", "qgk_coarsen",
            "qgk_subgrid_forcing", "qgk_pack_inputs", "qgk_add_forcing_device",
+           "qgk_fused_row_net", "qgk_fused_col_forcing",
            "qgk_diag_set_params", "qgk_diag_accumulate", "qgk_diag_read", "qgk_diag_reset",
            "qgk_param_smagorinsky", "qgk_smagorinsky_viscosity", "qgk_param_zb2020",
            "qgk_param_backscatter", "qgk_apply_q_param", "qgk_apply_uv_param",
@@ -129,6 +130,9 @@ def _load():
         "qgk_coarsen": (I, [P, P, I, I, P]),
         "qgk_subgrid_forcing": (I, [P, P, P]),
         "qgk_pack_inputs": (I, [P, P, I, P]),
+        "qgk_fused_row_net": (I, [P, P, I, P]),
+        "qgk_fused_col_forcing": (I, [P, P, I, P, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double]),
         "qgk_add_forcing_device": (I, [P, P, I, P]),
         "qgk_diag_set_params": (I, [P, I, P]),
         "qgk_diag_accumulate": (I, [P, ctypes.c_uint, I, D, D, D, D]),

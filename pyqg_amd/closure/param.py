@@ -67,9 +67,15 @@ class ClosedEnsemble:
     """Coarse two-layer QG ensemble stepped with the CNN closure on the GPU.
 
     With `param=None` it is the plain coarse model (fused steps), the baseline
-    the closure is compared with."""
+    the closure is compared with.  With the network, `fused=True` (the default
+    where the grid allows it: two layers, n <= 128) steps through the fused
+    two-kernel step: the row kernel writes the network input, the column
+    kernel adds the network's forcing to the tendency (qgk_fused_row_net,
+    qgk_fused_col_forcing); `fused=False` runs the reference's phases one by
+    one with the forcing added after the friction (the same arithmetic up to
+    the transforms' rounding)."""
 
-    def __init__(self, nmembers, n, dt, phys, param=None, seed=0):
+    def __init__(self, nmembers, n, dt, phys, param=None, seed=0, fused=True):
         from ..ensemble import Ensemble
         from ..parameterizations import Parameterization
         self.builtin = isinstance(param, Parameterization)
@@ -81,10 +87,16 @@ class ClosedEnsemble:
         self.n = n
         rng = np.random.default_rng(seed)
         self.ens.set_q([1e-7 * rng.standard_normal((2, n, n)) for _ in range(nmembers)])
+        self.fused = False
         if param is not None and not self.builtin:
             self.cinp = param.net.chp[0]
-            self.xin = torch.empty((nmembers, n, n, self.cinp), dtype=torch.bfloat16,
+            # zeros: the fused row kernel writes only the 3*nz input channels
+            self.xin = torch.zeros((nmembers, n, n, self.cinp), dtype=torch.bfloat16,
                                    device="cuda")
+            self.fused = bool(fused) and n <= 128 and self.cinp % 8 == 0
+            if self.fused:
+                self.b.set_layer_split(False)
+                self.fused = self.b.exchange_columns() == n // 2
             self.acts = param.net.activation_buffers(nmembers, n, n)
             self.inv_in = np.ascontiguousarray(1.0 / param.scales.in_scale)
             self.out_scale = np.ascontiguousarray(param.scales.out_scale)
@@ -149,6 +161,13 @@ class ClosedEnsemble:
         """Invert, advect, friction, the network's forcing, time step (weights w)."""
         L = _lib.lib
         h = self.b.h
+        if self.fused:
+            _lib.check(L.qgk_fused_row_net(h, _lib.ptr(self.inv_in), self.cinp,
+                                           ctypes_p(self.xin)))
+            out = self.param.net.forward_into(self.xin, self.acts, self.stream)
+            _lib.check(L.qgk_fused_col_forcing(h, ctypes_p(out), out.shape[-1],
+                                               _lib.ptr(self.out_scale), *w))
+            return
         _lib.check(L.qgk_invert(h))
         _lib.check(L.qgk_advect(h))
         _lib.check(L.qgk_friction(h))

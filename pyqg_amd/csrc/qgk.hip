@@ -766,11 +766,26 @@ __device__ __forceinline__ void row_pair_extend(const cd (&h)[2][E / 2],
 template <int E>
 constexpr int row_flux_waves() { return E == 8 ? 3 : 1; }
 
-template <int NX, int E, bool PK>
+// The BF16 rounding of k_pack_nhwc (nearest even), for the network input.
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+  unsigned int bits = __float_as_uint(f);
+  bits += 0x7fffu + ((bits >> 16) & 1u);
+  return (unsigned short)(bits >> 16);
+}
+
+// NET: the kernel also writes the learned closure's input from the real q, u,
+// v it has formed, as qgk_pack_inputs does from the real fields: NHWC BF16,
+// channel f * nz + z (f = q, u, v) of pixel (row, x) of the member is the
+// value times net_scale[f * nz + z]; a block writes the three channels of its
+// layer.  The padding channels are left as they are (zero in the caller's
+// buffer).
+template <int NX, int E, bool PK, bool NET = false>
 __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP), row_flux_waves<E>()) void k_row_flux(
     const cd* __restrict__ Qm, const cd* __restrict__ Um, const cd* __restrict__ Vm,
     cd* __restrict__ UQm, cd* __restrict__ VQm, const MemberParams* __restrict__ params,
-    int nz, int ny, double scale, const cd* __restrict__ tw) {
+    int nz, int ny, double scale, const cd* __restrict__ tw,
+    const double* __restrict__ net_scale = nullptr, unsigned short* __restrict__ net_in = nullptr,
+    int cinp = 0) {
   using G = FGeo<NX, E>;
   constexpr int NK = ldk_of(NX);  // row stride
   __shared__ cd lds[NX * G::RP];
@@ -814,6 +829,19 @@ __global__ __launch_bounds__((FGeo<NX, E>::T * FGeo<NX, E>::RP), row_flux_waves<
     const double q0 = zq[s].x * scale, q1 = zq[s].y * scale;
     const double u0 = zu[s].x * scale, u1 = zu[s].y * scale;
     const double v0 = zv[s].x * scale, v1 = zv[s].y * scale;
+    if constexpr (NET) {
+      if (ok) {
+        const size_t px0 = (((size_t)m * ny + j0) * NX + t + s * G::T) * cinp;
+        const size_t px1 = px0 + (size_t)NX * cinp;  // row j0 + 1
+        const double sq = net_scale[z], su = net_scale[nz + z], sv = net_scale[2 * nz + z];
+        net_in[px0 + z] = bf16_rne((float)(q0 * sq));
+        net_in[px1 + z] = bf16_rne((float)(q1 * sq));
+        net_in[px0 + nz + z] = bf16_rne((float)(u0 * su));
+        net_in[px1 + nz + z] = bf16_rne((float)(u1 * su));
+        net_in[px0 + 2 * nz + z] = bf16_rne((float)(v0 * sv));
+        net_in[px1 + 2 * nz + z] = bf16_rne((float)(v1 * sv));
+      }
+    }
     zu[s] = {(u0 + ubg) * q0, (u1 + ubg) * q1};
     zv[s] = {v0 * q0, v1 * q1};
   }
@@ -1630,12 +1658,15 @@ __device__ __forceinline__ double std_ll(const MemberParams& P, int l) {
 // 496-517): S = ik UQ + il VQ from the y-transformed fluxes, dqhdt = -(S + ik Qy
 // ph) (+ rek kappa^2 ph in the bottom layer), qh_new = filtr (qh + dt1 dqhdt +
 // dt2 dqhdt_p + dt3 dqhdt_pp), in the reference's operation order.
+// With f_pt, the q parameterization's dqh at the point is added after the
+// drag (kernel.pyx:424-442, called after _do_friction by model.py:420-426).
 template <bool KW = false>
 __device__ __forceinline__ cd tend_ab3_point(const MemberParams& P, cd UQ, cd VQ, cd ph, cd q,
                                              double kk, double kq, double ll, bool drag,
                                              size_t pt, const cd* __restrict__ p_pt,
                                              const cd* __restrict__ pp_pt, StepCoeffs cf,
-                                             cd& d_out, double wk = 0.0) {
+                                             cd& d_out, double wk = 0.0,
+                                             const cd* __restrict__ f_pt = nullptr) {
 #pragma clang fp contract(off)
   const cd S = {-kk * UQ.y + -ll * VQ.y, kk * UQ.x + ll * VQ.x};
   cd d = {S.x + -kq * ph.y, S.y + kq * ph.x};
@@ -1643,6 +1674,10 @@ __device__ __forceinline__ cd tend_ab3_point(const MemberParams& P, cd UQ, cd VQ
   if (drag) {
     const double cr = P.rek * (KW ? wk : gld(P.k2l2, pt));
     d = {d.x + cr * ph.x, d.y + cr * ph.y};
+  }
+  if (f_pt) {
+    const cd f = ld_stream(f_pt);
+    d = {d.x + f.x, d.y + f.y};
   }
   const double f = gld(P.filtr, pt);
   const cd pv = ld_stream(p_pt), ppv = ld_stream(pp_pt);
@@ -1742,13 +1777,15 @@ constexpr int col_reg_waves() { return NZ == 2 && E == 8 ? 2 : 1; }
 // LDS; its qh, dqhdt_p and dqhdt_pp from memory) and keeps the new qh and the
 // derived u, v spectra in LDS for the packed prologue.  The next transform's
 // first LDS write follows a barrier, so columns 2-5 live until then.
-template <int NY, int NZ, int E, bool PK>
+// FORCE: dqh (the q parameterization's spectral forcing, [M][nz][ny][ldk])
+// is added to the tendency after the drag.
+template <int NY, int NZ, int E, bool PK, bool FORCE = false>
 __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK), (col_reg_waves<NZ, E>())) void k_col_step_reg(
     const cd* __restrict__ qh_old, cd* __restrict__ qh_new, const cd* __restrict__ p,
     cd* __restrict__ pp_to_newp, const cd* __restrict__ UQm, const cd* __restrict__ VQm,
     cd* __restrict__ Qm, cd* __restrict__ Um, cd* __restrict__ Vm,
     const MemberParams* __restrict__ params, StepCoeffs cf, int nk, int ldk,
-    const cd* __restrict__ tw) {
+    const cd* __restrict__ tw, const cd* __restrict__ dqh = nullptr) {
 #pragma clang fp contract(off)
   using G = FGeo<NY, E>;
   static_assert(!PK || G::TK >= 6, "the packed exchange uses six LDS columns");
@@ -1850,7 +1887,7 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK), (col_reg_waves<
         }
         cd d;
         const cd qn = tend_ab3_point<PK>(P, xa[s], xb[s], ph, Q[z][s], kk, kq, ll, drag, pt, p + o,
-                                         pp_to_newp + o, cf, d, w);
+                                         pp_to_newp + o, cf, d, w, FORCE ? dqh + o : nullptr);
         st_stream(qh_new + o, qn);
         st_stream(pp_to_newp + o, d);
         Q[z][s] = qn;
@@ -1882,7 +1919,8 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK), (col_reg_waves<
           cd dN;
           const cd qnN = tend_ab3_point<true>(P, lds[4 * NY + l], lds[5 * NY + l], phN, qs[z], kkN,
                                               kqN, ll, drag, ptN, p + zoff + ptN,
-                                              pp_to_newp + zoff + ptN, cf, dN, w);
+                                              pp_to_newp + zoff + ptN, cf, dN, w,
+                                              FORCE ? dqh + zoff + ptN : nullptr);
           st_stream(qh_new + zoff + ptN, qnN);
           st_stream(pp_to_newp + zoff + ptN, dN);
           nyq[z * NY + l] = qnN;
@@ -2700,6 +2738,7 @@ struct qgk_bucket {
   int ug_phase = -1;  // history rotation (iP) at capture
   long ug_gen = -1;   // gen at capture
   const cd* ug_qh = nullptr;  // qh buffer at capture (the fused step swaps qh buffers)
+  bool ug_fused = false;      // the captured steps are fused closed steps (qgk_fused_*)
   hipStream_t lst = nullptr;  // stream override for fused-step launches (graph capture)
   long gen = 0;               // bumped whenever launch arguments change (graph cache key)
   // hipGraph of two fused steps of a set of buckets, replayed while the AB
@@ -3272,6 +3311,39 @@ int launch_col_step(qgk_bucket* b, StepCoeffs cf) {
   HIPC(hipGetLastError());
   return 0;
 }
+
+// Fused closed step (qgk_fused_row_net / qgk_fused_col_forcing): the packed
+// register kernels with the network input written by the row kernel and the
+// forcing added by the column kernel, for two-layer grids up to 128.
+constexpr int kFusedNetMaxN = 128;
+
+struct RowFluxNetL {
+  template <int NX, int E>
+  static void go(qgk_bucket* b, const double* sc, unsigned short* net_in, int cinp) {
+    if constexpr (NX <= kFusedNetMaxN) {
+      using G = FGeo<NX, E>;
+      const double scale = 1.0 / ((double)b->nx * (double)b->ny);
+      dim3 grid((b->ny / 2 + G::RP - 1) / G::RP, b->M * b->nz);
+      hipLaunchKernelGGL((k_row_flux<NX, E, true, true>), grid, dim3(G::T * G::RP), 0,
+                         b->stream, b->Qm, b->Um, b->Vm, b->UQm, b->VQm, b->dp, b->nz, b->ny,
+                         scale, b->tw_x, sc, net_in, cinp);
+    }
+  }
+};
+
+struct ColStepForceL {
+  template <int NY, int E>
+  static void go(qgk_bucket* b, StepCoeffs cf) {
+    if constexpr (NY <= kFusedNetMaxN && pk_fits<NY, 2, E>()) {
+      using G = FGeo<NY, E>;
+      dim3 grid((col_count(b->nk, true) + G::TK - 1) / G::TK, b->M);
+      hipLaunchKernelGGL((k_col_step_reg<NY, 2, E, true, true>), grid, dim3(G::T * G::TK), 0,
+                         b->stream, b->field_c[QGK_QH], b->qh_alt, Pp(b), PP(b), b->UQm, b->VQm,
+                         b->Qm, b->Um, b->Vm, b->dp, cf, b->nk, b->ldk, b->tw_y,
+                         b->field_c[QGK_DQH]);
+    }
+  }
+};
 
 // Members' health words into w[M] (cleared first, by a fill kernel, so the
 // work can be captured): x inverse transforms of the primed Qm, Um, Vm into
@@ -3851,6 +3923,7 @@ int qgk_capture_begin(qgk_bucket* b) {
   b->ug_phase = b->iP;
   b->ug_gen = b->gen;
   b->ug_qh = b->field_c[QGK_QH];
+  b->ug_fused = false;
   return 0;
 }
 
@@ -3891,10 +3964,11 @@ int qgk_graph_replay(qgk_bucket* b, int n) {
   int rc = upload_params(b);
   if (rc) return rc;
   for (int i = 0; i < n; ++i) HIPC(hipGraphLaunch(b->ugexec, b->stream));
-  // the graph ends with a time step, as qgk_timestep leaves the bucket
+  // the graph ends with a time step, as qgk_timestep leaves the bucket, or
+  // with a fused closed step, as qgk_fused_col_forcing leaves it
   if (n > 0) {
-    b->primed = false;
-    b->q_stale = false;
+    b->primed = b->ug_fused;
+    b->q_stale = b->ug_fused;
   }
   HIPC(hipStreamSynchronize(b->stream));
   return 0;
@@ -4495,6 +4569,63 @@ int qgk_add_forcing_device(qgk_bucket* b, const float* net_out, int coutp, const
   hipLaunchKernelGGL(k_add, dim3(grid_1d(cpts)), dim3(256), 0, b->stream, D(b),
                      b->field_c[QGK_DQH], cpts);
   HIPC(hipGetLastError());
+  HIPC(stream_done(b));
+  return 0;
+}
+
+int qgk_fused_row_net(qgk_bucket* b, const double* inv_scale, int cinp, void* net_in) {
+  if (!b || !inv_scale || !net_in) return fail(QGK_E_ARG, "NULL argument");
+  if (b->nz != 2 || b->nx != b->ny || b->nx > kFusedNetMaxN)
+    return fail(QGK_E_ARG, "the fused closed step needs a two-layer square grid of n <= %d",
+                kFusedNetMaxN);
+  if (cinp < 3 * b->nz || cinp % 8) return fail(QGK_E_ARG, "cinp must be >= 6 and a multiple of 8");
+  if (reinterpret_cast<uintptr_t>(net_in) % 16) return fail(QGK_E_ARG, "net_in must be 16-byte aligned");
+  int rc = upload_params(b);
+  if (rc) return rc;
+  if (!packed_exchange(b))
+    return fail(QGK_E_STATE, "the fused closed step needs the packed register kernels "
+                             "(layer split off, standard wavenumber grid)");
+  double* sc = b->small;
+  if ((rc = upload_cached(b, inv_scale, 3 * b->nz, b->small_in_h, sc))) return rc;
+  if (!b->primed) {
+    if ((rc = launch_col_prologue(b))) return rc;
+    b->primed = true;
+  }
+  rc = with_ne<RowFluxNetL>(b->nx, b->erow, b, (const double*)sc,
+                            static_cast<unsigned short*>(net_in), cinp);
+  if (rc) return rc;
+  HIPC(hipGetLastError());
+  if (b->capturing) b->ug_fused = true;
+  HIPC(stream_done(b));
+  return 0;
+}
+
+int qgk_fused_col_forcing(qgk_bucket* b, const float* net_out, int coutp, const double* scale,
+                          double dt1, double dt2, double dt3) {
+  if (!b || !net_out || !scale) return fail(QGK_E_ARG, "NULL argument");
+  if (!b->field_r[QGK_DQ]) return fail(QGK_E_STATE, "dq is not allocated (no q parameterization)");
+  if (b->nz != 2 || b->nx != b->ny || b->nx > kFusedNetMaxN)
+    return fail(QGK_E_ARG, "the fused closed step needs a two-layer square grid of n <= %d",
+                kFusedNetMaxN);
+  if (coutp < b->nz) return fail(QGK_E_ARG, "coutp too small");
+  if (!b->primed || !packed_exchange(b))
+    return fail(QGK_E_STATE, "qgk_fused_row_net must begin the step");
+  double* sc = b->small + 128;
+  int rc = upload_cached(b, scale, b->nz, b->small_out_h, sc);
+  if (rc) return rc;
+  const int np = b->M * b->nz;
+  const size_t n = (size_t)np * b->nplane_r;
+  hipLaunchKernelGGL(k_unpack_forcing, dim3(grid_1d(n)), dim3(256), 0, b->stream, net_out, sc,
+                     b->field_r[QGK_DQ], b->M, b->nz, (int)b->nplane_r, coutp);
+  HIPC(hipGetLastError());
+  if ((rc = r2c2d(b, b->field_r[QGK_DQ], b->field_c[QGK_DQH], np))) return rc;
+  const StepCoeffs cf = {dt1, dt2, dt3};
+  if ((rc = with_ne<ColStepForceL>(b->ny, b->ecol, b, cf))) return rc;
+  HIPC(hipGetLastError());
+  rotate_history(b);
+  std::swap(b->field_c[QGK_QH], b->qh_alt);
+  b->q_stale = true;
+  if (b->capturing) b->ug_fused = true;
   HIPC(stream_done(b));
   return 0;
 }

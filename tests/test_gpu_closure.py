@@ -171,6 +171,71 @@ def test_closed_ensemble_graph_matches_step_by_step(mods):
             np.testing.assert_array_equal(a.ens.get("q", i), b.ens.get("q", i))
 
 
+def _small_closure(cnn, param, seed):
+    net = cnn.CNN(channels=(6, 32, 16, 2), seed=seed)
+    sc = param.Scales(in_scale=[1e-6, 1e-6, 1e-2, 1e-2, 1e-2, 1e-2], out_scale=[1e-12, 1e-12])
+    return param.CNNParameterization(net, sc)
+
+
+def test_fused_closed_step_input_matches_packing(mods):
+    """The fused row kernel writes the network input that qgk_pack_inputs
+    forms from the per-phase real fields: the same BF16 values, but for the
+    rare value whose rounding the transforms' last-bit differences move by one
+    BF16 step."""
+    import torch
+    from pyqg_amd import _lib
+    pq, cnn, _, param, _ = mods
+    p = _small_closure(cnn, param, 3)
+    a = param.ClosedEnsemble(4, 64, 3600., dict(QG_PHYS), param=p, seed=6, fused=True)
+    b = param.ClosedEnsemble(4, 64, 3600., dict(QG_PHYS), param=p, seed=6, fused=False)
+    assert a.fused and not b.fused
+    L = _lib.lib
+    _lib.check(L.qgk_fused_row_net(a.b.h, _lib.ptr(a.inv_in), a.cinp, param.ctypes_p(a.xin)))
+    _lib.check(L.qgk_invert(b.b.h))
+    _lib.check(L.qgk_pack_inputs(b.b.h, _lib.ptr(b.inv_in), b.cinp, param.ctypes_p(b.xin)))
+    torch.cuda.synchronize()
+    xa = a.xin.float().cpu().numpy()
+    xb = b.xin.float().cpu().numpy()
+    assert np.abs(xb[..., :6]).max() > 0
+    assert np.all(xa[..., 6:] == 0) and np.all(xb[..., 6:] == 0)
+    same = np.mean(xa == xb)
+    assert same > 0.999, same
+    step = np.abs(xb) * 2.0 ** -7 + 1e-30  # one BF16 step
+    assert np.all(np.abs(xa - xb) <= step)
+
+
+def test_fused_closed_step_matches_phases(mods):
+    """The fused closed step (network input from the row kernel, forcing added
+    in the column kernel) against the reference's phases one by one with the
+    same network: the same trajectory up to rounding."""
+    pq, cnn, _, param, _ = mods
+    p = _small_closure(cnn, param, 4)
+    a = param.ClosedEnsemble(3, 64, 3600., dict(QG_PHYS), param=p, seed=7, fused=True)
+    b = param.ClosedEnsemble(3, 64, 3600., dict(QG_PHYS), param=p, seed=7, fused=False)
+    for _ in range(6):
+        a._cnn_step()
+        b._cnn_step()
+    for i in range(3):
+        assert rel_l2(a.ens.get("q", i), b.ens.get("q", i)) < 1e-6
+        assert rel_l2(a.ens.get("dqhdt_p", i), b.ens.get("dqhdt_p", i)) < 1e-6
+
+
+def test_fused_closed_step_without_forcing_is_the_plain_step(mods):
+    """With the network's last layer zeroed the forcing is exactly zero, and
+    the fused closed step is the plain fused step."""
+    pq, cnn, _, param, _ = mods
+    p = _small_closure(cnn, param, 5)
+    p.net.w32[-1].zero_()
+    p.net.b32[-1].zero_()
+    p.net.refresh_bf16()
+    a = param.ClosedEnsemble(3, 64, 3600., dict(QG_PHYS), param=p, seed=8, fused=True)
+    b = param.ClosedEnsemble(3, 64, 3600., dict(QG_PHYS), param=None, seed=8)
+    a.step(10)
+    b.step(10)
+    for i in range(3):
+        assert rel_l2(a.ens.get("q", i), b.ens.get("q", i)) < 1e-12
+
+
 def test_trained_network_beats_no_closure(mods):
     """Offline: on held-out pairs the trained network's forcing error is well
     below the no-closure prediction (zero forcing), i.e. R^2 > 0.3."""
