@@ -15,6 +15,8 @@ Other workloads (--workload):
   layered10  64 LayeredModel members, nz = 10, nx = 256, L = 2e6 m,
              exponential stratification, first deformation radius 20 km
   cnn        the learned closure's CNN: forward and training-step throughput
+  closed     the learned closure's coarse step: 128 members at nx=64 with the CNN
+             forcing inside the fused step (step graphs)
   single512  one two-layer QGModel at nx=512, dt=1800 s (the single-GPU parity
              configuration): run() with fused stretches, a _step_forward loop and
              the oracle on one host core; no graphs, so it can run under a tracer
@@ -290,6 +292,44 @@ def bench_cnn(args):
     print(json.dumps(out))
 
 
+def bench_closed(args):
+    """The learned closure's coarse step: 128 two-layer members at nx=64,
+    dt = 4 h, each step the fused row kernel (writing the network input), the
+    8-layer network, and the fused column kernel with the network's forcing,
+    replayed from step graphs.  The network is untrained (seeded weights):
+    the timing does not depend on the values."""
+    import numpy as np
+    import torch
+    from pyqg_amd.closure import cnn as C
+    from pyqg_amd.closure import data as D
+    from pyqg_amd.closure import param as Pm
+    net = C.CNN(seed=0)
+    scales = Pm.Scales(np.full(6, 1e-6), np.full(2, 1e-12))
+    p = Pm.CNNParameterization(net, scales)
+    M, n, dt = 128, 64, 14400.
+    ce = Pm.ClosedEnsemble(M, n, dt, D.QG_EDDY, param=p, seed=0)
+    plain = Pm.ClosedEnsemble(M, n, dt, D.QG_EDDY, param=None, seed=0)
+    out = {}
+    for name, e in (("closed", ce), ("plain", plain)):
+        e.step(max(args.warmup, 4))
+        e.ens.buckets[0].sync()
+        t0 = time.perf_counter()
+        e.step(args.steps)
+        e.ens.buckets[0].sync()
+        out[name] = (time.perf_counter() - t0) * 1e3 / args.steps
+    ms = out["closed"]
+    print(json.dumps({
+        "metric": "learned-closure coarse step (128 members, nx=64, dt=4 h)",
+        "value": M * args.steps * dt / 86400. / (ms * 1e-3 * args.steps),
+        "unit": "model-days/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "vs_baseline": None, "dtype": "f64+bf16",
+        "data": "synthetic (seeded initial PV, seeded untrained network)",
+        "config": {"workload": "ClosedEnsemble: 128 two-layer QG members, nx=64, dt=14400 s, "
+                               "fused step with the 8-layer CNN forcing", "fused": ce.fused,
+                   "plain_ms_per_step": out["plain"],
+                   "us_per_member_day": ms * 1e3 / M * 86400. / dt}}))
+
+
 # ------------------------------------------------------------------ GPU leg
 # The reference's own example runs (docs/examples/parameterizations.ipynb:57-58, 88-89,
 # 137-138, 165-166): five model-years of a QGModel at dt = 3600 s, diagnostics averaged
@@ -392,7 +432,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--members", type=int, default=MEMBERS)
     ap.add_argument("--workload", choices=["qg256", "uneven", "layered10", "cnn", "dropin",
-                                           "single512"],
+                                           "single512", "closed"],
                     default="qg256")
     ap.add_argument("--cnn-library", action="store_true",
                     help="cnn workload: also time each layer with MIOpen (torch conv2d)")
@@ -412,6 +452,8 @@ def main():
 
     if args.workload == "cnn":
         return bench_cnn(args)
+    if args.workload == "closed":
+        return bench_closed(args)
     if args.workload == "dropin":
         return bench_dropin(args)
     if args.workload == "single512":
