@@ -87,6 +87,14 @@ struct StepCoeffs {
   double dt1, dt2, dt3;
 };
 
+// Loads of the member constants through global-address-space pointers: a
+// pointer read from MemberParams is generic, and generic (flat) loads count
+// against both the memory and the LDS wait counters, so every LDS wait of the
+// transforms would also wait for them.
+__device__ __forceinline__ double gld(const double* p, size_t i) {
+  return ((const __attribute__((address_space(1))) double*)p)[i];
+}
+
 // --------------------------------------------------------------------------
 // launch geometry per transform length
 // --------------------------------------------------------------------------
@@ -1120,7 +1128,7 @@ __device__ __forceinline__ void col_invert_points(const cd* __restrict__ qh, cd*
       // tridiagonal and, for kappa > 0, strictly diagonally dominant (S has a
       // negative diagonal and rows that sum to zero), so no pivoting is
       // needed.  ph = 0 at kappa = 0, where the reference zeroes a.
-      const double w = P.k2l2[pt];
+      const double w = gld(P.k2l2, pt);
       if (w == 0.0) {
         for (int z = 0; z < nz; ++z) phm[mbase + z * plane + pt] = cd{0.0, 0.0};
         continue;
@@ -1136,9 +1144,9 @@ __device__ __forceinline__ void col_invert_points(const cd* __restrict__ qh, cd*
       for (int j = 0; j < kTileNzMax; ++j) {
         if (j < nz) {
           const cd qj = qh[mbase + j * plane + pt];
-          const double l = lo[j];
-          const double r = 1.0 / ((dg[j] - w) - l * c_prev);
-          c_prev = hi[j] * r;
+          const double l = gld(lo, j);
+          const double r = 1.0 / ((gld(dg, j) - w) - l * c_prev);
+          c_prev = gld(hi, j) * r;
           d_prev = {(qj.x - l * d_prev.x) * r, (qj.y - l * d_prev.y) * r};
           cp[j] = c_prev;
           dp[j] = d_prev;
@@ -1170,7 +1178,7 @@ __device__ __forceinline__ void col_invert_points(const cd* __restrict__ qh, cd*
 #pragma unroll
         for (int j = 0; j < kTileNzMax; ++j) {
           if (j < nz) {
-            const double a = P.a[((size_t)z * nz + j) * plane + pt];
+            const double a = gld(P.a, ((size_t)z * nz + j) * plane + pt);
             acc.x = acc.x + a * q[j].x;
             acc.y = acc.y + a * q[j].y;
           }
@@ -1182,7 +1190,8 @@ __device__ __forceinline__ void col_invert_points(const cd* __restrict__ qh, cd*
 }
 
 // y IFFT of q, u = -il ph, v = ik ph of layer z from qh and the ph buffer
-template <int NY, int E, int TAG = 0>
+// (WQ false: u and v only, the step kernel having transformed q already)
+template <int NY, int E, int TAG = 0, bool WQ = true>
 __device__ __forceinline__ void col_prologue_layer_ph(const cd* __restrict__ qh,
                                                       const cd* __restrict__ phm,
                                                       cd* __restrict__ Qm, cd* __restrict__ Um,
@@ -1198,17 +1207,19 @@ __device__ __forceinline__ void col_prologue_layer_ph(const cd* __restrict__ qh,
 #pragma unroll
   for (int s = 0; s < G::E; ++s) {
     const size_t pt = (size_t)(t + s * G::T) * ldk + kc;
-    x[s] = ok ? qh[zoff + pt] : cd{0.0, 0.0};
+    if constexpr (WQ) x[s] = ok ? qh[zoff + pt] : cd{0.0, 0.0};
     ph[s] = ok ? phm[zoff + pt] : cd{0.0, 0.0};
   }
-  qgk::fft_block<NY, G::E, +1, G::TK, TAG>(x, lds, t, c, tw);
-  {  // whole tile rows (zeros past nk): full 128-byte lines
+  if constexpr (WQ) {
+    qgk::fft_block<NY, G::E, +1, G::TK, TAG>(x, lds, t, c, tw);
+    {  // whole tile rows (zeros past nk): full 128-byte lines
 #pragma unroll
-    for (int s = 0; s < G::E; ++s) st_stream(Qm + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
+      for (int s = 0; s < G::E; ++s) st_stream(Qm + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
+    }
   }
 #pragma unroll
   for (int s = 0; s < G::E; ++s) {
-    const double ll = P.ll[t + s * G::T];
+    const double ll = gld(P.ll, t + s * G::T);
     x[s] = {ll * ph[s].y, -ll * ph[s].x};
   }
   qgk::fft_block<NY, G::E, +1, G::TK, TAG>(x, lds, t, c, tw);
@@ -1216,7 +1227,7 @@ __device__ __forceinline__ void col_prologue_layer_ph(const cd* __restrict__ qh,
 #pragma unroll
     for (int s = 0; s < G::E; ++s) st_stream(Um + zoff + (size_t)(t + s * G::T) * ldk + k, x[s]);
   }
-  const double kk = P.kk[kc];
+  const double kk = gld(P.kk, kc);
 #pragma unroll
   for (int s = 0; s < G::E; ++s) x[s] = {-kk * ph[s].y, kk * ph[s].x};
   qgk::fft_block<NY, G::E, +1, G::TK, TAG>(x, lds, t, c, tw);
@@ -1266,11 +1277,11 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
   const MemberParams& P = params[m];
   const size_t plane = (size_t)NY * ldk;
   const size_t mbase = (size_t)m * nz * plane;
-  const double kk = P.kk[kc];
+  const double kk = gld(P.kk, kc);
 
   for (int z = 0; z < nz; ++z) {
     const size_t zoff = mbase + (size_t)z * plane;
-    cd xa[G::E], xb[G::E];
+    cd xa[G::E], xb[G::E], qn[G::E];
 #pragma unroll
     for (int s = 0; s < G::E; ++s) {
       const size_t pt = (size_t)(t + s * G::T) * ldk + kc;
@@ -1280,23 +1291,23 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
     qgk::fft_block<NY, G::E, -1, G::TK, kTagColPh>(xa, lds, t, c, tw);
     qgk::fft_block<NY, G::E, -1, G::TK, kTagColPh>(xb, lds, t, c, tw);
     if (ok) {
-      const double kq = kk * P.Qy[z];
+      const double kq = kk * gld(P.Qy, z);
       const bool drag = (z == nz - 1) && P.rek != 0.0;
 #pragma unroll
       for (int s = 0; s < G::E; ++s) {
         const int l = t + s * G::T;
         const size_t pt = (size_t)l * ldk + k;
         const size_t o = zoff + pt;
-        const double ll = P.ll[l];
+        const double ll = gld(P.ll, l);
         const cd S = {-kk * xa[s].y + -ll * xb[s].y, kk * xa[s].x + ll * xb[s].x};
         const cd ph = ld_stream(phm + o);
         cd d = {S.x + -kq * ph.y, S.y + kq * ph.x};
         d = {-d.x, -d.y};
         if (drag) {
-          const double cr = P.rek * P.k2l2[pt];
+          const double cr = P.rek * gld(P.k2l2, pt);
           d = {d.x + cr * ph.x, d.y + cr * ph.y};
         }
-        const double f = P.filtr[pt];
+        const double f = gld(P.filtr, pt);
         const cd q = ld_stream(qh_old + o), pv = ld_stream(p + o), ppv = ld_stream(pp_to_newp + o);
         double rx = q.x + cf.dt1 * d.x;
         double ry = q.y + cf.dt1 * d.y;
@@ -1304,7 +1315,8 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
         ry = ry + cf.dt2 * pv.y;
         rx = rx + cf.dt3 * ppv.x;
         ry = ry + cf.dt3 * ppv.y;
-        qh_new[o] = {f * rx, f * ry};  // read back by the inversion below
+        qn[s] = {f * rx, f * ry};
+        qh_new[o] = qn[s];  // read back by the inversion below
         st_stream(pp_to_newp + o, d);
       }
     } else {
@@ -1312,27 +1324,26 @@ __global__ __launch_bounds__((FGeo<NY, E>::T * FGeo<NY, E>::TK)) void k_col_step
 #pragma unroll
       for (int s = 0; s < G::E; ++s) {
         const size_t o = zoff + (size_t)(t + s * G::T) * ldk + k;
+        qn[s] = cd{0.0, 0.0};
         qh_new[o] = cd{0.0, 0.0};
         st_stream(pp_to_newp + o, cd{0.0, 0.0});
       }
     }
+    // the next step's Qm of this layer from the new qh still in registers
+    // (the prologue below then transforms u and v only)
+    qgk::fft_block<NY, G::E, +1, G::TK, kTagColPh>(qn, lds, t, c, tw);
+#pragma unroll
+    for (int s = 0; s < G::E; ++s) st_stream(Qm + zoff + (size_t)(t + s * G::T) * ldk + k, qn[s]);
   }
   col_invert_points<NY, E>(qh_new, phm, P, nz, t, k, ok, mbase, ldk);
   for (int z = 0; z < nz; ++z)
-    col_prologue_layer_ph<NY, E, kTagColPh>(qh_new, phm, Qm, Um, Vm, P, z, ldk, t, c, k, ok, mbase, lds,
-                                 tw);
+    col_prologue_layer_ph<NY, E, kTagColPh, false>(qh_new, phm, Qm, Um, Vm, P, z, ldk, t, c, k,
+                                                   ok, mbase, lds, tw);
 }
 
 // ---- nz <= 2: every field is read exactly once; the layers' qh and ph at the
 // thread's points stay in registers.
 
-// Loads of the member constants through global-address-space pointers: a
-// pointer read from MemberParams is generic, and generic (flat) loads count
-// against both the memory and the LDS wait counters, so every LDS wait of the
-// transforms would also wait for them.
-__device__ __forceinline__ double gld(const double* p, size_t i) {
-  return ((const __attribute__((address_space(1))) double*)p)[i];
-}
 
 // ph[z] = sum_k1 a[z,k1] q[k1] with the reference's summation order
 // KW: kappa^2 at the point is given (wk) instead of read from the k2l2 table
