@@ -82,6 +82,25 @@ def test_conv_forward_narrow_layers(cnn, cinp, coutp, H, relu, masked):
     assert torch.equal(y_bf.cpu(), y32.cpu().to(torch.bfloat16))
 
 
+@pytest.mark.parametrize("cinp,coutp,B", [(64, 64, 36), (32, 128, 20), (128, 32, 40)])
+def test_conv_forward_blocks_with_several_items(cnn, cinp, coutp, B):
+    """Batches whose (sample, 8-row group, channel group) items outnumber the
+    compute units, so that the resident blocks of the overlapped-staging kernel
+    take several items in turn (accumulators cleared between items, staging
+    buffers alternating across items and input chunks)."""
+    import torch
+    g = torch.Generator().manual_seed(cinp + 3 * coutp + B)
+    H, W = 64, 64
+    x = rnd_bf16((B, H, W, cinp), 1.0, g)
+    w = rnd_bf16((coutp, 25, cinp), 0.1, g)
+    b = torch.randn(coutp, generator=g) * 0.1
+    y_bf, y32 = cnn.conv_fwd(x.to(torch.bfloat16).cuda(), w.to(torch.bfloat16).cuda(), b.cuda(),
+                             coutp, relu=True, out_f32=True)
+    ref = torch.relu(ref_conv(x, w, b))
+    assert rel(y32.cpu(), ref) < 1e-5
+    assert torch.equal(y_bf.cpu(), y32.cpu().to(torch.bfloat16))
+
+
 @pytest.mark.parametrize("cin,coutp,H,relu", [(6, 128, 8, True), (8, 64, 16, True),
                                               (3, 32, 12, False), (6, 16, 5, True)])
 def test_first_layer_kernel(cnn, cin, coutp, H, relu):
