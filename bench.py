@@ -255,6 +255,19 @@ def bench_cnn(args):
         ms_train = e0.elapsed_time(e1) / nt
         train_tflops = 3 * Bt * fl / (ms_train * 1e-3) / 1e12
     library = cnn_library_reference(net.channels, B, args.steps) if args.cnn_library else None
+
+    def layer_kernel(l):
+        # the library's dispatch: first- and last-layer kernels where packed;
+        # 64-channel output groups over 32-channel input chunks on the
+        # overlapped-staging kernel (64-row images); the per-block kernel else
+        if l == 0 and net.first_packed:
+            return "k_conv_first<64, 4, 2>"
+        if l == net.nlayers - 1 and net.last_packed:
+            return "k_conv_last<64, 1>"
+        if net.chp[l] % 32 == 0 and net.chp[l + 1] % 64 == 0:
+            return "k_conv_fwd_ws<64, 4, 2, 4>"
+        mt = 4 if net.chp[l + 1] % 64 == 0 else (2 if net.chp[l + 1] % 32 == 0 else 1)
+        return f"k_conv_fwd_mt<64, {mt}, {32 if net.chp[l] % 32 == 0 else 16}, 2>"
     out = {
         "metric": "CNN subgrid-closure inference throughput (64x64 inputs, 8 conv layers)",
         "value": B * args.steps / wall,
@@ -273,11 +286,8 @@ def bench_cnn(args):
                    "batch": B},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": CNN_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / CNN_PEAK_TFLOPS, "traffic": None,
-                     "layer_kernels": ["k_conv_first" if (l == 0 and net.first_packed) else
-                                       "k_conv_last" if (l == net.nlayers - 1 and
-                                                         net.last_packed) else "k_conv_fwd_mt"
-                                       for l in range(net.nlayers)],
-                     "kernel": (f"k_conv_fwd_mt<64, 4, 32>: layer {dom + 1} "
+                     "layer_kernels": [layer_kernel(l) for l in range(net.nlayers)],
+                     "kernel": (f"{layer_kernel(dom)}: layer {dom + 1} "
                                 f"({net.channels[dom]}->{net.channels[dom + 1]} channels), "
                                 f"batch {B}"),
                      "algorithmic_flops_per_launch": layer_fl[dom],

@@ -43,6 +43,9 @@ int qgcnn_conv_first(const void* x, int B, int H, int W, int cinp, const void* w
  * output channel as rows of the matrix tile.  wl is w repacked by
  * qgcnn_pack_last: [16][5][cinp] BF16 (device, 160 * cinp bytes). */
 int qgcnn_pack_last(const void* w, int cinp, int coutp, int cout, void* wl, void* stream);
+/* The output may be narrower than the other layers' (coutp a multiple of 4,
+ * at least cout): with coutp = 4 the FP32 output of a 2-channel last layer
+ * writes 16 bytes per pixel instead of 64. */
 int qgcnn_conv_last(const void* x, int B, int H, int W, int cinp, const void* wl,
                     const float* bias, int coutp, int cout, void* y, float* y32, void* stream);
 /* dw[co][tap][ci] += sum_px dy[px][co] x[px + tap][ci] (FP32 accumulation) */

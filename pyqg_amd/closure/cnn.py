@@ -253,6 +253,8 @@ class CNN:
             return conv_first(h, self.w_first, self.b32[0], coutp, relu=relu, y=y, y32=y32,
                               out_bf16=out_bf16, out_f32=out_f32, stream=stream)
         if l == self.nlayers - 1 and self.last_packed and not relu:
+            if y32 is not None:  # a preallocated output may be narrower (out_width)
+                coutp = y32.shape[-1]
             return conv_last(h, self.w_last, self.b32[l], coutp, self.channels[-1], y=y, y32=y32,
                              out_bf16=out_bf16, out_f32=out_f32, stream=stream)
         return conv_fwd(h, self.wb[l], self.b32[l], coutp, relu=relu, out_bf16=out_bf16,
@@ -283,10 +285,16 @@ class CNN:
                     acts.append(y)
         return (out, acts) if keep else out
 
+    def out_width(self):
+        """Channels of forward_into's FP32 output: the last-layer kernel writes
+        4 (of which channels[-1] are real), any other kernel the padded count."""
+        return 4 if self.last_packed and self.channels[-1] <= 4 else self.chp[-1]
+
     def activation_buffers(self, B, H, W):
         """Persistent outputs of every layer for forward_into: BF16 for the
-        hidden layers, FP32 for the last."""
-        return [torch.empty((B, H, W, self.chp[l + 1]),
+        hidden layers, FP32 for the last (out_width channels)."""
+        return [torch.empty((B, H, W, self.out_width() if l == self.nlayers - 1
+                             else self.chp[l + 1]),
                             dtype=torch.float32 if l == self.nlayers - 1 else torch.bfloat16,
                             device=self.device) for l in range(self.nlayers)]
 

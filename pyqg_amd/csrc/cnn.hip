@@ -1247,9 +1247,12 @@ int qgcnn_pack_last(const void* w, int cinp, int coutp, int cout, void* wl, void
 int qgcnn_conv_last(const void* x, int B, int H, int W, int cinp, const void* wl,
                     const float* bias, int coutp, int cout, void* y, float* y32, void* stream) {
   if (!x || !wl || (!y && !y32)) return fail(QGCNN_E_ARG, "NULL argument");
-  if (!mult16(cinp) || cinp % 32 != 0 || !mult16(coutp) || cout < 1 || KS * cout > 16 ||
-      cout > coutp)
-    return fail(QGCNN_E_ARG, "the last-layer kernel needs cinp a multiple of 32 and 5*cout <= 16");
+  if (!mult16(cinp) || cinp % 32 != 0 || coutp % 4 != 0 || coutp < 4 || coutp > 128 ||
+      cout < 1 || KS * cout > 16 || cout > coutp)
+    return fail(QGCNN_E_ARG, "the last-layer kernel needs cinp a multiple of 32, 5*cout <= 16 "
+                             "and coutp a multiple of 4 in [4, 128]");
+  if (!all_aligned16({x, wl, bias, y32}) || ((uintptr_t)y & 7u))
+    return fail(QGCNN_E_ARG, "tensors must start on 16-byte boundaries (y on 8-byte ones)");
   if (B < 1 || H < KS || W != 64)
     return fail(QGCNN_E_ARG, "images must be W=64 wide and at least 5 rows (got %dx%d)", H, W);
   dim3 grid((H + 3) / 4, B);

@@ -126,23 +126,24 @@ def test_first_layer_kernel(cnn, cin, coutp, H, relu):
     assert torch.equal(y_bf.cpu(), y32.cpu().to(torch.bfloat16))
 
 
-@pytest.mark.parametrize("cinp,cout,H", [(32, 2, 8), (64, 3, 12), (32, 1, 5), (128, 2, 64)])
-def test_last_layer_kernel(cnn, cinp, cout, H):
+@pytest.mark.parametrize("cinp,cout,H,coutp", [(32, 2, 8, 16), (64, 3, 12, 16), (32, 1, 5, 16),
+                                               (128, 2, 64, 16), (32, 2, 8, 4), (64, 3, 16, 4)])
+def test_last_layer_kernel(cnn, cinp, cout, H, coutp):
     """The last-layer kernel (horizontal taps as matrix rows, the horizontal sum
     through LDS) against the float reference; the padded output channels carry
-    their bias."""
+    their bias.  coutp = 4: the narrow output of the closed step."""
     import torch
     g = torch.Generator().manual_seed(cinp + 10 * cout + H)
-    B, W, coutp = 3, 64, 16
+    B, W = 3, 64
     x = rnd_bf16((B, H, W, cinp), 1.0, g)
-    w = torch.zeros((coutp, 25, cinp))
+    w = torch.zeros((16, 25, cinp))     # the layer's weights: 16 padded output channels
     w[:cout] = rnd_bf16((cout, 25, cinp), 0.1, g)
-    b = torch.zeros(coutp)
+    b = torch.zeros(16)
     b[:cout] = torch.randn(cout, generator=g) * 0.1
     wl = cnn.pack_last(w.to(torch.bfloat16).cuda(), cout)
     y_bf, y32 = cnn.conv_last(x.to(torch.bfloat16).cuda(), wl, b.cuda(), coutp, cout,
                               out_bf16=True, out_f32=True)
-    ref = ref_conv(x, w, b)
+    ref = ref_conv(x, w, b)[..., :coutp]
     assert rel(y32.cpu(), ref) < 1e-5
     assert rel(y_bf.float().cpu(), ref) < 1e-2
     assert torch.equal(y_bf.cpu(), y32.cpu().to(torch.bfloat16))
