@@ -340,34 +340,41 @@ __global__ __launch_bounds__(256, RW) void k_conv_fwd_mt(const bf16_t* __restric
 }
 
 // --------------------------------------------------------------------------
-// Forward convolution with overlapped staging, for 64-output-channel groups
-// (the K=32 form, 64-pixel rows, heights a multiple of 8).  k_conv_fwd_mt
+// Forward convolution with overlapped staging (the K=32 form, 64-pixel rows,
+// heights a multiple of 8, 32- or 64-channel output groups).  k_conv_fwd_mt
 // stages a chunk's input rows, waits, and only then runs the chunk's matrix
 // ops, so each block alternates between a memory phase and a compute phase.
-// Here one block per CU is resident for the whole layer and walks its share of
-// the work items (sample, 8-row group, 64-channel group), each as cinp/32
+// Here the blocks stay resident for the whole layer, each walking its share of
+// the work items (sample, 8-row group, output-channel group) as cinp/32
 // stages.  Warp roles:
-//   waves 0-3 (compute): the matrix ops on the stage in LDS buffer s%2, the
-//     next tap's weight operands and the next row's LDS operands loaded while
-//     the current ones feed the matrix cores;
-//   waves 4..4+NLW-1 (loaders): read stage s+1's input rows from HBM and write
-//     them to the other LDS buffer meanwhile.
-// One barrier per stage: the loaders write stage s into buffer s%2 and arrive
-// at barrier s; the compute waves pass barrier s and then read buffer s%2.  A
-// loader's write of stage s+1 (after barrier s) goes to buffer (s+1)%2, whose
-// last readers (stage s-1) all arrived at barrier s before that.  The roles
-// have separate memory counters, so a compute wave's weight loads never wait
-// for the staging loads.
+//   waves 0-3 (compute): the matrix ops on the stage in LDS, the next tap's
+//     weight operands and the next row's LDS operands loaded while the current
+//     ones feed the matrix cores.  With four weight tiles (64 channels) the
+//     waves pair up, two tiles x four rows each; with two, each wave takes
+//     both tiles for two rows;
+//   waves 4..4+NLW-1 (loaders): read the next stage's input rows from HBM.
+// DB (four weight tiles: one block per CU, two LDS buffers): the loaders write
+// stage s into buffer s%2 and arrive at barrier s; the compute waves pass
+// barrier s and then read buffer s%2.  A loader's write of stage s+1 (after
+// barrier s) goes to buffer (s+1)%2, whose last readers (stage s-1) all
+// arrived at barrier s before that.  One buffer (two weight tiles: two blocks
+// per CU, so that each SIMD also interleaves two compute waves): the loaders
+// hold the next stage in registers; per stage, barrier A (the compute waves
+// are done with the previous stage), the write, barrier B, then the loads of
+// the following stage, which overlap this stage's matrix ops.
+// The roles have separate memory counters, so a compute wave's weight loads
+// never wait for the staging loads.
 // Measured at 128 -> 64 channels, batch 128 (variants timed outside the
 // library): the kernel ~210 us (k_conv_fwd_mt ~255 us); 43 us with the matrix
 // ops removed; 134 us with the staging removed (LDS never written); ~170 us
 // with pseudo-random data written to LDS each stage and no HBM loads.  The
 // matrix ops run slower on operand data that toggles (a matrix-op-only loop:
 // 1.39 PFLOP/s on random operands, 1.56 on constant ones) and slower again
-// beside the HBM traffic of the staging.
+// beside the HBM traffic of the staging.  At 32 -> 32 channels: 30 us against
+// 33 for k_conv_fwd_mt (the DB form with one compute wave per SIMD: 37-42).
 // --------------------------------------------------------------------------
-template <int W, int MT, int RW, int NLW>
-__global__ __launch_bounds__(256 + 64 * NLW, 1) void k_conv_fwd_ws(
+template <int W, int MT, int RW, int NLW, bool DB>
+__global__ __launch_bounds__(256 + 64 * NLW, DB ? 1 : 2) void k_conv_fwd_ws(
     const bf16_t* __restrict__ x, const bf16_t* __restrict__ w, const float* __restrict__ bias,
     int B, int H, int cinp, int coutp, int relu, const bf16_t* __restrict__ mask,
     bf16_t* __restrict__ y, float* __restrict__ y32) {
@@ -388,7 +395,7 @@ __global__ __launch_bounds__(256 + 64 * NLW, 1) void k_conv_fwd_ws(
   // per-read wrap-around arithmetic)
   constexpr int WP = W + 4;
   constexpr int BUF = SR * PIECES * WP * 8;
-  __shared__ __attribute__((aligned(16))) bf16_t lds[2 * BUF];
+  __shared__ __attribute__((aligned(16))) bf16_t lds[(DB ? 2 : 1) * BUF];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int nrg = H / ROWS, nch = cinp / KC, ncog = coutp / (16 * MT);
   const int items = B * nrg * ncog;
@@ -400,39 +407,73 @@ __global__ __launch_bounds__(256 + 64 * NLW, 1) void k_conv_fwd_ws(
     // covers 16 pixels x 4 K groups (1 KB of global memory) and writes 16
     // consecutive entries of each K group's plane.  The row offsets are
     // wave-uniform (scalar registers); the lane's part is fixed.
+    // With DB, stage s is loaded and written to buffer s%2, then barrier s.
+    // Without (one buffer, two blocks per CU): the loaders hold the next
+    // stage in registers; per stage, barrier A (the compute waves are done
+    // with the previous stage), the write, barrier B, then the loads of the
+    // stage after it, which overlap this stage's matrix ops.
     constexpr int NL = 64 * NLW;
-    static_assert(NL == PIECES * W, "one staged row per loader pass");
+    constexpr int K = PIECES * W / NL;  // loader passes per staged row
+    static_assert(K * NL == PIECES * W, "whole rows per loader passes");
     const int lt = threadIdx.x - 256;
-    const int pix = ((lt / 64) % (W / 16)) * 16 + lt % 16, pc = (lt / 16) % PIECES;
     const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc(
         (void*)x, (short)0, B * H * W * cinp * (int)sizeof(bf16_t), 0x00020000);
-    const int xlane = (pix * cinp + pc * 8) * (int)sizeof(bf16_t);
-    bf16_t* lw = lds + (pc * WP + pix + 2) * 8;
-    // the halo copies: pixels W-2, W-1 also left of pixel 0, pixels 0, 1 also
-    // right of pixel W-1
-    const int halo = pix >= W - 2 ? -W : (pix < 2 ? W : 0);
-    for (int s = 0; s < S; ++s) {
-      const int it = blockIdx.x + (s / nch) * gridDim.x;
-      const int ci0 = (s % nch) * KC;
+    int xlane[K], halo[K];
+    bf16_t* lw[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+      const int qq = lt + NL * j;
+      const int pix = ((qq / 64) % (W / 16)) * 16 + qq % 16, pc = (qq / 16) % PIECES;
+      xlane[j] = (pix * cinp + pc * 8) * (int)sizeof(bf16_t);
+      lw[j] = lds + (pc * WP + pix + 2) * 8;
+      // the halo copies: pixels W-2, W-1 also left of pixel 0, pixels 0, 1
+      // also right of pixel W-1
+      halo[j] = pix >= W - 2 ? -W : (pix < 2 ? W : 0);
+    }
+    uint4 v[SR * K];
+    auto load = [&](int st) {
+      const int it = blockIdx.x + (st / nch) * gridDim.x;
+      const int ci0 = (st % nch) * KC;
       const int rest = it / ncog;
       const int b = rest / nrg, row0 = (rest % nrg) * ROWS;
-      uint4 v[SR];
 #pragma unroll
       for (int r = 0; r < SR; ++r) {
         const int yy = (row0 + r - 2 + H) % H;
-        v[r] = __builtin_bit_cast(
-            uint4, __builtin_amdgcn_raw_buffer_load_b128(
-                       xrs, xlane, ((b * H + yy) * W * cinp + ci0) * (int)sizeof(bf16_t), 0));
-      }
-      bf16_t* dst = lw + (s & 1) * BUF;
+        const int so = ((b * H + yy) * W * cinp + ci0) * (int)sizeof(bf16_t);
 #pragma unroll
-      for (int r = 0; r < SR; ++r) *reinterpret_cast<uint4*>(dst + r * PIECES * WP * 8) = v[r];
-      if (halo) {
+        for (int j = 0; j < K; ++j)
+          v[r * K + j] = __builtin_bit_cast(
+              uint4, __builtin_amdgcn_raw_buffer_load_b128(xrs, xlane[j], so, 0));
+      }
+    };
+    auto write = [&](int boff) {
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        bf16_t* dst = lw[j] + boff;
 #pragma unroll
         for (int r = 0; r < SR; ++r)
-          *reinterpret_cast<uint4*>(dst + (r * PIECES * WP + halo) * 8) = v[r];
+          *reinterpret_cast<uint4*>(dst + r * PIECES * WP * 8) = v[r * K + j];
+        if (halo[j]) {
+#pragma unroll
+          for (int r = 0; r < SR; ++r)
+            *reinterpret_cast<uint4*>(dst + (r * PIECES * WP + halo[j]) * 8) = v[r * K + j];
+        }
       }
-      __syncthreads();
+    };
+    if constexpr (DB) {
+      for (int s = 0; s < S; ++s) {
+        load(s);
+        write((s & 1) * BUF);
+        __syncthreads();
+      }
+    } else {
+      if (S > 0) load(0);
+      for (int s = 0; s < S; ++s) {
+        __syncthreads();  // A
+        write(0);
+        __syncthreads();  // B
+        if (s + 1 < S) load(s + 1);
+      }
     }
     return;
   }
@@ -444,11 +485,13 @@ __global__ __launch_bounds__(256 + 64 * NLW, 1) void k_conv_fwd_ws(
   // every wave loading all four, the weight loads, 16 lines each, kept the
   // address unit stalled on the L1 for ~45 % of the kernel) and each LDS
   // operand feeds two matrix ops.
-  static_assert(MT == 4 && RW == 2, "paired waves over four weight tiles");
-  constexpr int MTW = MT / 2;  // weight tiles per wave
-  constexpr int RWW = 2 * RW;  // output rows per wave
-  const int mo = (wave & 1) * MTW;     // the wave's first tile
-  const int wr0 = (wave >> 1) * RWW;   // the wave's first row
+  // (with two weight tiles each wave takes both for RW rows)
+  static_assert(RW == 2, "the LDS operand ring alternates between the wave's rows");
+  constexpr bool PAIR = MT >= 4;
+  constexpr int MTW = PAIR ? MT / 2 : MT;  // weight tiles per wave
+  constexpr int RWW = PAIR ? 2 * RW : RW;  // output rows per wave
+  const int mo = PAIR ? (wave & 1) * MTW : 0;             // the wave's first tile
+  const int wr0 = PAIR ? (wave >> 1) * RWW : wave * RWW;  // the wave's first row
   // Operand pipeline of a compute wave, one step per (tap, output row): the
   // weights are loaded PD taps ahead (a ring of PD + 1 register sets; an L2
   // hit takes longer than one tap's matrix ops; the last taps of a stage load
@@ -490,12 +533,13 @@ __global__ __launch_bounds__(256 + 64 * NLW, 1) void k_conv_fwd_ws(
   }
   for (int s = 0; s < S; ++s) {
     __syncthreads();
+    if constexpr (!DB) __syncthreads();  // B: the loaders' write of stage s is done
     const int it = blockIdx.x + (s / nch) * gridDim.x;
     const int ch = s % nch;
     const int cog = it % ncog, rest = it / ncog;
     const int b = rest / nrg, row0 = (rest % nrg) * ROWS;
     const int co0 = cog * 16 * MT;
-    const bf16_t* buf = lds + (s & 1) * BUF;
+    const bf16_t* buf = lds + (DB ? (s & 1) * BUF : 0);
     const int wr = stage_w(s);
     const bool more = s + 1 < S;
     const int wn = more ? stage_w(s + 1) : wr;
@@ -1162,14 +1206,26 @@ int qgcnn_conv_fwd(const void* x, int B, int H, int W, int cinp, const void* w,
   if (cinp % 32 == 0 && H % 8 == 0 && mt == 4 &&
       (size_t)B * H * W * (cinp > coutp ? cinp : coutp) * sizeof(bf16_t) < (1u << 31)) {
     // 64-channel output groups: overlapped staging, one resident block per CU
-    // over the layer's work items (measured 15-20 % faster at 128 -> 64
-    // channels; with fewer output channels the per-block kernel below is as
-    // fast or faster)
+    // over the layer's work items (128 -> 64 channels: 189 against 235 us)
     const int items = B * (H / 8) * (coutp / 64);
     const int grid = items < num_cus() ? items : num_cus();
-    hipLaunchKernelGGL((k_conv_fwd_ws<64, 4, 2, kLoaderWaves>), dim3(grid),
+    hipLaunchKernelGGL((k_conv_fwd_ws<64, 4, 2, kLoaderWaves, true>), dim3(grid),
                        dim3(256 + 64 * kLoaderWaves), 0, st, (const bf16_t*)x, (const bf16_t*)w,
                        bias, B, H, cinp, coutp, relu, (const bf16_t*)mask, (bf16_t*)y, y32);
+    HIPC(hipGetLastError());
+    return 0;
+  }
+  if (cinp % 32 == 0 && H % 8 == 0 && mt == 2 &&
+      (size_t)B * H * W * (cinp > coutp ? cinp : coutp) * sizeof(bf16_t) < (1u << 31)) {
+    // 32-channel output groups: overlapped staging with two resident blocks
+    // per CU, each with one LDS stage (51 KB; 4 KB of unused dynamic LDS keep
+    // a third block off the CU, so that the grid of two blocks per CU spreads
+    // evenly).  32 -> 32 channels 30 against 33 us, 64 -> 32 74 against 81.
+    const int items = B * (H / 8) * (coutp / 32);
+    const int grid = items < 2 * num_cus() ? items : 2 * num_cus();
+    hipLaunchKernelGGL((k_conv_fwd_ws<64, 2, 2, 4, false>), dim3(grid), dim3(256 + 64 * 4), 4096,
+                       st, (const bf16_t*)x, (const bf16_t*)w, bias, B, H, cinp, coutp, relu,
+                       (const bf16_t*)mask, (bf16_t*)y, y32);
     HIPC(hipGetLastError());
     return 0;
   }

@@ -82,12 +82,14 @@ def test_conv_forward_narrow_layers(cnn, cinp, coutp, H, relu, masked):
     assert torch.equal(y_bf.cpu(), y32.cpu().to(torch.bfloat16))
 
 
-@pytest.mark.parametrize("cinp,coutp,B", [(64, 64, 36), (32, 128, 20), (128, 32, 40)])
+@pytest.mark.parametrize("cinp,coutp,B", [(64, 64, 36), (32, 128, 20), (64, 32, 72),
+                                           (32, 32, 70), (96, 32, 66)])
 def test_conv_forward_blocks_with_several_items(cnn, cinp, coutp, B):
     """Batches whose (sample, 8-row group, channel group) items outnumber the
-    compute units, so that the resident blocks of the overlapped-staging kernel
-    take several items in turn (accumulators cleared between items, staging
-    buffers alternating across items and input chunks)."""
+    resident blocks of the overlapped-staging kernel (one per CU for 64-channel
+    groups, two for 32-channel ones), so that each takes several items in turn
+    (accumulators cleared between items, staging buffers reused across items
+    and input chunks)."""
     import torch
     g = torch.Generator().manual_seed(cinp + 3 * coutp + B)
     H, W = 64, 64
