@@ -258,14 +258,16 @@ def bench_cnn(args):
 
     def layer_kernel(l):
         # the library's dispatch: first- and last-layer kernels where packed;
-        # 64-channel output groups over 32-channel input chunks on the
+        # 64- and 32-channel output groups over 32-channel input chunks on the
         # overlapped-staging kernel (64-row images); the per-block kernel else
         if l == 0 and net.first_packed:
             return "k_conv_first<64, 4, 2>"
         if l == net.nlayers - 1 and net.last_packed:
             return "k_conv_last<64, 1>"
         if net.chp[l] % 32 == 0 and net.chp[l + 1] % 64 == 0:
-            return "k_conv_fwd_ws<64, 4, 2, 4>"
+            return "k_conv_fwd_ws<64, 4, 2, 4, true>"
+        if net.chp[l] % 32 == 0 and net.chp[l + 1] % 32 == 0:
+            return "k_conv_fwd_ws<64, 2, 2, 4, false>"
         mt = 4 if net.chp[l + 1] % 64 == 0 else (2 if net.chp[l + 1] % 32 == 0 else 1)
         return f"k_conv_fwd_mt<64, {mt}, {32 if net.chp[l] % 32 == 0 else 16}, 2>"
     out = {
