@@ -709,6 +709,153 @@ __global__ __launch_bounds__(256, RW) void k_conv_first(const bf16_t* __restrict
   }
 }
 
+// First layer with overlapped staging (64-output-channel groups, 64-pixel
+// rows, heights a multiple of 8): one resident block per CU walks its share of
+// the (sample, 8-row group) items of one output-channel group, whose packed
+// weights (4 tiles x 7 matrix steps x 64 lanes x 16 bytes = 28 KB) stay in LDS
+// for the whole layer.  Four loader waves stage the next item's 12 input rows
+// (8 channels, the periodic halo stored twice: [row][pixel - 2 .. 65]) into one
+// of two LDS buffers while the four compute waves run the current item and
+// store its outputs; the compute waves load nothing from global memory, so
+// their stores drain while the next item's matrix ops run.  Operands are
+// read one (step, row) ahead from LDS.  K layout as k_conv_first.
+template <int W, int NLW>
+__global__ __launch_bounds__(256 + 64 * NLW, 1) void k_conv_first_ws(
+    const bf16_t* __restrict__ x, const bf16_t* __restrict__ w8, const float* __restrict__ bias,
+    int B, int H, int cinp, int coutp, int relu, bf16_t* __restrict__ y,
+    float* __restrict__ y32) {
+  typedef v8bf op;
+  constexpr int MT = 4, RW = 2, NT = W / 16;
+  constexpr int ROWS = 4 * RW, SR = ROWS + KS - 1, CH = kFirstCh;
+  constexpr int WP = W + 4;
+  constexpr int BUF = SR * WP * CH;                // one input stage (bf16 elements)
+  constexpr int NSTEP = kFirstTaps / 4;            // K=32 matrix steps
+  constexpr int WSZ = MT * NSTEP * 64 * 8;         // the group's packed weights
+  __shared__ __attribute__((aligned(16))) bf16_t lds[WSZ + 2 * BUF];
+  bf16_t* const lw = lds;
+  bf16_t* const lin = lds + WSZ;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int nrg = H / ROWS, ncog = coutp / (16 * MT);
+  const int items = B * nrg * ncog;
+  // the grid is a multiple of ncog, so every item of a block has its group
+  const int cog = (int)blockIdx.x % ncog, co0 = cog * 16 * MT;
+  const int mine = (int)blockIdx.x < items ? (items - 1 - (int)blockIdx.x) / (int)gridDim.x + 1 : 0;
+  {  // the group's weights in operand order: entry ((m * NSTEP + c) * 64 + lane)
+    for (int e = threadIdx.x; e < MT * NSTEP * 64; e += blockDim.x) {
+      const int ln = e % 64, c = (e / 64) % NSTEP, m = e / (64 * NSTEP);
+      const int li = ln % 16, lg = ln / 16;
+      *reinterpret_cast<uint4*>(lw + e * 8) = *reinterpret_cast<const uint4*>(
+          w8 + ((size_t)tile_channel<MT>(co0, m, li) * kFirstTaps + 4 * c + lg) * CH);
+    }
+  }
+  if (wave >= 4) {  // loader waves: stage of item k into buffer k%2, then barrier k
+    constexpr int NL = 64 * NLW;
+    constexpr int TOTAL = SR * W;
+    constexpr int NQ = (TOTAL + NL - 1) / NL;
+    const int lt = threadIdx.x - 256;
+    for (int k = 0; k < mine; ++k) {
+      const int it = blockIdx.x + k * gridDim.x;
+      const int rest = it / ncog;
+      const int b = rest / nrg, row0 = (rest % nrg) * ROWS;
+      uint4 v[NQ];
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) {
+        const int q = lt + NL * i;
+        if (TOTAL % NL == 0 || q < TOTAL) {
+          const int pix = q % W, r = q / W;
+          const int yy = (row0 + r - 2 + H) % H;
+          v[i] = *reinterpret_cast<const uint4*>(x + (((size_t)b * H + yy) * W + pix) * cinp);
+        }
+      }
+      bf16_t* buf = lin + (k & 1) * BUF;
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) {
+        const int q = lt + NL * i;
+        if (TOTAL % NL == 0 || q < TOTAL) {
+          const int pix = q % W, r = q / W;
+          bf16_t* dst = buf + (r * WP + pix + 2) * CH;
+          *reinterpret_cast<uint4*>(dst) = v[i];
+          const int halo = pix >= W - 2 ? -W : (pix < 2 ? W : 0);
+          if (halo) *reinterpret_cast<uint4*>(dst + halo * CH) = v[i];
+        }
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  const int li = lane % 16, lg = lane / 16;
+  const int wr0 = wave * RW;
+  // per matrix step c the lane's tap 4c + lg (taps 25-27 have zero weights;
+  // their operand reads stay on tap 24)
+  int boff[NSTEP];
+#pragma unroll
+  for (int c = 0; c < NSTEP; ++c) {
+    const int tap = 4 * c + lg < NTAP ? 4 * c + lg : NTAP - 1;
+    boff[c] = ((wr0 + tap / KS) * WP + li + tap % KS) * CH;
+  }
+  const int cb = tile_channel<MT>(co0, 0, 4 * lg);
+  float bvv[4 * MT];
+  load_bias<MT>(bias, cb, bvv);
+  v4f acc[MT][RW][NT];
+  op a[2][MT];
+  op bq[2][NT];
+  for (int k = 0; k < mine; ++k) {
+    __syncthreads();
+    const int it = blockIdx.x + k * gridDim.x;
+    const int rest = it / ncog;
+    const int b = rest / nrg, row0 = (rest % nrg) * ROWS;
+    const bf16_t* buf = lin + (k & 1) * BUF;
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int rr = 0; rr < RW; ++rr)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[m][rr][t] = v4f{0.f, 0.f, 0.f, 0.f};
+    auto read_a = [&](op (&av)[MT], int c) {
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+        av[m] = *reinterpret_cast<const op*>(lw + ((m * NSTEP + c) * 64 + lane) * 8);
+    };
+    auto read_b = [&](op (&bv)[NT], int c, int rr) {
+      const bf16_t* p = buf + boff[c] + rr * WP * CH;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const op*>(p + t * 16 * CH);
+    };
+    read_a(a[0], 0);
+    read_b(bq[0], 0, 0);
+    static_for<0, NSTEP * RW>([&](auto ic) {
+      constexpr int idx = decltype(ic)::value;
+      constexpr int c = idx / RW, rr = idx % RW;
+      constexpr bool lda = rr == 0 && c + 1 < NSTEP;
+      constexpr bool ldb = idx + 1 < NSTEP * RW;
+      if constexpr (lda) read_a(a[(c + 1) & 1], c + 1);
+      if constexpr (ldb) read_b(bq[(idx + 1) & 1], (idx + 1) / RW, (idx + 1) % RW);
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+          acc[m][rr][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[c & 1][m], bq[idx & 1][t],
+                                                                  acc[m][rr][t], 0, 0, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, (lda ? MT : 0) + (ldb ? NT : 0), 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, MT * NT, 0);
+    });
+#pragma unroll
+    for (int rr = 0; rr < RW; ++rr) {
+      const int row = row0 + wr0 + rr;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const size_t o = (((size_t)b * H + row) * W + t * 16 + li) * coutp + cb;
+        float v[4 * MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) v[4 * m + j] = acc[m][rr][t][j];
+        store_channels<MT>(v, bvv, bias != nullptr, relu, nullptr, y, y32, o);
+      }
+    }
+  }
+}
+
 // w8[co][tap][c] = w[co][tap][c] for tap < 25 and c < 8, else 0
 __global__ void k_pack_first(const bf16_t* __restrict__ w, bf16_t* __restrict__ w8, int cinp,
                              int coutp) {
@@ -1273,6 +1420,19 @@ int qgcnn_conv_first(const void* x, int B, int H, int W, int cinp, const void* w
     return fail(QGCNN_E_ARG, "tensors must start on 16-byte boundaries");
   hipStream_t st = (hipStream_t)stream;
   const int mt = coutp % 64 == 0 ? 4 : (coutp % 32 == 0 ? 2 : 1);
+  if (mt == 4 && H % 8 == 0) {
+    // overlapped staging, one resident block per CU; the grid a multiple of
+    // the output-channel groups so that each block keeps one group's weights
+    const int ncog = coutp / 64;
+    const int items = B * (H / 8) * ncog;
+    int grid = items < num_cus() ? items : num_cus() / ncog * ncog;
+    if (grid < ncog) grid = ncog;
+    hipLaunchKernelGGL((k_conv_first_ws<64, kLoaderWaves>), dim3(grid),
+                       dim3(256 + 64 * kLoaderWaves), 0, st, (const bf16_t*)x, (const bf16_t*)w8,
+                       bias, B, H, cinp, coutp, relu, (bf16_t*)y, y32);
+    HIPC(hipGetLastError());
+    return 0;
+  }
   const int rw = H % 8 == 0 ? 2 : 1;
   dim3 grid((H + 4 * rw - 1) / (4 * rw), B, coutp / (16 * mt));
 #define QGCNN_FIRST(MT, RW)                                                                    \

@@ -103,6 +103,28 @@ def test_conv_forward_blocks_with_several_items(cnn, cinp, coutp, B):
     assert torch.equal(y_bf.cpu(), y32.cpu().to(torch.bfloat16))
 
 
+@pytest.mark.parametrize("B", [48, 80])
+def test_first_layer_kernel_several_items_per_block(cnn, B):
+    """The first layer at the closure's shape (6 -> 128 channels, 64 x 64) with
+    more (sample, row group, channel group) items than resident blocks: each
+    block keeps one channel group's weights and takes several items."""
+    import torch
+    g = torch.Generator().manual_seed(B)
+    H = W = 64
+    cin, coutp = 6, 128
+    x = torch.zeros((B, H, W, 16))
+    x[..., :cin] = rnd_bf16((B, H, W, cin), 1.0, g)
+    w = torch.zeros((coutp, 25, 16))
+    w[..., :cin] = rnd_bf16((coutp, 25, cin), 0.1, g)
+    b = torch.randn(coutp, generator=g) * 0.1
+    w8 = cnn.pack_first(w.to(torch.bfloat16).cuda())
+    y_bf, y32 = cnn.conv_first(x.to(torch.bfloat16).cuda(), w8, b.cuda(), coutp, relu=True,
+                               out_f32=True)
+    ref = torch.relu(ref_conv(x, w, b))
+    assert rel(y32.cpu(), ref) < 1e-5
+    assert torch.equal(y_bf.cpu(), y32.cpu().to(torch.bfloat16))
+
+
 @pytest.mark.parametrize("cin,coutp,H,relu", [(6, 128, 8, True), (8, 64, 16, True),
                                               (3, 32, 12, False), (6, 16, 5, True)])
 def test_first_layer_kernel(cnn, cin, coutp, H, relu):
