@@ -207,18 +207,29 @@ def bench_cnn(args):
     B = 128
     xp = net.pad_input(torch.randn(B, 64, 64, net.channels[0], device="cuda"))
     st = torch.cuda.current_stream()
+    # inference as the closed step runs it: every layer's output preallocated
+    # (forward_into), so the launches queue back to back; the allocating
+    # forward (net.forward, a new output tensor per layer) is timed beside it
+    bufs = net.activation_buffers(B, 64, 64)
     for _ in range(args.warmup):
+        net.forward_into(xp, bufs, None)
         net.forward(xp)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     e0.record(st)
     for _ in range(args.steps):
-        net.forward(xp)
+        net.forward_into(xp, bufs, None)
     e1.record(st)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_fwd = e0.elapsed_time(e1) / args.steps
+    e0.record(st)
+    for _ in range(args.steps):
+        net.forward(xp)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms_fwd_alloc = e0.elapsed_time(e1) / args.steps
     # per-layer times of the same forward pass (events on the launch stream)
     layer_ms = []
     h = xp
@@ -296,6 +307,7 @@ def bench_cnn(args):
                      "kernel_ms": layer_ms[dom], "layer_ms": layer_ms,
                      "forward_ms": ms_fwd,
                      "forward_tflops": B * fl / (ms_fwd * 1e-3) / 1e12,
+                     "forward_allocating_ms": ms_fwd_alloc,
                      "train_step_ms_batch64": ms_train, "train_tflops": train_tflops},
         "cpu_baseline": cpu,
     }

@@ -963,6 +963,139 @@ __global__ __launch_bounds__(256) void k_conv_last(const bf16_t* __restrict__ x,
   }
 }
 
+// Last layer with overlapped staging (64-pixel rows, heights a multiple of
+// 8): the work of k_conv_last, one resident block per CU walking its share of
+// the (sample, 8-row group) items as cinp/32 stages.  Four loader waves stage
+// the next stage's 12 input rows ([row][K group][pixel], 16-byte entries; no
+// halo: the horizontal taps are summed in the epilogue) into one of two LDS
+// buffers while the four compute waves run the current one (barrier pattern
+// of k_conv_fwd_ws); the compute waves' stores drain under the next stage.
+template <int W, int NLW>
+__global__ __launch_bounds__(256 + 64 * NLW, 1) void k_conv_last_ws(
+    const bf16_t* __restrict__ x, const bf16_t* __restrict__ wl, const float* __restrict__ bias,
+    int B, int H, int cinp, int coutp, int cout, bf16_t* __restrict__ y,
+    float* __restrict__ y32) {
+  typedef v8bf op;
+  constexpr int KC = 32, RW = 2, NT = W / 16;
+  constexpr int ROWS = 4 * RW, SR = ROWS + KS - 1, PIECES = KC / 8;
+  constexpr int BUF = SR * PIECES * W * 8;
+  __shared__ __attribute__((aligned(16))) bf16_t lds[2 * BUF];
+  __shared__ float part[4][RW][16][W + 4];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int nrg = H / ROWS, nch = cinp / KC;
+  const int items = B * nrg;
+  const int mine = (int)blockIdx.x < items ? (items - 1 - (int)blockIdx.x) / (int)gridDim.x + 1 : 0;
+  const int S = mine * nch;
+  if (wave >= 4) {  // loader waves: stage s into buffer s%2, then barrier s
+    constexpr int NL = 64 * NLW;
+    constexpr int K = PIECES * W / NL;
+    static_assert(K * NL == PIECES * W, "whole rows per loader passes");
+    const int lt = threadIdx.x - 256;
+    for (int s = 0; s < S; ++s) {
+      const int it = blockIdx.x + (s / nch) * gridDim.x;
+      const int ci0 = (s % nch) * KC;
+      const int b = it / nrg, row0 = (it % nrg) * ROWS;
+      uint4 v[SR * K];
+#pragma unroll
+      for (int r = 0; r < SR; ++r) {
+        const int yy = (row0 + r - 2 + H) % H;
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+          const int qq = lt + NL * j;
+          const int pix = ((qq / 64) % (W / 16)) * 16 + qq % 16, pc = (qq / 16) % PIECES;
+          v[r * K + j] = *reinterpret_cast<const uint4*>(
+              x + (((size_t)b * H + yy) * W + pix) * cinp + ci0 + pc * 8);
+        }
+      }
+      bf16_t* buf = lds + (s & 1) * BUF;
+#pragma unroll
+      for (int r = 0; r < SR; ++r)
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+          const int qq = lt + NL * j;
+          const int pix = ((qq / 64) % (W / 16)) * 16 + qq % 16, pc = (qq / 16) % PIECES;
+          *reinterpret_cast<uint4*>(buf + ((r * PIECES + pc) * W + pix) * 8) = v[r * K + j];
+        }
+      __syncthreads();
+    }
+    return;
+  }
+  const int li = lane % 16, lg = lane / 16;
+  const int wr0 = wave * RW;
+  v4f acc[RW][NT];
+#pragma unroll
+  for (int rr = 0; rr < RW; ++rr)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[rr][t] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int s = 0; s < S; ++s) {
+    __syncthreads();
+    const int it = blockIdx.x + (s / nch) * gridDim.x;
+    const int ch = s % nch;
+    const int b = it / nrg, row0 = (it % nrg) * ROWS;
+    const bf16_t* buf = lds + (s & 1) * BUF;
+    op a[KS];
+#pragma unroll
+    for (int dy = 0; dy < KS; ++dy)
+      a[dy] = *reinterpret_cast<const op*>(wl + ((size_t)li * KS + dy) * cinp + ch * KC + lg * 8);
+#pragma unroll
+    for (int dy = 0; dy < KS; ++dy)
+#pragma unroll
+      for (int rr = 0; rr < RW; ++rr) {
+        const bf16_t* lrow = buf + ((wr0 + rr + dy) * PIECES + lg) * W * 8 + li * 8;
+        op bv[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) bv[t] = *reinterpret_cast<const op*>(lrow + t * 16 * 8);
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          acc[rr][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[dy], bv[t], acc[rr][t], 0, 0, 0);
+      }
+    if (ch != nch - 1) continue;
+    // D[4 lg + j][p] of the wave's rows into its own LDS region, then the
+    // horizontal sums; every lane handles one pixel
+#pragma unroll
+    for (int rr = 0; rr < RW; ++rr)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) part[wave][rr][4 * lg + j][t * 16 + li] = acc[rr][t][j];
+        acc[rr][t] = v4f{0.f, 0.f, 0.f, 0.f};
+      }
+    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the wave's LDS writes are done
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int rr = 0; rr < RW; ++rr) {
+      const int row = row0 + wr0 + rr;
+      const int p = lane;
+      const size_t o = (((size_t)b * H + row) * W + p) * coutp;
+      for (int c0 = 0; c0 < coutp; c0 += 4) {
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int c = c0 + i;
+          v[i] = bias ? bias[c] : 0.f;
+          if (c < cout) {
+            float sum = 0.f;
+#pragma unroll
+            for (int j = 0; j < KS; ++j) sum += part[wave][rr][c * KS + j][(p + j - 2 + W) % W];
+            v[i] += sum;
+          }
+        }
+        if (y) {
+          v4s out;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) out[i] = (short)f2bf(v[i]);
+          *reinterpret_cast<v4s*>(y + o + c0) = out;
+        }
+        if (y32) *reinterpret_cast<v4f*>(y32 + o + c0) = v4f{v[0], v[1], v[2], v[3]};
+      }
+    }
+    // the next item's partial sums go to the same region: every lane has
+    // read this item's before any lane writes
+    __builtin_amdgcn_s_waitcnt(0xc07f);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 // wl[r][dy][ci] = w[r / 5][dy * 5 + r % 5][ci] for r < 5 cout, else 0
 __global__ void k_pack_last(const bf16_t* __restrict__ w, bf16_t* __restrict__ wl, int cinp,
                             int cout) {
@@ -1471,6 +1604,17 @@ int qgcnn_conv_last(const void* x, int B, int H, int W, int cinp, const void* wl
     return fail(QGCNN_E_ARG, "tensors must start on 16-byte boundaries (y on 8-byte ones)");
   if (B < 1 || H < KS || W != 64)
     return fail(QGCNN_E_ARG, "images must be W=64 wide and at least 5 rows (got %dx%d)", H, W);
+  if (H % 8 == 0) {
+    // overlapped staging, one resident block per CU over the (sample, 8-row
+    // group) items
+    const int items = B * (H / 8);
+    const int grid = items < num_cus() ? items : num_cus();
+    hipLaunchKernelGGL((k_conv_last_ws<64, kLoaderWaves>), dim3(grid),
+                       dim3(256 + 64 * kLoaderWaves), 0, (hipStream_t)stream, (const bf16_t*)x,
+                       (const bf16_t*)wl, bias, B, H, cinp, coutp, cout, (bf16_t*)y, y32);
+    HIPC(hipGetLastError());
+    return 0;
+  }
   dim3 grid((H + 3) / 4, B);
   hipLaunchKernelGGL((k_conv_last<64, 1>), grid, dim3(256), 0, (hipStream_t)stream,
                      (const bf16_t*)x, (const bf16_t*)wl, bias, H, cinp, coutp, cout, (bf16_t*)y,

@@ -174,6 +174,28 @@ def test_last_layer_kernel(cnn, cinp, cout, H, coutp):
     assert not y32[..., cout:].any()
 
 
+@pytest.mark.parametrize("cinp,B", [(32, 40), (64, 36)])
+def test_last_layer_kernel_several_items_per_block(cnn, cinp, B):
+    """The last layer at the closure's shape (2 outputs, the 4-channel FP32
+    output of the closed step, 64 x 64) with more (sample, row group) items
+    than resident blocks, so that each block takes several in turn."""
+    import torch
+    g = torch.Generator().manual_seed(cinp + B)
+    H = W = 64
+    cout, coutp = 2, 4
+    x = rnd_bf16((B, H, W, cinp), 1.0, g)
+    w = torch.zeros((16, 25, cinp))
+    w[:cout] = rnd_bf16((cout, 25, cinp), 0.1, g)
+    b = torch.zeros(16)
+    b[:cout] = torch.randn(cout, generator=g) * 0.1
+    wl = cnn.pack_last(w.to(torch.bfloat16).cuda(), cout)
+    _, y32 = cnn.conv_last(x.to(torch.bfloat16).cuda(), wl, b.cuda(), coutp, cout,
+                           out_bf16=False, out_f32=True)
+    ref = ref_conv(x, w, b)[..., :coutp]
+    assert rel(y32.cpu(), ref) < 1e-5
+    assert not y32[..., cout:].any()
+
+
 def test_backward_data_with_relu_mask(cnn):
     import torch
     g = torch.Generator().manual_seed(1)
