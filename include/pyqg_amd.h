@@ -220,6 +220,20 @@ int qgk_fused_row_net(qgk_bucket* b, const double* inv_scale, int cinp, void* ne
 int qgk_fused_col_forcing(qgk_bucket* b, const float* net_out, int coutp, const double* scale,
                           double dt1, double dt2, double dt3);
 
+/* Stochastic forcing added to the network's (no counterpart in the reference;
+ * for the learned closure): each step, after dqh = rfft2(dq) in
+ * qgk_add_forcing_device and qgk_fused_col_forcing,
+ *   xi <- phi xi + sqrt(1 - phi^2) amp w,   dqh += xi,
+ * per member, layer and spectral point, w complex normal (E|w|^2 = 1) drawn
+ * from a counter-based hash of (seed, step, point); the kx = 0 and kx = nx/2
+ * columns are drawn Hermitian in l, so xi is the spectrum of a real field when
+ * amp is symmetric there.  amp: nz * ny * nk doubles (layer, row, column; the
+ * stationary rms of xi, in the units of dqh, i.e. of rfft2 of dq), phi in
+ * [0, 1) the one-step autocorrelation.  amp NULL switches it off.  The process
+ * restarts from zero at every call; captured steps are recaptured. */
+int qgk_set_spectral_noise(qgk_bucket* b, const double* amp, double phi,
+                           unsigned long long seed);
+
 /* Two-layer column steps by the layer-split kernel (each layer its own
  * thread group) instead of the register kernel (no counterpart in the
  * reference).  Default: on for one member or members x max(nx, ny)^2 <= 2^19, off

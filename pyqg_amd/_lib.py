@@ -35,7 +35,7 @@ EXPORTS = ["qgk_create", "qgk_destroy", "qgk_last_error", "qgk_set_const", "qgk_
            "qgk_get_layer_split", "qgk_autotune",
            "qgk_device_name", "qgk_put_device", "qgk_get_device", "qgk_coarsen",
            "qgk_subgrid_forcing", "qgk_pack_inputs", "qgk_add_forcing_device",
-           "qgk_fused_row_net", "qgk_fused_col_forcing",
+           "qgk_fused_row_net", "qgk_fused_col_forcing", "qgk_set_spectral_noise",
            "qgk_diag_set_params", "qgk_diag_accumulate", "qgk_diag_read", "qgk_diag_reset",
            "qgk_param_smagorinsky", "qgk_smagorinsky_viscosity", "qgk_param_zb2020",
            "qgk_param_backscatter", "qgk_apply_q_param", "qgk_apply_uv_param",
@@ -134,6 +134,7 @@ def _load():
         "qgk_fused_col_forcing": (I, [P, P, I, P, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_double]),
         "qgk_add_forcing_device": (I, [P, P, I, P]),
+        "qgk_set_spectral_noise": (I, [P, P, ctypes.c_double, ctypes.c_ulonglong]),
         "qgk_diag_set_params": (I, [P, I, P]),
         "qgk_diag_accumulate": (I, [P, ctypes.c_uint, I, D, D, D, D]),
         "qgk_diag_read": (I, [P, I, I, I, P]),
@@ -301,6 +302,19 @@ class Bucket:
 
     def set_packed_exchange(self, on):
         check(lib.qgk_set_packed_exchange(self.h, int(bool(on))))
+
+    def set_spectral_noise(self, amp, phi=0.0, seed=0):
+        """Stochastic forcing added to the network's each step (AR(1) per
+        spectral point, amp (nz, ny, nk) its stationary rms in dqh units, phi
+        the one-step autocorrelation); amp None switches it off."""
+        if amp is None:
+            check(lib.qgk_set_spectral_noise(self.h, None, 0.0, 0))
+            return
+        a = np.ascontiguousarray(amp, dtype=np.float64)
+        if a.shape != (self.nz, self.ny, self.nk):
+            raise ValueError(f"amp must have shape {(self.nz, self.ny, self.nk)}")
+        check(lib.qgk_set_spectral_noise(self.h, a.ctypes.data_as(ctypes.c_void_p), float(phi),
+                                         int(seed)))
 
     def exchange_columns(self):
         """Columns per row of the x-mixed fields the fused kernels exchange:

@@ -46,12 +46,13 @@ BUDGET = ["KEflux", "APEflux", "APEgenspec", "KEfrictionspec", "Dissspec", "EKEd
 def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=64, lr=1e-3,
         online_members=128, online_days=720, avg_days=360, dt_lo=14400., seed=0, ema=0.998,
         ema_from=0.5, closures=("backscatter", "zb2020", "smagorinsky"), calibrate="none",
-        filter_kind="gaussian", log=print):
+        filter_kind="gaussian", stochastic=False, log=print):
     out = {"config": dict(members=members, samples=samples, spinup_days=spinup_days,
                           every=every, epochs=epochs, batch=batch, lr=lr,
                           online_members=online_members, online_days=online_days,
                           avg_days=avg_days, dt_hi=3600., dt_lo=dt_lo, seed=seed, ema=ema,
-                          ema_from=ema_from, calibrate=calibrate, filter=filter_kind)}
+                          ema_from=ema_from, calibrate=calibrate, filter=filter_kind,
+                          stochastic=stochastic)}
     L = data_mod.QG_EDDY["L"]
     g_hi, g_lo = tools.grid(256, L), tools.grid(64, L)
     # ---------------------------------------------------------- 1. data
@@ -111,6 +112,18 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
         scales = param_mod.Scales(scales.in_scale, scales.out_scale * gain)
         out["offline"]["gain"] = gain.tolist()
         log(f"variance calibration gain {gain}")
+    noise = None
+    if stochastic:
+        # the forcing the network leaves out, as an AR(1) process per spectral
+        # point with the held-out residual's spectrum and its correlation over
+        # one sampling interval (one day) scaled to the coarse step
+        amp, rho = train_mod.residual_statistics(net, x_te, y_te, scales, lag=members)
+        r1 = float(max(np.mean(rho), 0.0))
+        phi = r1 ** (dt_lo / (every * 3600.)) if r1 > 0 else 0.0
+        noise = (amp, phi)
+        out["offline"]["residual_lag_correlation"] = rho.tolist()
+        out["offline"]["noise_phi"] = phi
+        log(f"residual: one-interval correlation {rho}, phi per coarse step {phi:.3f}")
     flop_per_sample = 2 * 64 * 64 * 25 * sum(
         net.chp[l] * net.chp[l + 1] for l in range(net.nlayers))
     out["train"]["tflops"] = 3 * flop_per_sample * info["steps"] * batch / info["seconds"] / 1e12
@@ -129,9 +142,14 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
     choices = {"backscatter": builtin.BackscatterBiharmonic(), "zb2020": builtin.ZannaBolton2020(),
                "smagorinsky": builtin.Smagorinsky()}
     views = {}
-    for name, prm in [("plain", None), ("cnn", p)] + [(c, choices[c]) for c in closures]:
+    plan = [("plain", None), ("cnn", p)]
+    if noise is not None:
+        plan.append(("cnn_stochastic", p))
+    for name, prm in plan + [(c, choices[c]) for c in closures]:
         ce = param_mod.ClosedEnsemble(online_members, 64, dt_lo, gen.phys, param=prm, seed=seed)
         ce.ens.set_q([qlo[i % members] for i in range(online_members)])
+        if name == "cnn_stochastic":
+            ce.set_noise(noise[0], noise[1], seed=seed + 1)
         per_day = int(round(86400. / dt_lo))
         nsteps = online_days * per_day
         navg = avg_days * per_day
@@ -203,13 +221,16 @@ def main():
                          "mean-square forcing equals the target's")
     ap.add_argument("--filter", choices=["gaussian", "sharp"], default="gaussian",
                     help="coarse-graining after the spectral truncation")
+    ap.add_argument("--stochastic", action="store_true",
+                    help="also run the network with an AR(1) stochastic forcing that has the "
+                         "held-out residual's spectrum and time correlation")
     ap.add_argument("--out", default=None, help="also append the JSON line to this file")
     a = ap.parse_args()
     res = run(members=a.members, samples=a.samples, spinup_days=a.spinup_days, every=a.every,
               epochs=a.epochs, batch=a.batch, lr=a.lr, online_members=a.online_members,
               online_days=a.online_days, avg_days=a.avg_days, dt_lo=a.dt_lo, seed=a.seed,
               ema=a.ema, ema_from=a.ema_from, closures=[c for c in a.closures.split(",") if c],
-              calibrate=a.calibrate, filter_kind=a.filter,
+              calibrate=a.calibrate, filter_kind=a.filter, stochastic=a.stochastic,
               log=lambda s: print(s, flush=True))
     line = json.dumps(res)
     print(line)

@@ -107,6 +107,15 @@ class ClosedEnsemble:
     def t(self):
         return self.ens.t
 
+    def set_noise(self, amp, phi=0.0, seed=0):
+        """Stochastic forcing beside the network's (qgk_set_spectral_noise):
+        amp (nz, n, n/2 + 1) the stationary rms per spectral point in dqh
+        units, phi the one-step autocorrelation; None switches it off."""
+        if self.param is None or self.builtin:
+            raise ValueError("the stochastic forcing goes with the network closure")
+        self.b.set_spectral_noise(amp, phi, seed)
+        self._graph_key = None
+
     def step(self, nsteps=1):
         if self.param is None:
             self.ens.step(nsteps)

@@ -108,3 +108,32 @@ def amplitude(net: CNN, x, y, scales: Scales, batch=128):
         n += yb.shape[0] * yb.shape[1] * yb.shape[2]
     return {"target_ms": (yy / n).tolist(), "pred_ms": (pp / n).tolist(),
             "cross": (yp / n).tolist()}
+
+
+@torch.no_grad()
+def residual_statistics(net: CNN, x, y, scales: Scales, lag, batch=128):
+    """The part of the held-out forcing the network leaves out, r = y - net(x)
+    in the forcing's physical units: per layer, the rms of its unnormalised 2-D
+    real transform (the units of the model's dqh) over samples, shape
+    (nz, ny, nx/2 + 1), and the correlation of r between samples `lag` apart
+    (the same member one sampling interval later when the pairs are stored
+    snapshot by snapshot, `lag` members per snapshot)."""
+    nout = y.shape[-1]
+    osc = torch.as_tensor(scales.out_scale[:nout], dtype=torch.float64, device=x.device)
+    res = []
+    for s in range(0, x.shape[0], batch):
+        idx = torch.arange(s, min(s + batch, x.shape[0]), device=x.device)
+        xb, yb = _normalized(x, y, scales, idx, net)
+        out = net.forward(xb)[..., :nout]
+        res.append(((yb - out).double() * osc).float())
+    r = torch.cat(res)                                  # (N, ny, nx, nz)
+    spec = torch.zeros((nout, r.shape[1], r.shape[2] // 2 + 1), dtype=torch.float64,
+                       device=x.device)
+    for s in range(0, r.shape[0], batch):
+        rh = torch.fft.rfft2(r[s:s + batch].permute(0, 3, 1, 2).double())
+        spec += (rh.real ** 2 + rh.imag ** 2).sum(dim=0)
+    amp = torch.sqrt(spec / r.shape[0]).cpu().numpy()
+    a, b = r[:-lag].double(), r[lag:].double()
+    rho = ((a * b).sum(dim=(0, 1, 2)) / torch.sqrt((a * a).sum(dim=(0, 1, 2)) *
+                                                   (b * b).sum(dim=(0, 1, 2)))).cpu().numpy()
+    return amp, rho

@@ -575,6 +575,63 @@ __global__ void k_add(cd* __restrict__ dqhdt, const cd* __restrict__ dqh, size_t
   }
 }
 
+// Stochastic forcing: a first-order autoregressive process per spectral point,
+//   xi <- phi xi + sqrt(1 - phi^2) amp w,   dqh += xi,
+// w complex normal (E|w|^2 = 1) from a counter-based hash of (seed, step
+// counter, point), so that a captured step draws new numbers at every replay
+// (the counter lives in device memory; k_noise_count advances it after each
+// draw).  The columns kx = 0 and kx = nx/2 of a real field's half spectrum
+// are Hermitian in l: the draw at row ny - l is the conjugate of row l's (the
+// self-conjugate rows draw a real w of unit variance), so xi stays the
+// spectrum of a real field as long as amp is symmetric in l there.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__global__ void k_spectral_noise(cd* __restrict__ dqh, cd* __restrict__ xi,
+                                 const double* __restrict__ amp, double phi, double sig,
+                                 unsigned long long seed, const unsigned long long* __restrict__ ctr,
+                                 int M, int nz, int ny, int nx, int nk, int ldk) {
+  const unsigned long long step = *ctr;
+  const size_t total = (size_t)M * nz * ny * nk;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int k = (int)(i % nk);
+    const int l = (int)((i / nk) % ny);
+    const size_t zm = i / ((size_t)nk * ny);  // member * nz + layer
+    const int z = (int)(zm % nz);
+    const bool herm = k == 0 || (nx % 2 == 0 && k == nx / 2);
+    const int lm = l == 0 ? 0 : ny - l;
+    const int lc = herm && lm < l ? lm : l;  // the row whose draw this point uses
+    const unsigned long long key =
+        mix64(seed ^ mix64(step * 0x100000001B3ull + ((zm * ny + lc) * nk + k)));
+    const double u1 = ((key >> 11) + 1) * (1.0 / 9007199254740993.0);   // (0, 1]
+    const double u2 = (mix64(key) >> 11) * (1.0 / 9007199254740992.0);  // [0, 1)
+    const double r = sqrt(-log(u1));  // |w|^2 exponential with mean 1
+    double s, c;
+    sincos(6.283185307179586 * u2, &s, &c);
+    cd w = {r * c, r * s};
+    if (herm) {
+      if (lc == lm && lm == l) w = {w.x * 1.4142135623730951, 0.0};  // self-conjugate row
+      else if (lc != l) w.y = -w.y;                                  // conjugate of row lc
+    }
+    const double a = amp[((size_t)z * ny + l) * nk + k];
+    const size_t o = (zm * ny + l) * ldk + k;
+    const cd x = xi[o];
+    const cd xn = {phi * x.x + sig * a * w.x, phi * x.y + sig * a * w.y};
+    xi[o] = xn;
+    const cd d = dqh[o];
+    dqh[o] = {d.x + xn.x, d.y + xn.y};
+  }
+}
+
+__global__ void k_noise_count(unsigned long long* __restrict__ ctr) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ctr = *ctr + 1;
+}
+
 // uv parameterization add (kernel.pyx:412-421): dqhdt += (-il duh + ik dvh)
 __global__ void k_add_curl(cd* __restrict__ dqhdt, const cd* __restrict__ duh,
                            const cd* __restrict__ dvh, const MemberParams* __restrict__ params,
@@ -2750,6 +2807,13 @@ struct qgk_bucket {
   long ug_gen = -1;   // gen at capture
   const cd* ug_qh = nullptr;  // qh buffer at capture (the fused step swaps qh buffers)
   bool ug_fused = false;      // the captured steps are fused closed steps (qgk_fused_*)
+  // stochastic forcing (qgk_set_spectral_noise): AR(1) state, amplitudes, step counter
+  cd* noise_xi = nullptr;
+  double* noise_amp = nullptr;
+  unsigned long long* noise_ctr = nullptr;
+  bool noise_on = false;
+  double noise_phi = 0.0;
+  unsigned long long noise_seed = 0;
   hipStream_t lst = nullptr;  // stream override for fused-step launches (graph capture)
   long gen = 0;               // bumped whenever launch arguments change (graph cache key)
   // hipGraph of two fused steps of a set of buckets, replayed while the AB
@@ -4561,6 +4625,56 @@ int qgk_pack_inputs(qgk_bucket* b, const double* inv_scale, int cinp, void* dst)
   return 0;
 }
 
+// the stochastic forcing, when set, added to dqh after the transform of the
+// network's forcing (both forcing paths)
+static int apply_noise(qgk_bucket* b) {
+  if (!b->noise_on) return 0;
+  const double phi = b->noise_phi, sig = std::sqrt(1.0 - phi * phi);
+  const size_t n = (size_t)b->M * b->nz * b->ny * b->nk;
+  hipLaunchKernelGGL(k_spectral_noise, dim3(grid_1d(n)), dim3(256), 0, b->stream,
+                     b->field_c[QGK_DQH], b->noise_xi, b->noise_amp, phi, sig, b->noise_seed,
+                     b->noise_ctr, b->M, b->nz, b->ny, b->nx, b->nk, b->ldk);
+  hipLaunchKernelGGL(k_noise_count, dim3(1), dim3(64), 0, b->stream, b->noise_ctr);
+  HIPC(hipGetLastError());
+  return 0;
+}
+
+int qgk_set_spectral_noise(qgk_bucket* b, const double* amp, double phi,
+                           unsigned long long seed) {
+  if (!b) return fail(QGK_E_ARG, "NULL handle");
+  if (int rc0 = refuse_in_capture(b, "qgk_set_spectral_noise")) return rc0;
+  if (!amp) {
+    b->noise_on = false;
+    b->gen++;
+    return 0;
+  }
+  if (!(phi >= 0.0 && phi < 1.0)) return fail(QGK_E_ARG, "phi must be in [0, 1)");
+  if (!b->field_c[QGK_DQH])
+    return fail(QGK_E_STATE, "dqh is not allocated (no q parameterization)");
+  const size_t na = (size_t)b->nz * b->ny * b->nk;
+  for (size_t i = 0; i < na; ++i)
+    if (!(amp[i] >= 0.0) || !std::isfinite(amp[i]))
+      return fail(QGK_E_ARG, "amplitudes must be finite and non-negative");
+  int rc;
+  if (!b->noise_xi) {
+    if ((rc = dev_alloc(b, (void**)&b->noise_xi, (size_t)b->M * b->nz * b->nplane_c * sizeof(cd))))
+      return rc;
+    if ((rc = dev_alloc(b, (void**)&b->noise_amp, na * sizeof(double)))) return rc;
+    if ((rc = dev_alloc(b, (void**)&b->noise_ctr, sizeof(unsigned long long)))) return rc;
+  } else {  // a new process starts from zero
+    HIPC(hipMemsetAsync(b->noise_xi, 0, (size_t)b->M * b->nz * b->nplane_c * sizeof(cd),
+                        b->stream));
+    HIPC(hipMemsetAsync(b->noise_ctr, 0, sizeof(unsigned long long), b->stream));
+  }
+  HIPC(hipMemcpyAsync(b->noise_amp, amp, na * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  HIPC(hipStreamSynchronize(b->stream));
+  b->noise_on = true;
+  b->noise_phi = phi;
+  b->noise_seed = seed;
+  b->gen++;  // captured steps are recaptured
+  return 0;
+}
+
 int qgk_add_forcing_device(qgk_bucket* b, const float* net_out, int coutp, const double* scale) {
   if (!b || !net_out || !scale) return fail(QGK_E_ARG, "NULL argument");
   if (!b->field_r[QGK_DQ]) return fail(QGK_E_STATE, "dq is not allocated (no q parameterization)");
@@ -4576,6 +4690,7 @@ int qgk_add_forcing_device(qgk_bucket* b, const float* net_out, int coutp, const
   HIPC(hipGetLastError());
   rc = r2c2d(b, b->field_r[QGK_DQ], b->field_c[QGK_DQH], np);
   if (rc) return rc;
+  if ((rc = apply_noise(b))) return rc;
   const size_t cpts = (size_t)np * b->nplane_c;
   hipLaunchKernelGGL(k_add, dim3(grid_1d(cpts)), dim3(256), 0, b->stream, D(b),
                      b->field_c[QGK_DQH], cpts);
@@ -4630,6 +4745,7 @@ int qgk_fused_col_forcing(qgk_bucket* b, const float* net_out, int coutp, const 
                      b->field_r[QGK_DQ], b->M, b->nz, (int)b->nplane_r, coutp);
   HIPC(hipGetLastError());
   if ((rc = r2c2d(b, b->field_r[QGK_DQ], b->field_c[QGK_DQH], np))) return rc;
+  if ((rc = apply_noise(b))) return rc;
   const StepCoeffs cf = {dt1, dt2, dt3};
   if ((rc = with_ne<ColStepForceL>(b->ny, b->ecol, b, cf))) return rc;
   HIPC(hipGetLastError());

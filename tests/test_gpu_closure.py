@@ -276,3 +276,91 @@ def test_online_closure_at_the_experiments_resolution(mods):
         assert on["cnn"]["ke_rel_err"][layer] < 0.8 * on["plain"]["ke_rel_err"][layer], layer
     assert on["cnn"]["budget_similarity_mean"] > 0.0
     assert on["cnn"]["finite"] and on["plain"]["finite"]
+
+
+def _noise_draws(pq, amp, phi, ndraw, seed=11, M=8, n=64):
+    """dqh after each of ndraw forcing calls with a zero network output: the
+    stochastic term alone (qgk_set_spectral_noise), one draw per call."""
+    import ctypes
+    import torch
+    from pyqg_amd import _lib
+    ens = pq.Ensemble.qg(np.full(M, n), full_state=True, q_param=True)
+    b = ens.buckets[0]
+    b.set_spectral_noise(amp, phi, seed)
+    net0 = torch.zeros((M, n, n, 4), dtype=torch.float32, device="cuda")
+    sc = np.ones(2)
+    out = []
+    for _ in range(ndraw):
+        _lib.check(_lib.lib.qgk_add_forcing_device(b.h, ctypes.c_void_p(net0.data_ptr()), 4,
+                                                   _lib.ptr(sc)))
+        out.append(b.get("dqh"))
+    return np.array(out)             # (ndraw, M, nz, ny, nk)
+
+
+def test_spectral_noise_variance_symmetry_and_fresh_draws(mods):
+    """phi = 0: every point's draw has mean square amp^2 (here a spectrum that
+    varies by layer and wavenumber), the kx = 0 and kx = nx/2 columns are the
+    half spectrum of a real field (row ny - l the conjugate of row l, rows 0
+    and ny/2 real), and successive calls draw afresh."""
+    pq = mods[0]
+    n = 64
+    nk = n // 2 + 1
+    l = np.fft.fftfreq(n, 1.0 / n)[:, None]
+    k = np.arange(nk)[None, :]
+    amp = np.stack([np.exp(-(k ** 2 + l ** 2) / 200.0) + 0.5, np.full((n, nk), 2.0)])
+    d = _noise_draws(pq, amp, 0.0, 12)
+    ms = (np.abs(d) ** 2).mean(axis=(0, 1))               # (nz, ny, nk)
+    ratio = ms.sum(axis=(1, 2)) / (amp ** 2).sum(axis=(1, 2))
+    np.testing.assert_allclose(ratio, 1.0, atol=0.02)
+    # per wavenumber band, within the sampling error of 12 x 8 draws per point
+    band = ms[:, 1:n // 2, 1:nk - 1].mean(axis=1) / (amp[:, 1:n // 2, 1:nk - 1] ** 2).mean(axis=1)
+    assert np.abs(band - 1.0).max() < 0.12
+    for kc in (0, n // 2):
+        col = d[..., kc]                                   # (ndraw, M, nz, ny)
+        np.testing.assert_array_equal(col[..., n - np.arange(1, n)], np.conj(col[..., 1:]))
+        assert not col[..., 0].imag.any() and not col[..., n // 2].imag.any()
+        assert np.abs(col[..., 0].real).max() > 0
+    c = np.vdot(d[1:], d[:-1]).real / np.vdot(d, d).real
+    assert abs(c) < 0.01
+
+
+def test_spectral_noise_ar1_correlation_and_seed(mods):
+    """phi = 0.8: after the start-up the process keeps the prescribed mean
+    square, and one call to the next correlates by phi; the same seed draws
+    the same sequence, another seed another."""
+    pq = mods[0]
+    n, nk = 64, 33
+    amp = np.full((2, n, nk), 3.0)
+    d = _noise_draws(pq, amp, 0.8, 50)
+    s = d[20:]
+    ms = (np.abs(s) ** 2).mean()
+    assert abs(ms / 9.0 - 1.0) < 0.03
+    rho = np.vdot(s[:-1], s[1:]).real / np.vdot(s[:-1], s[:-1]).real
+    assert abs(rho - 0.8) < 0.02
+    again = _noise_draws(pq, amp, 0.8, 3)
+    np.testing.assert_array_equal(again, d[:3])
+    other = _noise_draws(pq, amp, 0.8, 3, seed=12)
+    assert np.abs(other - d[:3]).max() > 1.0
+
+
+def test_closed_ensemble_with_noise_graph_matches_step_by_step(mods):
+    """With the stochastic forcing on, the replayed step graphs draw the same
+    numbers as steps taken one at a time (the step counter lives on the
+    device), and the forcing changes the trajectory."""
+    pq, cnn, _, param, _ = mods
+    p = _small_closure(cnn, param, 4)
+    phys = dict(QG_PHYS)
+    amp = np.full((2, 64, 33), 1e-9)
+    a = param.ClosedEnsemble(3, 64, 3600., phys, param=p, seed=5)
+    b = param.ClosedEnsemble(3, 64, 3600., phys, param=p, seed=5)
+    c = param.ClosedEnsemble(3, 64, 3600., phys, param=p, seed=5)
+    a.set_noise(amp, 0.5, seed=3)
+    b.set_noise(amp, 0.5, seed=3)
+    for chunk in (7, 6):
+        a.step(chunk)
+        c.step(chunk)
+        for _ in range(chunk):
+            b._cnn_step()
+        for i in range(3):
+            np.testing.assert_array_equal(a.ens.get("q", i), b.ens.get("q", i))
+    assert np.abs(a.ens.get("q", 0) - c.ens.get("q", 0)).max() > 0
