@@ -5,8 +5,12 @@
 // 16, FP32 accumulation in the matrix cores (v_mfma_f32_16x16x32_bf16, and the
 // K=16 form for 16-channel inputs).
 //
-//   forward        y = act(conv(x, W) + b)            k_conv_fwd_mt
-//   backward data  dx = conv(dy, W') * [x_prev > 0]   k_conv_fwd_mt on W' = flip/transpose(W)
+//   forward        y = act(conv(x, W) + b)            k_conv_fwd_ws (persistent, overlapped
+//                                                      staging; 64-pixel rows, heights a
+//                                                      multiple of 8), k_conv_fwd_mt else;
+//                                                      first / last layers k_conv_first_ws,
+//                                                      k_conv_last_ws (k_conv_first/_last)
+//   backward data  dx = conv(dy, W') * [x_prev > 0]   the forward kernels on W' = flip/transpose(W)
 //   backward wts   dW = sum_px dy (x) x_shifted        k_conv_wgrad_mt (+ k_wgrad_reduce),
 //                                                      or k_conv_wgrad with atomics
 //   bias grad, MSE loss / grad, Adam, casts           small element-wise kernels
