@@ -3853,6 +3853,13 @@ int qgk_get(qgk_bucket* b, int field, void* host) {
 
 // End of an entry point that only enqueues device work: wait for it, unless
 // the stream is being captured into a graph (then nothing runs yet).
+// The four phases of a plain step (qgk_invert, qgk_advect, qgk_friction,
+// qgk_timestep) do not wait: they take no host data, every later kernel is
+// ordered after them on the bucket's stream, and every entry point that hands
+// data to or from the host waits for the stream first, so a loop of
+// _step_forward calls queues its kernels back to back instead of waiting for
+// each phase (enqueue-only phases; launch errors are still returned here,
+// errors during execution by the next waiting call).
 hipError_t stream_done(qgk_bucket* b) {
   return b->capturing ? hipSuccess : hipStreamSynchronize(b->stream);
 }
@@ -3872,7 +3879,7 @@ int qgk_invert(qgk_bucket* b) {
     double* out[2] = {b->field_r[QGK_U], b->field_r[QGK_V]};
     if ((rc = c2r2d_n(b, 2, in, out, np))) return rc;
   }
-  HIPC(stream_done(b));
+  HIPC(hipGetLastError());  // no wait: see enqueue-only phases above
   return 0;
 }
 
@@ -3896,7 +3903,7 @@ int qgk_advect(qgk_bucket* b) {
                      b->field_c[QGK_UQH], b->field_c[QGK_VQH], b->field_c[QGK_PH], D(b), b->dp,
                      b->nz, b->nl, b->ldk, b->M);
   HIPC(hipGetLastError());
-  HIPC(stream_done(b));
+  HIPC(hipGetLastError());  // no wait: see enqueue-only phases above
   return 0;
 }
 
@@ -3908,7 +3915,7 @@ int qgk_friction(qgk_bucket* b) {
   hipLaunchKernelGGL(k_friction, dim3(grid_1d(pts)), dim3(256), 0, b->stream,
                      b->field_c[QGK_PH], D(b), b->dp, b->nz, b->nl, b->ldk, b->M);
   HIPC(hipGetLastError());
-  HIPC(stream_done(b));
+  HIPC(hipGetLastError());  // no wait: see enqueue-only phases above
   return 0;
 }
 
@@ -3983,7 +3990,7 @@ int qgk_timestep(qgk_bucket* b, double dt1, double dt2, double dt3) {
   b->primed = false;
   b->q_stale = false;
   if ((rc = c2r2d(b, b->field_c[QGK_QH], b->field_r[QGK_Q], np))) return rc;
-  HIPC(stream_done(b));
+  HIPC(hipGetLastError());  // no wait: see enqueue-only phases above
   return 0;
 }
 
