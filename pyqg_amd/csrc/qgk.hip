@@ -4935,6 +4935,8 @@ int qgk_diag_set_params(qgk_bucket* b, int member, const double* params) {
   }
   double* dst = b->dprm_h.data() + (size_t)member * plen;
   if (!b->dprm_set[member] || std::memcmp(dst, params, plen * sizeof(double)) != 0) {
+    // an accumulation may still be copying the table (accumulations do not wait)
+    HIPC(hipStreamSynchronize(b->stream));
     std::memcpy(dst, params, plen * sizeof(double));
     b->dprm_dirty = true;  // uploaded with the next accumulation
   }
@@ -5076,7 +5078,8 @@ int qgk_diag_accumulate(qgk_bucket* b, unsigned int mask, int instant, double dt
                        apegen ? b->dslot[QGK_D_APEGEN] : -1);
     HIPC(hipGetLastError());
   }
-  HIPC(hipStreamSynchronize(b->stream));
+  // no wait: the sums stay on the device until qgk_diag_read, which waits
+  HIPC(hipGetLastError());
   return 0;
 }
 
