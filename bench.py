@@ -107,8 +107,9 @@ def cpu_baseline(workload="qg256", nsteps=None):
     os.environ["OPENBLAS_NUM_THREADS"] = "1"
     if nsteps is None:
         nsteps = {"qg256": 500, "uneven": 500, "layered10": 60}[workload]
-    # one member per core of the CPU share a GPU box allots to one GPU (16
-    # cores; the host may show more, see affinity and cpu_count below)
+    # one member per core of the host's share per GPU (16 cores on the
+    # MI355X hosts this runs on; the host may show more: affinity and
+    # cpu_count are reported beside the result)
     affinity = len(os.sched_getaffinity(0))
     cores = min(affinity, 16)
     with cf.ProcessPoolExecutor(max_workers=cores, mp_context=mp.get_context("spawn")) as ex:
