@@ -103,6 +103,24 @@ def test_conv_forward_blocks_with_several_items(cnn, cinp, coutp, B):
     assert torch.equal(y_bf.cpu(), y32.cpu().to(torch.bfloat16))
 
 
+@pytest.mark.parametrize("cinp,coutp,B", [(64, 128, 20), (32, 32, 70)])
+def test_masked_convolution_several_items_per_block(cnn, cinp, coutp, B):
+    """The backward-data form (no ReLU, the previous layer's ReLU mask, no
+    bias) at batch sizes where the overlapped-staging kernel's blocks take
+    several items each."""
+    import torch
+    g = torch.Generator().manual_seed(cinp + coutp + B)
+    H = W = 64
+    x = rnd_bf16((B, H, W, cinp), 1.0, g)
+    w = rnd_bf16((coutp, 25, cinp), 0.1, g)
+    mk = torch.relu(rnd_bf16((B, H, W, coutp), 1.0, g))
+    y_bf, y32 = cnn.conv_fwd(x.to(torch.bfloat16).cuda(), w.to(torch.bfloat16).cuda(), None,
+                             coutp, relu=False, out_f32=True, mask=mk.to(torch.bfloat16).cuda())
+    ref = ref_conv(x, w) * (mk > 0)
+    assert rel(y32.cpu(), ref) < 1e-5
+    assert torch.equal(y_bf.cpu(), y32.cpu().to(torch.bfloat16))
+
+
 @pytest.mark.parametrize("B", [48, 80])
 def test_first_layer_kernel_several_items_per_block(cnn, B):
     """The first layer at the closure's shape (6 -> 128 channels, 64 x 64) with
