@@ -1302,7 +1302,16 @@ __global__ void k_wgrad_partial(const float* __restrict__ ws, float* __restrict_
   const int k0 = blockIdx.y * kSplitGroup;
   const int k1 = k0 + kSplitGroup < nsplit ? k0 + kSplitGroup : nsplit;
   float s = 0.f;
-  for (int k = k0; k < k1; ++k) s += ws[(size_t)k * n + i];
+  int k = k0;
+  // eight loads in flight, added in the same order as one at a time
+  for (; k + 8 <= k1; k += 8) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = ws[(size_t)(k + j) * n + i];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += v[j];
+  }
+  for (; k < k1; ++k) s += ws[(size_t)k * n + i];
   part[(size_t)blockIdx.y * n + i] = s;
 }
 
@@ -1344,9 +1353,17 @@ __global__ __launch_bounds__(256) void k_bias_grad(const bf16_t* __restrict__ dy
   const int lanes_per_c = blockDim.x / coutp;
   const int sub = threadIdx.x / coutp;
   float s = 0.f;
-  for (size_t p = blockIdx.x * (size_t)lanes_per_c + sub; p < npix;
-       p += (size_t)gridDim.x * lanes_per_c)
-    s += bf2f(dy[p * coutp + c]);
+  const size_t stride = (size_t)gridDim.x * lanes_per_c;
+  size_t p = blockIdx.x * (size_t)lanes_per_c + sub;
+  // eight loads in flight, added in the same order as one at a time
+  for (; p + 7 * stride < npix; p += 8 * stride) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = bf2f(dy[(p + j * stride) * coutp + c]);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += v[j];
+  }
+  for (; p < npix; p += stride) s += bf2f(dy[p * coutp + c]);
   red[threadIdx.x] = s;
   __syncthreads();
   if (sub == 0) {
@@ -1366,9 +1383,17 @@ __global__ __launch_bounds__(256) void k_bias_grad_part(const bf16_t* __restrict
   const int lanes_per_c = blockDim.x / coutp;
   const int sub = threadIdx.x / coutp;
   float s = 0.f;
-  for (size_t p = blockIdx.x * (size_t)lanes_per_c + sub; p < npix;
-       p += (size_t)gridDim.x * lanes_per_c)
-    s += bf2f(dy[p * coutp + c]);
+  const size_t stride = (size_t)gridDim.x * lanes_per_c;
+  size_t p = blockIdx.x * (size_t)lanes_per_c + sub;
+  // eight loads in flight, added in the same order as one at a time
+  for (; p + 7 * stride < npix; p += 8 * stride) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = bf2f(dy[(p + j * stride) * coutp + c]);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += v[j];
+  }
+  for (; p < npix; p += stride) s += bf2f(dy[p * coutp + c]);
   red[threadIdx.x] = s;
   __syncthreads();
   if (sub == 0) {
