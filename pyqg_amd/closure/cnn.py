@@ -194,6 +194,29 @@ def mse(pred, target, grad=None, gscale=1.0):
     return loss
 
 
+def spectral_loss_grad(out, target):
+    """Gradient with respect to the network output of the log-spectral
+    distance between prediction and target: mean over channels and 2-D
+    wavenumbers of (log E_pred - log E_target)^2, E the batch-mean power of the
+    real 2-D transform.  An MSE fit leaves out the part of the forcing its
+    inputs do not determine and with it power at every scale; this term asks
+    for the target's power scale by scale.  (torch autograd on the output
+    only; the network's own backward pass takes the result.)"""
+    nch = target.shape[-1]
+    o = out[..., :nch].detach().clone().requires_grad_(True)
+    with torch.enable_grad():
+        po = torch.fft.rfft2(o.permute(0, 3, 1, 2))
+        pt = torch.fft.rfft2(target.permute(0, 3, 1, 2))
+        eo = (po.real ** 2 + po.imag ** 2).mean(dim=0)
+        et = (pt.real ** 2 + pt.imag ** 2).mean(dim=0)
+        eps = 1e-6 * et.mean()
+        loss = ((torch.log(eo + eps) - torch.log(et + eps)) ** 2).mean()
+        g, = torch.autograd.grad(loss, o)
+    full = torch.zeros_like(out)
+    full[..., :nch] = g
+    return full
+
+
 # ---------------------------------------------------------------- network
 class CNN:
     """5x5 periodic convolutions with ReLU between them; channels[0] inputs,
@@ -342,14 +365,18 @@ class CNN:
                                  _stream()))
         self._repack()
 
-    def train_step(self, xp, target, lr=1e-3, clip=None):
+    def train_step(self, xp, target, lr=1e-3, clip=None, spec_weight=0.0):
         """One Adam step on the MSE between output channels and target
         (B,H,W,channels[-1]) FP32; returns the loss tensor.  The global
         gradient norm is kept in `last_grad_norm` (a device scalar); with
-        `clip` the gradients are scaled down to at most that norm first."""
+        `clip` the gradients are scaled down to at most that norm first.
+        spec_weight > 0 adds that times spectral_loss to the loss."""
         out, acts = self.forward(xp, keep=True)
         grad = torch.empty(out.shape, dtype=torch.bfloat16, device=out.device)
         loss = mse(out, target, grad)
+        if spec_weight:
+            g = spectral_loss_grad(out, target)
+            grad = (grad.float() + spec_weight * g).to(torch.bfloat16)
         dws, dbs = self.backward(acts, grad)
         gn = torch.sqrt(sum((d * d).sum() for d in dws) + sum((d * d).sum() for d in dbs))
         self.last_grad_norm = gn

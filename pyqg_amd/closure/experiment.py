@@ -46,13 +46,13 @@ BUDGET = ["KEflux", "APEflux", "APEgenspec", "KEfrictionspec", "Dissspec", "EKEd
 def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=64, lr=1e-3,
         online_members=128, online_days=720, avg_days=360, dt_lo=14400., seed=0, ema=0.998,
         ema_from=0.5, closures=("backscatter", "zb2020", "smagorinsky"), calibrate="none",
-        filter_kind="gaussian", stochastic=False, log=print):
+        filter_kind="gaussian", stochastic=False, spec_weight=0.0, log=print):
     out = {"config": dict(members=members, samples=samples, spinup_days=spinup_days,
                           every=every, epochs=epochs, batch=batch, lr=lr,
                           online_members=online_members, online_days=online_days,
                           avg_days=avg_days, dt_hi=3600., dt_lo=dt_lo, seed=seed, ema=ema,
                           ema_from=ema_from, calibrate=calibrate, filter=filter_kind,
-                          stochastic=stochastic)}
+                          stochastic=stochastic, spectral_loss=spec_weight)}
     L = data_mod.QG_EDDY["L"]
     g_hi, g_lo = tools.grid(256, L), tools.grid(64, L)
     # ---------------------------------------------------------- 1. data
@@ -98,7 +98,8 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
     net = cnn_mod.CNN(seed=seed)
     out["n_params"] = net.n_params()
     info = train_mod.train(net, x_tr, y_tr, scales, epochs=epochs, batch=batch, lr=lr,
-                           log=log, seed=seed, ema=ema, ema_from=ema_from)
+                           log=log, seed=seed, ema=ema, ema_from=ema_from,
+                           spec_weight=spec_weight)
     out["train"] = {"steps": info["steps"], "seconds": info["seconds"],
                     "final_loss": info["history"][-1][1] if info["history"] else None}
     out["offline"] = train_mod.evaluate(net, x_te, y_te, scales)
@@ -224,6 +225,9 @@ def main():
     ap.add_argument("--stochastic", action="store_true",
                     help="also run the network with an AR(1) stochastic forcing that has the "
                          "held-out residual's spectrum and time correlation")
+    ap.add_argument("--spectral-loss", type=float, default=0.0,
+                    help="weight of the log-spectral distance between predicted and target "
+                         "forcing added to the MSE in training (0: MSE only)")
     ap.add_argument("--out", default=None, help="also append the JSON line to this file")
     a = ap.parse_args()
     res = run(members=a.members, samples=a.samples, spinup_days=a.spinup_days, every=a.every,
@@ -231,6 +235,7 @@ def main():
               online_days=a.online_days, avg_days=a.avg_days, dt_lo=a.dt_lo, seed=a.seed,
               ema=a.ema, ema_from=a.ema_from, closures=[c for c in a.closures.split(",") if c],
               calibrate=a.calibrate, filter_kind=a.filter, stochastic=a.stochastic,
+              spec_weight=a.spectral_loss,
               log=lambda s: print(s, flush=True))
     line = json.dumps(res)
     print(line)

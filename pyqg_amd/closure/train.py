@@ -17,7 +17,7 @@ def _normalized(x, y, scales, idx, net):
 
 
 def train(net: CNN, x, y, scales: Scales, epochs=4, batch=64, lr=1e-3, lr_min=1e-5, seed=0,
-          log=None, clip_factor=4.0, ema=0.998, ema_from=0.5):
+          log=None, clip_factor=4.0, ema=0.998, ema_from=0.5, spec_weight=0.0):
     """Adam on the MSE of normalised forcing, cosine learning-rate decay.
     x (N, H, W, 6) and y (N, H, W, 2) FP32 tensors on the GPU.
 
@@ -45,7 +45,7 @@ def train(net: CNN, x, y, scales: Scales, epochs=4, batch=64, lr=1e-3, lr_min=1e
             xb, yb = _normalized(x, y, scales, idx, net)
             cur = lr_min + 0.5 * (lr - lr_min) * (1 + math.cos(math.pi * it / steps))
             clip = clip_factor * gn_mean if (clip_factor and it >= 50) else None
-            loss = net.train_step(xb, yb, lr=cur, clip=clip)
+            loss = net.train_step(xb, yb, lr=cur, clip=clip, spec_weight=spec_weight)
             if ema and it >= int(ema_from * steps):
                 params = list(net.w32) + list(net.b32)
                 if avg is None:
