@@ -1187,11 +1187,13 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const bf16_t* __restrict__ d
 // 25 taps (each wave one 16 x 16 tile, 25 accumulators) and the rows
 // [r_begin, r_end) of one sample.  Per output row it stages dy (transposed to
 // [co][px]) and one new x row (transposed to [ci][px] with two periodic pixels
-// of padding on each side) into a five-row LDS ring.  For each 32-pixel K step
+// of padding on each side) into a five-row LDS ring, the row's loads issued one
+// row ahead so that they overlap the matrix ops.  For each 32-pixel K step
 // one A read (dy) and two aligned 16-byte B reads per x row give the B
 // operands of all five horizontal shifts (byte-aligned in registers), so one
 // dy read feeds 25 MFMAs.  The block's sums go to its own slice of a workspace
-// ([split][tap][co][ci]) that k_wgrad_reduce adds into dW.
+// ([split][16x16 tile][tap][lane][4]) that k_wgrad_partial and k_wgrad_reduce
+// add into dW.
 // --------------------------------------------------------------------------
 __device__ __forceinline__ v8bf window8(const uint4& c0, const uint4& c1, int dx) {
   unsigned w[4];
@@ -1220,7 +1222,6 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_mt(const bf16_t* __restrict_
   __shared__ __attribute__((aligned(16))) bf16_t sdy[CO * LS];
   __shared__ __attribute__((aligned(16))) bf16_t sx[5 * CI * LS];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int nthreads = blockDim.x;
   const int tiles_ci = cinp / CI;
   const int cot = blockIdx.x / tiles_ci, cit = blockIdx.x % tiles_ci;
   const int co_base = cot * CO, ci_base = cit * CI;
@@ -1234,36 +1235,81 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_mt(const bf16_t* __restrict_
 #pragma unroll
   for (int t = 0; t < NTAP; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  // stage x row q (logical, periodic) into ring slot q mod 5
-  auto stage_x = [&](int q) {
+  // Staging: thread tid holds x pieces tid + NTH i (16 bytes: 8 channels of
+  // one padded pixel) and dy pieces tid + NTH i (8 channels of one pixel).
+  // All of a row's loads are issued before any of its LDS writes, and the
+  // next row's loads are in flight during this row's matrix ops.
+  constexpr int NTH = 64 * WC * WI;
+  constexpr int XPC = XP * (CI / 8), DPC = W * (CO / 8);
+  constexpr int NXQ = (XPC + NTH - 1) / NTH, NDQ = (DPC + NTH - 1) / NTH;
+  typedef unsigned v4u __attribute__((ext_vector_type(4)));
+  v4u xv[NXQ], dv[NDQ];
+  const int tid = threadIdx.x;
+  auto load_x = [&](int q) {
     const int yy = ((q % H) + H) % H;
-    const int slot = ((q % 5) + 5) % 5;
-    for (int pc = threadIdx.x; pc < XP * (CI / 8); pc += nthreads) {
-      const int e = pc / (CI / 8), c8 = (pc % (CI / 8)) * 8;
-      const int px = (e - 2 + W) % W;
-      const uint4 v = *reinterpret_cast<const uint4*>(
-          x + (((size_t)b * H + yy) * W + px) * cinp + ci_base + c8);
-      const bf16_t* hv = reinterpret_cast<const bf16_t*>(&v);
-#pragma unroll
-      for (int c = 0; c < 8; ++c) sx[(slot * CI + c8 + c) * LS + e] = hv[c];
-    }
+    static_for<0, NXQ>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int pc = tid + NTH * i;
+      if (XPC % NTH == 0 || pc < XPC) {
+        const int e = pc / (CI / 8), c8 = (pc % (CI / 8)) * 8;
+        const int px = (e - 2 + W) % W;
+        xv[i] = *reinterpret_cast<const v4u*>(x + (((size_t)b * H + yy) * W + px) * cinp +
+                                              ci_base + c8);
+      }
+    });
   };
-  stage_x(r_begin - 2);
-  stage_x(r_begin - 1);
-  stage_x(r_begin);
-  stage_x(r_begin + 1);
+  // x row q into ring slot q mod 5
+  auto write_x = [&](int q) {
+    const int slot = ((q % 5) + 5) % 5;
+    static_for<0, NXQ>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int pc = tid + NTH * i;
+      if (XPC % NTH == 0 || pc < XPC) {
+        const int e = pc / (CI / 8), c8 = (pc % (CI / 8)) * 8;
+        const bf16_t* hv = reinterpret_cast<const bf16_t*>(&xv[i]);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) sx[(slot * CI + c8 + c) * LS + e] = hv[c];
+      }
+    });
+  };
+  auto load_dy = [&](int r) {
+    static_for<0, NDQ>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int pc = tid + NTH * i;
+      if (DPC % NTH == 0 || pc < DPC) {
+        const int px = pc / (CO / 8), c8 = (pc % (CO / 8)) * 8;
+        dv[i] = *reinterpret_cast<const v4u*>(dy + (((size_t)b * H + r) * W + px) * coutp +
+                                              co_base + c8);
+      }
+    });
+  };
+  auto write_dy = [&]() {
+    static_for<0, NDQ>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int pc = tid + NTH * i;
+      if (DPC % NTH == 0 || pc < DPC) {
+        const int px = pc / (CO / 8), c8 = (pc % (CO / 8)) * 8;
+        const bf16_t* hv = reinterpret_cast<const bf16_t*>(&dv[i]);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) sdy[(c8 + c) * LS + px] = hv[c];
+      }
+    });
+  };
+  for (int q = r_begin - 2; q < r_begin + 2; ++q) {
+    load_x(q);
+    write_x(q);
+  }
+  load_x(r_begin + 2);
+  load_dy(r_begin);
   for (int r = r_begin; r < r_end; ++r) {
     __syncthreads();  // previous row's reads are done
-    stage_x(r + 2);
-    for (int pc = threadIdx.x; pc < W * (CO / 8); pc += nthreads) {
-      const int px = pc / (CO / 8), c8 = (pc % (CO / 8)) * 8;
-      const uint4 v = *reinterpret_cast<const uint4*>(
-          dy + (((size_t)b * H + r) * W + px) * coutp + co_base + c8);
-      const bf16_t* hv = reinterpret_cast<const bf16_t*>(&v);
-#pragma unroll
-      for (int c = 0; c < 8; ++c) sdy[(c8 + c) * LS + px] = hv[c];
-    }
+    write_x(r + 2);
+    write_dy();
     __syncthreads();
+    if (r + 1 < r_end) {
+      load_x(r + 3);
+      load_dy(r + 1);
+    }
 #pragma unroll
     for (int kk = 0; kk < W / 32; ++kk) {
       const int p0 = kk * 32 + lg * 8;
@@ -1281,15 +1327,13 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_mt(const bf16_t* __restrict_
       }
     }
   }
-  // lane holds D[co = 4 lg + j][ci = li] of its wave's tile, for every tap
-  float* out = ws + (size_t)split * NTAP * coutp * cinp;
+  // lane holds D[co = 4 lg + j][ci = li] of its wave's tile, for every tap:
+  // the workspace keeps it as [16x16 tile][tap][lane][j], so that each lane
+  // stores 16 bytes per tap and the wave 1 KB in a row
+  const int tile = (cot * WC + wc) * (cinp / 16) + cit * WI + wi;
+  float* out = ws + (size_t)split * NTAP * coutp * cinp + (size_t)tile * NTAP * 256;
 #pragma unroll
-  for (int t = 0; t < NTAP; ++t)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int co = co_base + wc * 16 + lg * 4 + j, ci = ci_base + wi * 16 + li;
-      out[((size_t)t * coutp + co) * cinp + ci] = acc[t][j];
-    }
+  for (int t = 0; t < NTAP; ++t) *reinterpret_cast<v4f*>(out + (t * 64 + lane) * 4) = acc[t];
 }
 
 // First stage of the workspace reduction: part[g][i] = sum of ws[k][i] over
@@ -1321,9 +1365,12 @@ __global__ void k_wgrad_reduce(const float* __restrict__ part, float* __restrict
   const size_t n = (size_t)NTAP * coutp * cinp;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
-    const int ci = (int)(i % cinp);
-    const int co = (int)((i / cinp) % coutp);
-    const int tap = (int)(i / ((size_t)cinp * coutp));
+    // workspace order [16x16 (co, ci) tile][tap][lane][j]: lane = li + 16 lg
+    // holds co = 16 tco + 4 lg + j, ci = 16 tci + li
+    const int j = (int)(i % 4), lane = (int)((i / 4) % 64);
+    const int tap = (int)((i / 256) % NTAP), tile = (int)(i / (256 * NTAP));
+    const int co = (tile / (cinp / 16)) * 16 + (lane / 16) * 4 + j;
+    const int ci = (tile % (cinp / 16)) * 16 + lane % 16;
     float s = 0.f;
     for (int g = 0; g < ngroups; ++g) s += part[(size_t)g * n + i];
     dw[((size_t)co * NTAP + tap) * cinp + ci] += s;
@@ -1678,14 +1725,19 @@ struct WgradPlan {
 
 WgradPlan wgrad_plan(int B, int H, int cinp, int coutp) {
   WgradPlan p;
-  p.wc = coutp >= 32 ? 2 : 1;
-  p.wi = cinp >= 32 ? 2 : 1;
-  if (p.wi == 1 && coutp >= 64) p.wc = 4;
-  if (p.wc == 1 && cinp >= 64) p.wi = 4;
+  // waves per block along co and ci: counts that divide the 16-channel tiles
+  const int nco = coutp / 16, nci = cinp / 16;
+  p.wc = nco % 2 == 0 ? 2 : 1;
+  p.wi = nci % 2 == 0 ? 2 : 1;
+  if (p.wi == 1 && nco % 4 == 0) p.wc = 4;
+  if (p.wc == 1 && nci % 4 == 0) p.wi = 4;
   p.tiles = (coutp / (16 * p.wc)) * (cinp / (16 * p.wi));
+  // rows split until about two blocks per CU: each block writes a partial
+  // of 25 x coutp x cinp floats, which the reductions read back, so more
+  // splits cost more than they hide
   p.rsplit = 1;
   while (p.rsplit < 8 && H % (2 * p.rsplit) == 0 && H / (2 * p.rsplit) >= 4 &&
-         (size_t)p.tiles * B * p.rsplit < 1024)
+         (size_t)p.tiles * B * p.rsplit < 512)
     p.rsplit *= 2;
   p.nsplit = B * p.rsplit;
   return p;

@@ -251,7 +251,8 @@ def test_weight_and_bias_gradient(cnn, cinp, coutp):
 
 
 @pytest.mark.parametrize("B,H,cinp,coutp", [(3, 10, 16, 128), (2, 12, 128, 64),
-                                            (1, 64, 32, 32), (2, 8, 32, 16), (2, 16, 64, 16)])
+                                            (1, 64, 32, 32), (2, 8, 32, 16), (2, 16, 64, 16),
+                                            (1, 8, 32, 48), (1, 8, 48, 32), (2, 8, 96, 80)])
 def test_blocked_weight_gradient_matches_atomic(cnn, B, H, cinp, coutp):
     """The matrix-core weight gradient (workspace, K split over rows) equals the
     atomic kernel and the float reference on odd batch and row counts."""
