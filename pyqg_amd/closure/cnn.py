@@ -336,11 +336,25 @@ class CNN:
     def backward(self, acts, dout):
         """dout: bf16 gradient of the loss w.r.t. the last (pre-activation) output.
         Returns FP32 (dW, db) per layer."""
+        # every gradient is a view of one zeroed buffer (segments on 256-byte
+        # boundaries), so that the norm and the clip are one launch each
+        def seg(n):
+            return (n + 63) // 64 * 64
+        sizes = [(self.w32[l].numel(), self.b32[l].numel()) for l in range(self.nlayers)]
+        flat = torch.zeros(sum(seg(a) + seg(b) for a, b in sizes), dtype=torch.float32,
+                           device=dout.device)
+        views, o = [], 0
+        for l, (a, b) in enumerate(sizes):
+            dw_l = flat[o:o + a].view(self.w32[l].shape)
+            o += seg(a)
+            db_l = flat[o:o + b].view(self.b32[l].shape)
+            o += seg(b)
+            views.append((dw_l, db_l))
+        self._grad_flat = flat
         dws, dbs = [], []
         d = dout
         for l in reversed(range(self.nlayers)):
-            dw = torch.zeros_like(self.w32[l])
-            db = torch.zeros_like(self.b32[l])
+            dw, db = views[l]
             conv_wgrad(d, acts[l], dw)
             bias_grad(d, db)
             dws.append(dw)
@@ -378,12 +392,11 @@ class CNN:
             g = spectral_loss_grad(out, target)
             grad = (grad.float() + spec_weight * g).to(torch.bfloat16)
         dws, dbs = self.backward(acts, grad)
-        gn = torch.sqrt(sum((d * d).sum() for d in dws) + sum((d * d).sum() for d in dbs))
+        # the gradients are views of one buffer whose padding stays zero
+        gn = torch.linalg.vector_norm(self._grad_flat)
         self.last_grad_norm = gn
         if clip is not None:
-            s = torch.clamp(clip / (gn + 1e-30), max=1.0)
-            for d in list(dws) + list(dbs):
-                d.mul_(s)
+            self._grad_flat.mul_(torch.clamp(clip / (gn + 1e-30), max=1.0))
         self.adam_step(dws, dbs, lr=lr)
         return loss
 

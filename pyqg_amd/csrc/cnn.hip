@@ -1471,17 +1471,42 @@ __global__ void k_mse(const float* __restrict__ pred, const float* __restrict__ 
                       size_t npix, int coutp, int nch, float inv_n, float gscale,
                       bf16_t* __restrict__ grad, float* __restrict__ loss) {
   float s = 0.f;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < npix * coutp;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i % coutp);
-    const size_t p = i / coutp;
-    float g = 0.f;
-    if (c < nch) {
-      const float d = pred[i] - target[p * nch + c];
-      s += d * d;
-      g = 2.f * d * inv_n * gscale;
+  if (coutp % 8 == 0) {
+    // 8 channels of one pixel per thread and step: 32-byte reads and a
+    // 16-byte gradient store, consecutive lanes on consecutive pieces
+    const int cg = coutp / 8;
+    for (size_t i8 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i8 < npix * cg;
+         i8 += (size_t)gridDim.x * blockDim.x) {
+      const size_t p = i8 / cg;
+      const int c0 = (int)(i8 - p * cg) * 8;
+      const v4f a0 = *reinterpret_cast<const v4f*>(pred + i8 * 8);
+      const v4f a1 = *reinterpret_cast<const v4f*>(pred + i8 * 8 + 4);
+      v8s g8;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        float g = 0.f;
+        if (c0 + j < nch) {
+          const float d = (j < 4 ? a0[j] : a1[j - 4]) - target[p * nch + c0 + j];
+          s += d * d;
+          g = 2.f * d * inv_n * gscale;
+        }
+        g8[j] = (short)f2bf(g);
+      }
+      if (grad) *reinterpret_cast<v8s*>(grad + i8 * 8) = g8;
     }
-    if (grad) grad[i] = f2bf(g);
+  } else {
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < npix;
+         p += (size_t)gridDim.x * blockDim.x) {
+      for (int c = 0; c < coutp; ++c) {
+        float g = 0.f;
+        if (c < nch) {
+          const float d = pred[p * coutp + c] - target[p * nch + c];
+          s += d * d;
+          g = 2.f * d * inv_n * gscale;
+        }
+        if (grad) grad[p * coutp + c] = f2bf(g);
+      }
+    }
   }
   // wave reduction, then the block's waves through LDS, one atomic per block
   __shared__ float red[16];
@@ -1823,7 +1848,10 @@ int qgcnn_mse(const float* pred, const float* target, size_t npix, int coutp, in
               float gscale, void* grad, float* loss, void* stream) {
   if (!pred || !target || nch < 1 || nch > coutp) return fail(QGCNN_E_ARG, "bad arguments");
   const float inv_n = 1.0f / (float)((double)npix * nch);
-  const int blocks = grid1(npix * coutp) < 1024 ? grid1(npix * coutp) : 1024;
+  if (coutp % 8 == 0 && !all_aligned16({pred, grad}))
+    return fail(QGCNN_E_ARG, "pred and grad must start on 16-byte boundaries");
+  const size_t work = coutp % 8 == 0 ? npix * (coutp / 8) : npix;
+  const int blocks = grid1(work) < 1024 ? grid1(work) : 1024;
   hipLaunchKernelGGL(k_mse, dim3(blocks), dim3(256), 0, (hipStream_t)stream, pred, target, npix,
                      coutp, nch, inv_n, gscale, (bf16_t*)grad, loss);
   HIPC(hipGetLastError());

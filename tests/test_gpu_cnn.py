@@ -275,18 +275,24 @@ def test_blocked_weight_gradient_matches_atomic(cnn, B, H, cinp, coutp):
     assert rel(d2.cpu(), wr.grad) < 1e-5
 
 
-def test_mse_loss_and_gradient(cnn):
+@pytest.mark.parametrize("coutp,nch", [(16, 2), (4, 2), (24, 10)])
+def test_mse_loss_and_gradient(cnn, coutp, nch):
+    """Loss and gradient over the first nch of coutp channels (16-byte path
+    for coutp a multiple of 8, with real channels across a piece boundary;
+    per-channel path for 4)."""
     import torch
-    g = torch.Generator().manual_seed(3)
-    pred = torch.randn((2, 8, 64, 16), generator=g)
-    target = torch.randn((2, 8, 64, 2), generator=g)
+    g = torch.Generator().manual_seed(3 + coutp)
+    pred = torch.randn((2, 8, 64, coutp), generator=g)
+    target = torch.randn((2, 8, 64, nch), generator=g)
     grad = torch.empty(pred.shape, dtype=torch.bfloat16, device="cuda")
     loss = cnn.mse(pred.cuda(), target.cuda(), grad)
-    ref = ((pred[..., :2] - target) ** 2).mean()
+    ref = ((pred[..., :nch] - target) ** 2).mean()
     assert abs(float(loss) - float(ref)) < 1e-5 * float(ref)
     gref = torch.zeros_like(pred)
-    gref[..., :2] = 2 * (pred[..., :2] - target) / target.numel()
+    gref[..., :nch] = 2 * (pred[..., :nch] - target) / target.numel()
     assert rel(grad.float().cpu(), gref) < 1e-2
+    if nch < coutp:
+        assert grad[..., nch:].float().abs().max().item() == 0.0
 
 
 def test_network_gradients_match_torch(cnn):
