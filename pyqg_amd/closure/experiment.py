@@ -46,13 +46,13 @@ BUDGET = ["KEflux", "APEflux", "APEgenspec", "KEfrictionspec", "Dissspec", "EKEd
 def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=64, lr=1e-3,
         online_members=128, online_days=720, avg_days=360, dt_lo=14400., seed=0, ema=0.998,
         ema_from=0.5, closures=("backscatter", "zb2020", "smagorinsky"), calibrate="none",
-        filter_kind="gaussian", stochastic=False, spec_weight=0.0, log=print):
+        filter_kind="gaussian", stochastic=False, spec_weight=0.0, nets=1, log=print):
     out = {"config": dict(members=members, samples=samples, spinup_days=spinup_days,
                           every=every, epochs=epochs, batch=batch, lr=lr,
                           online_members=online_members, online_days=online_days,
                           avg_days=avg_days, dt_hi=3600., dt_lo=dt_lo, seed=seed, ema=ema,
                           ema_from=ema_from, calibrate=calibrate, filter=filter_kind,
-                          stochastic=stochastic, spectral_loss=spec_weight)}
+                          stochastic=stochastic, spectral_loss=spec_weight, nets=nets)}
     L = data_mod.QG_EDDY["L"]
     g_hi, g_lo = tools.grid(256, L), tools.grid(64, L)
     # ---------------------------------------------------------- 1. data
@@ -125,6 +125,16 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
         out["offline"]["residual_lag_correlation"] = rho.tolist()
         out["offline"]["noise_phi"] = phi
         log(f"residual: one-interval correlation {rho}, phi per coarse step {phi:.3f}")
+    extra = []
+    for k in range(1, nets):
+        # more networks on the same pairs, other initial weights and batch orders
+        nk = cnn_mod.CNN(seed=seed + 1000 * k)
+        train_mod.train(nk, x_tr, y_tr, scales, epochs=epochs, batch=batch, lr=lr,
+                        log=lambda s: None, seed=seed + 1000 * k, ema=ema, ema_from=ema_from,
+                        spec_weight=spec_weight)
+        out.setdefault("offline_extra", []).append(train_mod.evaluate(nk, x_te, y_te, scales))
+        extra.append(nk)
+        log(f"network {k + 1} of {nets}: offline R^2 = {out['offline_extra'][-1]['r2']:.3f}")
     flop_per_sample = 2 * 64 * 64 * 25 * sum(
         net.chp[l] * net.chp[l + 1] for l in range(net.nlayers))
     out["train"]["tflops"] = 3 * flop_per_sample * info["steps"] * batch / info["seconds"] / 1e12
@@ -144,6 +154,8 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
                "smagorinsky": builtin.Smagorinsky()}
     views = {}
     plan = [("plain", None), ("cnn", p)]
+    if extra:
+        plan.append(("cnn_avg", param_mod.CNNAverageParameterization([net] + extra, scales)))
     if noise is not None:
         plan.append(("cnn_stochastic", p))
     for name, prm in plan + [(c, choices[c]) for c in closures]:
@@ -228,6 +240,9 @@ def main():
     ap.add_argument("--spectral-loss", type=float, default=0.0,
                     help="weight of the log-spectral distance between predicted and target "
                          "forcing added to the MSE in training (0: MSE only)")
+    ap.add_argument("--nets", type=int, default=1,
+                    help="also train this many networks in all (other seeds) and run the "
+                         "mean of their forcings online as cnn_avg")
     ap.add_argument("--out", default=None, help="also append the JSON line to this file")
     a = ap.parse_args()
     res = run(members=a.members, samples=a.samples, spinup_days=a.spinup_days, every=a.every,
@@ -235,7 +250,7 @@ def main():
               online_days=a.online_days, avg_days=a.avg_days, dt_lo=a.dt_lo, seed=a.seed,
               ema=a.ema, ema_from=a.ema_from, closures=[c for c in a.closures.split(",") if c],
               calibrate=a.calibrate, filter_kind=a.filter, stochastic=a.stochastic,
-              spec_weight=a.spectral_loss,
+              spec_weight=a.spectral_loss, nets=a.nets,
               log=lambda s: print(s, flush=True))
     line = json.dumps(res)
     print(line)

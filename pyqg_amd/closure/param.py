@@ -63,6 +63,29 @@ class CNNParameterization:
         return y[0].permute(2, 0, 1).double().cpu().numpy()
 
 
+class CNNAverageParameterization(CNNParameterization):
+    """The mean of several trained networks' forcings (same inputs and
+    scales): averaging independently trained networks narrows the spread the
+    training seed leaves in the learned closure.  `net` is the first network
+    (its input layout is the common one); the closed ensemble runs every
+    network and adds each forcing with weight 1/len(nets)."""
+
+    def __init__(self, nets, scales: Scales):
+        if not nets:
+            raise ValueError("no networks to average")
+        if any(n.chp[0] != nets[0].chp[0] or n.channels[-1] != nets[0].channels[-1]
+               for n in nets):
+            raise ValueError("the averaged networks need the same input and output channels")
+        super().__init__(nets[0], scales)
+        self.nets = list(nets)
+
+    def predict(self, x):
+        xin = x / torch.as_tensor(self.scales.in_scale, dtype=torch.float32, device=x.device)
+        nout = self.net.channels[-1]
+        out = sum(n.forward(n.pad_input(xin))[..., :nout] for n in self.nets) / len(self.nets)
+        return out * torch.as_tensor(self.scales.out_scale, dtype=torch.float32, device=x.device)
+
+
 class ClosedEnsemble:
     """Coarse two-layer QG ensemble stepped with the CNN closure on the GPU.
 
@@ -93,13 +116,19 @@ class ClosedEnsemble:
             # zeros: the fused row kernel writes only the 3*nz input channels
             self.xin = torch.zeros((nmembers, n, n, self.cinp), dtype=torch.bfloat16,
                                    device="cuda")
-            self.fused = bool(fused) and n <= 128 and self.cinp % 8 == 0
+            self.nets = getattr(param, "nets", [param.net])
+            # several networks: their forcings are added one by one after the
+            # friction (the phase-by-phase form)
+            self.fused = (bool(fused) and n <= 128 and self.cinp % 8 == 0 and
+                          len(self.nets) == 1)
             if self.fused:
                 self.b.set_layer_split(False)
                 self.fused = self.b.exchange_columns() == n // 2
-            self.acts = param.net.activation_buffers(nmembers, n, n)
+            self.actss = [net.activation_buffers(nmembers, n, n) for net in self.nets]
+            self.acts = self.actss[0]
             self.inv_in = np.ascontiguousarray(1.0 / param.scales.in_scale)
             self.out_scale = np.ascontiguousarray(param.scales.out_scale)
+            self.out_scale_k = np.ascontiguousarray(param.scales.out_scale / len(self.nets))
             self.stream = ctypes.c_void_p(_lib.lib.qgk_stream(self.b.h))
             self._graph_key = None
 
@@ -113,6 +142,9 @@ class ClosedEnsemble:
         units, phi the one-step autocorrelation; None switches it off."""
         if self.param is None or self.builtin:
             raise ValueError("the stochastic forcing goes with the network closure")
+        if len(self.nets) > 1:
+            raise ValueError("the stochastic forcing goes with a single network (each "
+                             "network's forcing add would draw it again)")
         self.b.set_spectral_noise(amp, phi, seed)
         self._graph_key = None
 
@@ -134,9 +166,8 @@ class ClosedEnsemble:
             nsteps -= 1
         if nsteps >= 2:
             w, self.ens.ablevel = C.ab_weights(self.ens.ablevel, self.ens.dt)
-            net = self.param.net
-            key = (tuple(t.data_ptr() for t in net.wb), tuple(t.data_ptr() for t in net.b32),
-                   tuple(w))
+            key = (tuple(t.data_ptr() for net in self.nets for t in net.wb),
+                   tuple(t.data_ptr() for net in self.nets for t in net.b32), tuple(w))
             ok = self._graph_key == key
             if not ok or L.qgk_graph_replay(h, 0) != 0:
                 # allocations and uploads before a capture; the history phase
@@ -181,9 +212,11 @@ class ClosedEnsemble:
         _lib.check(L.qgk_advect(h))
         _lib.check(L.qgk_friction(h))
         _lib.check(L.qgk_pack_inputs(h, _lib.ptr(self.inv_in), self.cinp, ctypes_p(self.xin)))
-        out = self.param.net.forward_into(self.xin, self.acts, self.stream)
-        _lib.check(L.qgk_add_forcing_device(h, ctypes_p(out), out.shape[-1],
-                                            _lib.ptr(self.out_scale)))
+        scale = self.out_scale if len(self.nets) == 1 else self.out_scale_k
+        for net, acts in zip(self.nets, self.actss):
+            out = net.forward_into(self.xin, acts, self.stream)
+            _lib.check(L.qgk_add_forcing_device(h, ctypes_p(out), out.shape[-1],
+                                                _lib.ptr(scale)))
         _lib.check(L.qgk_timestep(h, *w))
 
     def _cnn_step(self):

@@ -220,6 +220,44 @@ def test_fused_closed_step_matches_phases(mods):
         assert rel_l2(a.ens.get("dqhdt_p", i), b.ens.get("dqhdt_p", i)) < 1e-6
 
 
+def test_averaged_networks_closure(mods):
+    """Several networks' forcings averaged in the closed step: two copies of
+    one network give that network's trajectory; two different networks give,
+    over one step from the same state, the mean of their two tendencies; the
+    graph-replayed steps equal the step-by-step ones; the stochastic forcing
+    is refused (each forcing add would draw it again)."""
+    pq, cnn, _, param, _ = mods
+    p1 = _small_closure(cnn, param, 4)
+    p2 = param.CNNParameterization(_small_closure(cnn, param, 9).net, p1.scales)
+    twice = param.CNNAverageParameterization([p1.net, p1.net], p1.scales)
+    a = param.ClosedEnsemble(3, 64, 3600., dict(QG_PHYS), param=p1, seed=7, fused=False)
+    b = param.ClosedEnsemble(3, 64, 3600., dict(QG_PHYS), param=twice, seed=7)
+    assert not b.fused
+    for _ in range(6):
+        a._cnn_step()
+        b._cnn_step()
+    for i in range(3):
+        assert rel_l2(a.ens.get("q", i), b.ens.get("q", i)) < 1e-12
+    avg = param.CNNAverageParameterization([p1.net, p2.net], p1.scales)
+    e1, e2, ea = (param.ClosedEnsemble(3, 64, 3600., dict(QG_PHYS), param=p, seed=8,
+                                       fused=False) for p in (p1, p2, avg))
+    for e in (e1, e2, ea):
+        e._cnn_step()
+    for i in range(3):
+        mean = 0.5 * (e1.ens.get("dqhdt_p", i) + e2.ens.get("dqhdt_p", i))
+        assert rel_l2(ea.ens.get("dqhdt_p", i), mean) < 1e-10
+        assert rel_l2(e1.ens.get("dqhdt_p", i), e2.ens.get("dqhdt_p", i)) > 1e-6
+    g = param.ClosedEnsemble(3, 64, 3600., dict(QG_PHYS), param=avg, seed=8)
+    h = param.ClosedEnsemble(3, 64, 3600., dict(QG_PHYS), param=avg, seed=8)
+    g.step(10)
+    for _ in range(10):
+        h._cnn_step()
+    for i in range(3):
+        assert np.array_equal(g.ens.get("q", i), h.ens.get("q", i))
+    with pytest.raises(ValueError):
+        g.set_noise(np.zeros((2, 64, 33)), 0.5)
+
+
 def test_fused_closed_step_without_forcing_is_the_plain_step(mods):
     """With the network's last layer zeroed the forcing is exactly zero, and
     the fused closed step is the plain fused step."""
