@@ -105,6 +105,7 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
     out["offline"] = train_mod.evaluate(net, x_te, y_te, scales)
     amp = train_mod.amplitude(net, x_te, y_te, scales)
     out["offline"]["amplitude"] = amp
+    scales_fit = scales
     if calibrate == "variance":
         # restore the forcing's held-out mean square, layer by layer: the MSE fit
         # leaves out the part of the forcing its inputs do not determine, and
@@ -129,12 +130,21 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
     for k in range(1, nets):
         # more networks on the same pairs, other initial weights and batch orders
         nk = cnn_mod.CNN(seed=seed + 1000 * k)
-        train_mod.train(nk, x_tr, y_tr, scales, epochs=epochs, batch=batch, lr=lr,
+        train_mod.train(nk, x_tr, y_tr, scales_fit, epochs=epochs, batch=batch, lr=lr,
                         log=lambda s: None, seed=seed + 1000 * k, ema=ema, ema_from=ema_from,
                         spec_weight=spec_weight)
-        out.setdefault("offline_extra", []).append(train_mod.evaluate(nk, x_te, y_te, scales))
+        out.setdefault("offline_extra", []).append(train_mod.evaluate(nk, x_te, y_te,
+                                                                      scales_fit))
         extra.append(nk)
         log(f"network {k + 1} of {nets}: offline R^2 = {out['offline_extra'][-1]['r2']:.3f}")
+    scales_avg = scales_fit
+    if extra and calibrate == "variance":
+        # the mean of the networks, calibrated as one network would be
+        amp_avg = train_mod.amplitude([net] + extra, x_te, y_te, scales_fit)
+        gain_avg = np.sqrt(np.asarray(amp_avg["target_ms"]) / np.asarray(amp_avg["pred_ms"]))
+        scales_avg = param_mod.Scales(scales_fit.in_scale, scales_fit.out_scale * gain_avg)
+        out["offline"]["gain_mean_of_networks"] = gain_avg.tolist()
+        log(f"variance calibration gain of the mean of {nets} networks {gain_avg}")
     flop_per_sample = 2 * 64 * 64 * 25 * sum(
         net.chp[l] * net.chp[l + 1] for l in range(net.nlayers))
     out["train"]["tflops"] = 3 * flop_per_sample * info["steps"] * batch / info["seconds"] / 1e12
@@ -155,7 +165,8 @@ def run(members=256, samples=51200, spinup_days=1825, every=24, epochs=8, batch=
     views = {}
     plan = [("plain", None), ("cnn", p)]
     if extra:
-        plan.append(("cnn_avg", param_mod.CNNAverageParameterization([net] + extra, scales)))
+        plan.append(("cnn_avg", param_mod.CNNAverageParameterization([net] + extra,
+                                                                     scales_avg)))
     if noise is not None:
         plan.append(("cnn_stochastic", p))
     for name, prm in plan + [(c, choices[c]) for c in closures]:

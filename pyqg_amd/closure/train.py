@@ -87,11 +87,14 @@ def evaluate(net: CNN, x, y, scales: Scales, batch=128):
 
 
 @torch.no_grad()
-def amplitude(net: CNN, x, y, scales: Scales, batch=128):
+def amplitude(net, x, y, scales: Scales, batch=128):
     """Per output channel (layer) on held-out pairs: mean square of the target,
     of the prediction, and their mean product (normalised units).  An MSE
     fit shrinks the prediction towards the conditional mean: its mean square
-    falls short of the target's by the unexplained variance."""
+    falls short of the target's by the unexplained variance.  `net` may be a
+    list of networks with one input layout: the prediction is their mean."""
+    nets = list(net) if isinstance(net, (list, tuple)) else [net]
+    net = nets[0]
     nout = y.shape[-1]
     yy = torch.zeros(nout, dtype=torch.float64, device=x.device)
     pp = torch.zeros_like(yy)
@@ -100,7 +103,7 @@ def amplitude(net: CNN, x, y, scales: Scales, batch=128):
     for s in range(0, x.shape[0], batch):
         idx = torch.arange(s, min(s + batch, x.shape[0]), device=x.device)
         xb, yb = _normalized(x, y, scales, idx, net)
-        out = net.forward(xb)[..., :nout].double()
+        out = sum(n.forward(xb)[..., :nout].double() for n in nets) / len(nets)
         yb = yb.double()
         yy += (yb ** 2).sum(dim=(0, 1, 2))
         pp += (out ** 2).sum(dim=(0, 1, 2))
